@@ -1,0 +1,350 @@
+"""fastpoly_amd — MI355X (gfx950) build of fastpoly's exact NTT convolution path.
+
+Python mirror of the reference API (fpx::poly::conv / inv / log / exp / sqrt,
+NttPlan<M>, release_scratch_memory / scratch_memory_stats) over the C ABI of
+libfpxg.so (include/fpxg.h).  Coefficients are the reference's raw Montgomery
+limbs (x * 2^32 mod p) as uint32 numpy arrays; results are bit-identical to
+the reference CPU library.
+
+Errors map onto the reference's exceptions: DomainError (poly::domain_error),
+NttSizeError (ntt_size_error / std::length_error), MemoryError (bad_alloc).
+There is no CPU fallback: importing works without a GPU, but every compute
+call needs the built library and a device, and raises otherwise.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+__all__ = [
+    "DomainError", "NttSizeError", "DeviceError", "UnsupportedModulus", "lib", "lib_path",
+    "conv", "inv", "log", "exp", "sqrt", "series", "ntt_forward", "ntt_inverse", "NttPlan",
+    "release_scratch_memory", "scratch_memory_stats", "from_int", "to_int", "Mont", "DeviceArray",
+    "EXPORTED_SYMBOLS",
+]
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_NAME = "libfpxg.so"
+
+OK, E_DOMAIN, E_SIZE, E_NOMEM, E_DEVICE, E_UNSUPPORTED, E_ARG = range(7)
+OPS = {"inv": 0, "log": 1, "exp": 2, "sqrt": 3}
+NTT_FORWARD, NTT_FORWARD_PREFIX, NTT_INVERSE, NTT_INVERSE_PREFIX, NTT_INVERSE_HIGHHALF = range(5)
+SIZE_MAX = (1 << 64) - 1
+
+# Every extern "C" entry point of include/fpxg.h.
+EXPORTED_SYMBOLS = [
+    "fpxg_init", "fpxg_shutdown", "fpxg_strerror", "fpxg_last_error", "fpxg_arena_stats",
+    "fpxg_release_scratch", "fpxg_conv_u32", "fpxg_series_u32", "fpxg_ntt_u32", "fpxg_conv_dev",
+    "fpxg_series_dev", "fpxg_ntt_dev", "fpxg_dev_alloc", "fpxg_dev_free", "fpxg_memcpy_h2d",
+    "fpxg_memcpy_d2h", "fpxg_synchronize", "fpxg_stream", "fpxg_event_record", "fpxg_event_elapsed",
+]
+
+
+class FpxgError(RuntimeError):
+    code = E_DEVICE
+
+
+class DomainError(FpxgError, ValueError):
+    """poly::domain_error (detail/poly/base.hpp:26-30)."""
+    code = E_DOMAIN
+
+
+class NttSizeError(FpxgError):
+    """ntt_size_error (ntt.hpp:59-63) and std::length_error."""
+    code = E_SIZE
+
+
+class DeviceError(FpxgError):
+    code = E_DEVICE
+
+
+class UnsupportedModulus(FpxgError, ValueError):
+    code = E_UNSUPPORTED
+
+
+class ArgumentError(FpxgError, ValueError):
+    code = E_ARG
+
+
+_ERRORS = {E_DOMAIN: DomainError, E_SIZE: NttSizeError, E_DEVICE: DeviceError,
+           E_UNSUPPORTED: UnsupportedModulus, E_ARG: ArgumentError}
+
+_u32p = ctypes.POINTER(ctypes.c_uint32)
+_size_p = ctypes.POINTER(ctypes.c_size_t)
+
+
+class Stats(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_size_t) for n in (
+        "cached_bytes", "cached_blocks", "system_allocations", "cache_hits", "in_use_bytes", "capacity_bytes")]
+
+
+def lib_path() -> str:
+    return os.path.join(HERE, LIB_NAME)
+
+
+_lib = None
+
+
+def lib():
+    """The loaded libfpxg.so; raises if it was not built (no fallback path)."""
+    global _lib
+    if _lib is None:
+        path = lib_path()
+        if not os.path.exists(path):
+            raise ImportError(f"{path} is missing: build it with `python -m fastpoly_amd.build`")
+        L = ctypes.CDLL(path)
+        L.fpxg_init.argtypes = [ctypes.c_int, ctypes.c_size_t]
+        L.fpxg_strerror.restype = ctypes.c_char_p
+        L.fpxg_strerror.argtypes = [ctypes.c_int]
+        L.fpxg_last_error.restype = ctypes.c_char_p
+        L.fpxg_arena_stats.argtypes = [ctypes.POINTER(Stats)]
+        L.fpxg_conv_u32.argtypes = [ctypes.c_uint32, _u32p, ctypes.c_size_t, _u32p, ctypes.c_size_t,
+                                    ctypes.c_size_t, _u32p, ctypes.c_size_t, _size_p]
+        L.fpxg_conv_dev.argtypes = [ctypes.c_uint32, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_void_p,
+                                    ctypes.c_size_t, ctypes.c_size_t, ctypes.c_void_p, ctypes.c_size_t, _size_p]
+        L.fpxg_series_u32.argtypes = [ctypes.c_int, ctypes.c_uint32, _u32p, ctypes.c_size_t, ctypes.c_size_t, _u32p]
+        L.fpxg_series_dev.argtypes = [ctypes.c_int, ctypes.c_uint32, ctypes.c_void_p, ctypes.c_size_t,
+                                      ctypes.c_size_t, ctypes.c_void_p]
+        L.fpxg_ntt_u32.argtypes = [ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_int, ctypes.c_size_t,
+                                   _u32p]
+        L.fpxg_ntt_dev.argtypes = [ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_int, ctypes.c_size_t,
+                                   ctypes.c_void_p]
+        L.fpxg_dev_alloc.restype = ctypes.c_void_p
+        L.fpxg_dev_alloc.argtypes = [ctypes.c_size_t]
+        L.fpxg_dev_free.argtypes = [ctypes.c_void_p]
+        L.fpxg_memcpy_h2d.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t]
+        L.fpxg_memcpy_d2h.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t]
+        L.fpxg_stream.restype = ctypes.c_void_p
+        L.fpxg_event_record.argtypes = [ctypes.c_int]
+        L.fpxg_event_elapsed.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_float)]
+        _lib = L
+    return _lib
+
+
+def _check(rc: int, what: str) -> None:
+    if rc == OK:
+        return
+    msg = lib().fpxg_last_error().decode(errors="replace")
+    if rc == E_NOMEM:
+        raise MemoryError(f"{what}: {msg}")
+    raise _ERRORS.get(rc, FpxgError)(f"{what}: {msg}")
+
+
+def _u32(a) -> np.ndarray:
+    return np.ascontiguousarray(np.asarray(a, dtype=np.uint32))
+
+
+def _ptr(a: np.ndarray):
+    return a.ctypes.data_as(_u32p)
+
+
+# ---------------------------------------------------------------------------
+# Montgomery helpers on the host (modint.hpp:57-180): conversion of plain
+# integers to and from the limb representation.
+# ---------------------------------------------------------------------------
+def from_int(p: int, x) -> np.ndarray | int:
+    if p == 2:
+        conv = lambda v: int(v) & 1  # noqa: E731
+    else:
+        one = (1 << 32) % p
+        conv = lambda v: (int(v) % p) * one % p  # noqa: E731
+    if np.isscalar(x):
+        return conv(x)
+    return np.array([conv(v) for v in np.asarray(x).ravel()], dtype=np.uint32)
+
+
+def to_int(p: int, limbs) -> np.ndarray | int:
+    if p == 2:
+        conv = lambda v: int(v) & 1  # noqa: E731
+    else:
+        rinv = pow(1 << 32, p - 2, p)
+        conv = lambda v: int(v) * rinv % p  # noqa: E731
+    if np.isscalar(limbs):
+        return conv(limbs)
+    return np.array([conv(v) for v in np.asarray(limbs).ravel()], dtype=np.uint64)
+
+
+class Mont:
+    """Field descriptor mirroring fpx::Mont<Mod, Root> (root 0 = smallest primitive root)."""
+
+    ALIASES = {998244353: 3, 1004535809: 3, 469762049: 3, 167772161: 3, 754974721: 11, 1224736769: 3}
+
+    def __init__(self, mod: int, root: int = 0):
+        self.mod = int(mod)
+        self.root = int(root) if root else self.ALIASES.get(self.mod, 0)
+
+    def __repr__(self):
+        return f"Mont<{self.mod}, {self.root}>"
+
+
+# ---------------------------------------------------------------------------
+# The reference API.
+# ---------------------------------------------------------------------------
+def init(device: int = 0, arena_bytes: int = 0) -> None:
+    _check(lib().fpxg_init(device, arena_bytes), "fpxg_init")
+
+
+def conv(p: int, a, b, n: int | None = None) -> np.ndarray:
+    """poly::conv(a, b, n) (detail/poly/convolution.hpp:145-147)."""
+    a, b = _u32(a), _u32(b)
+    lim = SIZE_MAX if n is None else int(n)
+    la, lb = min(a.size, lim), min(b.size, lim)
+    cap = min(la + lb - 1, lim) if la and lb else 0
+    out = np.empty(max(cap, 1), dtype=np.uint32)
+    got = ctypes.c_size_t(0)
+    same = a is b
+    rc = lib().fpxg_conv_u32(p, _ptr(a), a.size, _ptr(a) if same else _ptr(b), b.size, lim, _ptr(out),
+                             out.size, ctypes.byref(got))
+    _check(rc, "conv")
+    return out[: got.value].copy()
+
+
+def series(op: str, p: int, a, n: int) -> np.ndarray:
+    a = _u32(a)
+    out = np.empty(max(int(n), 1), dtype=np.uint32)
+    rc = lib().fpxg_series_u32(OPS[op], p, _ptr(a), a.size, int(n), _ptr(out))
+    _check(rc, op)
+    return out[: int(n)].copy()
+
+
+def inv(p: int, a, n: int) -> np.ndarray:
+    """poly::inv (detail/poly/inverse.hpp:163)."""
+    return series("inv", p, a, n)
+
+
+def log(p: int, a, n: int) -> np.ndarray:
+    """poly::log (detail/poly/exponential.hpp:85)."""
+    return series("log", p, a, n)
+
+
+def exp(p: int, a, n: int) -> np.ndarray:
+    """poly::exp (detail/poly/exponential.hpp:107)."""
+    return series("exp", p, a, n)
+
+
+def sqrt(p: int, a, n: int) -> np.ndarray:
+    """poly::sqrt (detail/poly/square_root.hpp:12)."""
+    return series("sqrt", p, a, n)
+
+
+def _ntt(p: int, root: int, data, variant: int, count: int = 0) -> np.ndarray:
+    x = _u32(data).copy()
+    _check(lib().fpxg_ntt_u32(p, root, x.size, variant, count, _ptr(x)), "ntt")
+    return x
+
+
+def ntt_forward(p: int, data, root: int = 0) -> np.ndarray:
+    """NttPlan<M>::forward (ntt.hpp:142): canonical limbs, bit-reversed order."""
+    return _ntt(p, root, data, NTT_FORWARD)
+
+
+def ntt_inverse(p: int, data, root: int = 0) -> np.ndarray:
+    """NttPlan<M>::inverse (ntt.hpp:179), normalised by 1/n."""
+    return _ntt(p, root, data, NTT_INVERSE)
+
+
+class NttPlan:
+    """NttPlan<Mont<mod, root>>::get(n) (ntt.hpp:130-192) backed by the device."""
+
+    def __init__(self, field: Mont, n: int):
+        self.field, self.n = field, int(n)
+
+    def forward(self, a):
+        return _ntt(self.field.mod, self.field.root, a, NTT_FORWARD)
+
+    forward_lazy = forward  # canonical values satisfy the [0, 2p) contract
+
+    def forward_prefix(self, a, count: int):
+        return _ntt(self.field.mod, self.field.root, a, NTT_FORWARD_PREFIX, count)
+
+    def inverse(self, a):
+        return _ntt(self.field.mod, self.field.root, a, NTT_INVERSE)
+
+    def inverse_prefix(self, a, count: int):
+        return _ntt(self.field.mod, self.field.root, a, NTT_INVERSE_PREFIX, count)
+
+    def inverse_highhalf(self, a):
+        return _ntt(self.field.mod, self.field.root, a, NTT_INVERSE_HIGHHALF)
+
+
+def release_scratch_memory() -> None:
+    """release_scratch_memory (memory.hpp:251-255): return idle arena memory."""
+    _check(lib().fpxg_release_scratch(), "release_scratch")
+
+
+def scratch_memory_stats() -> dict:
+    """scratch_memory_stats (memory.hpp:256-258) for the device arena."""
+    s = Stats()
+    _check(lib().fpxg_arena_stats(ctypes.byref(s)), "arena_stats")
+    return {name: getattr(s, name) for name, _ in Stats._fields_}
+
+
+# ---------------------------------------------------------------------------
+# Device-resident buffers for the *_dev entry points.
+# ---------------------------------------------------------------------------
+class DeviceArray:
+    """uint32 buffer in the library's device arena."""
+
+    def __init__(self, n: int):
+        self.n = int(n)
+        self.ptr = lib().fpxg_dev_alloc(max(self.n, 1) * 4)
+        if not self.ptr:
+            _check(E_NOMEM, "dev_alloc")
+
+    @classmethod
+    def from_host(cls, a) -> "DeviceArray":
+        a = _u32(a)
+        d = cls(a.size)
+        if a.size:
+            _check(lib().fpxg_memcpy_h2d(d.ptr, a.ctypes.data, a.size * 4), "h2d")
+        return d
+
+    def to_host(self, n: int | None = None) -> np.ndarray:
+        n = self.n if n is None else int(n)
+        out = np.empty(max(n, 1), dtype=np.uint32)
+        if n:
+            _check(lib().fpxg_memcpy_d2h(out.ctypes.data, self.ptr, n * 4), "d2h")
+        return out[:n]
+
+    def free(self) -> None:
+        if self.ptr:
+            lib().fpxg_dev_free(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def conv_dev(p: int, a: DeviceArray, la: int, b: DeviceArray, lb: int, out: DeviceArray,
+             n: int | None = None) -> int:
+    got = ctypes.c_size_t(0)
+    lim = SIZE_MAX if n is None else int(n)
+    _check(lib().fpxg_conv_dev(p, a.ptr, la, b.ptr, lb, lim, out.ptr, out.n, ctypes.byref(got)), "conv_dev")
+    return got.value
+
+
+def series_dev(op: str, p: int, a: DeviceArray, la: int, n: int, out: DeviceArray) -> None:
+    _check(lib().fpxg_series_dev(OPS[op], p, a.ptr, la, n, out.ptr), op + "_dev")
+
+
+def ntt_dev(p: int, data: DeviceArray, variant: int = NTT_FORWARD, count: int = 0, root: int = 0) -> None:
+    _check(lib().fpxg_ntt_dev(p, root, data.n, variant, count, data.ptr), "ntt_dev")
+
+
+def synchronize() -> None:
+    _check(lib().fpxg_synchronize(), "synchronize")
+
+
+def event_record(slot: int) -> None:
+    _check(lib().fpxg_event_record(slot), "event_record")
+
+
+def event_elapsed_ms(a: int, b: int) -> float:
+    ms = ctypes.c_float(0)
+    _check(lib().fpxg_event_elapsed(a, b, ctypes.byref(ms)), "event_elapsed")
+    return float(ms.value)

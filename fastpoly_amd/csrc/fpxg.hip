@@ -1,0 +1,824 @@
+// fastpoly_amd: the one translation unit of libfpxg.so (hipcc, gfx950).
+//
+// Host runtime (context, arena, per-prime device plans), the transform and
+// convolution drivers, the Newton series drivers (series.hpp) and the C ABI
+// declared in include/fpxg.h.
+#include "runtime.hpp"
+
+#include <cstdio>
+#include <cstring>
+
+namespace fpxg {
+
+// ===========================================================================
+// Field helpers
+// ===========================================================================
+
+static uint64_t powmod_u64(uint64_t a, uint64_t e, uint64_t m) {
+  uint64_t r = 1 % m;
+  a %= m;
+  for (; e; e >>= 1, a = (unsigned __int128)a * a % m)
+    if (e & 1) r = (unsigned __int128)r * a % m;
+  return r;
+}
+
+bool is_prime_u32(uint32_t p) {
+  if (p < 2) return false;
+  for (uint32_t q : {2u, 3u, 5u, 7u, 11u, 13u})
+    if (p % q == 0) return p == q;
+  uint32_t d = p - 1;
+  int s = 0;
+  while ((d & 1) == 0) { d >>= 1; ++s; }
+  for (uint32_t a : {2u, 7u, 61u}) {  // deterministic below 2^32
+    if (a % p == 0) continue;
+    uint64_t x = powmod_u64(a, d, p);
+    if (x == 1 || x == p - 1) continue;
+    bool comp = true;
+    for (int r = 1; r < s; ++r) {
+      x = x * x % p;
+      if (x == p - 1) { comp = false; break; }
+    }
+    if (comp) return false;
+  }
+  return true;
+}
+
+// primitive_root (modint.hpp:28-52): smallest g of full order.
+uint32_t smallest_primitive_root(uint32_t p) {
+  if (p == 2) return 1;
+  uint32_t factors[16];
+  int count = 0;
+  uint32_t rem = p - 1;
+  for (uint32_t d = 2; (uint64_t)d * d <= rem; ++d) {
+    if (rem % d) continue;
+    factors[count++] = d;
+    do rem /= d; while (rem % d == 0);
+  }
+  if (rem > 1) factors[count++] = rem;
+  for (uint32_t g = 2; g < p; ++g) {
+    bool ok = true;
+    for (int i = 0; i < count && ok; ++i) ok = powmod_u64(g, (p - 1) / factors[i], p) != 1;
+    if (ok) return g;
+  }
+  return 0;
+}
+
+// Roots of the reference's named aliases (modint.hpp:195-205).
+uint32_t alias_root(uint32_t p) {
+  switch (p) {
+    case 998244353u: case 1004535809u: case 469762049u: case 167772161u: case 1224736769u: return 3;
+    case 754974721u: return 11;
+    default: return 0;
+  }
+}
+
+bool HostField::sqrt(uint32_t a, uint32_t* out) const {
+  if (p == 2) { *out = a; return true; }
+  const uint32_t o = from_int(1);
+  if (a == 0) { *out = 0; return true; }
+  if (pow(a, (p - 1) / 2) != o) return false;
+  if (p % 4 == 3) { *out = pow(a, ((uint64_t)p + 1) / 4); return true; }
+  uint64_t q = p - 1;
+  int s = 0;
+  while (q % 2 == 0) { q /= 2; ++s; }
+  uint32_t z = from_int(2);
+  while (pow(z, (p - 1) / 2) != neg(o)) z = add(z, o);
+  uint32_t cc = pow(z, q), x = pow(a, (q + 1) / 2), t = pow(a, q);
+  int m = s;
+  while (t != o) {
+    int i = 0;
+    for (uint32_t t2 = t; t2 != o; t2 = mul(t2, t2)) ++i;
+    const uint32_t bb = pow(cc, (uint64_t)1 << (m - i - 1));
+    x = mul(x, bb);
+    cc = mul(bb, bb);
+    t = mul(t, cc);
+    m = i;
+  }
+  *out = x;
+  return true;
+}
+
+static void init_field(HostField& F, uint32_t p, uint32_t root) {
+  F.p = p;
+  uint32_t inv = 1;
+  for (int i = 0; i < 5; ++i) inv *= 2u - p * inv;
+  F.ninv = 0u - inv;
+  F.one = p == 2 ? 1u : (uint32_t)((((uint64_t)1) << 32) % p);
+  F.r2 = (uint32_t)((uint64_t)F.one * F.one % p);
+  F.root = root ? root : (alias_root(p) ? alias_root(p) : smallest_primitive_root(p));
+  uint32_t t = p - 1;
+  F.maxlog = 0;
+  while (t && !(t & 1)) { t >>= 1; ++F.maxlog; }
+  F.lazy = p < (1u << 30);
+  F.dev.p = p;
+  F.dev.ninv = F.ninv;
+  F.dev.r2 = F.r2;
+  F.dev.one = F.one;
+  if (p == 2 || F.maxlog < 1) return;
+  // Device tables for transforms of up to 2^logmax points.
+  const int logmax = std::min(F.maxlog, 26);
+  const int lvl_log = std::min(logmax, kLvlLog);
+  const int split = std::min(logmax, 12);
+  const size_t lvl_n = (size_t)1 << lvl_log, lo_n = (size_t)1 << split, hi_n = (size_t)1 << (logmax - split);
+  std::vector<uint32_t> h(2 * lvl_n + 2 * lo_n + 2 * hi_n, 0);
+  uint32_t* lvl_f = h.data();
+  uint32_t* lvl_i = lvl_f + lvl_n;
+  uint32_t* lo_f = lvl_i + lvl_n;
+  uint32_t* lo_i = lo_f + lo_n;
+  uint32_t* hi_f = lo_i + lo_n;
+  uint32_t* hi_i = hi_f + hi_n;
+  const uint32_t g = F.from_int(F.root);
+  const uint32_t wmax = F.pow(g, (uint64_t)(p - 1) >> logmax), wmax_i = F.inv(wmax);
+  for (int k = 1; k <= lvl_log; ++k) {
+    const uint32_t L = 1u << k;
+    const uint32_t w = F.pow(wmax, (uint64_t)1 << (logmax - k)), wi = F.inv(w);
+    uint32_t cf = F.one, ci = F.one;
+    for (uint32_t m = 0; m < L / 2; ++m) {
+      lvl_f[L / 2 + m] = cf;
+      lvl_i[L / 2 + m] = ci;
+      cf = F.mul(cf, w);
+      ci = F.mul(ci, wi);
+    }
+  }
+  {
+    uint32_t cf = F.one, ci = F.one;
+    for (size_t k = 0; k < lo_n; ++k) {
+      lo_f[k] = cf; lo_i[k] = ci;
+      cf = F.mul(cf, wmax); ci = F.mul(ci, wmax_i);
+    }
+    const uint32_t sf = F.pow(wmax, lo_n), si = F.pow(wmax_i, lo_n);
+    cf = F.one; ci = F.one;
+    for (size_t k = 0; k < hi_n; ++k) {
+      hi_f[k] = cf; hi_i[k] = ci;
+      cf = F.mul(cf, sf); ci = F.mul(ci, si);
+    }
+  }
+  FPXG_HIP(hipMalloc(&F.tables, h.size() * 4));
+  FPXG_HIP(hipMemcpy(F.tables, h.data(), h.size() * 4, hipMemcpyHostToDevice));
+  F.dev.logmax = logmax;
+  F.dev.split = split;
+  F.dev.lvl_fwd = F.tables;
+  F.dev.lvl_inv = F.tables + lvl_n;
+  F.dev.tw_lo = F.tables + 2 * lvl_n;
+  F.dev.tw_lo_inv = F.dev.tw_lo + lo_n;
+  F.dev.tw_hi = F.dev.tw_lo_inv + lo_n;
+  F.dev.tw_hi_inv = F.dev.tw_hi + hi_n;
+}
+
+static uint32_t inv_pow2(const HostField& F, int L) {
+  return F.pow(F.from_int(((uint64_t)F.p + 1) / 2), (uint64_t)L);
+}
+
+// ===========================================================================
+// Arena
+// ===========================================================================
+
+Arena::~Arena() {
+  for (auto& c : chunks_) (void)hipFree(c.base);
+}
+
+static int size_class(size_t bytes) {
+  int k = 8;
+  while (((size_t)1 << k) < bytes) ++k;
+  return k;
+}
+
+void* Arena::alloc(size_t bytes) {
+  std::lock_guard<std::mutex> g(mu_);
+  const int k = size_class(bytes);
+  const size_t sz = (size_t)1 << k;
+  auto& fl = free_[k];
+  if (!fl.empty()) {
+    void* p = fl.back();
+    fl.pop_back();
+    ++hits_;
+    idle_bytes_ -= sz;
+    --idle_blocks_;
+    in_use_ += sz;
+    return p;
+  }
+  for (auto& c : chunks_) {
+    const size_t off = (c.used + 255) & ~(size_t)255;
+    if (off + sz <= c.size) {
+      c.used = off + sz;
+      void* p = c.base + off;
+      owner_[p] = k;
+      in_use_ += sz;
+      return p;
+    }
+  }
+  const size_t chunk = std::max(sz, (size_t)256 << 20);
+  if (reserved_ + chunk > cap_) {
+    if (reserved_ + sz > cap_) throw Error(E_NOMEM, "device arena exhausted");
+  }
+  const size_t take = reserved_ + chunk > cap_ ? sz : chunk;
+  char* base = nullptr;
+  if (hipMalloc(&base, take) != hipSuccess) {
+    (void)hipGetLastError();
+    throw Error(E_NOMEM, "hipMalloc failed for the device arena");
+  }
+  ++sys_allocs_;
+  reserved_ += take;
+  chunks_.push_back({base, take, sz});
+  owner_[base] = k;
+  in_use_ += sz;
+  return base;
+}
+
+void Arena::free(void* p) {
+  if (!p) return;
+  std::lock_guard<std::mutex> g(mu_);
+  auto it = owner_.find(p);
+  if (it == owner_.end()) return;
+  const size_t sz = (size_t)1 << it->second;
+  free_[it->second].push_back(p);
+  in_use_ -= sz;
+  idle_bytes_ += sz;
+  ++idle_blocks_;
+}
+
+void Arena::release_idle() {
+  std::lock_guard<std::mutex> g(mu_);
+  if (in_use_ != 0) return;  // chunks are carved; only an idle arena can return them
+  for (auto& c : chunks_) (void)hipFree(c.base);
+  chunks_.clear();
+  free_.clear();
+  owner_.clear();
+  reserved_ = idle_bytes_ = idle_blocks_ = 0;
+}
+
+fpxg_stats Arena::stats() const {
+  std::lock_guard<std::mutex> g(mu_);
+  fpxg_stats s{};
+  s.cached_bytes = idle_bytes_;
+  s.cached_blocks = idle_blocks_;
+  s.system_allocations = sys_allocs_;
+  s.cache_hits = hits_;
+  s.in_use_bytes = in_use_;
+  s.capacity_bytes = cap_;
+  return s;
+}
+
+// ===========================================================================
+// Context and launch helpers
+// ===========================================================================
+
+static std::unique_ptr<Context> g_ctx;
+static std::mutex g_init_mu;
+static size_t g_arena_bytes = (size_t)16 << 30;
+static int g_device = 0;
+
+template <bool LAZY, int MODE>
+static bool set_rows_attr(size_t bytes) {
+  return hipFuncSetAttribute((const void*)k_rows<LAZY, MODE>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                             (int)bytes) == hipSuccess;
+}
+
+#define FPXG_ROW_MODES(X)                                                                      \
+  X(MODE_FWD_A) X(MODE_INV) X(MODE_INV | MODE_SCALE) X(MODE_FWD_A | MODE_B_RAW | MODE_INV)     \
+  X(MODE_FWD_A | MODE_B_RAW | MODE_INV | MODE_SCALE) X(MODE_FWD_A | MODE_B_SPEC | MODE_INV)    \
+  X(MODE_FWD_A | MODE_B_SPEC | MODE_INV | MODE_SCALE) X(MODE_B_SPEC | MODE_INV)                \
+  X(MODE_B_SPEC | MODE_INV | MODE_SCALE) X(MODE_FWD_A | MODE_B_RAW | MODE_STORE_B | MODE_INV)  \
+  X(MODE_FWD_A | MODE_B_RAW | MODE_STORE_B | MODE_INV | MODE_SCALE)
+
+static void context_init(Context& c) {
+  FPXG_HIP(hipSetDevice(c.device));
+  FPXG_HIP(hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking));
+  for (auto& e : c.events) FPXG_HIP(hipEventCreate(&e));
+  hipDeviceProp_t prop;
+  FPXG_HIP(hipGetDeviceProperties(&prop, c.device));
+  c.cus = prop.multiProcessorCount;
+  // Use up to 2^14-word tiles when the device grants more than 64 KB of LDS
+  // to one workgroup (gfx950: 160 KB per CU).
+  const size_t want = (size_t)lds_words(1u << 14) * 4;
+  size_t limit = prop.sharedMemPerBlock;
+  if (prop.maxSharedMemoryPerMultiProcessor > limit) limit = prop.maxSharedMemoryPerMultiProcessor;
+  bool big = limit >= want;
+  if (big) {
+#define X(m) big = set_rows_attr<true, m>(want) && set_rows_attr<false, m>(want) && big;
+    FPXG_ROW_MODES(X)
+#undef X
+    for (const void* k : {(const void*)k_cols<true, false>, (const void*)k_cols<true, true>,
+                          (const void*)k_cols<false, false>, (const void*)k_cols<false, true>})
+      if (hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)want) != hipSuccess) big = false;
+    (void)hipGetLastError();
+  }
+  c.lds_bytes = big ? want : 65536;
+  c.row_log_max = big ? 14 : 13;
+  c.tile_log_max = big ? 14 : 13;
+  c.arena = std::make_unique<Arena>(g_arena_bytes);
+}
+
+Context& ctx() {
+  std::lock_guard<std::mutex> g(g_init_mu);
+  if (!g_ctx) {
+    auto c = std::make_unique<Context>();
+    c->device = g_device;
+    context_init(*c);
+    g_ctx = std::move(c);
+  }
+  return *g_ctx;
+}
+
+HostField& Context::field(uint32_t p, uint32_t root) {
+  const uint64_t key = ((uint64_t)p << 32) | root;
+  auto it = fields.find(key);
+  if (it != fields.end()) return *it->second;
+  auto F = std::make_unique<HostField>();
+  init_field(*F, p, root);
+  HostField& ref = *F;
+  fields[key] = std::move(F);
+  return ref;
+}
+
+PassPlan Context::plan(int L) const {
+  if (L <= row_log_max) return {L, L, 0, 0};
+  const int b = row_log_max;
+  const int S = L - b;
+  int logC = std::min(4, tile_log_max - S);
+  if (logC < 0) throw Error(E_SIZE, "transform too large for two passes");
+  logC = std::min(logC, b);
+  return {L, b, S, logC};
+}
+
+template <bool LAZY>
+static void launch_rows_t(Context& c, int mode, const RowJob& j, int logM, uint32_t nblocks, uint32_t njobs,
+                          const RowJob* dev_jobs) {
+  const size_t lds = (size_t)lds_words(1u << logM) * 4;
+  dim3 grid(nblocks, njobs), block(kThreads);
+  switch (mode) {
+#define X(m) \
+  case (m): hipLaunchKernelGGL((k_rows<LAZY, (m)>), grid, block, lds, c.stream, j, dev_jobs, logM); break;
+    FPXG_ROW_MODES(X)
+#undef X
+    default: throw Error(E_DEVICE, "unsupported row-pass mode");
+  }
+  FPXG_HIP(hipGetLastError());
+}
+
+static void launch_rows(Context& c, bool lazy, int mode, const RowJob& j, int logM, uint32_t nblocks,
+                        uint32_t njobs = 1, const RowJob* dev_jobs = nullptr) {
+  if (lazy) launch_rows_t<true>(c, mode, j, logM, nblocks, njobs, dev_jobs);
+  else launch_rows_t<false>(c, mode, j, logM, nblocks, njobs, dev_jobs);
+}
+
+static void launch_cols(Context& c, bool lazy, bool inv, const ColJob& j, const PassPlan& P, uint32_t njobs = 1,
+                        const ColJob* dev_jobs = nullptr) {
+  const size_t lds = (size_t)lds_words(1u << (P.S + P.logC)) * 4;
+  dim3 grid(1u << (P.b - P.logC), njobs), block(kThreads);
+  if (lazy) {
+    if (inv) hipLaunchKernelGGL((k_cols<true, true>), grid, block, lds, c.stream, j, dev_jobs, P.L, P.S, P.logC);
+    else hipLaunchKernelGGL((k_cols<true, false>), grid, block, lds, c.stream, j, dev_jobs, P.L, P.S, P.logC);
+  } else {
+    if (inv) hipLaunchKernelGGL((k_cols<false, true>), grid, block, lds, c.stream, j, dev_jobs, P.L, P.S, P.logC);
+    else hipLaunchKernelGGL((k_cols<false, false>), grid, block, lds, c.stream, j, dev_jobs, P.L, P.S, P.logC);
+  }
+  FPXG_HIP(hipGetLastError());
+}
+
+__global__ void k_fill(uint32_t* p, size_t n, uint32_t v) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+    p[i] = v;
+}
+
+static unsigned grid_for(size_t n) {
+  size_t g = (n + 255) / 256;
+  return (unsigned)std::max<size_t>(1, std::min<size_t>(g, 8192));
+}
+
+static void fill(Context& c, uint32_t* p, size_t n, uint32_t v) {
+  if (n == 0) return;
+  hipLaunchKernelGGL(k_fill, dim3(grid_for(n)), dim3(256), 0, c.stream, p, n, v);
+  FPXG_HIP(hipGetLastError());
+}
+
+static int ceil_log2(uint64_t n) {
+  int k = 0;
+  while (((uint64_t)1 << k) < n) ++k;
+  return k;
+}
+
+// ===========================================================================
+// Transforms
+// ===========================================================================
+
+void ntt_forward(Context& c, const HostField& F, uint32_t* dst, const uint32_t* src, uint32_t in_count, int L,
+                 bool lift, bool canon) {
+  const PassPlan P = c.plan(L);
+  const uint32_t N = 1u << L;
+  if (P.S == 0) {
+    RowJob j{};
+    j.src_a = src; j.dst = dst; j.in_a = in_count; j.out_lo = 0; j.out_hi = N;
+    j.lift = lift; j.canon = canon; j.f = F.dev;
+    launch_rows(c, F.lazy, MODE_FWD_A, j, L, 1);
+    return;
+  }
+  ColJob cj{};
+  cj.src = src; cj.dst = dst; cj.in_count = in_count; cj.lift = lift; cj.f = F.dev;
+  launch_cols(c, F.lazy, false, cj, P);
+  RowJob rj{};
+  rj.src_a = dst; rj.dst = dst; rj.in_a = N; rj.out_lo = 0; rj.out_hi = N; rj.canon = canon; rj.f = F.dev;
+  launch_rows(c, F.lazy, MODE_FWD_A, rj, P.b, 1u << P.S);
+}
+
+void ntt_inverse(Context& c, const HostField& F, uint32_t* dst, const uint32_t* src, int L, uint32_t out_lo,
+                 uint32_t out_hi, uint32_t out_shift) {
+  // src is clobbered (row pass in place); dst may alias src only if out_shift == 0.
+  const PassPlan P = c.plan(L);
+  const uint32_t N = 1u << L;
+  const uint32_t inv_n = inv_pow2(F, L);
+  if (P.S == 0) {
+    RowJob j{};
+    j.src_a = src; j.dst = dst; j.in_a = N; j.out_lo = out_lo; j.out_hi = out_hi; j.out_shift = out_shift;
+    j.inv_n = inv_n; j.canon = 1; j.f = F.dev;
+    launch_rows(c, F.lazy, MODE_INV | MODE_SCALE, j, L, 1);
+    return;
+  }
+  uint32_t* work = const_cast<uint32_t*>(src);
+  RowJob rj{};
+  rj.src_a = work; rj.dst = work; rj.in_a = N; rj.out_lo = 0; rj.out_hi = N; rj.f = F.dev;
+  launch_rows(c, F.lazy, MODE_INV, rj, P.b, 1u << P.S);
+  ColJob cj{};
+  cj.src = work; cj.dst = dst; cj.out_lo = out_lo; cj.out_hi = out_hi; cj.out_shift = out_shift;
+  cj.inv_n = inv_n; cj.canon = 1; cj.f = F.dev;
+  launch_cols(c, F.lazy, true, cj, P);
+}
+
+// Product window [0, outc) of the cyclic-free product over one NTT field F;
+// with lift the inputs are raw limbs of another field (CRT auxiliary use).
+static void conv_core(Context& c, const HostField& F, const uint32_t* a, size_t la, const uint32_t* b, size_t lb,
+                      uint32_t* out, size_t outc, bool lift) {
+  const int L = std::max(1, ceil_log2(la + lb - 1));
+  const uint32_t N = 1u << L;
+  const PassPlan P = c.plan(L);
+  const uint32_t inv_n = inv_pow2(F, L);
+  if (P.S == 0) {
+    RowJob j{};
+    j.src_a = a; j.src_b = b; j.dst = out; j.in_a = (uint32_t)la; j.in_b = (uint32_t)lb;
+    j.out_lo = 0; j.out_hi = (uint32_t)outc; j.inv_n = inv_n; j.lift = lift; j.canon = 1; j.f = F.dev;
+    launch_rows(c, F.lazy, MODE_FWD_A | MODE_B_RAW | MODE_INV | MODE_SCALE, j, L, 1);
+    return;
+  }
+  DevBuf A = c.buf(N), B = c.buf(N);
+  ColJob cj{};
+  cj.src = a; cj.dst = A.get(); cj.in_count = (uint32_t)la; cj.lift = lift; cj.f = F.dev;
+  launch_cols(c, F.lazy, false, cj, P);
+  cj.src = b; cj.dst = B.get(); cj.in_count = (uint32_t)lb;
+  launch_cols(c, F.lazy, false, cj, P);
+  RowJob rj{};
+  rj.src_a = A.get(); rj.src_b = B.get(); rj.dst = A.get(); rj.in_a = N; rj.in_b = N;
+  rj.out_lo = 0; rj.out_hi = N; rj.f = F.dev;
+  launch_rows(c, F.lazy, MODE_FWD_A | MODE_B_RAW | MODE_INV, rj, P.b, 1u << P.S);
+  ColJob ij{};
+  ij.src = A.get(); ij.dst = out; ij.out_lo = 0; ij.out_hi = (uint32_t)outc; ij.inv_n = inv_n; ij.canon = 1;
+  ij.f = F.dev;
+  launch_cols(c, F.lazy, true, ij, P);
+}
+
+static const uint32_t kCrtP[3] = {167772161u, 469762049u, 754974721u};  // crt_convolution.hpp:14-19
+static const size_t kCrtBlock = (size_t)1 << 23;                          // crt_convolution.hpp:19
+
+// crt_prime_count (crt_convolution.hpp:78-83).
+static int crt_prime_count(uint32_t p, size_t terms) {
+  const uint64_t sq = (uint64_t)(p - 1) * (p - 1);
+  if (terms == 0) terms = 1;
+  if (sq <= (kCrtP[0] - 1) / terms) return 1;
+  if (sq <= ((uint64_t)kCrtP[0] * kCrtP[1] - 1) / terms) return 2;
+  return 3;
+}
+
+static GarnerConst garner_const(Context& c, const HostField& T, int primes, bool accumulate) {
+  GarnerConst g{};
+  const HostField* P[3];
+  for (int k = 0; k < 3; ++k) {
+    P[k] = &c.field(kCrtP[k]);
+    g.P[k] = P[k]->p;
+    g.Pninv[k] = P[k]->ninv;
+  }
+  const uint64_t p1 = kCrtP[0], p2 = kCrtP[1], p3 = kCrtP[2];
+  g.inv12 = P[1]->inv(P[1]->from_int(p1));
+  g.p1_in3 = P[2]->from_int(p1);
+  g.inv123 = P[2]->inv(P[2]->from_int(p1 * p2 % p3));
+  g.p = T.p;
+  g.ninv = T.ninv;
+  g.p1_t = (uint32_t)(p1 % T.p);
+  g.p12_t = (uint32_t)((p1 * p2) % T.p);
+  g.primes = (uint32_t)primes;
+  g.accumulate = accumulate;
+  return g;
+}
+
+static void crt_window(Context& c, const HostField& T, const uint32_t* a, size_t la, const uint32_t* b, size_t lb,
+                       uint32_t* out, size_t outc, bool accumulate) {
+  const int primes = crt_prime_count(T.p, std::min(la, lb));
+  DevBuf res[3];
+  for (int k = 0; k < primes; ++k) {
+    res[k] = c.buf(outc);
+    conv_core(c, c.field(kCrtP[k]), a, la, b, lb, res[k].get(), outc, true);
+  }
+  const GarnerConst g = garner_const(c, T, primes, accumulate);
+  hipLaunchKernelGGL(k_garner, dim3(grid_for(outc)), dim3(256), 0, c.stream, res[0].get(),
+                     primes > 1 ? res[1].get() : res[0].get(), primes > 2 ? res[2].get() : res[0].get(), out,
+                     (uint32_t)outc, g);
+  FPXG_HIP(hipGetLastError());
+}
+
+static bool native_ok(const HostField& T, size_t full) {
+  if (T.p == 2 || T.maxlog < 6) return false;  // ntt_max_size <= naive_threshold: CRT (convolution.hpp:97-100)
+  const int L = std::max(1, ceil_log2(full));
+  return L <= std::min(T.maxlog, 26);
+}
+
+void conv_dev(Context& c, const HostField& T, const uint32_t* a, size_t la, const uint32_t* b, size_t lb,
+              uint32_t* out, size_t outc) {
+  if (la == 0 || lb == 0 || outc == 0) return;
+  la = std::min(la, outc);
+  lb = std::min(lb, outc);
+  const size_t full = la + lb - 1;
+  if (native_ok(T, full)) {
+    conv_core(c, T, a, la, b, lb, out, outc, false);
+    return;
+  }
+  if (la <= kCrtBlock && lb <= kCrtBlock) {
+    crt_window(c, T, a, la, b, lb, out, outc, false);
+    return;
+  }
+  // crt_convolution block pairs (crt_convolution.hpp:187-205).
+  fill(c, out, outc, 0);
+  for (size_t ai = 0; ai < la;) {
+    const size_t na = std::min(kCrtBlock, la - ai);
+    for (size_t bi = 0; bi < lb && bi < outc - ai;) {
+      const size_t nb = std::min(kCrtBlock, lb - bi);
+      const size_t cnt = std::min(na + nb - 1, outc - ai - bi);
+      crt_window(c, T, a + ai, na, b + bi, nb, out + ai + bi, cnt, true);
+      bi += nb;
+    }
+    ai += na;
+  }
+}
+
+}  // namespace fpxg
+
+#include "series.hpp"
+
+// ===========================================================================
+// C ABI
+// ===========================================================================
+
+using namespace fpxg;
+
+namespace {
+thread_local std::string t_last_error;
+
+template <class F>
+int guarded(F&& f) {
+  try {
+    Context& c = ctx();
+    std::lock_guard<std::recursive_mutex> g(c.mu);
+    f(c);
+    return OK;
+  } catch (const Error& e) {
+    t_last_error = e.what();
+    return e.code;
+  } catch (const std::bad_alloc&) {
+    t_last_error = "host allocation failed";
+    return E_NOMEM;
+  } catch (const std::exception& e) {
+    t_last_error = e.what();
+    return E_DEVICE;
+  }
+}
+
+void check_modulus(uint32_t p) {
+  if (p < 2 || p >= (1u << 31) || !is_prime_u32(p))
+    throw Error(E_UNSUPPORTED, "modulus must be a prime in [2, 2^31)");
+}
+
+// conv_limbs length rules (convolution.hpp:17-23).
+size_t conv_len(size_t la, size_t lb, size_t n, size_t* pla, size_t* plb) {
+  if (la == 0 || lb == 0 || n == 0) return 0;
+  la = std::min(la, n);
+  lb = std::min(lb, n);
+  if (la > SIZE_MAX - (lb - 1)) throw Error(E_SIZE, "poly::conv: result length overflows size_t");
+  if (la > ((size_t)1 << 31) || lb > ((size_t)1 << 31))
+    throw Error(E_SIZE, "operands above 2^31 coefficients are not supported");
+  *pla = la;
+  *plb = lb;
+  return std::min(la + lb - 1, n);
+}
+
+struct Upload {
+  DevBuf buf;
+  Upload(Context& c, const uint32_t* h, size_t n) : buf(c.buf(n)) {
+    if (n) FPXG_HIP(hipMemcpyAsync(buf.get(), h, n * 4, hipMemcpyHostToDevice, c.stream));
+  }
+};
+}  // namespace
+
+extern "C" {
+
+int fpxg_init(int device, size_t arena_bytes) {
+  {
+    std::lock_guard<std::mutex> g(g_init_mu);
+    if (g_ctx) return g_ctx->device == device ? OK : E_ARG;
+    g_device = device;
+    if (arena_bytes) g_arena_bytes = arena_bytes;
+  }
+  return guarded([](Context&) {});
+}
+
+void fpxg_shutdown(void) {
+  std::lock_guard<std::mutex> g(g_init_mu);
+  if (!g_ctx) return;
+  (void)hipStreamSynchronize(g_ctx->stream);
+  for (auto& kv : g_ctx->fields)
+    if (kv.second->tables) (void)hipFree(kv.second->tables);
+  for (auto& e : g_ctx->events) (void)hipEventDestroy(e);
+  (void)hipStreamDestroy(g_ctx->stream);
+  g_ctx.reset();
+}
+
+const char* fpxg_strerror(int code) {
+  switch (code) {
+    case OK: return "ok";
+    case E_DOMAIN: return "domain error (poly::domain_error)";
+    case E_SIZE: return "size error (ntt_size_error / length_error)";
+    case E_NOMEM: return "device memory exhausted (bad_alloc)";
+    case E_DEVICE: return "HIP device error";
+    case E_UNSUPPORTED: return "unsupported modulus";
+    case E_ARG: return "invalid argument";
+    default: return "unknown error";
+  }
+}
+
+const char* fpxg_last_error(void) { return t_last_error.c_str(); }
+
+int fpxg_arena_stats(fpxg_stats* out) {
+  if (!out) return E_ARG;
+  return guarded([&](Context& c) { *out = c.arena->stats(); });
+}
+
+int fpxg_release_scratch(void) {
+  return guarded([&](Context& c) {
+    FPXG_HIP(hipStreamSynchronize(c.stream));
+    c.arena->release_idle();
+  });
+}
+
+int fpxg_conv_u32(uint32_t p, const uint32_t* a, size_t la, const uint32_t* b, size_t lb, size_t n,
+                  uint32_t* out, size_t out_cap, size_t* out_len) {
+  return guarded([&](Context& c) {
+    check_modulus(p);
+    if (!out_len) throw Error(E_ARG, "out_len is null");
+    *out_len = 0;
+    size_t xa = 0, xb = 0;
+    const size_t outc = conv_len(la, lb, n, &xa, &xb);
+    if (outc == 0) return;
+    if (!a || !b || !out || out_cap < outc) throw Error(E_ARG, "null pointer or output capacity too small");
+    const HostField& T = c.field(p);
+    Upload ua(c, a, xa);
+    const bool same = a == b && xa == xb;
+    Upload ub(c, same ? nullptr : b, same ? 0 : xb);
+    DevBuf o = c.buf(outc);
+    conv_dev(c, T, ua.buf.get(), xa, same ? ua.buf.get() : ub.buf.get(), xb, o.get(), outc);
+    FPXG_HIP(hipMemcpyAsync(out, o.get(), outc * 4, hipMemcpyDeviceToHost, c.stream));
+    FPXG_HIP(hipStreamSynchronize(c.stream));
+    *out_len = outc;
+  });
+}
+
+int fpxg_conv_dev(uint32_t p, const uint32_t* a, size_t la, const uint32_t* b, size_t lb, size_t n,
+                  uint32_t* out, size_t out_cap, size_t* out_len) {
+  return guarded([&](Context& c) {
+    check_modulus(p);
+    if (!out_len) throw Error(E_ARG, "out_len is null");
+    *out_len = 0;
+    size_t xa = 0, xb = 0;
+    const size_t outc = conv_len(la, lb, n, &xa, &xb);
+    if (outc == 0) return;
+    if (!a || !b || !out || out_cap < outc) throw Error(E_ARG, "null pointer or output capacity too small");
+    conv_dev(c, c.field(p), a, xa, b, xb, out, outc);
+    *out_len = outc;
+  });
+}
+
+static void ntt_check(const HostField& F, uint32_t n, int variant, size_t count) {
+  if (n < 2 || (n & (n - 1))) throw Error(E_SIZE, "NTT size must be a power of two");
+  if (F.p == 2 || ceil_log2(n) > F.maxlog) throw Error(E_SIZE, "NTT size exceeds 2^v2(mod-1) for this modulus");
+  if (ceil_log2(n) > 26) throw Error(E_SIZE, "NTT sizes above 2^26 are not supported on the device");
+  if ((variant == FPXG_NTT_FORWARD_PREFIX || variant == FPXG_NTT_INVERSE_PREFIX) && count > n)
+    throw Error(E_SIZE, "NTT prefix exceeds plan size");
+  if (variant < 0 || variant > FPXG_NTT_INVERSE_HIGHHALF) throw Error(E_ARG, "unknown NTT variant");
+}
+
+static void ntt_run(Context& c, const HostField& F, uint32_t n, int variant, size_t count, uint32_t* d) {
+  const int L = ceil_log2(n);
+  switch (variant) {
+    case FPXG_NTT_FORWARD: ntt_forward(c, F, d, d, n, L, false, true); break;
+    case FPXG_NTT_FORWARD_PREFIX: ntt_forward(c, F, d, d, (uint32_t)count, L, false, true); break;
+    case FPXG_NTT_INVERSE: ntt_inverse(c, F, d, d, L, 0, n, 0); break;
+    case FPXG_NTT_INVERSE_PREFIX: ntt_inverse(c, F, d, d, L, 0, (uint32_t)count, 0); break;
+    case FPXG_NTT_INVERSE_HIGHHALF: ntt_inverse(c, F, d, d, L, n / 2, n, 0); break;
+  }
+}
+
+int fpxg_ntt_u32(uint32_t p, uint32_t root, uint32_t n, int variant, size_t count, uint32_t* data) {
+  return guarded([&](Context& c) {
+    check_modulus(p);
+    const HostField& F = c.field(p, root);
+    ntt_check(F, n, variant, count);
+    if (!data) throw Error(E_ARG, "data is null");
+    Upload u(c, data, n);
+    ntt_run(c, F, n, variant, count, u.buf.get());
+    FPXG_HIP(hipMemcpyAsync(data, u.buf.get(), (size_t)n * 4, hipMemcpyDeviceToHost, c.stream));
+    FPXG_HIP(hipStreamSynchronize(c.stream));
+  });
+}
+
+int fpxg_ntt_dev(uint32_t p, uint32_t root, uint32_t n, int variant, size_t count, uint32_t* data) {
+  return guarded([&](Context& c) {
+    check_modulus(p);
+    const HostField& F = c.field(p, root);
+    ntt_check(F, n, variant, count);
+    if (!data) throw Error(E_ARG, "data is null");
+    ntt_run(c, F, n, variant, count, data);
+  });
+}
+
+int fpxg_series_u32(int op, uint32_t p, const uint32_t* a, size_t la, size_t n, uint32_t* out) {
+  return guarded([&](Context& c) {
+    check_modulus(p);
+    if (n == 0) return;
+    if (!out || (la && !a)) throw Error(E_ARG, "null pointer");
+    const HostField& T = c.field(p);
+    const size_t use = std::min(la, n);
+    Upload u(c, a, use);
+    DevBuf o = c.buf(n);
+    series_dev(c, T, op, u.buf.get(), use, n, o.get());
+    FPXG_HIP(hipMemcpyAsync(out, o.get(), n * 4, hipMemcpyDeviceToHost, c.stream));
+    FPXG_HIP(hipStreamSynchronize(c.stream));
+  });
+}
+
+int fpxg_series_dev(int op, uint32_t p, const uint32_t* a, size_t la, size_t n, uint32_t* out) {
+  return guarded([&](Context& c) {
+    check_modulus(p);
+    if (n == 0) return;
+    if (!out || (la && !a)) throw Error(E_ARG, "null pointer");
+    series_dev(c, c.field(p), op, a, std::min(la, n), n, out);
+  });
+}
+
+void* fpxg_dev_alloc(size_t bytes) {
+  void* p = nullptr;
+  int rc = guarded([&](Context& c) { p = c.arena->alloc(std::max<size_t>(bytes, 4)); });
+  return rc == OK ? p : nullptr;
+}
+
+int fpxg_dev_free(void* ptr) {
+  return guarded([&](Context& c) {
+    FPXG_HIP(hipStreamSynchronize(c.stream));
+    c.arena->free(ptr);
+  });
+}
+
+int fpxg_memcpy_h2d(void* dst, const void* src, size_t bytes) {
+  return guarded([&](Context& c) {
+    FPXG_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, c.stream));
+    FPXG_HIP(hipStreamSynchronize(c.stream));
+  });
+}
+
+int fpxg_memcpy_d2h(void* dst, const void* src, size_t bytes) {
+  return guarded([&](Context& c) {
+    FPXG_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, c.stream));
+    FPXG_HIP(hipStreamSynchronize(c.stream));
+  });
+}
+
+int fpxg_synchronize(void) {
+  return guarded([&](Context& c) { FPXG_HIP(hipStreamSynchronize(c.stream)); });
+}
+
+void* fpxg_stream(void) {
+  void* s = nullptr;
+  (void)guarded([&](Context& c) { s = (void*)c.stream; });
+  return s;
+}
+
+int fpxg_event_record(int slot) {
+  return guarded([&](Context& c) {
+    if (slot < 0 || slot >= 16) throw Error(E_ARG, "event slot out of range");
+    FPXG_HIP(hipEventRecord(c.events[slot], c.stream));
+  });
+}
+
+int fpxg_event_elapsed(int a, int b, float* ms) {
+  return guarded([&](Context& c) {
+    if (a < 0 || a >= 16 || b < 0 || b >= 16 || !ms) throw Error(E_ARG, "bad event slot");
+    FPXG_HIP(hipEventSynchronize(c.events[b]));
+    FPXG_HIP(hipEventElapsedTime(ms, c.events[a], c.events[b]));
+  });
+}
+
+}  // extern "C"

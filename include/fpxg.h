@@ -1,0 +1,101 @@
+/* fpxg — C ABI of the MI355X (gfx950) build of fastpoly's NTT convolution path.
+ *
+ * Every coefficient crosses the boundary as the raw Montgomery limb of the
+ * reference representation (x * 2^32 mod p, include/fastpoly/modint.hpp:54-98),
+ * so reinterpret_cast<const uint32_t*>(std::vector<fpx::Mont<P>>::data()) is
+ * zero-copy.  Results are bit-identical to the reference CPU library.
+ *
+ * Host-pointer entry points copy in, compute on the GPU and copy out before
+ * returning.  *_dev entry points take device pointers (library-allocated via
+ * fpxg_dev_alloc or any HIP allocation on the same device), enqueue on the
+ * library stream and return immediately; fpxg_synchronize() waits.
+ *
+ * Return codes map onto the reference's exceptions (see fpxg_strerror).
+ * Thread safety: calls are serialised by an internal mutex.
+ */
+#ifndef FPXG_H
+#define FPXG_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+  FPXG_OK = 0,
+  FPXG_E_DOMAIN = 1,      /* poly::domain_error (detail/poly/base.hpp:26-30) */
+  FPXG_E_SIZE = 2,        /* ntt_size_error (ntt.hpp:59-63), std::length_error (convolution.hpp:20-21) */
+  FPXG_E_NOMEM = 3,       /* std::bad_alloc: device arena exhausted */
+  FPXG_E_DEVICE = 4,      /* HIP runtime failure (std::runtime_error) */
+  FPXG_E_UNSUPPORTED = 5, /* modulus not prime or outside [2, 2^31) */
+  FPXG_E_ARG = 6          /* null pointer / output capacity too small */
+};
+
+enum { FPXG_INV = 0, FPXG_LOG = 1, FPXG_EXP = 2, FPXG_SQRT = 3 };
+
+/* NttPlan variants (ntt.hpp:142-192). */
+enum {
+  FPXG_NTT_FORWARD = 0,         /* forward(): canonical, bit-reversed order */
+  FPXG_NTT_FORWARD_PREFIX = 1,  /* forward_prefix(a, count): a[count:n) treated as zero */
+  FPXG_NTT_INVERSE = 2,         /* inverse(): normalised by 1/n */
+  FPXG_NTT_INVERSE_PREFIX = 3,  /* inverse_prefix(a, count): only [0, count) valid */
+  FPXG_NTT_INVERSE_HIGHHALF = 4 /* inverse_highhalf(): only [n/2, n) valid */
+};
+
+typedef struct {
+  size_t cached_bytes;       /* idle bytes held by the arena (ScratchMemoryStats, memory.hpp:21-26) */
+  size_t cached_blocks;
+  size_t system_allocations; /* hipMalloc calls made by the arena */
+  size_t cache_hits;
+  size_t in_use_bytes;
+  size_t capacity_bytes;
+} fpxg_stats;
+
+/* Library lifetime.  fpxg_init is implicit on first use (device 0, default
+ * arena); call it explicitly to choose the device or cap the arena. */
+int fpxg_init(int device, size_t arena_bytes);
+void fpxg_shutdown(void);
+const char* fpxg_strerror(int code);
+const char* fpxg_last_error(void);
+int fpxg_arena_stats(fpxg_stats* out);
+/* release_scratch_memory (memory.hpp:251-258): drop idle arena blocks. */
+int fpxg_release_scratch(void);
+
+/* poly::conv(a, b, n) (detail/poly/convolution.hpp:145-147).  Writes
+ * min(min(la,n)+min(lb,n)-1, n) limbs (0 if an input is empty or n == 0);
+ * pass n = SIZE_MAX for the full product. */
+int fpxg_conv_u32(uint32_t p, const uint32_t* a, size_t la, const uint32_t* b, size_t lb,
+                  size_t n, uint32_t* out, size_t out_cap, size_t* out_len);
+
+/* poly::inv / log / exp / sqrt (inverse.hpp:163, exponential.hpp:85 and :107,
+ * square_root.hpp:12): n output limbs. */
+int fpxg_series_u32(int op, uint32_t p, const uint32_t* a, size_t la, size_t n, uint32_t* out);
+
+/* NttPlan<Mont<p, root>>::get(n) transforms in place (ntt.hpp:130-192);
+ * root == 0 selects the smallest primitive root (modint.hpp:28-52). */
+int fpxg_ntt_u32(uint32_t p, uint32_t root, uint32_t n, int variant, size_t count, uint32_t* data);
+
+/* Device-resident forms of the three calls above (asynchronous). */
+int fpxg_conv_dev(uint32_t p, const uint32_t* a, size_t la, const uint32_t* b, size_t lb,
+                  size_t n, uint32_t* out, size_t out_cap, size_t* out_len);
+int fpxg_series_dev(int op, uint32_t p, const uint32_t* a, size_t la, size_t n, uint32_t* out);
+int fpxg_ntt_dev(uint32_t p, uint32_t root, uint32_t n, int variant, size_t count, uint32_t* data);
+
+/* Device memory and stream plumbing for the *_dev entry points. */
+void* fpxg_dev_alloc(size_t bytes);
+int fpxg_dev_free(void* ptr);
+int fpxg_memcpy_h2d(void* dst, const void* src, size_t bytes);
+int fpxg_memcpy_d2h(void* dst, const void* src, size_t bytes);
+int fpxg_synchronize(void);
+void* fpxg_stream(void);
+/* HIP events on the library stream: record slot (0..15), elapsed ms a -> b. */
+int fpxg_event_record(int slot);
+int fpxg_event_elapsed(int slot_a, int slot_b, float* ms);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* FPXG_H */

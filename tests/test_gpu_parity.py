@@ -1,0 +1,184 @@
+"""GPU parity: every result of the HIP path (through the C ABI) is bit-identical
+to the oracle / the reference's published checksums on the same seeded inputs.
+"""
+import random
+
+import numpy as np
+import pytest
+
+import golden_inputs as gi
+
+pytestmark = pytest.mark.gpu
+
+P = 998244353
+Q = 1000000007
+
+
+@pytest.fixture(scope="module")
+def fa():
+    import fastpoly_amd
+    fastpoly_amd.init(0, 0)
+    return fastpoly_amd
+
+
+def _rnd(po, p, n, seed):
+    return po.MT(seed).rnd(p, n)
+
+
+# ------------------------------------------------------------------ NTT order
+@pytest.mark.parametrize("p", [P, 167772161, 469762049, 754974721, 1224736769])
+def test_ntt_forward_inverse_exact(fa, oracle, p):
+    root = oracle.lib().fpo_primitive_root(p)
+    for lg in list(range(1, 15)) + [15, 16, 17, 18, 20]:
+        n = 1 << lg
+        x = _rnd(oracle, p, n, 1 + n)
+        want = oracle.ntt(p, x, root=root)
+        got = fa.ntt_forward(p, x, root=root)
+        assert np.array_equal(got, want), (p, n)
+        back = fa.ntt_inverse(p, want, root=root)
+        assert np.array_equal(back, x), (p, n)
+
+
+def test_ntt_windows(fa, oracle):
+    po = oracle
+    for lg in (4, 10, 14, 15, 17):
+        n = 1 << lg
+        x = _rnd(po, P, n, 7 + n)
+        plan = fa.NttPlan(fa.Mont(P), n)
+        for count in (1, n // 4, n // 2, n // 2 + 3, n):
+            pre = x.copy()
+            pre[count:] = 0
+            assert np.array_equal(plan.forward_prefix(x, count), po.ntt(P, pre)), (n, count)
+        spec = po.ntt(P, x)
+        full = po.ntt(P, spec, inverse=True)
+        for count in (1, n // 4, n // 2 + 1, n):
+            got = plan.inverse_prefix(spec, count)
+            assert np.array_equal(got[:count], full[:count])
+        hh = plan.inverse_highhalf(spec)
+        assert np.array_equal(hh[n // 2:], full[n // 2:])
+
+
+def test_ntt_size_errors(fa):
+    with pytest.raises(fa.NttSizeError):
+        fa.ntt_forward(P, np.zeros(3, np.uint32))
+    with pytest.raises(fa.NttSizeError):
+        fa.ntt_forward(Q, np.zeros(4, np.uint32))
+    with pytest.raises(fa.UnsupportedModulus):
+        fa.conv(1000000000, [1], [1])
+
+
+@pytest.mark.parametrize("p,op,n,want", [
+    (int(m), op, int(s), ck)
+    for m, t in gi.reference_checksums().items() for op, sizes in t.items()
+    if op == "ntt-forward" for s, ck in sizes.items() if int(s) <= (1 << 20)])
+def test_golden_ntt_forward(fa, oracle, p, op, n, want):
+    root = oracle.lib().fpo_primitive_root(p)
+    x = gi.bench_case(oracle, p, op, n)
+    assert gi.hexsum(oracle, fa.ntt_forward(p, x, root=root)) == want
+
+
+# ------------------------------------------------------------------ conv
+def test_config1_conv_65536(fa, oracle):
+    a, b = gi.bench_case(oracle, P, "conv", 65536)
+    assert gi.hexsum(oracle, fa.conv(P, a, b)) == "841755dfc72581be"
+
+
+@pytest.mark.parametrize("p", [P, Q, 1224736769, 167772161, 100003, 3, 2])
+def test_conv_random_shapes(fa, oracle, p):
+    rng = random.Random(p)
+    shapes = [(1, 1), (1, 7), (2, 2), (5, 40), (40, 41), (64, 64), (100, 3000), (4097, 4096),
+              (8191, 8193), (16384, 16385), (70000, 3), (33000, 40000)]
+    for la, lb in shapes:
+        a = _rnd(oracle, p, la, rng.randrange(1 << 30))
+        b = _rnd(oracle, p, lb, rng.randrange(1 << 30))
+        want = oracle.conv(p, a, b)
+        assert np.array_equal(fa.conv(p, a, b), want), (p, la, lb)
+        n = (la + lb) // 3 + 1
+        assert np.array_equal(fa.conv(p, a, b, n), want[:n]), (p, la, lb, n)
+    x = _rnd(oracle, p, 5000, 11)
+    assert np.array_equal(fa.conv(p, x, x), oracle.conv(p, x, x))
+    assert fa.conv(p, [], x).size == 0 and fa.conv(p, x, x, 0).size == 0
+
+
+@pytest.mark.parametrize("p,op,n,want", [
+    (int(m), op, int(s), ck)
+    for m, t in gi.reference_checksums().items() for op, sizes in t.items()
+    if op in ("conv", "conv-trunc", "conv-tail", "square", "skinny") for s, ck in sizes.items()
+    if int(s) <= (1 << 20)])
+def test_golden_conv(fa, oracle, p, op, n, want):
+    from test_oracle import run_conv_op
+    assert gi.hexsum(oracle, run_conv_op(oracle, p, op, n, conv=fa.conv)) == want
+
+
+# ------------------------------------------------------------------ series
+SERIES_OPS = ("inv", "log", "exp", "sqrt", "inv-short", "inv-linear", "exp-linear")
+
+
+@pytest.mark.parametrize("p,op,n,want", [
+    (int(m), op, int(s), ck)
+    for m, t in gi.reference_checksums().items() for op, sizes in t.items()
+    if op in SERIES_OPS for s, ck in sizes.items() if int(s) <= (1 << 20)])
+def test_golden_series(fa, oracle, p, op, n, want):
+    from test_oracle import run_series_op
+    got = run_series_op(oracle, p, op, n, series=fa.series, conv=fa.conv)
+    assert gi.hexsum(oracle, got) == want
+
+
+@pytest.mark.parametrize("key", sorted(gi.ref_vectors()["series"].keys()))
+def test_ref_series_vectors(fa, key):
+    p, op, n = key.split("/")
+    p, n = int(p), int(n)
+    case = gi.ref_vectors()["series"][key]
+    base = "sqrt" if op.startswith("sqrt") else op
+    assert [int(v) for v in fa.series(base, p, case["a"], n)] == case["out"]
+
+
+@pytest.mark.parametrize("p", [P, Q, 1224736769, 100003])
+def test_series_random_vs_oracle(fa, oracle, p):
+    rng = random.Random(p + 5)
+    for n in (1, 2, 3, 9, 31, 32, 33, 40, 41, 64, 65, 100, 1000, 4097, 20000):
+        a = _rnd(oracle, p, n, rng.randrange(1 << 30))
+        a[0] = oracle.from_int(p, 1 + rng.randrange(p - 1))
+        assert np.array_equal(fa.inv(p, a, n), oracle.series("inv", p, a, n)), ("inv", p, n)
+        a1 = a.copy(); a1[0] = oracle.from_int(p, 1)
+        if n <= p:
+            assert np.array_equal(fa.log(p, a1, n), oracle.series("log", p, a1, n)), ("log", p, n)
+            a0 = a.copy(); a0[0] = 0
+            assert np.array_equal(fa.exp(p, a0, n), oracle.series("exp", p, a0, n)), ("exp", p, n)
+        sq = oracle.conv(p, a, a, n)
+        assert np.array_equal(fa.sqrt(p, sq, n), oracle.series("sqrt", p, sq, n)), ("sqrt", p, n)
+
+
+def test_series_edge_cases(fa, oracle):
+    po = oracle
+    one = po.from_int(P, 1)
+    for op, a in [("inv", [0, one]), ("inv", []), ("log", [po.from_int(P, 2)]), ("exp", [one]),
+                  ("sqrt", [0, one]), ("sqrt", [po.from_int(P, 3)])]:
+        with pytest.raises(fa.DomainError):
+            fa.series(op, P, a, 10)
+    # short inputs, zero tails, valuations (inverse.hpp:169-177, square_root.hpp:26-44)
+    cases = [("inv", [one] + [0] * 5), ("inv", [one, po.from_int(P, 5)]), ("inv", [one, 0, 0, one]),
+             ("exp", [0]), ("exp", [0, one]), ("exp", []), ("sqrt", [0, 0, 0, 0, po.from_int(P, 4)]),
+             ("sqrt", [0] * 6), ("sqrt", [0, 0, one, one]), ("log", [one])]
+    for op, a in cases:
+        for n in (1, 5, 70, 3000):
+            assert np.array_equal(fa.series(op, P, a, n), po.series(op, P, a, n)), (op, a, n)
+    assert fa.inv(P, [one], 0).size == 0
+
+
+# ------------------------------------------------------------------ workload sizes
+@pytest.mark.parametrize("key", sorted(gi.ref_vectors()["checksums"].keys()))
+def test_workload_checksums(fa, oracle, key):
+    po = oracle
+    p, op, n = key.split("/")
+    p, n = int(p), int(n)
+    want = gi.ref_vectors()["checksums"][key]
+    if op == "conv":
+        a, b = gi.bench_case(po, p, "conv", n)
+        got = fa.conv(p, a, b)
+    elif op == "inv-random-a0":
+        got = fa.inv(p, gi.series_input(po, p, n, True), n)
+    else:
+        from test_oracle import run_series_op
+        got = run_series_op(po, p, op, n, series=fa.series, conv=fa.conv)
+    assert gi.hexsum(po, got) == want
