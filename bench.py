@@ -1,0 +1,218 @@
+#!/usr/bin/env python3
+"""Benchmark of the fastpoly NTT convolution path on one MI355X.
+
+Workload (one "step"), mod 998244353 with synthetic uniform [0, p) limbs:
+  * conv of two 2^23-coefficient polynomials: product of 2^24 - 1 limbs through
+    three auxiliary-prime 2^24-point NTTs and Garner reconstruction;
+  * Newton series at n = 2^20: inv (random a0), log (a0 = 1), exp (b0 = 0),
+    sqrt (c = d^2 mod x^n, d0 = 1).
+Inputs are resident in HBM before the timed region; `value` counts output
+coefficients per second of the whole step.
+
+Also reported on the same line:
+  roofline      2^24-point NTT passes (column pass / row pass) timed with HIP
+                events on the library stream; algorithmic bytes 8N per pass
+                (read + write N uint32), 16N per transform;
+  cpu_baseline  the reference library (compiled from the reference sources into
+                oracle/_ref) timed on this host, one thread, on the same step;
+                its outputs are compared limb for limb with the GPU's.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
+"""
+import argparse
+import json
+import os
+import platform
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+P = 998244353
+AUX = 167772161
+N_CONV = 1 << 23
+N_SERIES = 1 << 20
+HBM_PEAK_GBS = 8000.0
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--ntt-reps", type=int, default=50)
+    return ap.parse_args()
+
+
+def montgomery_one(p):
+    return (1 << 32) % p
+
+
+def make_inputs(rng):
+    a = rng.integers(0, P, N_CONV, dtype=np.uint32)
+    b = rng.integers(0, P, N_CONV, dtype=np.uint32)
+    s = rng.integers(0, P, N_SERIES, dtype=np.uint32)
+    inv_in = s.copy()
+    inv_in[0] = int(rng.integers(1, P))
+    log_in = s.copy()
+    log_in[0] = montgomery_one(P)
+    exp_in = s.copy()
+    exp_in[0] = 0
+    d = rng.integers(0, P, N_SERIES, dtype=np.uint32)
+    d[0] = montgomery_one(P)
+    return a, b, inv_in, log_in, exp_in, d
+
+
+def output_coeffs():
+    return (2 * N_CONV - 1) + 4 * N_SERIES
+
+
+def traffic_from_profiles():
+    path = os.path.join(ROOT, "profiles", "traffic.json")
+    try:
+        with open(path) as fh:
+            return json.load(fh)
+    except (OSError, ValueError):
+        return {}
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not offered "
+              "(batches shard across GPUs with no exchange; see DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+    import fastpoly_amd as fa
+
+    fa.init(0, 0)
+    rng = np.random.default_rng(20261203)
+    a, b, inv_in, log_in, exp_in, d = make_inputs(rng)
+    D = fa.DeviceArray
+    da, db = D.from_host(a), D.from_host(b)
+    dinv, dlog, dexp = D.from_host(inv_in), D.from_host(log_in), D.from_host(exp_in)
+    dd = D.from_host(d)
+    dsq = D(N_SERIES)
+    fa.conv_dev(P, dd, N_SERIES, dd, N_SERIES, dsq, N_SERIES)  # sqrt input, untimed setup
+    out_conv = D(2 * N_CONV - 1)
+    out_s = [D(N_SERIES) for _ in range(4)]
+    fa.synchronize()
+
+    def step():
+        fa.conv_dev(P, da, N_CONV, db, N_CONV, out_conv)
+        fa.series_dev("inv", P, dinv, N_SERIES, N_SERIES, out_s[0])
+        fa.series_dev("log", P, dlog, N_SERIES, N_SERIES, out_s[1])
+        fa.series_dev("exp", P, dexp, N_SERIES, N_SERIES, out_s[2])
+        fa.series_dev("sqrt", P, dsq, N_SERIES, N_SERIES, out_s[3])
+
+    for _ in range(args.warmup):
+        step()
+    fa.synchronize()
+    fa.event_record(0)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    fa.synchronize()
+    t1 = time.perf_counter()
+    fa.event_record(1)
+    wall = t1 - t0
+    ms_per_step = 1e3 * wall / args.steps
+    value = output_coeffs() * args.steps / wall
+
+    gpu_out = {"conv": out_conv.to_host(), "inv": out_s[0].to_host(), "log": out_s[1].to_host(),
+               "exp": out_s[2].to_host(), "sqrt": out_s[3].to_host()}
+    sq_host = dsq.to_host()
+
+    # ---- roofline: the 2^24-point transform's two passes, HIP events on the library stream
+    n = 1 << 24
+    plan = fa.ntt_plan_info(n)
+    x = D.from_host(rng.integers(0, AUX, n, dtype=np.uint32))
+    passes = {}
+    for pidx, name in ((0, "ntt_col_pass"), (1, "ntt_row_pass")):
+        for _ in range(3):
+            fa.ntt_pass_dev(AUX, x, pidx)
+        fa.synchronize()
+        fa.event_record(2)
+        for _ in range(args.ntt_reps):
+            fa.ntt_pass_dev(AUX, x, pidx)
+        fa.event_record(3)
+        ms = fa.event_elapsed_ms(2, 3) / args.ntt_reps
+        passes[name] = ms
+    alg_pass = 8 * n
+    dominant = max(passes, key=passes.get)
+    achieved = alg_pass / (passes[dominant] * 1e-3) / 1e9
+    traffic = traffic_from_profiles().get(dominant)
+    t_transform = sum(passes.values())
+    roofline = {
+        "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+        "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+        "kernel": dominant, "alg_bytes_per_launch": alg_pass, "avg_launch_ms": round(passes[dominant], 5),
+        "ntt_2p24": {
+            "plan": plan, "pass_ms": {k: round(v, 5) for k, v in passes.items()},
+            "transform_ms": round(t_transform, 5),
+            "achieved_GBps": round(16 * n / (t_transform * 1e-3) / 1e9, 1),
+            "frac": round(16 * n / (t_transform * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+        },
+    }
+
+    result = {
+        "metric": "bit-exact coeffs/s: batched vs per-call GPU; 2^24 NTT HBM GB/s frac; p99 latency",
+        "value": value, "unit": "coeffs/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": ms_per_step, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "u32", "data": "synthetic (uniform [0,p) Montgomery limbs, numpy PCG64 seed 20261203)",
+        "config": {"workload": "single large transforms mod 998244353: conv 2^23 x 2^23 (2^24-point aux NTTs) "
+                               "+ inv/log/exp/sqrt at n = 2^20",
+                   "modulus": P, "conv_len": N_CONV, "series_n": N_SERIES},
+        "roofline": roofline,
+        "cpu_baseline": None,
+    }
+
+    if not args.no_cpu:
+        result["cpu_baseline"] = cpu_baseline(args, a, b, inv_in, log_in, exp_in, sq_host, gpu_out)
+    print(json.dumps(result), flush=True)
+
+
+def cpu_baseline(args, a, b, inv_in, log_in, exp_in, sq_host, gpu_out):
+    """The reference library on this host (1 thread), same step, same inputs."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle as po
+
+    kind = "reference" if po.ref() is not None else "port"
+    conv = po.ref_conv if kind == "reference" else po.conv
+    series = po.ref_series if kind == "reference" else po.series
+    outs = {}
+    t0 = time.perf_counter()
+    reps = 0
+    while True:
+        outs["conv"] = conv(P, a, b)
+        outs["inv"] = series("inv", P, inv_in, N_SERIES)
+        outs["log"] = series("log", P, log_in, N_SERIES)
+        outs["exp"] = series("exp", P, exp_in, N_SERIES)
+        outs["sqrt"] = series("sqrt", P, sq_host, N_SERIES)
+        reps += 1
+        el = time.perf_counter() - t0
+        if el >= args.cpu_seconds or reps >= 20:
+            break
+    exact = all(np.array_equal(outs[k], gpu_out[k]) for k in outs)
+    cpu = platform.processor() or platform.machine()
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    cpu = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    backend = po.ref().fpref_backend().decode() if kind == "reference" else "scalar C"
+    return {"value": output_coeffs() * reps / el, "unit": "coeffs/s", "cores": 1, "kind": kind,
+            "sample": f"{reps} full steps (conv 2^23x2^23 + inv/log/exp/sqrt n=2^20), {el:.1f} s, "
+                      f"backend {backend}, {cpu}",
+            "bit_exact_vs_gpu": bool(exact), "ms_per_step": 1e3 * el / reps}
+
+
+if __name__ == "__main__":
+    main()

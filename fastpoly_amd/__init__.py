@@ -39,6 +39,7 @@ EXPORTED_SYMBOLS = [
     "fpxg_release_scratch", "fpxg_conv_u32", "fpxg_series_u32", "fpxg_ntt_u32", "fpxg_conv_dev",
     "fpxg_series_dev", "fpxg_ntt_dev", "fpxg_dev_alloc", "fpxg_dev_free", "fpxg_memcpy_h2d",
     "fpxg_memcpy_d2h", "fpxg_synchronize", "fpxg_stream", "fpxg_event_record", "fpxg_event_elapsed",
+    "fpxg_ntt_pass_dev", "fpxg_ntt_plan_info",
 ]
 
 
@@ -111,6 +112,10 @@ def lib():
                                    _u32p]
         L.fpxg_ntt_dev.argtypes = [ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_int, ctypes.c_size_t,
                                    ctypes.c_void_p]
+        L.fpxg_ntt_pass_dev.argtypes = [ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_int,
+                                        ctypes.c_void_p]
+        _ip = ctypes.POINTER(ctypes.c_int)
+        L.fpxg_ntt_plan_info.argtypes = [ctypes.c_uint32, _ip, _ip, _ip]
         L.fpxg_dev_alloc.restype = ctypes.c_void_p
         L.fpxg_dev_alloc.argtypes = [ctypes.c_size_t]
         L.fpxg_dev_free.argtypes = [ctypes.c_void_p]
@@ -348,3 +353,13 @@ def event_elapsed_ms(a: int, b: int) -> float:
     ms = ctypes.c_float(0)
     _check(lib().fpxg_event_elapsed(a, b, ctypes.byref(ms)), "event_elapsed")
     return float(ms.value)
+
+
+def ntt_pass_dev(p: int, data: DeviceArray, pass_index: int, root: int = 0) -> None:
+    _check(lib().fpxg_ntt_pass_dev(p, root, data.n, pass_index, data.ptr), "ntt_pass_dev")
+
+
+def ntt_plan_info(n: int) -> dict:
+    r, c, t = ctypes.c_int(0), ctypes.c_int(0), ctypes.c_int(0)
+    _check(lib().fpxg_ntt_plan_info(n, ctypes.byref(r), ctypes.byref(c), ctypes.byref(t)), "plan_info")
+    return {"row_log": r.value, "col_levels": c.value, "tile_cols_log": t.value}

@@ -745,6 +745,39 @@ int fpxg_ntt_dev(uint32_t p, uint32_t root, uint32_t n, int variant, size_t coun
   });
 }
 
+int fpxg_ntt_pass_dev(uint32_t p, uint32_t root, uint32_t n, int pass, uint32_t* data) {
+  return guarded([&](Context& c) {
+    check_modulus(p);
+    const HostField& F = c.field(p, root);
+    ntt_check(F, n, FPXG_NTT_FORWARD, 0);
+    if (!data) throw Error(E_ARG, "data is null");
+    const int L = ceil_log2(n);
+    const PassPlan P = c.plan(L);
+    if (pass == 0) {
+      if (P.S == 0) throw Error(E_ARG, "single-pass size has no column pass");
+      ColJob cj{};
+      cj.src = data; cj.dst = data; cj.in_count = n; cj.f = F.dev;
+      launch_cols(c, F.lazy, false, cj, P);
+    } else if (pass == 1) {
+      RowJob rj{};
+      rj.src_a = data; rj.dst = data; rj.in_a = n; rj.out_lo = 0; rj.out_hi = n; rj.canon = 1; rj.f = F.dev;
+      launch_rows(c, F.lazy, MODE_FWD_A, rj, P.b, 1u << P.S);
+    } else {
+      throw Error(E_ARG, "pass must be 0 or 1");
+    }
+  });
+}
+
+int fpxg_ntt_plan_info(uint32_t n, int* row_log, int* col_levels, int* tile_cols_log) {
+  return guarded([&](Context& c) {
+    if (n < 2 || (n & (n - 1))) throw Error(E_SIZE, "NTT size must be a power of two");
+    const PassPlan P = c.plan(ceil_log2(n));
+    if (row_log) *row_log = P.b;
+    if (col_levels) *col_levels = P.S;
+    if (tile_cols_log) *tile_cols_log = P.logC;
+  });
+}
+
 int fpxg_series_u32(int op, uint32_t p, const uint32_t* a, size_t la, size_t n, uint32_t* out) {
   return guarded([&](Context& c) {
     check_modulus(p);

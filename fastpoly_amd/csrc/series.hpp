@@ -156,6 +156,18 @@ __global__ void k_scan_nonzero(const uint32_t* a, size_t lo, size_t hi, unsigned
     }
 }
 
+// Characteristic two (square_root.hpp:15-25): (sum b_i x^i)^2 = sum b_i x^(2i).
+// out[i/2] = a[i] for even i; any nonzero odd coefficient sets *bad.
+__global__ void k_frobenius_sqrt(const uint32_t* a, size_t cnt, uint32_t* out, unsigned int* bad) {
+  FPXG_GRID_STRIDE(i, cnt) {
+    if (i & 1) {
+      if (a[i] != 0) atomicOr(bad, 1u);
+    } else {
+      out[i / 2] = a[i];
+    }
+  }
+}
+
 // ---------------------------------------------------------------------------
 // Host side.
 // ---------------------------------------------------------------------------
@@ -386,7 +398,14 @@ static void series_exp(Context& c, const HostField& T, const uint32_t* a, size_t
 // poly::sqrt (square_root.hpp:12-112).
 static void series_sqrt(Context& c, const HostField& T, const uint32_t* a, size_t la, size_t n, uint32_t* out) {
   fill(c, out, n, 0);
-  if (T.p == 2) throw Error(E_UNSUPPORTED, "sqrt in characteristic 2 is not offered on the device");
+  if (T.p == 2) {
+    const size_t cnt = std::min(la, n);
+    DevBuf flag = c.buf(1);
+    fill(c, flag.get(), 1, 0);
+    launch_ew(c, k_frobenius_sqrt, cnt, a, cnt, out, (unsigned int*)flag.get());
+    if (read_word(c, flag.get())) throw Error(E_DOMAIN, "poly::sqrt: nonzero odd coefficient in characteristic 2");
+    return;
+  }
   const auto nz = scan_nonzero(c, a, 0, la);
   const size_t v = nz.first;
   if (v >= la) return;

@@ -83,6 +83,15 @@ int fpxg_conv_dev(uint32_t p, const uint32_t* a, size_t la, const uint32_t* b, s
 int fpxg_series_dev(int op, uint32_t p, const uint32_t* a, size_t la, size_t n, uint32_t* out);
 int fpxg_ntt_dev(uint32_t p, uint32_t root, uint32_t n, int variant, size_t count, uint32_t* data);
 
+/* One pass of a device-resident forward transform (0 = column pass with the
+ * four-step twiddle, 1 = row pass; a single-pass size only has pass 1), for
+ * per-kernel timing and tuning.  Returns FPXG_E_ARG for a pass the plan of
+ * this size does not have. */
+int fpxg_ntt_pass_dev(uint32_t p, uint32_t root, uint32_t n, int pass, uint32_t* data);
+/* The plan chosen for an n-point transform: row words 2^*row_log, column
+ * levels *col_levels in tiles of 2^*tile_cols_log columns. */
+int fpxg_ntt_plan_info(uint32_t n, int* row_log, int* col_levels, int* tile_cols_log);
+
 /* Device memory and stream plumbing for the *_dev entry points. */
 void* fpxg_dev_alloc(size_t bytes);
 int fpxg_dev_free(void* ptr);

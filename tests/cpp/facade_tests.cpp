@@ -1,0 +1,134 @@
+// C++ drop-in checks: the fpx:: facade (include/fastpoly_amd/fastpoly.hpp)
+// used exactly as reference code uses fastpoly, on the reference benchmark's
+// inputs (std::mt19937_64, bench/bench.cpp:216-220) against its published
+// FNV-1a checksums (bench/bench.cpp:152-159).  Built into a shared object by
+// tests/cpp/build.sh; tests/test_facade.py calls fpx_facade_tests on the GPU.
+#include <cstdio>
+#include <cstring>
+#include <random>
+#include <string>
+
+#include "fastpoly_amd/fastpoly.hpp"
+
+namespace {
+
+int failures = 0;
+std::string log_text;
+
+void check(bool ok, const char* what) {
+  char line[256];
+  std::snprintf(line, sizeof line, "%s %s\n", ok ? "ok  " : "FAIL", what);
+  log_text += line;
+  if (!ok) ++failures;
+}
+
+template <class V>
+unsigned long long checksum(const V& a) {
+  unsigned long long h = 14695981039346656037ull;
+  for (const auto& x : a) {
+    h ^= x.raw_val();
+    h *= 1099511628211ull;
+  }
+  return h ^ a.size();
+}
+
+template <class M>
+std::vector<M> rnd(size_t n, std::mt19937_64& rng) {
+  std::vector<M> a(n);
+  for (auto& x : a) x = M::from_int(rng() % M::mod);
+  return a;
+}
+
+template <class F>
+bool throws_domain(F&& f) {
+  try { f(); } catch (const fpx::poly::domain_error&) { return true; } catch (...) { return false; }
+  return false;
+}
+template <class F>
+bool throws_size(F&& f) {
+  try { f(); } catch (const fpx::ntt_size_error&) { return true; } catch (...) { return false; }
+  return false;
+}
+
+void run_all() {
+  using M = fpx::mod998244353;
+  {
+    const size_t n = 65536;
+    std::mt19937_64 rng(2 + n);
+    const fpx::Poly<M> a(rnd<M>(n, rng)), b(rnd<M>(n, rng));
+    check(checksum(a * b) == 0x841755dfc72581beull, "Poly operator* 65536 (bench conv checksum)");
+    check(checksum(a.mul(b, n)) == 0x6548fcfc353988f8ull, "Poly::mul(b, n) 65536 (conv-trunc checksum)");
+    check(checksum(fpx::poly::conv(a, a)) == 0x7069bfe1ae6aad95ull, "conv(a, a) (square checksum)");
+  }
+  {
+    const size_t n = 65536;
+    std::mt19937_64 rng(3 + n);
+    fpx::Poly<M> a(rnd<M>(n, rng));
+    a[0] = M::from_int(1);
+    check(checksum(a.inv(n)) == 0x81c1f8a79506b8d2ull, "inv 65536");
+    check(checksum(a.log(n)) == 0x605ffdc345fa9039ull, "log 65536");
+    fpx::Poly<M> e = a;
+    e[0] = M();
+    check(checksum(e.exp(n)) == 0xff4d4e75dae0ae43ull, "exp 65536");
+    const fpx::Poly<M> sq = a.mul(a, n);
+    check(checksum(sq.sqrt(n)) == 0xd32ea1d3f38db84aull, "sqrt 65536");
+    const auto one = a.mul(a.inv(n), n);
+    bool unit = one[0] == M::from_int(1);
+    for (size_t i = 1; i < one.size(); ++i) unit = unit && one[i].is_zero();
+    check(unit, "a * inv(a) == 1 mod x^n");
+  }
+  {
+    using Q = fpx::mod1000000007;
+    const size_t n = 16384;
+    std::mt19937_64 rng(2 + n);
+    const auto a = rnd<Q>(n, rng), b = rnd<Q>(n, rng);
+    check(checksum(fpx::poly::conv(a, b)) == 0x80dc26cc95895f00ull, "conv mod 1e9+7 16384 (CRT)");
+    std::mt19937_64 r3(3 + n);
+    auto s = rnd<Q>(n, r3);
+    s[0] = Q::from_int(1);
+    check(checksum(fpx::poly::inv(s, n)) == 0x0b6d10016dec1f5bull, "inv mod 1e9+7 16384");
+  }
+  {
+    const uint32_t n = 65536;
+    std::mt19937_64 rng(1 + n);
+    const auto in = rnd<M>(n, rng);
+    std::vector<uint32_t> a(n);
+    for (uint32_t i = 0; i < n; ++i) a[i] = in[i].raw_val();
+    const auto& plan = fpx::NttPlan<M>::get_ref(n);
+    plan.forward(a.data());
+    std::vector<M> fa(n);
+    for (uint32_t i = 0; i < n; ++i) fa[i] = M::raw(a[i]);
+    check(checksum(fa) == 0xb83db2bcacb85d37ull, "NttPlan::forward 65536 (ntt-forward checksum)");
+    plan.inverse(a.data());
+    bool same = true;
+    for (uint32_t i = 0; i < n; ++i) same = same && a[i] == in[i].raw_val();
+    check(same, "NttPlan::inverse roundtrip");
+  }
+  check(throws_domain([] { fpx::poly::inv(std::vector<M>{M(), M(1)}, 4); }), "inv(a0 == 0) -> domain_error");
+  check(throws_domain([] { fpx::poly::log(std::vector<M>{M(2)}, 4); }), "log(a0 != 1) -> domain_error");
+  check(throws_domain([] { fpx::poly::exp(std::vector<M>{M(1)}, 4); }), "exp(a0 != 0) -> domain_error");
+  check(throws_domain([] { fpx::poly::sqrt(std::vector<M>{M(), M(1)}, 4); }), "sqrt odd valuation -> domain_error");
+  check(throws_size([] { fpx::NttPlan<M>::get(3); }), "NttPlan::get(3) -> ntt_size_error");
+  check(throws_size([] { fpx::NttPlan<M>::get(1u << 24); }), "NttPlan::get(2^24) -> ntt_size_error");
+  check(fpx::poly::conv(std::vector<M>{}, std::vector<M>{M(1)}).empty(), "conv with an empty operand");
+  const auto st = fpx::scratch_memory_stats();
+  check(st.system_allocations >= 1, "scratch_memory_stats reports the device arena");
+}
+
+}  // namespace
+
+extern "C" int fpx_facade_tests(char* report, size_t cap) {
+  failures = 0;
+  log_text.clear();
+  try {
+    run_all();
+  } catch (const std::exception& e) {
+    log_text += std::string("FAIL exception: ") + e.what() + "\n";
+    ++failures;
+  }
+  if (report && cap) {
+    std::strncpy(report, log_text.c_str(), cap - 1);
+    report[cap - 1] = 0;
+  }
+  return failures;
+}
