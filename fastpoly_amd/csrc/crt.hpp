@@ -1,0 +1,59 @@
+// Three-prime CRT reconstruction kernel (restore_crt, crt_convolution.hpp:88-127).
+#pragma once
+
+#include "device.hpp"
+
+namespace fpxg {
+
+// Garner reconstruction of restore_crt (crt_convolution.hpp:88-127).  c_k are
+// canonical Montgomery residues of the exact integer product of the raw target
+// limbs modulo the auxiliary primes; the integer x1 + p1 x2 + p1 p2 x3 carries
+// two target Montgomery factors and one REDC leaves the product's limb.
+struct GarnerConst {
+  uint32_t P[3], Pninv[3];
+  uint32_t inv12;    // Montgomery (mod P2) of p1^-1
+  uint32_t p1_in3;   // Montgomery (mod P3) of p1
+  uint32_t inv123;   // Montgomery (mod P3) of (p1 p2)^-1
+  uint32_t p, ninv;  // target
+  uint32_t p1_t, p12_t;  // p1 mod p, p1 p2 mod p (plain)
+  uint32_t primes, accumulate;
+};
+
+__global__ void k_garner(const uint32_t* __restrict__ c1, const uint32_t* __restrict__ c2,
+                         const uint32_t* __restrict__ c3, uint32_t* __restrict__ out, uint32_t n,
+                         GarnerConst g) {
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const uint32_t x1 = redc_full(c1[i], g.P[0], g.Pninv[0]);
+    uint32_t x2 = 0, x3 = 0;
+    if (g.primes >= 2) {
+      const uint32_t r2 = redc_full(c2[i], g.P[1], g.Pninv[1]);
+      const uint32_t d2 = r2 >= x1 ? r2 - x1 : r2 + g.P[1] - x1;
+      x2 = mulmod_full(d2, g.inv12, g.P[1], g.Pninv[1]);
+      if (g.primes == 3) {
+        const uint32_t r3 = redc_full(c3[i], g.P[2], g.Pninv[2]);
+        uint32_t x12 = x1 + mulmod_full(x2, g.p1_in3, g.P[2], g.Pninv[2]);
+        x12 = x12 >= g.P[2] ? x12 - g.P[2] : x12;
+        const uint32_t d3 = r3 >= x12 ? r3 - x12 : r3 + g.P[2] - x12;
+        x3 = mulmod_full(d3, g.inv123, g.P[2], g.Pninv[2]);
+      }
+    }
+    uint32_t v;
+    if (g.p == 2) {
+      v = (x1 + x2 + x3) & 1u;
+      if (g.accumulate) v ^= out[i];
+    } else {
+      v = redc_full(x1, g.p, g.ninv);
+      uint32_t t = redc_full((uint64_t)x2 * g.p1_t, g.p, g.ninv);
+      v = v + t >= g.p ? v + t - g.p : v + t;
+      t = redc_full((uint64_t)x3 * g.p12_t, g.p, g.ninv);
+      v = v + t >= g.p ? v + t - g.p : v + t;
+      if (g.accumulate) {
+        const uint32_t o = out[i];
+        v = v + o >= g.p ? v + o - g.p : v + o;
+      }
+    }
+    out[i] = v;
+  }
+}
+
+}  // namespace fpxg

@@ -18,23 +18,24 @@
 
 namespace fpxg {
 
-// Device twiddle tables of one prime (built once per field, see runtime.cpp).
-//   lvl_fwd[(L>>1) + m] = w_L^m        for 2 <= L <= 2^kLvlLog, m < L/2
-//   lvl_inv[(L>>1) + m] = w_L^-m
-//   tw_lo[k] = w^k, tw_hi[k] = w^(k << split) with w of order 2^logmax;
-//   the *_inv planes hold the inverse powers.  All canonical Montgomery limbs.
+// Device twiddle tables of one prime (built once per field by the runtime).
+//   tw_lo[k] = w^k, tw_hi[k] = w^(k << split) with w of order 2^logmax, and
+//   the *_inv copies with inverse powers: w^x for any x < 2^logmax costs one
+//   REDC (the four-step twiddle of the column pass);
+//   planes_fwd / planes_inv: per-phase twiddle planes of blocks up to 2^14
+//   points (layout in ntt2.hpp).  All canonical Montgomery limbs.
 struct FieldDev {
   uint32_t p, ninv, r2, one;
   uint32_t logmax, split;
-  const uint32_t* lvl_fwd;
-  const uint32_t* lvl_inv;
   const uint32_t* tw_lo;
   const uint32_t* tw_hi;
   const uint32_t* tw_lo_inv;
   const uint32_t* tw_hi_inv;
+  const uint32_t* planes_fwd;
+  const uint32_t* planes_inv;
 };
 
-constexpr int kLvlLog = 14;  // per-level tables cover blocks up to 2^14 points
+constexpr int kPlaneLog = 14;  // twiddle planes cover blocks up to 2^14 points
 
 template <bool LAZY>
 struct Arith {
@@ -86,111 +87,8 @@ __device__ __forceinline__ uint32_t bitrev(uint32_t x, int bits) {
 template <bool LAZY>
 __device__ __forceinline__ uint32_t twiddle(const Arith<LAZY>& A, const uint32_t* lo,
                                             const uint32_t* hi, uint32_t split, uint32_t x) {
-  return A.mul(hi[x >> split], lo[x & ((1u << split) - 1)]);
-}
-
-// ---------------------------------------------------------------------------
-// LDS engine.  A tile is C = 2^logC columns x 2^S rows, row-major in LDS
-// (element (row i, column c) at pidx(i*C + c)).  Each column is an
-// independent 2^S-point transform whose level l (0 = outermost) pairs rows at
-// distance 2^(S-l-1) inside blocks of 2^(S-l) rows with twiddle w_{2^(S-l)}^m.
-// A radix-2^r phase covers levels [s, s+r): a thread holds the 2^r rows
-// hi*2^(S-s) + k*2^(S-s-r) + lo of one column in registers.
-// ---------------------------------------------------------------------------
-
-template <int r, bool LAZY>
-__device__ __forceinline__ void dif_phase(uint32_t* X, int S, int logC, int s, const uint32_t* lvl,
-                                          const Arith<LAZY>& A) {
-  const int lo_bits = S - s - r;
-  const uint32_t groups = 1u << (S - r + logC);
-  const uint32_t cmask = (1u << logC) - 1, lomask = (1u << lo_bits) - 1;
-  for (uint32_t g = threadIdx.x; g < groups; g += blockDim.x) {
-    const uint32_t c = g & cmask, gg = g >> logC;
-    const uint32_t lo = gg & lomask, hi = gg >> lo_bits;
-    const uint32_t base = (hi << (S - s)) | lo;
-    uint32_t x[1 << r];
-#pragma unroll
-    for (int k = 0; k < (1 << r); ++k) x[k] = X[pidx(((base + ((uint32_t)k << lo_bits)) << logC) | c)];
-#pragma unroll
-    for (int q = 0; q < r; ++q) {
-      const int h = 1 << (r - q - 1);
-      const uint32_t* tw = lvl + (1u << (S - s - q - 1));
-#pragma unroll
-      for (int k = 0; k < (1 << r); ++k) {
-        if (k & h) continue;
-        const uint32_t w = tw[((uint32_t)(k & (h - 1)) << lo_bits) | lo];
-        const uint32_t a0 = x[k], a1 = x[k + h];
-        x[k] = A.add(a0, a1);
-        x[k + h] = A.mul(a0 - a1 + A.R, w);
-      }
-    }
-#pragma unroll
-    for (int k = 0; k < (1 << r); ++k) X[pidx(((base + ((uint32_t)k << lo_bits)) << logC) | c)] = x[k];
-  }
-}
-
-template <int r, bool LAZY>
-__device__ __forceinline__ void dit_phase(uint32_t* X, int S, int logC, int s, const uint32_t* lvl,
-                                          const Arith<LAZY>& A) {
-  const int lo_bits = S - s - r;
-  const uint32_t groups = 1u << (S - r + logC);
-  const uint32_t cmask = (1u << logC) - 1, lomask = (1u << lo_bits) - 1;
-  for (uint32_t g = threadIdx.x; g < groups; g += blockDim.x) {
-    const uint32_t c = g & cmask, gg = g >> logC;
-    const uint32_t lo = gg & lomask, hi = gg >> lo_bits;
-    const uint32_t base = (hi << (S - s)) | lo;
-    uint32_t x[1 << r];
-#pragma unroll
-    for (int k = 0; k < (1 << r); ++k) x[k] = X[pidx(((base + ((uint32_t)k << lo_bits)) << logC) | c)];
-#pragma unroll
-    for (int q = r - 1; q >= 0; --q) {
-      const int h = 1 << (r - q - 1);
-      const uint32_t* tw = lvl + (1u << (S - s - q - 1));
-#pragma unroll
-      for (int k = 0; k < (1 << r); ++k) {
-        if (k & h) continue;
-        const uint32_t w = tw[((uint32_t)(k & (h - 1)) << lo_bits) | lo];
-        const uint32_t t = A.mul(x[k + h], w);
-        const uint32_t a0 = x[k];
-        x[k] = A.add(a0, t);
-        x[k + h] = A.sub(a0, t);
-      }
-    }
-#pragma unroll
-    for (int k = 0; k < (1 << r); ++k) X[pidx(((base + ((uint32_t)k << lo_bits)) << logC) | c)] = x[k];
-  }
-}
-
-// All S levels, radix-16 phases then one shorter phase; barriers between.
-template <bool LAZY>
-__device__ __forceinline__ void dif_all(uint32_t* X, int S, int logC, const uint32_t* lvl,
-                                        const Arith<LAZY>& A) {
-  for (int s = 0; s < S; s += 4) {
-    const int r = S - s < 4 ? S - s : 4;
-    switch (r) {
-      case 4: dif_phase<4, LAZY>(X, S, logC, s, lvl, A); break;
-      case 3: dif_phase<3, LAZY>(X, S, logC, s, lvl, A); break;
-      case 2: dif_phase<2, LAZY>(X, S, logC, s, lvl, A); break;
-      default: dif_phase<1, LAZY>(X, S, logC, s, lvl, A); break;
-    }
-    __syncthreads();
-  }
-}
-
-template <bool LAZY>
-__device__ __forceinline__ void dit_all(uint32_t* X, int S, int logC, const uint32_t* lvl,
-                                        const Arith<LAZY>& A) {
-  int last = ((S - 1) / 4) * 4;
-  for (int s = last; s >= 0; s -= 4) {
-    const int r = S - s < 4 ? S - s : 4;
-    switch (r) {
-      case 4: dit_phase<4, LAZY>(X, S, logC, s, lvl, A); break;
-      case 3: dit_phase<3, LAZY>(X, S, logC, s, lvl, A); break;
-      case 2: dit_phase<2, LAZY>(X, S, logC, s, lvl, A); break;
-      default: dit_phase<1, LAZY>(X, S, logC, s, lvl, A); break;
-    }
-    __syncthreads();
-  }
+  typedef __attribute__((address_space(1))) const uint32_t g32;
+  return A.mul(*(g32*)(hi + (x >> split)), *(g32*)(lo + (x & ((1u << split) - 1))));
 }
 
 }  // namespace fpxg

@@ -117,29 +117,19 @@ static void init_field(HostField& F, uint32_t p, uint32_t root) {
   if (p == 2 || F.maxlog < 1) return;
   // Device tables for transforms of up to 2^logmax points.
   const int logmax = std::min(F.maxlog, 26);
-  const int lvl_log = std::min(logmax, kLvlLog);
   const int split = std::min(logmax, 12);
-  const size_t lvl_n = (size_t)1 << lvl_log, lo_n = (size_t)1 << split, hi_n = (size_t)1 << (logmax - split);
-  std::vector<uint32_t> h(2 * lvl_n + 2 * lo_n + 2 * hi_n, 0);
-  uint32_t* lvl_f = h.data();
-  uint32_t* lvl_i = lvl_f + lvl_n;
-  uint32_t* lo_f = lvl_i + lvl_n;
+  const int plane_log = std::min(logmax, kPlaneLog);
+  const size_t lo_n = (size_t)1 << split, hi_n = (size_t)1 << (logmax - split);
+  const size_t tw_words = (2 * lo_n + 2 * hi_n + 3) & ~(size_t)3;  // keep the planes 16-byte aligned
+  std::vector<uint32_t> h(tw_words + 2 * kPlaneWords, 0);
+  uint32_t* lo_f = h.data();
   uint32_t* lo_i = lo_f + lo_n;
   uint32_t* hi_f = lo_i + lo_n;
   uint32_t* hi_i = hi_f + hi_n;
+  uint32_t* pl_f = h.data() + tw_words;
+  uint32_t* pl_i = pl_f + kPlaneWords;
   const uint32_t g = F.from_int(F.root);
   const uint32_t wmax = F.pow(g, (uint64_t)(p - 1) >> logmax), wmax_i = F.inv(wmax);
-  for (int k = 1; k <= lvl_log; ++k) {
-    const uint32_t L = 1u << k;
-    const uint32_t w = F.pow(wmax, (uint64_t)1 << (logmax - k)), wi = F.inv(w);
-    uint32_t cf = F.one, ci = F.one;
-    for (uint32_t m = 0; m < L / 2; ++m) {
-      lvl_f[L / 2 + m] = cf;
-      lvl_i[L / 2 + m] = ci;
-      cf = F.mul(cf, w);
-      ci = F.mul(ci, wi);
-    }
-  }
   {
     uint32_t cf = F.one, ci = F.one;
     for (size_t k = 0; k < lo_n; ++k) {
@@ -153,16 +143,47 @@ static void init_field(HostField& F, uint32_t p, uint32_t root) {
       cf = F.mul(cf, sf); ci = F.mul(ci, si);
     }
   }
+  // Powers of the roots of order 2^k, k <= plane_log, then the planes.
+  std::vector<std::vector<uint32_t>> pw_f(plane_log + 1), pw_i(plane_log + 1);
+  for (int k = 1; k <= plane_log; ++k) {
+    const uint32_t w = F.pow(wmax, (uint64_t)1 << (logmax - k)), wi = F.inv(w);
+    pw_f[k].resize((size_t)1 << (k - 1));
+    pw_i[k].resize((size_t)1 << (k - 1));
+    uint32_t cf = F.one, ci = F.one;
+    for (size_t m = 0; m < pw_f[k].size(); ++m) {
+      pw_f[k][m] = cf; pw_i[k][m] = ci;
+      cf = F.mul(cf, w); ci = F.mul(ci, wi);
+    }
+  }
+  for (int r = 1; r <= 4; ++r)
+    for (int logB = r; logB <= plane_log; ++logB) {
+      const int lo_bits = logB - r;
+      uint32_t* Pf = pl_f + plane_off(logB, r);
+      uint32_t* Pi = pl_i + plane_off(logB, r);
+      for (uint32_t lo = 0; lo < (1u << lo_bits); ++lo) {
+        int idx = 0;
+        for (int q = 0; q < r; ++q) {
+          const int Lg = logB - q, hh = 1 << (r - q - 1);
+          for (int k = 0; k < hh; ++k) {
+            const uint32_t m = ((uint32_t)k << lo_bits) | lo;
+            Pf[(lo << r) + idx + k] = pw_f[Lg][m];
+            Pi[(lo << r) + idx + k] = pw_i[Lg][m];
+          }
+          idx += hh;
+        }
+      }
+    }
   FPXG_HIP(hipMalloc(&F.tables, h.size() * 4));
   FPXG_HIP(hipMemcpy(F.tables, h.data(), h.size() * 4, hipMemcpyHostToDevice));
+  F.table_log = logmax;
   F.dev.logmax = logmax;
   F.dev.split = split;
-  F.dev.lvl_fwd = F.tables;
-  F.dev.lvl_inv = F.tables + lvl_n;
-  F.dev.tw_lo = F.tables + 2 * lvl_n;
-  F.dev.tw_lo_inv = F.dev.tw_lo + lo_n;
-  F.dev.tw_hi = F.dev.tw_lo_inv + lo_n;
-  F.dev.tw_hi_inv = F.dev.tw_hi + hi_n;
+  F.dev.tw_lo = F.tables;
+  F.dev.tw_lo_inv = F.tables + lo_n;
+  F.dev.tw_hi = F.tables + 2 * lo_n;
+  F.dev.tw_hi_inv = F.tables + 2 * lo_n + hi_n;
+  F.dev.planes_fwd = F.tables + tw_words;
+  F.dev.planes_inv = F.tables + tw_words + kPlaneWords;
 }
 
 static uint32_t inv_pow2(const HostField& F, int L) {
@@ -268,18 +289,95 @@ static std::mutex g_init_mu;
 static size_t g_arena_bytes = (size_t)16 << 30;
 static int g_device = 0;
 
-template <bool LAZY, int MODE>
-static bool set_rows_attr(size_t bytes) {
-  return hipFuncSetAttribute((const void*)k_rows<LAZY, MODE>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                             (int)bytes) == hipSuccess;
+// ---- launch dispatch over the compile-time shapes of ntt2.hpp ----
+
+template <int LOGM, bool LAZY, int MODE>
+static void rows2_launch(Context& c, const RowJob2& j, uint32_t nrows, uint32_t njobs, const RowJob2* dj) {
+  using RS = RowShape<LOGM>;
+  const size_t lds = (size_t)RS::U * RS::lds_words_per_row * 4;
+  dim3 grid((nrows + RS::U - 1) / RS::U, njobs), block(RS::T);
+  hipLaunchKernelGGL((k_rows2<LOGM, LAZY, MODE>), grid, block, lds, c.stream, j, dj);
 }
 
-#define FPXG_ROW_MODES(X)                                                                      \
-  X(MODE_FWD_A) X(MODE_INV) X(MODE_INV | MODE_SCALE) X(MODE_FWD_A | MODE_B_RAW | MODE_INV)     \
-  X(MODE_FWD_A | MODE_B_RAW | MODE_INV | MODE_SCALE) X(MODE_FWD_A | MODE_B_SPEC | MODE_INV)    \
-  X(MODE_FWD_A | MODE_B_SPEC | MODE_INV | MODE_SCALE) X(MODE_B_SPEC | MODE_INV)                \
-  X(MODE_B_SPEC | MODE_INV | MODE_SCALE) X(MODE_FWD_A | MODE_B_RAW | MODE_STORE_B | MODE_INV)  \
-  X(MODE_FWD_A | MODE_B_RAW | MODE_STORE_B | MODE_INV | MODE_SCALE)
+template <int LOGM, bool LAZY>
+static void rows2_mode(Context& c, int mode, const RowJob2& j, uint32_t nrows, uint32_t njobs, const RowJob2* dj) {
+  switch (mode) {
+    case R2M_FWD: rows2_launch<LOGM, LAZY, R2M_FWD>(c, j, nrows, njobs, dj); break;
+    case R2M_INV: rows2_launch<LOGM, LAZY, R2M_INV>(c, j, nrows, njobs, dj); break;
+    case R2M_CONV: rows2_launch<LOGM, LAZY, R2M_CONV>(c, j, nrows, njobs, dj); break;
+    case R2M_SPEC: rows2_launch<LOGM, LAZY, R2M_SPEC>(c, j, nrows, njobs, dj); break;
+    case R2M_SPEC_ONLY: rows2_launch<LOGM, LAZY, R2M_SPEC_ONLY>(c, j, nrows, njobs, dj); break;
+    default: throw Error(E_DEVICE, "unsupported row-pass mode");
+  }
+}
+
+template <bool LAZY>
+static void rows2_size(Context& c, int logM, int mode, const RowJob2& j, uint32_t nrows, uint32_t njobs,
+                       const RowJob2* dj) {
+  switch (logM) {
+#define FPXG_R(L) \
+  case L: rows2_mode<L, LAZY>(c, mode, j, nrows, njobs, dj); break;
+    FPXG_R(1) FPXG_R(2) FPXG_R(3) FPXG_R(4) FPXG_R(5) FPXG_R(6) FPXG_R(7)
+    FPXG_R(8) FPXG_R(9) FPXG_R(10) FPXG_R(11) FPXG_R(12) FPXG_R(13) FPXG_R(14)
+#undef FPXG_R
+    default: throw Error(E_SIZE, "row pass size out of range");
+  }
+}
+
+static void launch_rows(Context& c, bool lazy, int mode, const RowJob2& j, int logM, uint32_t nrows,
+                        uint32_t njobs = 1, const RowJob2* dj = nullptr) {
+  if (lazy) rows2_size<true>(c, logM, mode, j, nrows, njobs, dj);
+  else rows2_size<false>(c, logM, mode, j, nrows, njobs, dj);
+  FPXG_HIP(hipGetLastError());
+}
+
+static int tile_log_cols(int S) { return std::min(4, 14 - S); }
+
+template <int S, bool LAZY, bool INV>
+static void cols2_launch(Context& c, const ColJob2& j, const PassPlan& P, uint32_t njobs, const ColJob2* dj) {
+  constexpr int LOGC = S <= 10 ? 4 : 14 - S;
+  using CS = ColShape<S, LOGC>;
+  const size_t lds = (size_t)CS::words * 4;
+  dim3 grid(1u << (P.b - LOGC), njobs), block(CS::T);
+  hipLaunchKernelGGL((k_cols2<S, LOGC, LAZY, INV>), grid, block, lds, c.stream, j, dj, P.L);
+}
+
+template <bool LAZY, bool INV>
+static void cols2_size(Context& c, const ColJob2& j, const PassPlan& P, uint32_t njobs, const ColJob2* dj) {
+  switch (P.S) {
+#define FPXG_C(S) \
+  case S: cols2_launch<S, LAZY, INV>(c, j, P, njobs, dj); break;
+    FPXG_C(1) FPXG_C(2) FPXG_C(3) FPXG_C(4) FPXG_C(5) FPXG_C(6)
+    FPXG_C(7) FPXG_C(8) FPXG_C(9) FPXG_C(10) FPXG_C(11) FPXG_C(12)
+#undef FPXG_C
+    default: throw Error(E_SIZE, "column pass size out of range");
+  }
+}
+
+static void launch_cols(Context& c, bool lazy, bool inv, const ColJob2& j, const PassPlan& P, uint32_t njobs = 1,
+                        const ColJob2* dj = nullptr) {
+  if (lazy) {
+    if (inv) cols2_size<true, true>(c, j, P, njobs, dj);
+    else cols2_size<true, false>(c, j, P, njobs, dj);
+  } else {
+    if (inv) cols2_size<false, true>(c, j, P, njobs, dj);
+    else cols2_size<false, false>(c, j, P, njobs, dj);
+  }
+  FPXG_HIP(hipGetLastError());
+}
+
+template <int LOGM, bool LAZY, int MODE>
+static bool rows2_attr() {
+  using RS = RowShape<LOGM>;
+  return hipFuncSetAttribute((const void*)k_rows2<LOGM, LAZY, MODE>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                             (int)(RS::U * RS::lds_words_per_row * 4)) == hipSuccess;
+}
+template <int S, bool LAZY, bool INV>
+static bool cols2_attr() {
+  constexpr int LOGC = S <= 10 ? 4 : 14 - S;
+  return hipFuncSetAttribute((const void*)k_cols2<S, LOGC, LAZY, INV>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                             (int)(ColShape<S, LOGC>::words * 4)) == hipSuccess;
+}
 
 static void context_init(Context& c) {
   FPXG_HIP(hipSetDevice(c.device));
@@ -288,24 +386,20 @@ static void context_init(Context& c) {
   hipDeviceProp_t prop;
   FPXG_HIP(hipGetDeviceProperties(&prop, c.device));
   c.cus = prop.multiProcessorCount;
-  // Use up to 2^14-word tiles when the device grants more than 64 KB of LDS
-  // to one workgroup (gfx950: 160 KB per CU).
-  const size_t want = (size_t)lds_words(1u << 14) * 4;
-  size_t limit = prop.sharedMemPerBlock;
-  if (prop.maxSharedMemoryPerMultiProcessor > limit) limit = prop.maxSharedMemoryPerMultiProcessor;
-  bool big = limit >= want;
-  if (big) {
-#define X(m) big = set_rows_attr<true, m>(want) && set_rows_attr<false, m>(want) && big;
-    FPXG_ROW_MODES(X)
+  // Row passes of 2^14 words and column tiles of 2^14 words need 67.6 KB of
+  // LDS per workgroup (gfx950: 160 KB per CU).  Without it, stop at 2^13.
+  bool big = true;
+#define X(L, M) big = rows2_attr<L, true, M>() && rows2_attr<L, false, M>() && big;
+  X(13, R2M_FWD) X(13, R2M_INV) X(13, R2M_CONV) X(13, R2M_SPEC) X(13, R2M_SPEC_ONLY)
+  X(14, R2M_FWD) X(14, R2M_INV) X(14, R2M_CONV) X(14, R2M_SPEC) X(14, R2M_SPEC_ONLY)
 #undef X
-    for (const void* k : {(const void*)k_cols<true, false>, (const void*)k_cols<true, true>,
-                          (const void*)k_cols<false, false>, (const void*)k_cols<false, true>})
-      if (hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)want) != hipSuccess) big = false;
-    (void)hipGetLastError();
-  }
-  c.lds_bytes = big ? want : 65536;
+#define X(S) big = cols2_attr<S, true, false>() && cols2_attr<S, true, true>() && cols2_attr<S, false, false>() && \
+                   cols2_attr<S, false, true>() && big;
+  X(9) X(10) X(11) X(12)
+#undef X
+  (void)hipGetLastError();
+  c.lds_bytes = big ? (size_t)lds_words(1u << 14) * 4 : 65536;
   c.row_log_max = big ? 14 : 13;
-  c.tile_log_max = big ? 14 : 13;
   c.arena = std::make_unique<Arena>(g_arena_bytes);
 }
 
@@ -331,49 +425,17 @@ HostField& Context::field(uint32_t p, uint32_t root) {
   return ref;
 }
 
+// Pass split of a 2^L transform: one row pass up to 2^single_log_max; above,
+// S = L - b column levels over rows of 2^b words with b <= row_log_max and
+// S <= 12, balanced so both passes keep enough workgroups.
 PassPlan Context::plan(int L) const {
-  if (L <= row_log_max) return {L, L, 0, 0};
-  const int b = row_log_max;
-  const int S = L - b;
-  int logC = std::min(4, tile_log_max - S);
-  if (logC < 0) throw Error(E_SIZE, "transform too large for two passes");
-  logC = std::min(logC, b);
-  return {L, b, S, logC};
-}
-
-template <bool LAZY>
-static void launch_rows_t(Context& c, int mode, const RowJob& j, int logM, uint32_t nblocks, uint32_t njobs,
-                          const RowJob* dev_jobs) {
-  const size_t lds = (size_t)lds_words(1u << logM) * 4;
-  dim3 grid(nblocks, njobs), block(kThreads);
-  switch (mode) {
-#define X(m) \
-  case (m): hipLaunchKernelGGL((k_rows<LAZY, (m)>), grid, block, lds, c.stream, j, dev_jobs, logM); break;
-    FPXG_ROW_MODES(X)
-#undef X
-    default: throw Error(E_DEVICE, "unsupported row-pass mode");
-  }
-  FPXG_HIP(hipGetLastError());
-}
-
-static void launch_rows(Context& c, bool lazy, int mode, const RowJob& j, int logM, uint32_t nblocks,
-                        uint32_t njobs = 1, const RowJob* dev_jobs = nullptr) {
-  if (lazy) launch_rows_t<true>(c, mode, j, logM, nblocks, njobs, dev_jobs);
-  else launch_rows_t<false>(c, mode, j, logM, nblocks, njobs, dev_jobs);
-}
-
-static void launch_cols(Context& c, bool lazy, bool inv, const ColJob& j, const PassPlan& P, uint32_t njobs = 1,
-                        const ColJob* dev_jobs = nullptr) {
-  const size_t lds = (size_t)lds_words(1u << (P.S + P.logC)) * 4;
-  dim3 grid(1u << (P.b - P.logC), njobs), block(kThreads);
-  if (lazy) {
-    if (inv) hipLaunchKernelGGL((k_cols<true, true>), grid, block, lds, c.stream, j, dev_jobs, P.L, P.S, P.logC);
-    else hipLaunchKernelGGL((k_cols<true, false>), grid, block, lds, c.stream, j, dev_jobs, P.L, P.S, P.logC);
-  } else {
-    if (inv) hipLaunchKernelGGL((k_cols<false, true>), grid, block, lds, c.stream, j, dev_jobs, P.L, P.S, P.logC);
-    else hipLaunchKernelGGL((k_cols<false, false>), grid, block, lds, c.stream, j, dev_jobs, P.L, P.S, P.logC);
-  }
-  FPXG_HIP(hipGetLastError());
+  auto it = plan_override.find(L);
+  if (it != plan_override.end()) return it->second;
+  if (L <= single_log_max) return {L, L, 0, 0};
+  int S = std::max(L - row_log_max, std::min(L / 2, 10));
+  if (S > 12) throw Error(E_SIZE, "transform too large for two passes");
+  const int b = L - S;
+  return {L, b, S, tile_log_cols(S)};
 }
 
 __global__ void k_fill(uint32_t* p, size_t n, uint32_t v) {
@@ -406,19 +468,21 @@ void ntt_forward(Context& c, const HostField& F, uint32_t* dst, const uint32_t* 
                  bool lift, bool canon) {
   const PassPlan P = c.plan(L);
   const uint32_t N = 1u << L;
+  RowJob2 rj{};
+  rj.f = F.dev;
+  rj.out_lo = 0; rj.out_hi = N;
+  rj.flags = canon ? R2_CANON : 0;
   if (P.S == 0) {
-    RowJob j{};
-    j.src_a = src; j.dst = dst; j.in_a = in_count; j.out_lo = 0; j.out_hi = N;
-    j.lift = lift; j.canon = canon; j.f = F.dev;
-    launch_rows(c, F.lazy, MODE_FWD_A, j, L, 1);
+    rj.src_a = src; rj.dst = dst; rj.nrows = 1; rj.in_a = in_count;
+    if (lift) rj.flags |= R2_LIFT;
+    launch_rows(c, F.lazy, R2M_FWD, rj, L, 1);
     return;
   }
-  ColJob cj{};
-  cj.src = src; cj.dst = dst; cj.in_count = in_count; cj.lift = lift; cj.f = F.dev;
+  ColJob2 cj{};
+  cj.src = src; cj.dst = dst; cj.in_count = in_count; cj.flags = lift ? R2_LIFT : 0; cj.f = F.dev;
   launch_cols(c, F.lazy, false, cj, P);
-  RowJob rj{};
-  rj.src_a = dst; rj.dst = dst; rj.in_a = N; rj.out_lo = 0; rj.out_hi = N; rj.canon = canon; rj.f = F.dev;
-  launch_rows(c, F.lazy, MODE_FWD_A, rj, P.b, 1u << P.S);
+  rj.src_a = dst; rj.dst = dst; rj.nrows = 1u << P.S; rj.in_a = N;
+  launch_rows(c, F.lazy, R2M_FWD, rj, P.b, rj.nrows);
 }
 
 void ntt_inverse(Context& c, const HostField& F, uint32_t* dst, const uint32_t* src, int L, uint32_t out_lo,
@@ -427,20 +491,20 @@ void ntt_inverse(Context& c, const HostField& F, uint32_t* dst, const uint32_t* 
   const PassPlan P = c.plan(L);
   const uint32_t N = 1u << L;
   const uint32_t inv_n = inv_pow2(F, L);
+  RowJob2 rj{};
+  rj.f = F.dev;
   if (P.S == 0) {
-    RowJob j{};
-    j.src_a = src; j.dst = dst; j.in_a = N; j.out_lo = out_lo; j.out_hi = out_hi; j.out_shift = out_shift;
-    j.inv_n = inv_n; j.canon = 1; j.f = F.dev;
-    launch_rows(c, F.lazy, MODE_INV | MODE_SCALE, j, L, 1);
+    rj.src_a = src; rj.dst = dst; rj.nrows = 1; rj.out_lo = out_lo; rj.out_hi = out_hi; rj.out_shift = out_shift;
+    rj.inv_n = inv_n; rj.flags = R2_SCALE | R2_CANON;
+    launch_rows(c, F.lazy, R2M_INV, rj, L, 1);
     return;
   }
   uint32_t* work = const_cast<uint32_t*>(src);
-  RowJob rj{};
-  rj.src_a = work; rj.dst = work; rj.in_a = N; rj.out_lo = 0; rj.out_hi = N; rj.f = F.dev;
-  launch_rows(c, F.lazy, MODE_INV, rj, P.b, 1u << P.S);
-  ColJob cj{};
+  rj.src_a = work; rj.dst = work; rj.nrows = 1u << P.S; rj.out_lo = 0; rj.out_hi = N;
+  launch_rows(c, F.lazy, R2M_INV, rj, P.b, rj.nrows);
+  ColJob2 cj{};
   cj.src = work; cj.dst = dst; cj.out_lo = out_lo; cj.out_hi = out_hi; cj.out_shift = out_shift;
-  cj.inv_n = inv_n; cj.canon = 1; cj.f = F.dev;
+  cj.inv_n = inv_n; cj.f = F.dev;
   launch_cols(c, F.lazy, true, cj, P);
 }
 
@@ -452,26 +516,26 @@ static void conv_core(Context& c, const HostField& F, const uint32_t* a, size_t 
   const uint32_t N = 1u << L;
   const PassPlan P = c.plan(L);
   const uint32_t inv_n = inv_pow2(F, L);
+  RowJob2 rj{};
+  rj.f = F.dev;
   if (P.S == 0) {
-    RowJob j{};
-    j.src_a = a; j.src_b = b; j.dst = out; j.in_a = (uint32_t)la; j.in_b = (uint32_t)lb;
-    j.out_lo = 0; j.out_hi = (uint32_t)outc; j.inv_n = inv_n; j.lift = lift; j.canon = 1; j.f = F.dev;
-    launch_rows(c, F.lazy, MODE_FWD_A | MODE_B_RAW | MODE_INV | MODE_SCALE, j, L, 1);
+    rj.src_a = a; rj.src_b = b; rj.dst = out; rj.nrows = 1; rj.in_a = (uint32_t)la; rj.in_b = (uint32_t)lb;
+    rj.out_lo = 0; rj.out_hi = (uint32_t)outc; rj.inv_n = inv_n;
+    rj.flags = R2_SCALE | R2_CANON | (lift ? R2_LIFT : 0);
+    launch_rows(c, F.lazy, R2M_CONV, rj, L, 1);
     return;
   }
   DevBuf A = c.buf(N), B = c.buf(N);
-  ColJob cj{};
-  cj.src = a; cj.dst = A.get(); cj.in_count = (uint32_t)la; cj.lift = lift; cj.f = F.dev;
+  ColJob2 cj{};
+  cj.src = a; cj.dst = A.get(); cj.in_count = (uint32_t)la; cj.flags = lift ? R2_LIFT : 0; cj.f = F.dev;
   launch_cols(c, F.lazy, false, cj, P);
   cj.src = b; cj.dst = B.get(); cj.in_count = (uint32_t)lb;
   launch_cols(c, F.lazy, false, cj, P);
-  RowJob rj{};
-  rj.src_a = A.get(); rj.src_b = B.get(); rj.dst = A.get(); rj.in_a = N; rj.in_b = N;
-  rj.out_lo = 0; rj.out_hi = N; rj.f = F.dev;
-  launch_rows(c, F.lazy, MODE_FWD_A | MODE_B_RAW | MODE_INV, rj, P.b, 1u << P.S);
-  ColJob ij{};
-  ij.src = A.get(); ij.dst = out; ij.out_lo = 0; ij.out_hi = (uint32_t)outc; ij.inv_n = inv_n; ij.canon = 1;
-  ij.f = F.dev;
+  rj.src_a = A.get(); rj.src_b = B.get(); rj.dst = A.get(); rj.nrows = 1u << P.S; rj.in_a = N; rj.in_b = N;
+  rj.out_lo = 0; rj.out_hi = N;
+  launch_rows(c, F.lazy, R2M_CONV, rj, P.b, rj.nrows);
+  ColJob2 ij{};
+  ij.src = A.get(); ij.dst = out; ij.out_lo = 0; ij.out_hi = (uint32_t)outc; ij.inv_n = inv_n; ij.f = F.dev;
   launch_cols(c, F.lazy, true, ij, P);
 }
 
@@ -755,13 +819,14 @@ int fpxg_ntt_pass_dev(uint32_t p, uint32_t root, uint32_t n, int pass, uint32_t*
     const PassPlan P = c.plan(L);
     if (pass == 0) {
       if (P.S == 0) throw Error(E_ARG, "single-pass size has no column pass");
-      ColJob cj{};
+      ColJob2 cj{};
       cj.src = data; cj.dst = data; cj.in_count = n; cj.f = F.dev;
       launch_cols(c, F.lazy, false, cj, P);
     } else if (pass == 1) {
-      RowJob rj{};
-      rj.src_a = data; rj.dst = data; rj.in_a = n; rj.out_lo = 0; rj.out_hi = n; rj.canon = 1; rj.f = F.dev;
-      launch_rows(c, F.lazy, MODE_FWD_A, rj, P.b, 1u << P.S);
+      RowJob2 rj{};
+      rj.src_a = data; rj.dst = data; rj.nrows = 1u << P.S; rj.in_a = n; rj.out_lo = 0; rj.out_hi = n;
+      rj.flags = R2_CANON; rj.f = F.dev;
+      launch_rows(c, F.lazy, R2M_FWD, rj, P.b, rj.nrows);
     } else {
       throw Error(E_ARG, "pass must be 0 or 1");
     }

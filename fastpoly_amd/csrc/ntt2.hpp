@@ -1,0 +1,425 @@
+// Compile-time-shaped NTT passes for gfx950.
+//
+// A block of 2^LOGB points is transformed in radix-16 register phases: each
+// thread owns one 16-point group per phase (2^(4-r) groups of a shorter
+// radix-2^r phase).  The first phase is loaded straight from HBM into
+// registers, the last phase is stored straight from registers, and only the
+// phase transitions go through LDS.  Every phase reads its twiddles from a
+// precomputed "plane" laid out in the order the thread consumes them, with
+// 16-byte vector loads.  Because the last forward phase and the first inverse
+// phase own the same elements, a pointwise product between them never leaves
+// the registers.
+//
+// Level q of a phase starting at level s of a 2^LOGB block pairs elements at
+// distance 2^(LOGB-s-q-1) inside sub-blocks of 2^(LOGB-s-q) with twiddle
+// w_L^m, L = 2^(LOGB-s-q), m the upper element's offset: the decimation-in-
+// frequency order of NttPlan<M>::forward (ntt.hpp:283-308), read in
+// bit-reversed order, and its exact DIT inverse.
+#pragma once
+
+#include "device.hpp"
+
+namespace fpxg {
+
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+typedef __attribute__((address_space(1))) const uint32_t gu32_c;
+typedef __attribute__((address_space(1))) const u32x4 gu4_c;
+typedef __attribute__((address_space(1))) uint32_t gu32;
+
+__device__ __forceinline__ uint32_t gload(const uint32_t* p) { return *(gu32_c*)p; }
+__device__ __forceinline__ void gstore(uint32_t* p, uint32_t v) { *(gu32*)p = v; }
+
+// Twiddle planes.  plane(logB, r) starts at word plane_off(logB, r) and holds,
+// for lo < 2^(logB-r), the 2^r - 1 twiddles of group lo (level by level,
+// q = 0..r-1: w_{2^(logB-q)}^((k << (logB-r)) | lo), k < 2^(r-q-1)) in words
+// [lo*2^r, lo*2^r + 2^r - 1).  Planes exist for 1 <= r <= min(4, logB) <= 14.
+__host__ __device__ constexpr uint32_t plane_off(int logB, int r) {
+  return (uint32_t)(r - 1) * 32768u + (1u << logB) - (1u << r);
+}
+constexpr uint32_t kPlaneWords = 4u * 32768u;
+
+template <int r>
+__device__ __forceinline__ void load_tw(uint32_t* tw, const uint32_t* plane, uint32_t lo) {
+  const uint32_t* p = plane + (lo << r);
+  if constexpr (r >= 2) {
+#pragma unroll
+    for (int v = 0; v < (1 << r) / 4; ++v) {
+      const u32x4 q = *(gu4_c*)(p + 4 * v);
+      tw[4 * v] = q.x; tw[4 * v + 1] = q.y; tw[4 * v + 2] = q.z; tw[4 * v + 3] = q.w;
+    }
+  } else {
+    tw[0] = gload(p);
+  }
+}
+
+template <int r, bool LAZY>
+__device__ __forceinline__ void dif_block(uint32_t* x, const uint32_t* tw, const Arith<LAZY>& A) {
+  int idx = 0;
+#pragma unroll
+  for (int q = 0; q < r; ++q) {
+    const int h = 1 << (r - q - 1);
+#pragma unroll
+    for (int k = 0; k < (1 << r); ++k) {
+      if (k & h) continue;
+      const uint32_t a0 = x[k], a1 = x[k + h];
+      x[k] = A.add(a0, a1);
+      x[k + h] = A.mul(a0 - a1 + A.R, tw[idx + (k & (h - 1))]);
+    }
+    idx += h;
+  }
+}
+
+template <int r, bool LAZY>
+__device__ __forceinline__ void dit_block(uint32_t* x, const uint32_t* tw, const Arith<LAZY>& A) {
+#pragma unroll
+  for (int q = r - 1; q >= 0; --q) {
+    const int h = 1 << (r - q - 1);
+    const int idx = (1 << r) - (1 << (r - q));
+#pragma unroll
+    for (int k = 0; k < (1 << r); ++k) {
+      if (k & h) continue;
+      const uint32_t t = A.mul(x[k + h], tw[idx + (k & (h - 1))]);
+      const uint32_t a0 = x[k];
+      x[k] = A.add(a0, t);
+      x[k + h] = A.sub(a0, t);
+    }
+  }
+}
+
+// Phase schedule of a 2^LOGB block: radix-16 phases with the one shorter
+// phase (LOGB mod 4 levels) second, so that the first and last phases stay
+// radix 16 whenever LOGB >= 8.
+template <int LOGB>
+struct Sched {
+  static constexpr int a = LOGB / 4, b = LOGB % 4;
+  static constexpr int nph = a + (b ? 1 : 0);
+  static constexpr int short_at = a >= 1 ? 1 : 0;
+  static constexpr int r_of(int p) { return (b && p == short_at) ? b : 4; }
+  static constexpr int s_of(int p) {
+    int s = 0;
+    for (int i = 0; i < p; ++i) s += r_of(i);
+    return s;
+  }
+  // threads per unit (one 16-point group each in a radix-16 phase)
+  static constexpr int GT = LOGB >= 4 ? (1 << (LOGB - 4)) : 1;
+};
+
+// Element index within a unit for group id gid, register k of phase p.
+template <int LOGB, int P>
+__device__ __forceinline__ uint32_t elem(uint32_t gid, int k) {
+  constexpr int s = Sched<LOGB>::s_of(P), r = Sched<LOGB>::r_of(P);
+  constexpr int lo_bits = LOGB - s - r;
+  const uint32_t lo = gid & ((1u << lo_bits) - 1), hi = gid >> lo_bits;
+  return (hi << (LOGB - s)) | ((uint32_t)k << lo_bits) | lo;
+}
+template <int LOGB, int P>
+__device__ __forceinline__ uint32_t group_lo(uint32_t gid) {
+  constexpr int s = Sched<LOGB>::s_of(P), r = Sched<LOGB>::r_of(P);
+  constexpr int lo_bits = LOGB - s - r;
+  return gid & ((1u << lo_bits) - 1);
+}
+// Number of groups a thread owns in phase P, and its g-th group id.
+template <int LOGB, int P>
+struct PhaseGroups {
+  static constexpr int r = Sched<LOGB>::r_of(P);
+  static constexpr int NG = LOGB >= 4 ? (1 << (4 - r)) : 1;
+  __device__ static __forceinline__ uint32_t gid(uint32_t g, int u) { return (uint32_t)u * Sched<LOGB>::GT + g; }
+};
+
+// Compute phase P on registers (forward or inverse).
+template <int LOGB, int P, bool INV, bool LAZY>
+__device__ __forceinline__ void phase_compute(uint32_t* x, uint32_t g, const uint32_t* planes, const Arith<LAZY>& A) {
+  constexpr int r = Sched<LOGB>::r_of(P);
+  constexpr int NG = PhaseGroups<LOGB, P>::NG;
+  const uint32_t* plane = planes + plane_off(LOGB - Sched<LOGB>::s_of(P), r);
+#pragma unroll
+  for (int u = 0; u < NG; ++u) {
+    uint32_t tw[16];
+    load_tw<r>(tw, plane, group_lo<LOGB, P>(PhaseGroups<LOGB, P>::gid(g, u)));
+    if constexpr (INV) dit_block<r, LAZY>(x + u * (1 << r), tw, A);
+    else dif_block<r, LAZY>(x + u * (1 << r), tw, A);
+  }
+}
+
+// Register <-> LDS for phase P.  `lidx(e)` maps an element of the unit to its
+// (unpadded) LDS word.
+template <int LOGB, int P, class Idx>
+__device__ __forceinline__ void phase_to_lds(uint32_t* X, const uint32_t* x, uint32_t g, Idx lidx) {
+  constexpr int r = Sched<LOGB>::r_of(P);
+  constexpr int NG = PhaseGroups<LOGB, P>::NG;
+#pragma unroll
+  for (int u = 0; u < NG; ++u)
+#pragma unroll
+    for (int k = 0; k < (1 << r); ++k) X[pidx(lidx(elem<LOGB, P>(PhaseGroups<LOGB, P>::gid(g, u), k)))] = x[u * (1 << r) + k];
+}
+template <int LOGB, int P, class Idx>
+__device__ __forceinline__ void phase_from_lds(uint32_t* x, const uint32_t* X, uint32_t g, Idx lidx) {
+  constexpr int r = Sched<LOGB>::r_of(P);
+  constexpr int NG = PhaseGroups<LOGB, P>::NG;
+#pragma unroll
+  for (int u = 0; u < NG; ++u)
+#pragma unroll
+    for (int k = 0; k < (1 << r); ++k) x[u * (1 << r) + k] = X[pidx(lidx(elem<LOGB, P>(PhaseGroups<LOGB, P>::gid(g, u), k)))];
+}
+
+// Phases [P, END) of a forward transform whose registers already hold phase
+// P-1's output (P >= 1): each step writes LDS, syncs, reads phase P's groups
+// and computes.  All threads of the workgroup must call it.
+template <int LOGB, int P, int END, bool LAZY, class Idx>
+__device__ __forceinline__ void fwd_phases(uint32_t* x, uint32_t* X, uint32_t g, const uint32_t* planes,
+                                           const Arith<LAZY>& A, Idx lidx) {
+  if constexpr (P < END) {
+    phase_to_lds<LOGB, P - 1>(X, x, g, lidx);
+    __syncthreads();
+    phase_from_lds<LOGB, P>(x, X, g, lidx);
+    __syncthreads();
+    phase_compute<LOGB, P, false, LAZY>(x, g, planes, A);
+    fwd_phases<LOGB, P + 1, END, LAZY>(x, X, g, planes, A, lidx);
+  }
+}
+// Inverse phases P, P-1, ..., 0 where registers hold phase P+1's inverse output.
+template <int LOGB, int P, bool LAZY, class Idx>
+__device__ __forceinline__ void inv_phases(uint32_t* x, uint32_t* X, uint32_t g, const uint32_t* planes,
+                                           const Arith<LAZY>& A, Idx lidx) {
+  if constexpr (P >= 0) {
+    phase_to_lds<LOGB, P + 1>(X, x, g, lidx);
+    __syncthreads();
+    phase_from_lds<LOGB, P>(x, X, g, lidx);
+    __syncthreads();
+    phase_compute<LOGB, P, true, LAZY>(x, g, planes, A);
+    inv_phases<LOGB, P - 1, LAZY>(x, X, g, planes, A, lidx);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Row pass: U rows of 2^LOGM contiguous words per workgroup.
+// ---------------------------------------------------------------------------
+template <int LOGM>
+struct RowShape {
+  static constexpr int GT = Sched<LOGM>::GT;
+  static constexpr int U = GT >= 64 ? 1 : 64 / GT;
+  static constexpr int T = GT * U;
+  static constexpr uint32_t M = 1u << LOGM;
+  static constexpr uint32_t lds_words_per_row = lds_words(M);
+};
+
+struct RowJob2 {
+  const uint32_t* src_a;
+  const uint32_t* src_b;
+  uint32_t* dst;
+  uint32_t nrows;            // rows of 2^LOGM in this job
+  uint32_t in_a, in_b;       // zero-fill limits (array indices)
+  uint32_t out_lo, out_hi, out_shift;
+  uint32_t inv_n;            // flags & R2_SCALE: multiply by inv_n at the store
+  uint32_t flags;
+  FieldDev f;
+};
+enum : uint32_t { R2_LIFT = 1, R2_CANON = 2, R2_SCALE = 4, R2_NEG = 8 };
+enum : int { R2M_FWD = 1, R2M_INV = 2, R2M_CONV = 3, R2M_SPEC = 4, R2M_SPEC_ONLY = 5 };
+// R2M_FWD:  forward a                      R2M_INV:       inverse a
+// R2M_CONV: forward a, forward b, a*b, inverse
+// R2M_SPEC: forward a, a*spectrum(b), inverse
+// R2M_SPEC_ONLY: a (already a spectrum) * spectrum(b), inverse
+
+template <int LOGM, bool LAZY, int MODE>
+__global__ void __launch_bounds__(RowShape<LOGM>::T) k_rows2(RowJob2 j0, const RowJob2* __restrict__ jobs) {
+  using RS = RowShape<LOGM>;
+  using SC = Sched<LOGM>;
+  constexpr int NPH = SC::nph;
+  extern __shared__ uint32_t lds[];
+  const RowJob2 J = jobs ? jobs[blockIdx.y] : j0;
+  const Arith<LAZY> A(J.f);
+  const bool lift = (J.flags & R2_LIFT) != 0;
+  const uint32_t t = threadIdx.x;
+  const uint32_t g = t % RS::GT, ur = t / RS::GT;
+  const uint32_t row = blockIdx.x * RS::U + ur;
+  if (RS::U == 1 && row >= J.nrows) return;  // whole workgroup idle (uniform): no barrier is skipped
+  const bool active = RS::U == 1 || row < J.nrows;
+  const uint32_t base = row << LOGM;
+  uint32_t* X = lds + ur * RS::lds_words_per_row;
+  auto lidx = [](uint32_t e) { return e; };
+  uint32_t x[16];
+
+  // Branch-free first-phase load: indices past in_count (or of an inactive
+  // row) read a clamped valid word and are then replaced by zero.
+  auto load_first = [&](const uint32_t* src, uint32_t in_count, uint32_t* dst_regs) {
+    constexpr int r0 = SC::r_of(0);
+    const uint32_t last = in_count ? in_count - 1 : 0;
+#pragma unroll
+    for (int u = 0; u < PhaseGroups<LOGM, 0>::NG; ++u)
+#pragma unroll
+      for (int k = 0; k < (1 << r0); ++k) {
+        const uint32_t gi = base + elem<LOGM, 0>(PhaseGroups<LOGM, 0>::gid(g, u), k);
+        const bool live = active && gi < in_count;
+        uint32_t v = in_count ? gload(src + min(gi, last)) : 0u;
+        if (lift) v = A.mul(v, J.f.r2);
+        dst_regs[u * (1 << r0) + k] = live ? v : 0u;
+      }
+  };
+  // Forward transform of `src` ending with phase NPH-1's groups in regs.
+  auto forward = [&](const uint32_t* src, uint32_t in_count, uint32_t* regs) {
+    load_first(src, in_count, regs);
+    phase_compute<LOGM, 0, false, LAZY>(regs, g, J.f.planes_fwd, A);
+    fwd_phases<LOGM, 1, NPH, LAZY>(regs, X, g, J.f.planes_fwd, A, lidx);
+  };
+
+  constexpr int PL = NPH - 1;  // last forward phase = first inverse phase
+  constexpr int rl = SC::r_of(PL);
+  constexpr int NGL = PhaseGroups<LOGM, PL>::NG;
+
+  if constexpr (MODE == R2M_FWD) {
+    forward(J.src_a, J.in_a, x);
+#pragma unroll
+    for (int u = 0; u < NGL; ++u)
+#pragma unroll
+      for (int k = 0; k < (1 << rl); ++k) {
+        const uint32_t gi = base + elem<LOGM, PL>(PhaseGroups<LOGM, PL>::gid(g, u), k);
+        uint32_t v = x[u * (1 << rl) + k];
+        if (J.flags & R2_CANON) v = A.canon(v);
+        if (active) gstore(J.dst + gi, v);
+      }
+    return;
+  } else {
+    if constexpr (MODE == R2M_CONV) {
+      uint32_t y[16];
+      forward(J.src_b, J.in_b, y);
+      __syncthreads();
+      forward(J.src_a, J.in_a, x);
+#pragma unroll
+      for (int i = 0; i < NGL * (1 << rl); ++i) x[i] = A.mul(x[i], y[i]);
+    } else if constexpr (MODE == R2M_SPEC) {
+      forward(J.src_a, J.in_a, x);
+#pragma unroll
+      for (int u = 0; u < NGL; ++u)
+#pragma unroll
+        for (int k = 0; k < (1 << rl); ++k) {
+          const uint32_t gi = base + elem<LOGM, PL>(PhaseGroups<LOGM, PL>::gid(g, u), k);
+          const uint32_t w = gload(J.src_b + (active ? gi : 0u));
+          x[u * (1 << rl) + k] = A.mul(x[u * (1 << rl) + k], w);
+        }
+    } else {  // R2M_INV / R2M_SPEC_ONLY: load the spectrum in phase PL's grouping
+#pragma unroll
+      for (int u = 0; u < NGL; ++u)
+#pragma unroll
+        for (int k = 0; k < (1 << rl); ++k) {
+          const uint32_t gi = active ? base + elem<LOGM, PL>(PhaseGroups<LOGM, PL>::gid(g, u), k) : 0u;
+          uint32_t v = gload(J.src_a + gi);
+          if constexpr (MODE == R2M_SPEC_ONLY) v = A.mul(v, gload(J.src_b + gi));
+          x[u * (1 << rl) + k] = v;
+        }
+    }
+    // Inverse: phase PL on registers, then PL-1 .. 0 through LDS.
+    phase_compute<LOGM, PL, true, LAZY>(x, g, J.f.planes_inv, A);
+    inv_phases<LOGM, PL - 1, LAZY>(x, X, g, J.f.planes_inv, A, lidx);
+    constexpr int r0 = SC::r_of(0);
+#pragma unroll
+    for (int u = 0; u < PhaseGroups<LOGM, 0>::NG; ++u)
+#pragma unroll
+      for (int k = 0; k < (1 << r0); ++k) {
+        const uint32_t gi = base + elem<LOGM, 0>(PhaseGroups<LOGM, 0>::gid(g, u), k);
+        uint32_t v = x[u * (1 << r0) + k];
+        if (J.flags & R2_SCALE) v = A.mul(v, J.inv_n);
+        if (J.flags & R2_NEG) v = v == 0 ? 0u : A.R - v;
+        if (J.flags & R2_CANON) v = A.canon(v);
+        if (active && gi >= J.out_lo && gi < J.out_hi) gstore(J.dst + (gi - J.out_shift), v);
+      }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Column pass: a tile of C = 2^LOGC adjacent columns x 2^S rows of the
+// 2^S x M row-major view, with the four-step twiddle w_N^(c * bitrev_S(row))
+// applied after the forward levels (before the inverse ones).
+// ---------------------------------------------------------------------------
+struct ColJob2 {
+  const uint32_t* src;
+  uint32_t* dst;
+  uint32_t in_count;
+  uint32_t out_lo, out_hi, out_shift;
+  uint32_t inv_n;
+  uint32_t flags;  // R2_LIFT, R2_CANON, R2_NEG
+  FieldDev f;
+};
+
+template <int S, int LOGC>
+struct ColShape {
+  static constexpr int GT = Sched<S>::GT;
+  static constexpr int C = 1 << LOGC;
+  static constexpr int T = GT * C;
+  static constexpr uint32_t words = lds_words((1u << S) << LOGC);
+};
+
+template <int S, int LOGC, bool LAZY, bool INV>
+__global__ void __launch_bounds__((ColShape<S, LOGC>::T)) k_cols2(ColJob2 j0, const ColJob2* __restrict__ jobs, int L) {
+  using SC = Sched<S>;
+  constexpr int NPH = SC::nph;
+  constexpr int C = 1 << LOGC;
+  extern __shared__ uint32_t X[];
+  const ColJob2 J = jobs ? jobs[blockIdx.y] : j0;
+  const Arith<LAZY> A(J.f);
+  const bool lift = (J.flags & R2_LIFT) != 0;
+  const int b = L - S;
+  const uint32_t M = 1u << b;
+  const uint32_t tiles = M >> LOGC;
+  uint32_t tile = blockIdx.x;
+  if (tiles >= 64 && (tiles & 7) == 0) tile = (tile & 7) * (tiles >> 3) + (tile >> 3);
+  const uint32_t c = threadIdx.x % C, g = threadIdx.x / C;
+  const uint32_t col = (tile << LOGC) + c;
+  const uint32_t shift = J.f.logmax - L;
+  auto lidx = [c](uint32_t e) { return (e << LOGC) | c; };
+  uint32_t x[16];
+  constexpr int PL = NPH - 1;
+  if constexpr (!INV) {
+    constexpr int r0 = SC::r_of(0);
+    const uint32_t last = J.in_count ? J.in_count - 1 : 0;
+#pragma unroll
+    for (int u = 0; u < PhaseGroups<S, 0>::NG; ++u)
+#pragma unroll
+      for (int k = 0; k < (1 << r0); ++k) {
+        const uint32_t e = elem<S, 0>(PhaseGroups<S, 0>::gid(g, u), k);
+        const uint32_t gi = e * M + col;
+        uint32_t v = J.in_count ? gload(J.src + min(gi, last)) : 0u;
+        if (lift) v = A.mul(v, J.f.r2);
+        x[u * (1 << r0) + k] = gi < J.in_count ? v : 0u;
+      }
+    phase_compute<S, 0, false, LAZY>(x, g, J.f.planes_fwd, A);
+    fwd_phases<S, 1, NPH, LAZY>(x, X, g, J.f.planes_fwd, A, lidx);
+    constexpr int rl = SC::r_of(PL);
+#pragma unroll
+    for (int u = 0; u < PhaseGroups<S, PL>::NG; ++u)
+#pragma unroll
+      for (int k = 0; k < (1 << rl); ++k) {
+        const uint32_t e = elem<S, PL>(PhaseGroups<S, PL>::gid(g, u), k);
+        const uint32_t xx = (col * bitrev(e, S)) << shift;
+        uint32_t v = A.mul(x[u * (1 << rl) + k], twiddle<LAZY>(A, J.f.tw_lo, J.f.tw_hi, J.f.split, xx));
+        if (J.flags & R2_CANON) v = A.canon(v);
+        gstore(J.dst + e * M + col, v);
+      }
+  } else {
+    constexpr int rl = SC::r_of(PL);
+#pragma unroll
+    for (int u = 0; u < PhaseGroups<S, PL>::NG; ++u)
+#pragma unroll
+      for (int k = 0; k < (1 << rl); ++k) {
+        const uint32_t e = elem<S, PL>(PhaseGroups<S, PL>::gid(g, u), k);
+        const uint32_t xx = (col * bitrev(e, S)) << shift;
+        const uint32_t tw = A.mul(twiddle<LAZY>(A, J.f.tw_lo_inv, J.f.tw_hi_inv, J.f.split, xx), J.inv_n);
+        x[u * (1 << rl) + k] = A.mul(gload(J.src + e * M + col), tw);
+      }
+    phase_compute<S, PL, true, LAZY>(x, g, J.f.planes_inv, A);
+    inv_phases<S, PL - 1, LAZY>(x, X, g, J.f.planes_inv, A, lidx);
+    constexpr int r0 = SC::r_of(0);
+#pragma unroll
+    for (int u = 0; u < PhaseGroups<S, 0>::NG; ++u)
+#pragma unroll
+      for (int k = 0; k < (1 << r0); ++k) {
+        const uint32_t e = elem<S, 0>(PhaseGroups<S, 0>::gid(g, u), k);
+        const uint32_t gi = e * M + col;
+        uint32_t v = A.canon(x[u * (1 << r0) + k]);
+        if (J.flags & R2_NEG) v = v == 0 ? 0u : J.f.p - v;
+        if (gi >= J.out_lo && gi < J.out_hi) gstore(J.dst + (gi - J.out_shift), v);
+      }
+  }
+}
+
+}  // namespace fpxg

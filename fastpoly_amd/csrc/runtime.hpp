@@ -15,7 +15,8 @@
 #include <vector>
 
 #include "fpxg.h"
-#include "kernels.hpp"
+#include "crt.hpp"
+#include "ntt2.hpp"
 
 namespace fpxg {
 
@@ -44,6 +45,7 @@ struct HostField {
   bool lazy = false;
   FieldDev dev{};   // device tables (when maxlog >= 1)
   uint32_t* tables = nullptr;
+  int table_log = 0;  // transforms up to 2^table_log points have tables
 
   uint32_t reduce(uint64_t t) const {
     if (p == 2) return (uint32_t)(t & 1u);
@@ -131,7 +133,7 @@ class DevBuf {
 // Transform plan: how an N = 2^L transform splits into passes.
 // ---------------------------------------------------------------------------
 struct PassPlan {
-  int L, b, S, logC;  // rows of 2^b words; S column levels in tiles of 2^logC columns
+  int L, b, S, logC;  // rows of 2^b words; S column levels in tiles of 2^logC columns (S == 0: one row pass)
 };
 
 struct Context {
@@ -139,9 +141,10 @@ struct Context {
   hipStream_t stream = nullptr;
   hipEvent_t events[16] = {};
   size_t lds_bytes = 65536;  // usable dynamic LDS per workgroup
-  int row_log_max = 13;      // largest single row pass
-  int tile_log_max = 13;     // words per column tile
+  int row_log_max = 14;      // largest row pass
+  int single_log_max = 12;   // largest transform done as one row pass
   int cus = 256;
+  std::map<int, PassPlan> plan_override;  // autotuned splits by log2 n
   std::unique_ptr<Arena> arena;
   std::map<uint64_t, std::unique_ptr<HostField>> fields;
   std::recursive_mutex mu;

@@ -148,12 +148,29 @@ __global__ void k_copy(uint32_t* dst, const uint32_t* src, size_t n) {
   FPXG_GRID_STRIDE(i, n) dst[i] = src[i];
 }
 // First and last nonzero index of a[lo, hi): res[0] = min (init hi), res[1] = max + 1 (init lo).
-__global__ void k_scan_nonzero(const uint32_t* a, size_t lo, size_t hi, unsigned long long* res) {
+// Each block reduces in LDS and issues one atomic pair.
+__global__ void __launch_bounds__(256) k_scan_nonzero(const uint32_t* a, size_t lo, size_t hi, unsigned long long* res) {
+  __shared__ unsigned long long mn[256], mx[256];
+  unsigned long long first = ~0ull, last = 0;
   for (size_t i = lo + blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < hi; i += (size_t)gridDim.x * blockDim.x)
     if (a[i] != 0) {
-      atomicMin(&res[0], (unsigned long long)i);
-      atomicMax(&res[1], (unsigned long long)(i + 1));
+      if (first == ~0ull) first = i;
+      last = i + 1;
     }
+  mn[threadIdx.x] = first;
+  mx[threadIdx.x] = last;
+  __syncthreads();
+  for (int st = 128; st > 0; st >>= 1) {
+    if ((int)threadIdx.x < st) {
+      mn[threadIdx.x] = min(mn[threadIdx.x], mn[threadIdx.x + st]);
+      mx[threadIdx.x] = max(mx[threadIdx.x], mx[threadIdx.x + st]);
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    if (mn[0] != ~0ull) atomicMin(&res[0], mn[0]);
+    if (mx[0] != 0) atomicMax(&res[1], mx[0]);
+  }
 }
 
 // Characteristic two (square_root.hpp:15-25): (sum b_i x^i)^2 = sum b_i x^(2i).
@@ -246,11 +263,13 @@ struct Engine {
     }
   }
   // out[i - lo] = (x * y)[i] for i in [lo, hi) of the 2^L-point cyclic product,
-  // x given as target limbs (count of them), y as a spectrum.
-  void product(const uint32_t* x, size_t count, const Spec& Y, size_t lo, size_t hi, uint32_t* out) {
+  // x given as target limbs (count of them), y as a spectrum; neg negates.
+  void product(const uint32_t* x, size_t count, const Spec& Y, size_t lo, size_t hi, uint32_t* out,
+               bool neg = false) {
     const uint32_t N = 1u << L;
     const PassPlan P = c.plan(L);
     DevBuf res[3];
+    const uint32_t negf = (neg && native) ? R2_NEG : 0u;
     for (int i = 0; i < k; ++i) {
       const HostField& Fi = *F[i];
       uint32_t* dst = out;
@@ -259,24 +278,25 @@ struct Engine {
         dst = res[i].get();
       }
       const uint32_t inv_n = inv_pow2(Fi, L);
+      RowJob2 rj{};
+      rj.f = Fi.dev;
       if (P.S == 0) {
-        RowJob j{};
-        j.src_a = x; j.src_b = Y.b[i].get(); j.dst = dst; j.in_a = (uint32_t)count;
-        j.out_lo = (uint32_t)lo; j.out_hi = (uint32_t)hi; j.out_shift = (uint32_t)lo;
-        j.inv_n = inv_n; j.lift = !native; j.canon = 1; j.f = Fi.dev;
-        launch_rows(c, Fi.lazy, MODE_FWD_A | MODE_B_SPEC | MODE_INV | MODE_SCALE, j, L, 1);
+        rj.src_a = x; rj.src_b = Y.b[i].get(); rj.dst = dst; rj.nrows = 1; rj.in_a = (uint32_t)count;
+        rj.out_lo = (uint32_t)lo; rj.out_hi = (uint32_t)hi; rj.out_shift = (uint32_t)lo; rj.inv_n = inv_n;
+        rj.flags = R2_SCALE | R2_CANON | negf | (native ? 0u : R2_LIFT);
+        launch_rows(c, Fi.lazy, R2M_SPEC, rj, L, 1);
       } else {
         DevBuf W = c.buf(N);
-        ColJob cj{};
-        cj.src = x; cj.dst = W.get(); cj.in_count = (uint32_t)count; cj.lift = !native; cj.f = Fi.dev;
+        ColJob2 cj{};
+        cj.src = x; cj.dst = W.get(); cj.in_count = (uint32_t)count; cj.flags = native ? 0u : R2_LIFT;
+        cj.f = Fi.dev;
         launch_cols(c, Fi.lazy, false, cj, P);
-        RowJob rj{};
-        rj.src_a = W.get(); rj.src_b = Y.b[i].get(); rj.dst = W.get(); rj.in_a = N;
-        rj.out_lo = 0; rj.out_hi = N; rj.f = Fi.dev;
-        launch_rows(c, Fi.lazy, MODE_FWD_A | MODE_B_SPEC | MODE_INV, rj, P.b, 1u << P.S);
-        ColJob ij{};
+        rj.src_a = W.get(); rj.src_b = Y.b[i].get(); rj.dst = W.get(); rj.nrows = 1u << P.S; rj.in_a = N;
+        rj.out_lo = 0; rj.out_hi = N;
+        launch_rows(c, Fi.lazy, R2M_SPEC, rj, P.b, rj.nrows);
+        ColJob2 ij{};
         ij.src = W.get(); ij.dst = dst; ij.out_lo = (uint32_t)lo; ij.out_hi = (uint32_t)hi;
-        ij.out_shift = (uint32_t)lo; ij.inv_n = inv_n; ij.canon = 1; ij.f = Fi.dev;
+        ij.out_shift = (uint32_t)lo; ij.inv_n = inv_n; ij.flags = negf; ij.f = Fi.dev;
         launch_cols(c, Fi.lazy, true, ij, P);
       }
     }
@@ -285,6 +305,7 @@ struct Engine {
       hipLaunchKernelGGL(k_garner, dim3(grid_for(hi - lo)), dim3(256), 0, c.stream, res[0].get(), res[1].get(),
                          res[2].get(), out, (uint32_t)(hi - lo), g);
       FPXG_HIP(hipGetLastError());
+      if (neg) launch_ew(c, k_negate, hi - lo, out, hi - lo, scalar_of(T));
     }
   }
 };
@@ -301,8 +322,7 @@ static void extend_inverse(Context& c, const HostField& T, const uint32_t* a, si
   Spec Sc;
   E.forward(Sc, cc, k);
   DevBuf e = c.buf(hlen);
-  E.product(a, std::min(la, wanted), Sc, k, wanted, e.get());
-  launch_ew(c, k_negate, hlen, e.get(), hlen, scalar_of(T));
+  E.product(a, std::min(la, wanted), Sc, k, wanted, e.get(), true);
   E.product(e.get(), hlen, Sc, 0, hlen, cc + k);
 }
 
