@@ -141,53 +141,94 @@ __device__ __forceinline__ void phase_compute(uint32_t* x, uint32_t g, const uin
   }
 }
 
-// Register <-> LDS for phase P.  `lidx(e)` maps an element of the unit to its
-// (unpadded) LDS word.
-template <int LOGB, int P, class Idx>
-__device__ __forceinline__ void phase_to_lds(uint32_t* X, const uint32_t* x, uint32_t g, Idx lidx) {
-  constexpr int r = Sched<LOGB>::r_of(P);
+// Register <-> LDS for phase P.  The unit's element e lives at padded word
+// pidx(e << LOGC | c) (LOGC = 0 for rows, with a per-row base).  For a group
+// (hi, lo) of phase P the padded address is base(gid) + off(k) with off a
+// compile-time function of k whenever the block spans >= 32 words, so every
+// LDS access uses an immediate offset.
+template <int LOGB, int P, int LOGC>
+struct LdsMap {
+  static constexpr int s = Sched<LOGB>::s_of(P), r = Sched<LOGB>::r_of(P);
+  static constexpr int lo_bits = LOGB - s - r;
+  static constexpr int span = LOGB - s + LOGC;  // log2 words of one hi-block
+  static constexpr bool affine = span >= 5;
+  // unpadded word of (hi, k, lo, c)
+  __device__ static __forceinline__ uint32_t word(uint32_t gid, int k, uint32_t c) {
+    return (elem<LOGB, P>(gid, k) << LOGC) | c;
+  }
+  __device__ static __forceinline__ uint32_t base(uint32_t gid, uint32_t c) { return pidx(word(gid, 0, c)); }
+  // pidx(word(gid, k, c)) - base(gid, c), independent of gid and c when affine
+  static constexpr uint32_t off(int k) {
+    const uint32_t kw = (uint32_t)k << (lo_bits + LOGC);
+    if (lo_bits + LOGC >= 5) return kw + (kw >> 5);
+    // k*2^(lo_bits+LOGC) + (lo<<LOGC|c) stays below the next multiple of 32
+    return kw + (kw >> 5);
+  }
+};
+
+template <int LOGB, int P, int LOGC>
+__device__ __forceinline__ void phase_to_lds(uint32_t* X, const uint32_t* x, uint32_t g, uint32_t c) {
+  using LM = LdsMap<LOGB, P, LOGC>;
+  constexpr int r = LM::r;
   constexpr int NG = PhaseGroups<LOGB, P>::NG;
 #pragma unroll
-  for (int u = 0; u < NG; ++u)
+  for (int u = 0; u < NG; ++u) {
+    const uint32_t gid = PhaseGroups<LOGB, P>::gid(g, u);
+    if constexpr (LM::affine) {
+      uint32_t* b = X + LM::base(gid, c);
 #pragma unroll
-    for (int k = 0; k < (1 << r); ++k) X[pidx(lidx(elem<LOGB, P>(PhaseGroups<LOGB, P>::gid(g, u), k)))] = x[u * (1 << r) + k];
+      for (int k = 0; k < (1 << r); ++k) b[LM::off(k)] = x[u * (1 << r) + k];
+    } else {
+#pragma unroll
+      for (int k = 0; k < (1 << r); ++k) X[pidx(LM::word(gid, k, c))] = x[u * (1 << r) + k];
+    }
+  }
 }
-template <int LOGB, int P, class Idx>
-__device__ __forceinline__ void phase_from_lds(uint32_t* x, const uint32_t* X, uint32_t g, Idx lidx) {
-  constexpr int r = Sched<LOGB>::r_of(P);
+template <int LOGB, int P, int LOGC>
+__device__ __forceinline__ void phase_from_lds(uint32_t* x, const uint32_t* X, uint32_t g, uint32_t c) {
+  using LM = LdsMap<LOGB, P, LOGC>;
+  constexpr int r = LM::r;
   constexpr int NG = PhaseGroups<LOGB, P>::NG;
 #pragma unroll
-  for (int u = 0; u < NG; ++u)
+  for (int u = 0; u < NG; ++u) {
+    const uint32_t gid = PhaseGroups<LOGB, P>::gid(g, u);
+    if constexpr (LM::affine) {
+      const uint32_t* b = X + LM::base(gid, c);
 #pragma unroll
-    for (int k = 0; k < (1 << r); ++k) x[u * (1 << r) + k] = X[pidx(lidx(elem<LOGB, P>(PhaseGroups<LOGB, P>::gid(g, u), k)))];
+      for (int k = 0; k < (1 << r); ++k) x[u * (1 << r) + k] = b[LM::off(k)];
+    } else {
+#pragma unroll
+      for (int k = 0; k < (1 << r); ++k) x[u * (1 << r) + k] = X[pidx(LM::word(gid, k, c))];
+    }
+  }
 }
 
 // Phases [P, END) of a forward transform whose registers already hold phase
 // P-1's output (P >= 1): each step writes LDS, syncs, reads phase P's groups
 // and computes.  All threads of the workgroup must call it.
-template <int LOGB, int P, int END, bool LAZY, class Idx>
-__device__ __forceinline__ void fwd_phases(uint32_t* x, uint32_t* X, uint32_t g, const uint32_t* planes,
-                                           const Arith<LAZY>& A, Idx lidx) {
+template <int LOGB, int P, int END, int LOGC, bool LAZY>
+__device__ __forceinline__ void fwd_phases(uint32_t* x, uint32_t* X, uint32_t g, uint32_t c, const uint32_t* planes,
+                                           const Arith<LAZY>& A) {
   if constexpr (P < END) {
-    phase_to_lds<LOGB, P - 1>(X, x, g, lidx);
+    phase_to_lds<LOGB, P - 1, LOGC>(X, x, g, c);
     __syncthreads();
-    phase_from_lds<LOGB, P>(x, X, g, lidx);
+    phase_from_lds<LOGB, P, LOGC>(x, X, g, c);
     __syncthreads();
     phase_compute<LOGB, P, false, LAZY>(x, g, planes, A);
-    fwd_phases<LOGB, P + 1, END, LAZY>(x, X, g, planes, A, lidx);
+    fwd_phases<LOGB, P + 1, END, LOGC, LAZY>(x, X, g, c, planes, A);
   }
 }
 // Inverse phases P, P-1, ..., 0 where registers hold phase P+1's inverse output.
-template <int LOGB, int P, bool LAZY, class Idx>
-__device__ __forceinline__ void inv_phases(uint32_t* x, uint32_t* X, uint32_t g, const uint32_t* planes,
-                                           const Arith<LAZY>& A, Idx lidx) {
+template <int LOGB, int P, int LOGC, bool LAZY>
+__device__ __forceinline__ void inv_phases(uint32_t* x, uint32_t* X, uint32_t g, uint32_t c, const uint32_t* planes,
+                                           const Arith<LAZY>& A) {
   if constexpr (P >= 0) {
-    phase_to_lds<LOGB, P + 1>(X, x, g, lidx);
+    phase_to_lds<LOGB, P + 1, LOGC>(X, x, g, c);
     __syncthreads();
-    phase_from_lds<LOGB, P>(x, X, g, lidx);
+    phase_from_lds<LOGB, P, LOGC>(x, X, g, c);
     __syncthreads();
     phase_compute<LOGB, P, true, LAZY>(x, g, planes, A);
-    inv_phases<LOGB, P - 1, LAZY>(x, X, g, planes, A, lidx);
+    inv_phases<LOGB, P - 1, LOGC, LAZY>(x, X, g, c, planes, A);
   }
 }
 
@@ -229,7 +270,8 @@ __global__ void __launch_bounds__(RowShape<LOGM>::T) k_rows2(RowJob2 j0, const R
   extern __shared__ uint32_t lds[];
   const RowJob2 J = jobs ? jobs[blockIdx.y] : j0;
   const Arith<LAZY> A(J.f);
-  const bool lift = (J.flags & R2_LIFT) != 0;
+  const uint32_t flags = __builtin_amdgcn_readfirstlane(J.flags);  // uniform: scalar branches
+  const bool lift = (flags & R2_LIFT) != 0;
   const uint32_t t = threadIdx.x;
   const uint32_t g = t % RS::GT, ur = t / RS::GT;
   const uint32_t row = blockIdx.x * RS::U + ur;
@@ -237,7 +279,6 @@ __global__ void __launch_bounds__(RowShape<LOGM>::T) k_rows2(RowJob2 j0, const R
   const bool active = RS::U == 1 || row < J.nrows;
   const uint32_t base = row << LOGM;
   uint32_t* X = lds + ur * RS::lds_words_per_row;
-  auto lidx = [](uint32_t e) { return e; };
   uint32_t x[16];
 
   // Branch-free first-phase load: indices past in_count (or of an inactive
@@ -260,7 +301,7 @@ __global__ void __launch_bounds__(RowShape<LOGM>::T) k_rows2(RowJob2 j0, const R
   auto forward = [&](const uint32_t* src, uint32_t in_count, uint32_t* regs) {
     load_first(src, in_count, regs);
     phase_compute<LOGM, 0, false, LAZY>(regs, g, J.f.planes_fwd, A);
-    fwd_phases<LOGM, 1, NPH, LAZY>(regs, X, g, J.f.planes_fwd, A, lidx);
+    fwd_phases<LOGM, 1, NPH, 0, LAZY>(regs, X, g, 0u, J.f.planes_fwd, A);
   };
 
   constexpr int PL = NPH - 1;  // last forward phase = first inverse phase
@@ -275,7 +316,7 @@ __global__ void __launch_bounds__(RowShape<LOGM>::T) k_rows2(RowJob2 j0, const R
       for (int k = 0; k < (1 << rl); ++k) {
         const uint32_t gi = base + elem<LOGM, PL>(PhaseGroups<LOGM, PL>::gid(g, u), k);
         uint32_t v = x[u * (1 << rl) + k];
-        if (J.flags & R2_CANON) v = A.canon(v);
+        if (flags & R2_CANON) v = A.canon(v);
         if (active) gstore(J.dst + gi, v);
       }
     return;
@@ -310,7 +351,7 @@ __global__ void __launch_bounds__(RowShape<LOGM>::T) k_rows2(RowJob2 j0, const R
     }
     // Inverse: phase PL on registers, then PL-1 .. 0 through LDS.
     phase_compute<LOGM, PL, true, LAZY>(x, g, J.f.planes_inv, A);
-    inv_phases<LOGM, PL - 1, LAZY>(x, X, g, J.f.planes_inv, A, lidx);
+    inv_phases<LOGM, PL - 1, 0, LAZY>(x, X, g, 0u, J.f.planes_inv, A);
     constexpr int r0 = SC::r_of(0);
 #pragma unroll
     for (int u = 0; u < PhaseGroups<LOGM, 0>::NG; ++u)
@@ -318,9 +359,9 @@ __global__ void __launch_bounds__(RowShape<LOGM>::T) k_rows2(RowJob2 j0, const R
       for (int k = 0; k < (1 << r0); ++k) {
         const uint32_t gi = base + elem<LOGM, 0>(PhaseGroups<LOGM, 0>::gid(g, u), k);
         uint32_t v = x[u * (1 << r0) + k];
-        if (J.flags & R2_SCALE) v = A.mul(v, J.inv_n);
-        if (J.flags & R2_NEG) v = v == 0 ? 0u : A.R - v;
-        if (J.flags & R2_CANON) v = A.canon(v);
+        if (flags & R2_SCALE) v = A.mul(v, J.inv_n);
+        if (flags & R2_NEG) v = v == 0 ? 0u : A.R - v;
+        if (flags & R2_CANON) v = A.canon(v);
         if (active && gi >= J.out_lo && gi < J.out_hi) gstore(J.dst + (gi - J.out_shift), v);
       }
   }
@@ -357,7 +398,8 @@ __global__ void __launch_bounds__((ColShape<S, LOGC>::T)) k_cols2(ColJob2 j0, co
   extern __shared__ uint32_t X[];
   const ColJob2 J = jobs ? jobs[blockIdx.y] : j0;
   const Arith<LAZY> A(J.f);
-  const bool lift = (J.flags & R2_LIFT) != 0;
+  const uint32_t flags = __builtin_amdgcn_readfirstlane(J.flags);
+  const bool lift = (flags & R2_LIFT) != 0;
   const int b = L - S;
   const uint32_t M = 1u << b;
   const uint32_t tiles = M >> LOGC;
@@ -366,7 +408,6 @@ __global__ void __launch_bounds__((ColShape<S, LOGC>::T)) k_cols2(ColJob2 j0, co
   const uint32_t c = threadIdx.x % C, g = threadIdx.x / C;
   const uint32_t col = (tile << LOGC) + c;
   const uint32_t shift = J.f.logmax - L;
-  auto lidx = [c](uint32_t e) { return (e << LOGC) | c; };
   uint32_t x[16];
   constexpr int PL = NPH - 1;
   if constexpr (!INV) {
@@ -383,7 +424,7 @@ __global__ void __launch_bounds__((ColShape<S, LOGC>::T)) k_cols2(ColJob2 j0, co
         x[u * (1 << r0) + k] = gi < J.in_count ? v : 0u;
       }
     phase_compute<S, 0, false, LAZY>(x, g, J.f.planes_fwd, A);
-    fwd_phases<S, 1, NPH, LAZY>(x, X, g, J.f.planes_fwd, A, lidx);
+    fwd_phases<S, 1, NPH, LOGC, LAZY>(x, X, g, c, J.f.planes_fwd, A);
     constexpr int rl = SC::r_of(PL);
 #pragma unroll
     for (int u = 0; u < PhaseGroups<S, PL>::NG; ++u)
@@ -392,7 +433,7 @@ __global__ void __launch_bounds__((ColShape<S, LOGC>::T)) k_cols2(ColJob2 j0, co
         const uint32_t e = elem<S, PL>(PhaseGroups<S, PL>::gid(g, u), k);
         const uint32_t xx = (col * bitrev(e, S)) << shift;
         uint32_t v = A.mul(x[u * (1 << rl) + k], twiddle<LAZY>(A, J.f.tw_lo, J.f.tw_hi, J.f.split, xx));
-        if (J.flags & R2_CANON) v = A.canon(v);
+        if (flags & R2_CANON) v = A.canon(v);
         gstore(J.dst + e * M + col, v);
       }
   } else {
@@ -407,7 +448,7 @@ __global__ void __launch_bounds__((ColShape<S, LOGC>::T)) k_cols2(ColJob2 j0, co
         x[u * (1 << rl) + k] = A.mul(gload(J.src + e * M + col), tw);
       }
     phase_compute<S, PL, true, LAZY>(x, g, J.f.planes_inv, A);
-    inv_phases<S, PL - 1, LAZY>(x, X, g, J.f.planes_inv, A, lidx);
+    inv_phases<S, PL - 1, LOGC, LAZY>(x, X, g, c, J.f.planes_inv, A);
     constexpr int r0 = SC::r_of(0);
 #pragma unroll
     for (int u = 0; u < PhaseGroups<S, 0>::NG; ++u)
@@ -416,7 +457,7 @@ __global__ void __launch_bounds__((ColShape<S, LOGC>::T)) k_cols2(ColJob2 j0, co
         const uint32_t e = elem<S, 0>(PhaseGroups<S, 0>::gid(g, u), k);
         const uint32_t gi = e * M + col;
         uint32_t v = A.canon(x[u * (1 << r0) + k]);
-        if (J.flags & R2_NEG) v = v == 0 ? 0u : J.f.p - v;
+        if (flags & R2_NEG) v = v == 0 ? 0u : J.f.p - v;
         if (gi >= J.out_lo && gi < J.out_hi) gstore(J.dst + (gi - J.out_shift), v);
       }
   }

@@ -60,14 +60,38 @@ __device__ __forceinline__ uint32_t fs_inv(const FieldScalar& f, uint32_t a) {
 enum { REC_INV = 0, REC_EXP = 1, REC_SQRT = 2 };
 constexpr int kRecMax = 64;
 
-__global__ void __launch_bounds__(256) k_recur(int kind, FieldScalar f, const uint32_t* __restrict__ u, uint32_t lu,
+// Block-wide sum of one uint64 per thread (blockDim = TPB, a multiple of 64).
+template <int TPB>
+__device__ __forceinline__ unsigned long long block_sum(unsigned long long v, unsigned long long* red) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  if constexpr (TPB == 64) {
+    return v;
+  } else {
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+    __syncthreads();
+    unsigned long long t = 0;
+#pragma unroll
+    for (int w = 0; w < TPB / 64; ++w) t += red[w];
+    __syncthreads();
+    return t;
+  }
+}
+
+template <int TPB>
+__global__ void __launch_bounds__(TPB) k_recur(int kind, FieldScalar f, const uint32_t* __restrict__ u, uint32_t lu,
                                                const uint32_t* __restrict__ w, uint32_t lw, uint32_t* v, uint32_t lo,
                                                uint32_t hi, uint32_t init, uint32_t k) {
   // New values [lo, hi) (hi - lo <= kRecMax) live in LDS; older ones are read from v.
   __shared__ uint32_t nv[kRecMax];
-  __shared__ unsigned long long red[256];
+  __shared__ uint32_t recip[kRecMax];
+  __shared__ unsigned long long red[TPB / 64];
   __shared__ uint32_t v0s;
   const uint32_t base = lo;
+  if (kind == REC_EXP && threadIdx.x < hi - lo) {
+    const uint32_t i = lo + threadIdx.x;
+    recip[threadIdx.x] = i == 0 ? 0u : fs_inv(f, fs_from_int(f, i));
+  }
   if (threadIdx.x == 0) {
     uint32_t v0;
     if (lo == 0) {
@@ -88,21 +112,16 @@ __global__ void __launch_bounds__(256) k_recur(int kind, FieldScalar f, const ui
     else if (kind == REC_EXP) jmax = min(i, lu);
     else jmax = i - 1;
     unsigned long long s = 0;
-    for (uint32_t j = 1 + threadIdx.x; j <= jmax; j += blockDim.x) {
+    for (uint32_t j = 1 + threadIdx.x; j <= jmax; j += TPB) {
       const uint32_t x = kind == REC_EXP ? u[j - 1] : (kind == REC_INV ? u[j] : val(j));
       s += fs_mul(f, x, val(i - j));
     }
-    red[threadIdx.x] = s;
-    __syncthreads();
-    for (int st = 128; st > 0; st >>= 1) {
-      if ((int)threadIdx.x < st) red[threadIdx.x] += red[threadIdx.x + st];
-      __syncthreads();
-    }
+    s = block_sum<TPB>(s, red);
     if (threadIdx.x == 0) {
-      const uint32_t sum = (uint32_t)(red[0] % f.p);
+      const uint32_t sum = (uint32_t)(s % f.p);
       uint32_t r;
       if (kind == REC_INV) r = fs_mul(f, sum == 0 ? 0u : f.p - sum, v0);
-      else if (kind == REC_EXP) r = fs_mul(f, sum, fs_inv(f, fs_from_int(f, i)));
+      else if (kind == REC_EXP) r = fs_mul(f, sum, recip[i - lo]);
       else r = fs_mul(f, fs_sub(f, i < lw ? w[i] : 0u, sum), k);
       nv[i - base] = r;
       v[i] = r;
@@ -198,11 +217,18 @@ static void launch_ew(Context& c, K kernel, size_t n, Args... args) {
   FPXG_HIP(hipGetLastError());
 }
 
+// Seeds (dot products of at most ~128 terms) run on one wave; longer tails on
+// a 1024-thread workgroup.
 static void recur(Context& c, const HostField& T, int kind, const uint32_t* u, size_t lu, const uint32_t* w,
                   size_t lw, uint32_t* v, size_t lo, size_t hi, uint32_t init, uint32_t k) {
   if (hi <= lo) return;
-  hipLaunchKernelGGL(k_recur, dim3(1), dim3(256), 0, c.stream, kind, scalar_of(T), u, (uint32_t)lu, w,
-                     (uint32_t)lw, v, (uint32_t)lo, (uint32_t)hi, init, k);
+  if (hi <= 128) {
+    hipLaunchKernelGGL(k_recur<64>, dim3(1), dim3(64), 0, c.stream, kind, scalar_of(T), u, (uint32_t)lu, w,
+                       (uint32_t)lw, v, (uint32_t)lo, (uint32_t)hi, init, k);
+  } else {
+    hipLaunchKernelGGL(k_recur<1024>, dim3(1), dim3(1024), 0, c.stream, kind, scalar_of(T), u, (uint32_t)lu, w,
+                       (uint32_t)lw, v, (uint32_t)lo, (uint32_t)hi, init, k);
+  }
   FPXG_HIP(hipGetLastError());
 }
 
@@ -219,7 +245,8 @@ static std::pair<size_t, size_t> scan_nonzero(Context& c, const uint32_t* a, siz
   DevBuf r = c.buf(4);
   unsigned long long init[2] = {hi, lo}, out[2];
   FPXG_HIP(hipMemcpyAsync(r.get(), init, 16, hipMemcpyHostToDevice, c.stream));
-  hipLaunchKernelGGL(k_scan_nonzero, dim3(grid_for(hi - lo)), dim3(256), 0, c.stream, a, lo, hi,
+  const unsigned blocks = (unsigned)std::min<size_t>(256, (hi - lo + 255) / 256);
+  hipLaunchKernelGGL(k_scan_nonzero, dim3(blocks), dim3(256), 0, c.stream, a, lo, hi,
                      (unsigned long long*)r.get());
   FPXG_HIP(hipGetLastError());
   FPXG_HIP(hipMemcpyAsync(out, r.get(), 16, hipMemcpyDeviceToHost, c.stream));
