@@ -22,14 +22,14 @@ __all__ = [
     "DomainError", "NttSizeError", "DeviceError", "UnsupportedModulus", "lib", "lib_path",
     "conv", "inv", "log", "exp", "sqrt", "series", "ntt_forward", "ntt_inverse", "NttPlan",
     "release_scratch_memory", "scratch_memory_stats", "from_int", "to_int", "Mont", "DeviceArray",
-    "EXPORTED_SYMBOLS",
+    "EXPORTED_SYMBOLS", "Request", "batch", "batch_submit", "wait",
 ]
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_NAME = "libfpxg.so"
 
 OK, E_DOMAIN, E_SIZE, E_NOMEM, E_DEVICE, E_UNSUPPORTED, E_ARG = range(7)
-OPS = {"inv": 0, "log": 1, "exp": 2, "sqrt": 3}
+OPS = {"inv": 0, "log": 1, "exp": 2, "sqrt": 3, "conv": 4}
 NTT_FORWARD, NTT_FORWARD_PREFIX, NTT_INVERSE, NTT_INVERSE_PREFIX, NTT_INVERSE_HIGHHALF = range(5)
 SIZE_MAX = (1 << 64) - 1
 
@@ -39,7 +39,7 @@ EXPORTED_SYMBOLS = [
     "fpxg_release_scratch", "fpxg_conv_u32", "fpxg_series_u32", "fpxg_ntt_u32", "fpxg_conv_dev",
     "fpxg_series_dev", "fpxg_ntt_dev", "fpxg_dev_alloc", "fpxg_dev_free", "fpxg_memcpy_h2d",
     "fpxg_memcpy_d2h", "fpxg_synchronize", "fpxg_stream", "fpxg_event_record", "fpxg_event_elapsed",
-    "fpxg_ntt_pass_dev", "fpxg_ntt_plan_info",
+    "fpxg_ntt_pass_dev", "fpxg_ntt_plan_info", "fpxg_batch_submit", "fpxg_wait", "fpxg_batch_u32",
 ]
 
 
@@ -79,6 +79,14 @@ _size_p = ctypes.POINTER(ctypes.c_size_t)
 class Stats(ctypes.Structure):
     _fields_ = [(n, ctypes.c_size_t) for n in (
         "cached_bytes", "cached_blocks", "system_allocations", "cache_hits", "in_use_bytes", "capacity_bytes")]
+
+
+class Req(ctypes.Structure):
+    """fpxg_req (include/fpxg.h)."""
+    _fields_ = [("op", ctypes.c_int), ("p", ctypes.c_uint32), ("a", ctypes.c_void_p), ("la", ctypes.c_size_t),
+                ("b", ctypes.c_void_p), ("lb", ctypes.c_size_t), ("n", ctypes.c_size_t),
+                ("out", ctypes.c_void_p), ("out_cap", ctypes.c_size_t), ("out_len", ctypes.c_size_t),
+                ("status", ctypes.c_int)]
 
 
 def lib_path() -> str:
@@ -124,6 +132,9 @@ def lib():
         L.fpxg_stream.restype = ctypes.c_void_p
         L.fpxg_event_record.argtypes = [ctypes.c_int]
         L.fpxg_event_elapsed.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_float)]
+        L.fpxg_batch_submit.argtypes = [ctypes.POINTER(Req), ctypes.c_size_t, ctypes.POINTER(ctypes.c_uint64)]
+        L.fpxg_wait.argtypes = [ctypes.c_uint64]
+        L.fpxg_batch_u32.argtypes = [ctypes.POINTER(Req), ctypes.c_size_t]
         _lib = L
     return _lib
 
@@ -363,3 +374,76 @@ def ntt_plan_info(n: int) -> dict:
     r, c, t = ctypes.c_int(0), ctypes.c_int(0), ctypes.c_int(0)
     _check(lib().fpxg_ntt_plan_info(n, ctypes.byref(r), ctypes.byref(c), ctypes.byref(t)), "plan_info")
     return {"row_log": r.value, "col_levels": c.value, "tile_cols_log": t.value}
+
+
+# ---------------------------------------------------------------------------
+# Batches: many conv / series calls merged into shared launches.
+# ---------------------------------------------------------------------------
+class Request:
+    """One call of a batch: ("conv", p, a, b[, n]) or (op, p, a, n) for op in inv/log/exp/sqrt."""
+
+    def __init__(self, op: str, p: int, a, b=None, n: int | None = None):
+        self.op, self.p = op, int(p)
+        self.a = _u32(a)
+        self.b = None if b is None else _u32(b)
+        if op == "conv":
+            self.n = SIZE_MAX if n is None else int(n)
+        else:
+            if n is None:
+                raise ValueError("series requests need n")
+            self.n = int(n)
+        self.out = None
+        self.status = None
+
+    def out_capacity(self) -> int:
+        if self.op == "conv":
+            la, lb = min(self.a.size, self.n), min(self.b.size, self.n)
+            return 0 if la == 0 or lb == 0 else min(la + lb - 1, self.n)
+        return self.n
+
+
+def batch(requests) -> list:
+    """Runs host-array requests as one merged batch (fpxg_batch_u32).  Returns
+    each request's result array, or the exception the single call would raise."""
+    requests = list(requests)
+    k = len(requests)
+    arr = (Req * max(k, 1))()
+    outs = []
+    for i, r in enumerate(requests):
+        cap = r.out_capacity()
+        out = np.zeros(max(cap, 1), dtype=np.uint32)
+        outs.append(out)
+        q = arr[i]
+        q.op, q.p = OPS[r.op], r.p
+        q.a, q.la = (r.a.ctypes.data if r.a.size else None), r.a.size
+        if r.b is not None:
+            q.b, q.lb = (r.b.ctypes.data if r.b.size else None), r.b.size
+        q.n, q.out, q.out_cap = r.n, out.ctypes.data, cap
+    _check(lib().fpxg_batch_u32(arr, k), "batch")
+    results = []
+    for i, r in enumerate(requests):
+        st = arr[i].status
+        r.status = st
+        if st == OK:
+            r.out = outs[i][: arr[i].out_len].copy()
+            results.append(r.out)
+        else:
+            msg = fpxg_strerror(st)
+            results.append(MemoryError(msg) if st == E_NOMEM else _ERRORS.get(st, FpxgError)(msg))
+    return results
+
+
+def fpxg_strerror(code: int) -> str:
+    return lib().fpxg_strerror(code).decode()
+
+
+def batch_submit(reqs) -> int:
+    """Device-pointer batch: reqs is a ctypes array of Req (device addresses).
+    Returns the ticket; statuses and out_len are filled before return."""
+    t = ctypes.c_uint64(0)
+    _check(lib().fpxg_batch_submit(reqs, len(reqs), ctypes.byref(t)), "batch_submit")
+    return t.value
+
+
+def wait(ticket: int) -> None:
+    _check(lib().fpxg_wait(ticket), "wait")

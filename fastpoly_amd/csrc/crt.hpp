@@ -16,13 +16,13 @@ struct GarnerConst {
   uint32_t inv123;   // Montgomery (mod P3) of (p1 p2)^-1
   uint32_t p, ninv;  // target
   uint32_t p1_t, p12_t;  // p1 mod p, p1 p2 mod p (plain)
-  uint32_t primes, accumulate;
+  uint32_t primes, accumulate, neg;
 };
 
-__global__ void k_garner(const uint32_t* __restrict__ c1, const uint32_t* __restrict__ c2,
-                         const uint32_t* __restrict__ c3, uint32_t* __restrict__ out, uint32_t n,
-                         GarnerConst g) {
-  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+__device__ __forceinline__ void garner_one(const uint32_t* __restrict__ c1, const uint32_t* __restrict__ c2,
+                                           const uint32_t* __restrict__ c3, uint32_t* __restrict__ out, uint32_t i,
+                                           const GarnerConst& g) {
+  {
     const uint32_t x1 = redc_full(c1[i], g.P[0], g.Pninv[0]);
     uint32_t x2 = 0, x3 = 0;
     if (g.primes >= 2) {
@@ -47,6 +47,7 @@ __global__ void k_garner(const uint32_t* __restrict__ c1, const uint32_t* __rest
       v = v + t >= g.p ? v + t - g.p : v + t;
       t = redc_full((uint64_t)x3 * g.p12_t, g.p, g.ninv);
       v = v + t >= g.p ? v + t - g.p : v + t;
+      if (g.neg) v = v == 0 ? 0u : g.p - v;
       if (g.accumulate) {
         const uint32_t o = out[i];
         v = v + o >= g.p ? v + o - g.p : v + o;
@@ -54,6 +55,32 @@ __global__ void k_garner(const uint32_t* __restrict__ c1, const uint32_t* __rest
     }
     out[i] = v;
   }
+}
+
+__global__ void k_garner(const uint32_t* __restrict__ c1, const uint32_t* __restrict__ c2,
+                         const uint32_t* __restrict__ c3, uint32_t* __restrict__ out, uint32_t n,
+                         GarnerConst g) {
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    garner_one(c1, c2, c3, out, i, g);
+}
+
+// One record per CRT product window; the launch covers every record through a
+// chunk table (chunk -> record, first element), 1024 elements per chunk.
+struct GarnerRec {
+  const uint32_t* c1;
+  const uint32_t* c2;
+  const uint32_t* c3;
+  uint32_t* out;
+  uint32_t n;
+  GarnerConst g;
+};
+constexpr uint32_t kChunk = 1024;
+
+__global__ void __launch_bounds__(256) k_garner_batch(const GarnerRec* __restrict__ recs, const uint2* __restrict__ chunks) {
+  const uint2 ch = chunks[blockIdx.x];
+  const GarnerRec R = recs[ch.x];
+  const uint32_t end = min(R.n, ch.y + kChunk);
+  for (uint32_t i = ch.y + threadIdx.x; i < end; i += blockDim.x) garner_one(R.c1, R.c2, R.c3, R.out, i, R.g);
 }
 
 }  // namespace fpxg

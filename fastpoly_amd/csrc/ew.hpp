@@ -1,0 +1,313 @@
+// Small device kernels around the transforms: elementwise records, the
+// sequential seed/tail recurrences, the input scan, and their host emitters
+// (immediate launch, or a record appended to the active batch recorder).
+#pragma once
+
+namespace fpxg {
+
+__device__ __forceinline__ uint32_t fs_mul(const FieldScalar& f, uint32_t a, uint32_t b) {
+  if (f.p == 2) return a & b & 1u;
+  return mulmod_full(a, b, f.p, f.ninv);
+}
+__device__ __forceinline__ uint32_t fs_sub(const FieldScalar& f, uint32_t a, uint32_t b) {
+  return a >= b ? a - b : a - b + f.p;
+}
+__device__ __forceinline__ uint32_t fs_from_int(const FieldScalar& f, uint32_t x) {
+  if (f.p == 2) return x & 1u;
+  return mulmod_full(x % f.p, f.r2, f.p, f.ninv);
+}
+__device__ uint32_t fs_pow(const FieldScalar& f, uint32_t a, uint32_t e) {
+  uint32_t r = f.one, b = a;
+  for (; e; e >>= 1) {
+    if (e & 1) r = fs_mul(f, r, b);
+    b = fs_mul(f, b, b);
+  }
+  return r;
+}
+__device__ __forceinline__ uint32_t fs_inv(const FieldScalar& f, uint32_t a) {
+  return f.p == 2 ? a : fs_pow(f, a, f.p - 2);
+}
+
+// ---------------------------------------------------------------------------
+// Sequential recurrences (seeds and <= 8-coefficient tails), one workgroup per
+// record: new values are sequential, each dot product is spread over the block.
+//   INV : v[i] = -(sum_{j=1}^{min(i,lu-1)} u[j] v[i-j]) * v[0]     (inverse.hpp:68-77)
+//   EXP : v[i] = (sum_{j=1}^{min(i,lu)} u[j-1] v[i-j]) / i         (exponential.hpp:158-163)
+//   SQRT: v[i] = (w[i] - sum_{j=1}^{i-1} v[j] v[i-j]) * k          (square_root.hpp:46-52)
+// For INV with lo == 0, v[0] = u[0]^-1; for EXP/SQRT v[0] = init.
+// ---------------------------------------------------------------------------
+template <int TPB>
+__device__ __forceinline__ unsigned long long block_sum(unsigned long long v, unsigned long long* red) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  if constexpr (TPB == 64) {
+    return v;
+  } else {
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+    __syncthreads();
+    unsigned long long t = 0;
+#pragma unroll
+    for (int w = 0; w < TPB / 64; ++w) t += red[w];
+    __syncthreads();
+    return t;
+  }
+}
+
+template <int TPB>
+__global__ void __launch_bounds__(TPB) k_recur(const RecurRec* __restrict__ recs, RecurRec r0) {
+  __shared__ uint32_t nv[kRecMax];
+  __shared__ uint32_t recip[kRecMax];
+  __shared__ unsigned long long red[TPB / 64];
+  __shared__ uint32_t v0s;
+  const RecurRec R = recs ? recs[blockIdx.x] : r0;
+  const FieldScalar f = R.f;
+  const uint32_t lo = R.lo, hi = R.hi, base = R.lo;
+  uint32_t* v = R.v;
+  if (R.kind == REC_EXP && threadIdx.x < hi - lo) {
+    const uint32_t i = lo + threadIdx.x;
+    recip[threadIdx.x] = i == 0 ? 0u : fs_inv(f, fs_from_int(f, i));
+  }
+  if (threadIdx.x == 0) {
+    uint32_t v0;
+    if (lo == 0) {
+      v0 = R.kind == REC_INV ? fs_inv(f, R.u[0]) : R.init;
+      nv[0] = v0;
+      v[0] = v0;
+    } else {
+      v0 = v[0];
+    }
+    v0s = v0;
+  }
+  __syncthreads();
+  const uint32_t v0 = v0s;
+  auto val = [&](uint32_t idx) -> uint32_t { return idx < base ? v[idx] : nv[idx - base]; };
+  for (uint32_t i = lo == 0 ? 1 : lo; i < hi; ++i) {
+    uint32_t jmax;
+    if (R.kind == REC_INV) jmax = min(i, R.lu ? R.lu - 1 : 0);
+    else if (R.kind == REC_EXP) jmax = min(i, R.lu);
+    else jmax = i - 1;
+    unsigned long long s = 0;
+    for (uint32_t j = 1 + threadIdx.x; j <= jmax; j += TPB) {
+      const uint32_t x = R.kind == REC_EXP ? R.u[j - 1] : (R.kind == REC_INV ? R.u[j] : val(j));
+      s += fs_mul(f, x, val(i - j));
+    }
+    s = block_sum<TPB>(s, red);
+    if (threadIdx.x == 0) {
+      const uint32_t sum = (uint32_t)(s % f.p);
+      uint32_t r;
+      if (R.kind == REC_INV) r = fs_mul(f, sum == 0 ? 0u : f.p - sum, v0);
+      else if (R.kind == REC_EXP) r = fs_mul(f, sum, recip[i - lo]);
+      else r = fs_mul(f, fs_sub(f, i < R.lw ? R.w[i] : 0u, sum), R.k);
+      nv[i - base] = r;
+      v[i] = r;
+    }
+    __syncthreads();
+  }
+}
+
+__device__ __forceinline__ uint32_t ew_value(const EwRec& R, uint32_t i) {
+  const FieldScalar& f = R.f;
+  switch (R.op) {
+    case EW_FILL: return R.c;
+    case EW_NEGATE: { const uint32_t x = R.src[i]; return x == 0 ? 0u : f.p - x; }
+    case EW_SCALE: return fs_mul(f, R.src[i], R.c);
+    case EW_MUL_RECIP: return fs_mul(f, R.src[i], fs_inv(f, fs_from_int(f, R.c + i)));
+    case EW_DERIV: return fs_mul(f, R.src[i + 1], fs_from_int(f, i + 1));
+    case EW_INTEGRAL: return (i >= 1 && i - 1 < R.avail) ? fs_mul(f, R.src[i - 1], fs_inv(f, fs_from_int(f, i))) : 0u;
+    default: return fs_sub(f, i < R.avail ? R.src[i] : 0u, R.src2[i]);
+  }
+}
+
+// Elementwise records; one launch serves any number of them through a chunk
+// table of (record, first element) pairs, kChunk elements per chunk.
+__global__ void __launch_bounds__(256) k_ew(const EwRec* __restrict__ recs, const uint2* __restrict__ chunks, EwRec r0) {
+  uint32_t start = blockIdx.x * kChunk;
+  EwRec R = r0;
+  if (chunks) {
+    const uint2 ch = chunks[blockIdx.x];
+    R = recs[ch.x];
+    start = ch.y;
+  }
+  const uint32_t end = min(R.n, start + kChunk);
+  for (uint32_t i = start + threadIdx.x; i < end; i += blockDim.x) R.dst[i] = ew_value(R, i);
+}
+
+// Characteristic two (square_root.hpp:15-25): (sum b_i x^i)^2 = sum b_i x^(2i).
+// out[i/2] = a[i] for even i; any nonzero odd coefficient sets *bad.
+__global__ void k_frobenius_sqrt(const uint32_t* a, size_t cnt, uint32_t* out, unsigned int* bad) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < cnt; i += (size_t)gridDim.x * blockDim.x) {
+    if (i & 1) {
+      if (a[i] != 0) atomicOr(bad, 1u);
+    } else {
+      out[i / 2] = a[i];
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Input scan: for a[0, n): the first two nonzero indices, one past the last,
+// a[0] and a[first].  One workgroup per record.
+// ---------------------------------------------------------------------------
+struct ScanRec {
+  const uint32_t* a;
+  uint32_t n, pad;
+};
+struct ScanOut {
+  uint32_t first, second, last1, a0, afirst, pad[3];
+};
+
+struct ScanPart {
+  uint32_t m1, m2, last, pad;
+};
+
+// (first, second) nonzero indices of two disjoint ranges, merged.
+__device__ __forceinline__ void scan_merge(uint32_t& a1, uint32_t& a2, uint32_t b1, uint32_t b2) {
+  const uint32_t m2 = a1 < b1 ? min(a2, b1) : (b1 < a1 ? min(a1, b2) : min(a2, b2));
+  a1 = min(a1, b1);
+  a2 = m2;
+}
+
+// Stage 1: block x of record y scans slice x of B (indices are record-relative;
+// the sentinel "none" is the record's n).
+__global__ void __launch_bounds__(256) k_scan_part(const ScanRec* __restrict__ recs, ScanPart* __restrict__ parts,
+                                                  uint32_t B) {
+  __shared__ uint32_t m1s[256], m2s[256], lss[256];
+  const ScanRec R = recs[blockIdx.y];
+  const uint32_t per = (uint32_t)(((uint64_t)R.n + B - 1) / B);
+  const uint32_t lo = (uint32_t)min((uint64_t)blockIdx.x * per, (uint64_t)R.n);
+  const uint32_t hi = (uint32_t)min((uint64_t)lo + per, (uint64_t)R.n);
+  uint32_t m1 = R.n, m2 = R.n, last = 0;
+  for (uint32_t i = lo + threadIdx.x; i < hi; i += blockDim.x)
+    if (R.a[i] != 0) {
+      if (i < m1) { m2 = m1; m1 = i; } else if (i < m2) { m2 = i; }
+      last = i + 1;
+    }
+  m1s[threadIdx.x] = m1; m2s[threadIdx.x] = m2; lss[threadIdx.x] = last;
+  __syncthreads();
+  for (int st = 128; st > 0; st >>= 1) {
+    if ((int)threadIdx.x < st) {
+      uint32_t a1 = m1s[threadIdx.x], a2 = m2s[threadIdx.x];
+      scan_merge(a1, a2, m1s[threadIdx.x + st], m2s[threadIdx.x + st]);
+      m1s[threadIdx.x] = a1;
+      m2s[threadIdx.x] = a2;
+      lss[threadIdx.x] = max(lss[threadIdx.x], lss[threadIdx.x + st]);
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) parts[(size_t)blockIdx.y * B + blockIdx.x] = {m1s[0], m2s[0], lss[0], 0u};
+}
+
+// Stage 2: one workgroup per record merges its B <= 256 partials.
+__global__ void __launch_bounds__(256) k_scan_final(const ScanRec* __restrict__ recs,
+                                                   const ScanPart* __restrict__ parts, uint32_t B,
+                                                   ScanOut* __restrict__ out) {
+  __shared__ uint32_t m1s[256], m2s[256], lss[256];
+  const ScanRec R = recs[blockIdx.x];
+  ScanPart q{R.n, R.n, 0u, 0u};
+  if (threadIdx.x < B) q = parts[(size_t)blockIdx.x * B + threadIdx.x];
+  m1s[threadIdx.x] = q.m1; m2s[threadIdx.x] = q.m2; lss[threadIdx.x] = q.last;
+  __syncthreads();
+  for (int st = 128; st > 0; st >>= 1) {
+    if ((int)threadIdx.x < st) {
+      uint32_t a1 = m1s[threadIdx.x], a2 = m2s[threadIdx.x];
+      scan_merge(a1, a2, m1s[threadIdx.x + st], m2s[threadIdx.x + st]);
+      m1s[threadIdx.x] = a1;
+      m2s[threadIdx.x] = a2;
+      lss[threadIdx.x] = max(lss[threadIdx.x], lss[threadIdx.x + st]);
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    ScanOut o{};
+    o.first = m1s[0];
+    o.second = m2s[0];
+    o.last1 = lss[0];
+    o.a0 = R.n ? R.a[0] : 0u;
+    o.afirst = o.first < R.n ? R.a[o.first] : 0u;
+    out[blockIdx.x] = o;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Host side.
+// ---------------------------------------------------------------------------
+
+static FieldScalar scalar_of(const HostField& F) { return {F.p, F.ninv, F.r2, F.one}; }
+
+static void emit_ew(Context& c, const HostField& T, int op, uint32_t* dst, const uint32_t* src, const uint32_t* src2,
+                    size_t n, uint32_t avail, uint32_t cst) {
+  if (n == 0) return;
+  EwRec r{};
+  r.op = op; r.f = scalar_of(T); r.dst = dst; r.src = src; r.src2 = src2;
+  r.n = (uint32_t)n; r.avail = avail; r.c = cst;
+  if (c.rec) {
+    c.rec->push_ew(r);
+    return;
+  }
+  const unsigned blocks = (unsigned)((n + kChunk - 1) / kChunk);
+  hipLaunchKernelGGL(k_ew, dim3(blocks), dim3(256), 0, c.stream, (const EwRec*)nullptr, (const uint2*)nullptr, r);
+  FPXG_HIP(hipGetLastError());
+}
+
+// Seeds (dot products of at most ~128 terms) run on one wave; longer tails on
+// a 1024-thread workgroup.
+static void recur(Context& c, const HostField& T, int kind, const uint32_t* u, size_t lu, const uint32_t* w,
+                  size_t lw, uint32_t* v, size_t lo, size_t hi, uint32_t init, uint32_t k) {
+  if (hi <= lo) return;
+  RecurRec r{};
+  r.kind = kind; r.f = scalar_of(T); r.u = u; r.lu = (uint32_t)lu; r.w = w; r.lw = (uint32_t)lw; r.v = v;
+  r.lo = (uint32_t)lo; r.hi = (uint32_t)hi; r.init = init; r.k = k;
+  if (c.rec) {
+    c.rec->push_recur(r);
+    return;
+  }
+  if (hi <= 128) hipLaunchKernelGGL(k_recur<64>, dim3(1), dim3(64), 0, c.stream, (const RecurRec*)nullptr, r);
+  else hipLaunchKernelGGL(k_recur<1024>, dim3(1), dim3(1024), 0, c.stream, (const RecurRec*)nullptr, r);
+  FPXG_HIP(hipGetLastError());
+}
+
+static void emit_garner(Context& c, const GarnerRec& g) {
+  if (g.n == 0) return;
+  if (c.rec) {
+    c.rec->push_garner(g);
+    return;
+  }
+  hipLaunchKernelGGL(k_garner, dim3(grid_for(g.n)), dim3(256), 0, c.stream, g.c1, g.c2, g.c3, g.out, g.n, g.g);
+  FPXG_HIP(hipGetLastError());
+}
+
+// Scan of several inputs with one read-back.
+static std::vector<ScanOut> scan_inputs(Context& c, const std::vector<ScanRec>& recs) {
+  std::vector<ScanOut> out(recs.size());
+  if (recs.empty()) return out;
+  uint32_t maxn = 0;
+  for (const ScanRec& r : recs) maxn = std::max(maxn, r.n);
+  // Slices of >= 8192 words, at most 256 per record.
+  const uint32_t B = std::max<uint32_t>(1, std::min<uint32_t>(256, (maxn + 8191) / 8192));
+  DevBuf dr = c.buf(recs.size() * sizeof(ScanRec) / 4 + 1);
+  DevBuf dout = c.buf(recs.size() * sizeof(ScanOut) / 4 + 1);
+  DevBuf dpart = c.buf(recs.size() * B * sizeof(ScanPart) / 4 + 1);
+  FPXG_HIP(hipMemcpyAsync(dr.get(), recs.data(), recs.size() * sizeof(ScanRec), hipMemcpyHostToDevice, c.stream));
+  for (size_t s = 0; s < recs.size(); s += 65535) {
+    const size_t cnt = std::min<size_t>(65535, recs.size() - s);
+    const ScanRec* rp = (const ScanRec*)dr.get() + s;
+    hipLaunchKernelGGL(k_scan_part, dim3(B, (unsigned)cnt), dim3(256), 0, c.stream, rp,
+                       (ScanPart*)dpart.get() + s * B, B);
+    FPXG_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_scan_final, dim3((unsigned)cnt), dim3(256), 0, c.stream, rp,
+                       (const ScanPart*)dpart.get() + s * B, B, (ScanOut*)dout.get() + s);
+    FPXG_HIP(hipGetLastError());
+  }
+  FPXG_HIP(hipMemcpyAsync(out.data(), dout.get(), recs.size() * sizeof(ScanOut), hipMemcpyDeviceToHost, c.stream));
+  FPXG_HIP(hipStreamSynchronize(c.stream));
+  return out;
+}
+
+static uint32_t read_word(Context& c, const uint32_t* p) {
+  uint32_t h = 0;
+  FPXG_HIP(hipMemcpyAsync(&h, p, 4, hipMemcpyDeviceToHost, c.stream));
+  FPXG_HIP(hipStreamSynchronize(c.stream));
+  return h;
+}
+
+}  // namespace fpxg

@@ -246,9 +246,18 @@ void* Arena::alloc(size_t bytes) {
   return base;
 }
 
+size_t Arena::in_use() const {
+  std::lock_guard<std::mutex> g(mu_);
+  return in_use_;
+}
+
 void Arena::free(void* p) {
   if (!p) return;
   std::lock_guard<std::mutex> g(mu_);
+  if (defer_) {
+    defer_->push_back(p);
+    return;
+  }
   auto it = owner_.find(p);
   if (it == owner_.end()) return;
   const size_t sz = (size_t)1 << it->second;
@@ -286,7 +295,7 @@ fpxg_stats Arena::stats() const {
 
 static std::unique_ptr<Context> g_ctx;
 static std::mutex g_init_mu;
-static size_t g_arena_bytes = (size_t)16 << 30;
+static size_t g_arena_bytes = (size_t)64 << 30;
 static int g_device = 0;
 
 // ---- launch dispatch over the compile-time shapes of ntt2.hpp ----
@@ -326,6 +335,12 @@ static void rows2_size(Context& c, int logM, int mode, const RowJob2& j, uint32_
 
 static void launch_rows(Context& c, bool lazy, int mode, const RowJob2& j, int logM, uint32_t nrows,
                         uint32_t njobs = 1, const RowJob2* dj = nullptr) {
+  if (c.rec && !dj) {
+    RowJob2 jj = j;
+    jj.nrows = nrows;
+    c.rec->push_rows(logM, lazy, mode, jj);
+    return;
+  }
   if (lazy) rows2_size<true>(c, logM, mode, j, nrows, njobs, dj);
   else rows2_size<false>(c, logM, mode, j, nrows, njobs, dj);
   FPXG_HIP(hipGetLastError());
@@ -356,6 +371,10 @@ static void cols2_size(Context& c, const ColJob2& j, const PassPlan& P, uint32_t
 
 static void launch_cols(Context& c, bool lazy, bool inv, const ColJob2& j, const PassPlan& P, uint32_t njobs = 1,
                         const ColJob2* dj = nullptr) {
+  if (c.rec && !dj) {
+    c.rec->push_cols(P.S, lazy, inv, P.L, j);
+    return;
+  }
   if (lazy) {
     if (inv) cols2_size<true, true>(c, j, P, njobs, dj);
     else cols2_size<true, false>(c, j, P, njobs, dj);
@@ -383,6 +402,7 @@ static void context_init(Context& c) {
   FPXG_HIP(hipSetDevice(c.device));
   FPXG_HIP(hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking));
   for (auto& e : c.events) FPXG_HIP(hipEventCreate(&e));
+  for (auto& s : c.staging) FPXG_HIP(hipEventCreateWithFlags(&s.ev, hipEventDisableTiming));
   hipDeviceProp_t prop;
   FPXG_HIP(hipGetDeviceProperties(&prop, c.device));
   c.cus = prop.multiProcessorCount;
@@ -438,26 +458,34 @@ PassPlan Context::plan(int L) const {
   return {L, b, S, tile_log_cols(S)};
 }
 
-__global__ void k_fill(uint32_t* p, size_t n, uint32_t v) {
-  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
-    p[i] = v;
-}
-
 static unsigned grid_for(size_t n) {
   size_t g = (n + 255) / 256;
   return (unsigned)std::max<size_t>(1, std::min<size_t>(g, 8192));
-}
-
-static void fill(Context& c, uint32_t* p, size_t n, uint32_t v) {
-  if (n == 0) return;
-  hipLaunchKernelGGL(k_fill, dim3(grid_for(n)), dim3(256), 0, c.stream, p, n, v);
-  FPXG_HIP(hipGetLastError());
 }
 
 static int ceil_log2(uint64_t n) {
   int k = 0;
   while (((uint64_t)1 << k) < n) ++k;
   return k;
+}
+
+}  // namespace fpxg
+
+#include "ew.hpp"
+
+namespace fpxg {
+
+static void fill(Context& c, uint32_t* p, size_t n, uint32_t v) {
+  if (n == 0) return;
+  EwRec r{};
+  r.op = EW_FILL; r.f = {2u, 0u, 0u, 1u}; r.dst = p; r.n = (uint32_t)n; r.c = v;
+  if (c.rec) {
+    c.rec->push_ew(r);
+    return;
+  }
+  hipLaunchKernelGGL(k_ew, dim3((unsigned)((n + kChunk - 1) / kChunk)), dim3(256), 0, c.stream, (const EwRec*)nullptr,
+                     (const uint2*)nullptr, r);
+  FPXG_HIP(hipGetLastError());
 }
 
 // ===========================================================================
@@ -569,6 +597,7 @@ static GarnerConst garner_const(Context& c, const HostField& T, int primes, bool
   g.p12_t = (uint32_t)((p1 * p2) % T.p);
   g.primes = (uint32_t)primes;
   g.accumulate = accumulate;
+  g.neg = 0;
   return g;
 }
 
@@ -580,11 +609,14 @@ static void crt_window(Context& c, const HostField& T, const uint32_t* a, size_t
     res[k] = c.buf(outc);
     conv_core(c, c.field(kCrtP[k]), a, la, b, lb, res[k].get(), outc, true);
   }
-  const GarnerConst g = garner_const(c, T, primes, accumulate);
-  hipLaunchKernelGGL(k_garner, dim3(grid_for(outc)), dim3(256), 0, c.stream, res[0].get(),
-                     primes > 1 ? res[1].get() : res[0].get(), primes > 2 ? res[2].get() : res[0].get(), out,
-                     (uint32_t)outc, g);
-  FPXG_HIP(hipGetLastError());
+  GarnerRec g{};
+  g.c1 = res[0].get();
+  g.c2 = primes > 1 ? res[1].get() : res[0].get();
+  g.c3 = primes > 2 ? res[2].get() : res[0].get();
+  g.out = out;
+  g.n = (uint32_t)outc;
+  g.g = garner_const(c, T, primes, accumulate);
+  emit_garner(c, g);
 }
 
 static bool native_ok(const HostField& T, size_t full) {
@@ -624,6 +656,7 @@ void conv_dev(Context& c, const HostField& T, const uint32_t* a, size_t la, cons
 }  // namespace fpxg
 
 #include "series.hpp"
+#include "batch.hpp"
 
 // ===========================================================================
 // C ABI
@@ -698,6 +731,11 @@ void fpxg_shutdown(void) {
   for (auto& kv : g_ctx->fields)
     if (kv.second->tables) (void)hipFree(kv.second->tables);
   for (auto& e : g_ctx->events) (void)hipEventDestroy(e);
+  for (auto& s : g_ctx->staging) {
+    if (s.host) (void)hipHostFree(s.host);
+    (void)hipEventDestroy(s.ev);
+  }
+  for (auto& kv : g_ctx->tickets) (void)hipEventDestroy(kv.second);
   (void)hipStreamDestroy(g_ctx->stream);
   g_ctx.reset();
 }
@@ -916,6 +954,96 @@ int fpxg_event_elapsed(int a, int b, float* ms) {
     if (a < 0 || a >= 16 || b < 0 || b >= 16 || !ms) throw Error(E_ARG, "bad event slot");
     FPXG_HIP(hipEventSynchronize(c.events[b]));
     FPXG_HIP(hipEventElapsedTime(ms, c.events[a], c.events[b]));
+  });
+}
+
+// ---- merged batches ----
+
+static void prune_tickets(Context& c) {
+  for (auto it = c.tickets.begin(); it != c.tickets.end();) {
+    if (hipEventQuery(it->second) == hipSuccess) {
+      (void)hipEventDestroy(it->second);
+      it = c.tickets.erase(it);
+    } else {
+      ++it;
+    }
+  }
+}
+
+int fpxg_batch_submit(fpxg_req* reqs, size_t count, uint64_t* ticket) {
+  return guarded([&](Context& c) {
+    if (!reqs && count) throw Error(E_ARG, "reqs is null");
+    prune_tickets(c);
+    batch_dev(c, reqs, count);
+    hipEvent_t ev;
+    FPXG_HIP(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+    FPXG_HIP(hipEventRecord(ev, c.stream));
+    const uint64_t t = c.next_ticket++;
+    c.tickets[t] = ev;
+    if (ticket) *ticket = t;
+  });
+}
+
+int fpxg_wait(uint64_t ticket) {
+  hipEvent_t ev = nullptr;
+  int rc = guarded([&](Context& c) {
+    if (ticket == 0 || ticket >= c.next_ticket) throw Error(E_ARG, "unknown ticket");
+    auto it = c.tickets.find(ticket);
+    if (it == c.tickets.end()) return;  // already complete and pruned
+    ev = it->second;
+    c.tickets.erase(it);
+  });
+  if (rc != OK || !ev) return rc;
+  // Wait outside the library lock so other threads can keep submitting.
+  const hipError_t e = hipEventSynchronize(ev);
+  (void)hipEventDestroy(ev);
+  if (e != hipSuccess) {
+    t_last_error = hipGetErrorString(e);
+    return E_DEVICE;
+  }
+  return OK;
+}
+
+int fpxg_batch_u32(fpxg_req* reqs, size_t count) {
+  return guarded([&](Context& c) {
+    if (!reqs && count) throw Error(E_ARG, "reqs is null");
+    // One device block for every input and output, requests rewritten to it.
+    std::vector<fpxg_req> dq(reqs, reqs + count);
+    std::vector<size_t> in_a(count), in_b(count), out_n(count), off_a(count), off_b(count), off_o(count);
+    size_t total = 0;
+    for (size_t i = 0; i < count; ++i) {
+      const fpxg_req& q = reqs[i];
+      const bool conv = q.op == FPXG_CONV;
+      in_a[i] = q.a ? std::min(q.la, conv ? q.n : q.n) : 0;
+      in_b[i] = conv && q.b ? std::min(q.lb, q.n) : 0;
+      out_n[i] = q.out ? std::min(q.out_cap, conv ? std::min(q.n, q.la + q.lb) : q.n) : 0;
+      off_a[i] = total; total += in_a[i];
+      off_b[i] = total; total += in_b[i];
+      off_o[i] = total; total += out_n[i];
+    }
+    DevBuf dev = c.buf(std::max<size_t>(total, 1));
+    uint32_t* d = dev.get();
+    std::vector<uint32_t> host(total);
+    for (size_t i = 0; i < count; ++i) {
+      const fpxg_req& q = reqs[i];
+      if (in_a[i]) memcpy(host.data() + off_a[i], q.a, in_a[i] * 4);
+      if (in_b[i]) memcpy(host.data() + off_b[i], q.b, in_b[i] * 4);
+      dq[i].a = q.a ? d + off_a[i] : nullptr;
+      dq[i].la = q.a ? in_a[i] : q.la;
+      dq[i].b = q.b ? d + off_b[i] : nullptr;
+      dq[i].lb = q.b ? in_b[i] : q.lb;
+      dq[i].out = q.out ? d + off_o[i] : nullptr;
+      dq[i].out_cap = q.out ? out_n[i] : q.out_cap;
+    }
+    if (total) FPXG_HIP(hipMemcpyAsync(d, host.data(), total * 4, hipMemcpyHostToDevice, c.stream));
+    batch_dev(c, dq.data(), count);
+    if (total) FPXG_HIP(hipMemcpyAsync(host.data(), d, total * 4, hipMemcpyDeviceToHost, c.stream));
+    FPXG_HIP(hipStreamSynchronize(c.stream));
+    for (size_t i = 0; i < count; ++i) {
+      reqs[i].status = dq[i].status;
+      reqs[i].out_len = dq[i].status == OK ? dq[i].out_len : 0;
+      if (reqs[i].out_len) memcpy(reqs[i].out, host.data() + off_o[i], reqs[i].out_len * 4);
+    }
   });
 }
 

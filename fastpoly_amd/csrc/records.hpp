@@ -1,0 +1,47 @@
+// Launch records shared by the immediate drivers and the merged batch
+// executor: every elementwise op, recurrence and CRT reconstruction is a small
+// POD record, so one launch can serve any number of them.
+#pragma once
+
+#include <stdint.h>
+
+namespace fpxg {
+
+struct FieldScalar {
+  uint32_t p, ninv, r2, one;
+};
+
+// Sequential recurrences (seeds and <= 8-coefficient tails).
+enum { REC_INV = 0, REC_EXP = 1, REC_SQRT = 2 };
+constexpr int kRecMax = 64;
+struct RecurRec {
+  int kind;
+  FieldScalar f;
+  const uint32_t* u;
+  uint32_t lu;
+  const uint32_t* w;
+  uint32_t lw;
+  uint32_t* v;
+  uint32_t lo, hi, init, k;
+};
+
+// Elementwise ops on Montgomery limbs.
+enum {
+  EW_FILL = 0,      // dst[i] = c
+  EW_NEGATE = 1,    // dst[i] = -src[i]
+  EW_SCALE = 2,     // dst[i] = src[i] * c (Montgomery limb c)
+  EW_MUL_RECIP = 3, // dst[i] = src[i] / (c + i)                    (exponential.hpp:194-195)
+  EW_DERIV = 4,     // dst[i] = src[i+1] * (i+1)                    (calculus.hpp:17-39)
+  EW_INTEGRAL = 5,  // dst[i] = i in [1, avail] ? src[i-1] / i : 0  (calculus.hpp:97-116)
+  EW_SUB_FROM = 6   // dst[i] = (i < avail ? src[i] : 0) - src2[i]  (square_root.hpp:93-94)
+};
+struct EwRec {
+  int op;
+  FieldScalar f;
+  uint32_t* dst;
+  const uint32_t* src;
+  const uint32_t* src2;
+  uint32_t n, avail, c;
+};
+
+}  // namespace fpxg

@@ -17,6 +17,7 @@
 #include "fpxg.h"
 #include "crt.hpp"
 #include "ntt2.hpp"
+#include "records.hpp"
 
 namespace fpxg {
 
@@ -89,6 +90,17 @@ class Arena {
   void release_idle();
   fpxg_stats stats() const;
   size_t capacity() const { return cap_; }
+  size_t in_use() const;
+  // While a deferral list is installed, frees are parked on it (a recorded
+  // schedule still references the memory); flush_deferred() releases them.
+  void set_deferral(std::vector<void*>* list) { defer_ = list; }
+  void flush_deferred(std::vector<void*>& list) {
+    std::vector<void*>* saved = defer_;
+    defer_ = nullptr;
+    for (void* p : list) free(p);
+    list.clear();
+    defer_ = saved;
+  }
 
  private:
   struct Chunk { char* base; size_t size, used; };
@@ -97,6 +109,7 @@ class Arena {
   std::vector<Chunk> chunks_;
   std::map<int, std::vector<void*>> free_;  // class -> blocks
   std::map<void*, int> owner_;              // block -> class
+  std::vector<void*>* defer_ = nullptr;
   mutable std::mutex mu_;
 };
 
@@ -136,6 +149,47 @@ struct PassPlan {
   int L, b, S, logC;  // rows of 2^b words; S column levels in tiles of 2^logC columns (S == 0: one row pass)
 };
 
+// One recorded kernel launch of a request's schedule.
+enum { LK_ROWS = 0, LK_COLS = 1, LK_EW = 2, LK_RECUR = 3, LK_GARNER = 4 };
+struct LaunchRec {
+  int kind;
+  int k0, k1, k2, k3;  // shape: rows (logM, lazy, mode, -), cols (S, lazy, inv, L)
+  RowJob2 row;
+  ColJob2 col;
+  EwRec ew;
+  RecurRec rec;
+  GarnerRec gar;
+};
+
+struct Recorder {
+  std::vector<LaunchRec> recs;
+  void push_rows(int logM, bool lazy, int mode, const RowJob2& j) {
+    LaunchRec r{};
+    r.kind = LK_ROWS; r.k0 = logM; r.k1 = lazy; r.k2 = mode; r.row = j;
+    recs.push_back(r);
+  }
+  void push_cols(int S, bool lazy, bool inv, int L, const ColJob2& j) {
+    LaunchRec r{};
+    r.kind = LK_COLS; r.k0 = S; r.k1 = lazy; r.k2 = inv; r.k3 = L; r.col = j;
+    recs.push_back(r);
+  }
+  void push_ew(const EwRec& e) {
+    LaunchRec r{};
+    r.kind = LK_EW; r.ew = e;
+    recs.push_back(r);
+  }
+  void push_recur(const RecurRec& e) {
+    LaunchRec r{};
+    r.kind = LK_RECUR; r.rec = e;
+    recs.push_back(r);
+  }
+  void push_garner(const GarnerRec& g) {
+    LaunchRec r{};
+    r.kind = LK_GARNER; r.gar = g;
+    recs.push_back(r);
+  }
+};
+
 struct Context {
   int device = 0;
   hipStream_t stream = nullptr;
@@ -145,6 +199,20 @@ struct Context {
   int single_log_max = 12;   // largest transform done as one row pass
   int cus = 256;
   std::map<int, PassPlan> plan_override;  // autotuned splits by log2 n
+  Recorder* rec = nullptr;                // non-null: launches are recorded, not issued
+  // Pinned host slots for job-array uploads of the merged executor.
+  struct Staging {
+    uint8_t* host = nullptr;
+    size_t cap = 0;
+    hipEvent_t ev = nullptr;
+    bool used = false;
+  };
+  static constexpr int kStagingSlots = 4;
+  Staging staging[kStagingSlots];
+  int staging_next = 0;
+  // Outstanding batch tickets (completion events).
+  std::map<uint64_t, hipEvent_t> tickets;
+  uint64_t next_ticket = 1;
   std::unique_ptr<Arena> arena;
   std::map<uint64_t, std::unique_ptr<HostField>> fields;
   std::recursive_mutex mu;

@@ -33,7 +33,7 @@ enum {
   FPXG_E_ARG = 6          /* null pointer / output capacity too small */
 };
 
-enum { FPXG_INV = 0, FPXG_LOG = 1, FPXG_EXP = 2, FPXG_SQRT = 3 };
+enum { FPXG_INV = 0, FPXG_LOG = 1, FPXG_EXP = 2, FPXG_SQRT = 3, FPXG_CONV = 4 };
 
 /* NttPlan variants (ntt.hpp:142-192). */
 enum {
@@ -102,6 +102,38 @@ void* fpxg_stream(void);
 /* HIP events on the library stream: record slot (0..15), elapsed ms a -> b. */
 int fpxg_event_record(int slot);
 int fpxg_event_elapsed(int slot_a, int slot_b, float* ms);
+
+/* ---- Batches ------------------------------------------------------------
+ * One request = one poly::conv or poly::inv/log/exp/sqrt call, with that
+ * call's arguments, result and error code (status).  A batch runs every
+ * request's schedule merged with the others': one launch serves the same
+ * stage of all requests of the same kernel shape, so thousands of uneven
+ * requests over different moduli cost a few hundred launches.  Results are
+ * bit-identical to the single calls; a request that fails (status != 0)
+ * writes nothing and does not affect the others.
+ */
+typedef struct {
+  int op;              /* FPXG_CONV or FPXG_INV / LOG / EXP / SQRT */
+  uint32_t p;          /* modulus: a prime below 2^31 */
+  const uint32_t* a;
+  size_t la;
+  const uint32_t* b;   /* FPXG_CONV only */
+  size_t lb;
+  size_t n;            /* conv: truncation (SIZE_MAX: full product); series: output length */
+  uint32_t* out;
+  size_t out_cap;      /* capacity of out in limbs */
+  size_t out_len;      /* written: limbs produced */
+  int status;          /* written: FPXG_OK or the single call's error code */
+} fpxg_req;
+
+/* Device-pointer batch (buffers from fpxg_dev_alloc or HIP on this device,
+ * inputs complete in stream order).  Validation and status/out_len happen
+ * before return; the computation is enqueued and *ticket names it. */
+int fpxg_batch_submit(fpxg_req* reqs, size_t count, uint64_t* ticket);
+/* Blocks until the batch named by ticket has completed. */
+int fpxg_wait(uint64_t ticket);
+/* Host-pointer batch: uploads, runs and downloads before returning. */
+int fpxg_batch_u32(fpxg_req* reqs, size_t count);
 
 #ifdef __cplusplus
 }
