@@ -50,6 +50,10 @@ def parse():
     ap.add_argument("--batch-convs", type=int, default=8192)
     ap.add_argument("--batch-series", type=int, default=1024)
     ap.add_argument("--batch-reps", type=int, default=3)
+    ap.add_argument("--no-serve", action="store_true", help="skip the serving leg")
+    ap.add_argument("--serve-requests", type=int, default=1000)
+    ap.add_argument("--serve-rate", type=float, default=1000.0, help="offered load, requests/s (Poisson)")
+    ap.add_argument("--serve-levels", type=int, default=2)
     return ap.parse_args()
 
 
@@ -177,9 +181,67 @@ def main():
 
     if not args.no_batch:
         result["batched"] = batched_leg(fa, args)
+    if not args.no_serve:
+        result["serving"] = serving_leg(fa, args)
     if not args.no_cpu:
         result["cpu_baseline"] = cpu_baseline(args, a, b, inv_in, log_in, exp_in, sq_host, gpu_out)
     print(json.dumps(result), flush=True)
+
+
+def serving_leg(fa, args):
+    """Poisson stream of conv 50% / inv 20% / log 10% / exp 10% / sqrt 10%
+    requests, lengths log-uniform in [2^10, 2^20], moduli 998244353 (70%) /
+    1000000007 (30%), host inputs, submitted at their arrival times to the
+    continuous-batching server.  Latency = completion - submission on the
+    server's clock (includes upload, queueing, compute and download)."""
+    from fastpoly_amd.workloads import serving_stream
+
+    stream = serving_stream(args.serve_requests, args.serve_rate)
+    reqs = []
+    for _, t in stream:
+        reqs.append(fa.Request("conv", t[1], t[2], b=t[3]) if t[0] == "conv" else fa.Request(t[0], t[1], t[2], n=t[3]))
+    out_coeffs = sum(r.out_capacity() for r in reqs)
+    with fa.Server(levels_per_iteration=args.serve_levels) as srv:
+        # Warm-up: one request of each kind (fields, plans, pinned blocks).
+        for r in reqs[:16]:
+            srv.wait(srv.submit(r))
+        t0 = srv.now() + 0.01
+        ids, lag = [], []
+        for (arr, _), r in zip(stream, reqs):
+            due = t0 + arr
+            while True:
+                now = srv.now()
+                if now >= due:
+                    break
+                if due - now > 0.0005:
+                    time.sleep(due - now - 0.0003)
+            ids.append(srv.submit(r))
+            lag.append(srv.now() - due)
+        res = [srv.wait(i) for i in ids]
+        st = srv.stats()
+    lat = np.array([t["latency_s"] for _, t in res]) * 1e3
+    first = min(t["submit_s"] for _, t in res)
+    last = max(t["done_s"] for _, t in res)
+    errors = sum(isinstance(g, Exception) for g, _ in res)
+    # Each result against the same request run alone (single call, GPU).
+    exact = True
+    for r, (g, _) in zip(reqs, res):
+        want = fa.conv(r.p, r.a, r.b) if r.op == "conv" else fa.series(r.op, r.p, r.a, r.n)
+        if isinstance(g, Exception) or not np.array_equal(g, want):
+            exact = False
+            break
+    span = last - first
+    return {
+        "workload": "Poisson stream: conv 50% / inv 20% / log 10% / exp 10% / sqrt 10%, lengths log-uniform "
+                    "[2^10,2^20], moduli 998244353 70% / 1000000007 30%, host inputs",
+        "requests": len(reqs), "offered_rate_per_s": args.serve_rate,
+        "achieved_rate_per_s": round(len(reqs) / span, 1), "output_coeffs": int(out_coeffs),
+        "coeffs_per_s": out_coeffs / span, "p50_ms": round(float(np.percentile(lat, 50)), 3),
+        "p99_ms": round(float(np.percentile(lat, 99)), 3), "mean_ms": round(float(lat.mean()), 3),
+        "max_ms": round(float(lat.max()), 3), "submit_lag_p99_ms": round(float(np.percentile(lag, 99)) * 1e3, 3),
+        "errors": errors, "identical_to_single_calls": exact, "levels_per_iteration": args.serve_levels,
+        "server": st,
+    }
 
 
 def batched_leg(fa, args):

@@ -22,13 +22,13 @@ __all__ = [
     "DomainError", "NttSizeError", "DeviceError", "UnsupportedModulus", "lib", "lib_path",
     "conv", "inv", "log", "exp", "sqrt", "series", "ntt_forward", "ntt_inverse", "NttPlan",
     "release_scratch_memory", "scratch_memory_stats", "from_int", "to_int", "Mont", "DeviceArray",
-    "EXPORTED_SYMBOLS", "Request", "batch", "batch_submit", "wait",
+    "EXPORTED_SYMBOLS", "Request", "batch", "batch_submit", "wait", "Server",
 ]
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_NAME = "libfpxg.so"
 
-OK, E_DOMAIN, E_SIZE, E_NOMEM, E_DEVICE, E_UNSUPPORTED, E_ARG = range(7)
+OK, E_DOMAIN, E_SIZE, E_NOMEM, E_DEVICE, E_UNSUPPORTED, E_ARG, E_BUSY = range(8)
 OPS = {"inv": 0, "log": 1, "exp": 2, "sqrt": 3, "conv": 4}
 NTT_FORWARD, NTT_FORWARD_PREFIX, NTT_INVERSE, NTT_INVERSE_PREFIX, NTT_INVERSE_HIGHHALF = range(5)
 SIZE_MAX = (1 << 64) - 1
@@ -40,6 +40,8 @@ EXPORTED_SYMBOLS = [
     "fpxg_series_dev", "fpxg_ntt_dev", "fpxg_dev_alloc", "fpxg_dev_free", "fpxg_memcpy_h2d",
     "fpxg_memcpy_d2h", "fpxg_synchronize", "fpxg_stream", "fpxg_event_record", "fpxg_event_elapsed",
     "fpxg_ntt_pass_dev", "fpxg_ntt_plan_info", "fpxg_batch_submit", "fpxg_wait", "fpxg_batch_u32",
+    "fpxg_server_create", "fpxg_server_submit", "fpxg_server_wait", "fpxg_server_poll", "fpxg_server_now",
+    "fpxg_server_stats_get", "fpxg_server_destroy",
 ]
 
 
@@ -89,6 +91,19 @@ class Req(ctypes.Structure):
                 ("status", ctypes.c_int)]
 
 
+class Result(ctypes.Structure):
+    """fpxg_result (include/fpxg.h)."""
+    _fields_ = [("status", ctypes.c_int), ("out_len", ctypes.c_size_t), ("submit_s", ctypes.c_double),
+                ("admit_s", ctypes.c_double), ("done_s", ctypes.c_double)]
+
+
+class ServerStats(ctypes.Structure):
+    """fpxg_server_stats (include/fpxg.h)."""
+    _fields_ = [("submitted", ctypes.c_uint64), ("completed", ctypes.c_uint64), ("iterations", ctypes.c_uint64),
+                ("launches", ctypes.c_uint64), ("max_active", ctypes.c_uint64), ("max_queue", ctypes.c_uint64),
+                ("peak_in_use_bytes", ctypes.c_size_t)]
+
+
 def lib_path() -> str:
     return os.path.join(HERE, LIB_NAME)
 
@@ -135,6 +150,16 @@ def lib():
         L.fpxg_batch_submit.argtypes = [ctypes.POINTER(Req), ctypes.c_size_t, ctypes.POINTER(ctypes.c_uint64)]
         L.fpxg_wait.argtypes = [ctypes.c_uint64]
         L.fpxg_batch_u32.argtypes = [ctypes.POINTER(Req), ctypes.c_size_t]
+        L.fpxg_server_create.restype = ctypes.c_void_p
+        L.fpxg_server_create.argtypes = [ctypes.c_size_t, ctypes.c_int]
+        L.fpxg_server_submit.argtypes = [ctypes.c_void_p, ctypes.POINTER(Req), ctypes.POINTER(ctypes.c_uint64)]
+        L.fpxg_server_wait.argtypes = [ctypes.c_void_p, ctypes.c_uint64, ctypes.POINTER(Result)]
+        L.fpxg_server_poll.argtypes = [ctypes.c_void_p, ctypes.c_uint64, ctypes.POINTER(Result)]
+        L.fpxg_server_now.restype = ctypes.c_double
+        L.fpxg_server_now.argtypes = [ctypes.c_void_p]
+        L.fpxg_server_stats_get.argtypes = [ctypes.c_void_p, ctypes.POINTER(ServerStats)]
+        L.fpxg_server_destroy.restype = None
+        L.fpxg_server_destroy.argtypes = [ctypes.c_void_p]
         _lib = L
     return _lib
 
@@ -447,3 +472,79 @@ def batch_submit(reqs) -> int:
 
 def wait(ticket: int) -> None:
     _check(lib().fpxg_wait(ticket), "wait")
+
+
+# ---------------------------------------------------------------------------
+# Serving: continuous batching over host-pointer requests.
+# ---------------------------------------------------------------------------
+class Server:
+    """fpxg_server: submit() returns an id at once; wait(id) returns
+    (result array or exception, timings dict) once the request completes."""
+
+    def __init__(self, hbm_budget_bytes: int = 0, levels_per_iteration: int = 0):
+        self._h = lib().fpxg_server_create(hbm_budget_bytes, levels_per_iteration)
+        if not self._h:
+            _check(E_DEVICE, "server_create")
+        self._live = {}
+
+    def now(self) -> float:
+        return lib().fpxg_server_now(self._h)
+
+    def submit(self, r: "Request") -> int:
+        cap = r.out_capacity()
+        out = np.zeros(max(cap, 1), dtype=np.uint32)
+        q = Req()
+        q.op, q.p = OPS[r.op], r.p
+        q.a, q.la = (r.a.ctypes.data if r.a.size else None), r.a.size
+        if r.b is not None:
+            q.b, q.lb = (r.b.ctypes.data if r.b.size else None), r.b.size
+        q.n, q.out, q.out_cap = r.n, out.ctypes.data, cap
+        rid = ctypes.c_uint64(0)
+        _check(lib().fpxg_server_submit(self._h, ctypes.byref(q), ctypes.byref(rid)), "server_submit")
+        self._live[rid.value] = (r, q, out)  # keeps every buffer alive until completion
+        return rid.value
+
+    def _collect(self, rid: int, res: Result):
+        r, q, out = self._live.pop(rid)
+        t = {"submit_s": res.submit_s, "admit_s": res.admit_s, "done_s": res.done_s,
+             "latency_s": res.done_s - res.submit_s}
+        if res.status == OK:
+            return out[: res.out_len].copy(), t
+        msg = fpxg_strerror(res.status)
+        return (MemoryError(msg) if res.status == E_NOMEM else _ERRORS.get(res.status, FpxgError)(msg)), t
+
+    def wait(self, rid: int):
+        res = Result()
+        _check(lib().fpxg_server_wait(self._h, rid, ctypes.byref(res)), "server_wait")
+        return self._collect(rid, res)
+
+    def poll(self, rid: int):
+        """None while pending, else as wait()."""
+        res = Result()
+        rc = lib().fpxg_server_poll(self._h, rid, ctypes.byref(res))
+        if rc == E_BUSY:
+            return None
+        _check(rc, "server_poll")
+        return self._collect(rid, res)
+
+    def stats(self) -> dict:
+        s = ServerStats()
+        _check(lib().fpxg_server_stats_get(self._h, ctypes.byref(s)), "server_stats")
+        return {name: getattr(s, name) for name, _ in ServerStats._fields_}
+
+    def close(self) -> None:
+        if self._h:
+            lib().fpxg_server_destroy(self._h)
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -657,6 +657,7 @@ void conv_dev(Context& c, const HostField& T, const uint32_t* a, size_t la, cons
 
 #include "series.hpp"
 #include "batch.hpp"
+#include "serve.hpp"
 
 // ===========================================================================
 // C ABI
@@ -749,6 +750,7 @@ const char* fpxg_strerror(int code) {
     case E_DEVICE: return "HIP device error";
     case E_UNSUPPORTED: return "unsupported modulus";
     case E_ARG: return "invalid argument";
+    case E_BUSY: return "request still pending";
     default: return "unknown error";
   }
 }
@@ -1046,5 +1048,77 @@ int fpxg_batch_u32(fpxg_req* reqs, size_t count) {
     }
   });
 }
+
+// ---- serving ----
+
+}  // extern "C"
+
+struct fpxg_server {
+  std::unique_ptr<fpxg::Server> s;
+};
+
+extern "C" {
+
+fpxg_server* fpxg_server_create(size_t hbm_budget_bytes, int levels_per_iteration) {
+  fpxg_server* out = nullptr;
+  const int rc = guarded([&](Context& c) {
+    const size_t budget = hbm_budget_bytes ? hbm_budget_bytes : c.arena->capacity() / 2;
+    const int lv = levels_per_iteration > 0 ? levels_per_iteration : 2;
+    auto h = std::make_unique<fpxg_server>();
+    h->s = std::make_unique<Server>(c, budget, lv);
+    out = h.release();
+  });
+  return rc == OK ? out : nullptr;
+}
+
+int fpxg_server_submit(fpxg_server* h, fpxg_req* req, uint64_t* id) {
+  if (!h || !req || !id) return E_ARG;
+  Server& S = *h->s;
+  auto it = std::make_unique<SItem>();
+  it->user = req;
+  it->t_submit = S.now();
+  {
+    std::lock_guard<std::mutex> g(S.mu);
+    if (!S.fatal.empty() || S.stopping) {
+      t_last_error = S.fatal.empty() ? "server is stopping" : S.fatal;
+      return E_DEVICE;
+    }
+    it->id = S.next_id++;
+    *id = it->id;
+    S.outstanding.insert(it->id);
+    ++S.st.submitted;
+    S.incoming.push_back(std::move(it));
+  }
+  S.cv_work.notify_one();
+  return OK;
+}
+
+static int server_collect(fpxg_server* h, uint64_t id, fpxg_result* res, bool block) {
+  if (!h) return E_ARG;
+  Server& S = *h->s;
+  std::unique_lock<std::mutex> g(S.mu);
+  if (!S.outstanding.count(id)) return E_ARG;
+  if (block) S.cv_done.wait(g, [&] { return S.finished.count(id) > 0; });
+  auto f = S.finished.find(id);
+  if (f == S.finished.end()) return E_BUSY;
+  if (res) *res = f->second;
+  S.finished.erase(f);
+  S.outstanding.erase(id);
+  return OK;
+}
+
+int fpxg_server_wait(fpxg_server* h, uint64_t id, fpxg_result* res) { return server_collect(h, id, res, true); }
+int fpxg_server_poll(fpxg_server* h, uint64_t id, fpxg_result* res) { return server_collect(h, id, res, false); }
+
+double fpxg_server_now(fpxg_server* h) { return h ? h->s->now() : 0.0; }
+
+int fpxg_server_stats_get(fpxg_server* h, fpxg_server_stats* out) {
+  if (!h || !out) return E_ARG;
+  std::lock_guard<std::mutex> g(h->s->mu);
+  *out = h->s->st;
+  return OK;
+}
+
+void fpxg_server_destroy(fpxg_server* h) { delete h; }
 
 }  // extern "C"

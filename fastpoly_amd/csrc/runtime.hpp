@@ -28,7 +28,7 @@ struct Error : std::runtime_error {
   int code;
   Error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
 };
-enum { OK = 0, E_DOMAIN = 1, E_SIZE = 2, E_NOMEM = 3, E_DEVICE = 4, E_UNSUPPORTED = 5, E_ARG = 6 };
+enum { OK = 0, E_DOMAIN = 1, E_SIZE = 2, E_NOMEM = 3, E_DEVICE = 4, E_UNSUPPORTED = 5, E_ARG = 6, E_BUSY = 7 };
 
 #define FPXG_HIP(expr)                                                                    \
   do {                                                                                    \

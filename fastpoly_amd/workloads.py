@@ -35,9 +35,9 @@ def _coeffs(p: int, n: int, rng) -> np.ndarray:
 
 
 def series_input(op: str, p: int, n: int, rng) -> np.ndarray:
-    """A valid input for op at length n: inv has a0 in [1, p), log a0 = 1,
-    exp a0 = 0, sqrt a perfect square d^2 mod x^n with d0 = 1 (built by the
-    caller's conv; here d itself is returned with d0 = 1 and the flag set)."""
+    """A valid input for op at length n, other coefficients uniform: inv has
+    a0 uniform in [1, p), log a0 = 1, exp a0 = 0, sqrt a0 = 1 (a residue, so
+    the power-series square root exists)."""
     a = _coeffs(p, n, rng)
     if op == "inv":
         a[0] = int(rng.integers(1, p))
@@ -72,10 +72,7 @@ SERVE_MIX = (("conv", 0.5), ("inv", 0.2), ("log", 0.1), ("exp", 0.1), ("sqrt", 0
 
 
 def serving_stream(count: int, rate_per_s: float, seed: int = 20261204, length_log=(10, 20)):
-    """(arrival_s, request) pairs, arrivals Poisson at rate_per_s.  A sqrt
-    request carries d (d0 = 1); its input is d^2 mod x^n, formed on the device
-    by the caller before the timed stream (the input is a property of the
-    request, not of the schedule)."""
+    """(arrival_s, request) pairs, arrivals Poisson at rate_per_s."""
     rng = np.random.default_rng((seed, 2))
     t = 0.0
     out = []

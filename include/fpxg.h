@@ -30,7 +30,8 @@ enum {
   FPXG_E_NOMEM = 3,       /* std::bad_alloc: device arena exhausted */
   FPXG_E_DEVICE = 4,      /* HIP runtime failure (std::runtime_error) */
   FPXG_E_UNSUPPORTED = 5, /* modulus not prime or outside [2, 2^31) */
-  FPXG_E_ARG = 6          /* null pointer / output capacity too small */
+  FPXG_E_ARG = 6,         /* null pointer / output capacity too small */
+  FPXG_E_BUSY = 7         /* fpxg_server_poll: the request has not completed yet */
 };
 
 enum { FPXG_INV = 0, FPXG_LOG = 1, FPXG_EXP = 2, FPXG_SQRT = 3, FPXG_CONV = 4 };
@@ -134,6 +135,41 @@ int fpxg_batch_submit(fpxg_req* reqs, size_t count, uint64_t* ticket);
 int fpxg_wait(uint64_t ticket);
 /* Host-pointer batch: uploads, runs and downloads before returning. */
 int fpxg_batch_u32(fpxg_req* reqs, size_t count);
+
+/* ---- Serving -------------------------------------------------------------
+ * A request server with continuous batching: a scheduler thread admits
+ * submitted requests while the device arena stays within hbm_budget_bytes,
+ * and runs every admitted request's schedule merged with the others', a few
+ * levels (levels_per_iteration) per loop, so requests that arrive while others
+ * run join their launches.  Requests carry HOST pointers; the caller keeps a,
+ * b, out and the fpxg_req itself valid until the request completes, when
+ * status and out_len are written into it and out holds the result.  Results
+ * are bit-identical to the single calls.  Destroy servers before
+ * fpxg_shutdown.
+ */
+typedef struct fpxg_server fpxg_server;
+typedef struct {
+  int status;
+  size_t out_len;
+  double submit_s, admit_s, done_s; /* fpxg_server_now() clock */
+} fpxg_result;
+typedef struct {
+  uint64_t submitted, completed, iterations, launches, max_active, max_queue;
+  size_t peak_in_use_bytes;
+} fpxg_server_stats;
+
+/* hbm_budget_bytes 0: half the arena; levels_per_iteration 0: 2. */
+fpxg_server* fpxg_server_create(size_t hbm_budget_bytes, int levels_per_iteration);
+int fpxg_server_submit(fpxg_server* s, fpxg_req* req, uint64_t* id);
+/* Blocks until request id completes; fills *res (may be null). */
+int fpxg_server_wait(fpxg_server* s, uint64_t id, fpxg_result* res);
+/* FPXG_E_BUSY while request id is pending; else as fpxg_server_wait. */
+int fpxg_server_poll(fpxg_server* s, uint64_t id, fpxg_result* res);
+/* Seconds since the server was created (the clock of fpxg_result). */
+double fpxg_server_now(fpxg_server* s);
+int fpxg_server_stats_get(fpxg_server* s, fpxg_server_stats* out);
+/* Drains outstanding requests, stops the scheduler, frees the server. */
+void fpxg_server_destroy(fpxg_server* s);
 
 #ifdef __cplusplus
 }
