@@ -51,6 +51,8 @@ def parse():
     ap.add_argument("--batch-series", type=int, default=1024)
     ap.add_argument("--batch-reps", type=int, default=3)
     ap.add_argument("--no-serve", action="store_true", help="skip the serving leg")
+    ap.add_argument("--ntt-only", action="store_true",
+                    help="only the 2^24 transform passes and a 64 MiB device copy (for counter runs)")
     ap.add_argument("--serve-requests", type=int, default=1000)
     ap.add_argument("--serve-rate", type=float, default=1000.0, help="offered load, requests/s (Poisson)")
     ap.add_argument("--serve-levels", type=int, default=2)
@@ -98,6 +100,9 @@ def main():
     import fastpoly_amd as fa
 
     fa.init(0, 0)
+    if args.ntt_only:
+        ntt_only(fa, args)
+        return
     rng = np.random.default_rng(20261203)
     a, b, inv_in, log_in, exp_in, d = make_inputs(rng)
     D = fa.DeviceArray
@@ -186,6 +191,28 @@ def main():
     if not args.no_cpu:
         result["cpu_baseline"] = cpu_baseline(args, a, b, inv_in, log_in, exp_in, sq_host, gpu_out)
     print(json.dumps(result), flush=True)
+
+
+def ntt_only(fa, args):
+    """The two passes of the 2^24-point transform (args.ntt_reps each) and
+    args.ntt_reps device-to-device copies of the same 64 MiB: the copy is the
+    calibration kernel for the FETCH_SIZE / WRITE_SIZE counters."""
+    import ctypes
+    n = 1 << 24
+    rng = np.random.default_rng(1)
+    x = fa.DeviceArray.from_host(rng.integers(0, AUX, n, dtype=np.uint32))
+    y = fa.DeviceArray(n)
+    for pidx in (0, 1):
+        for _ in range(args.ntt_reps):
+            fa.ntt_pass_dev(AUX, x, pidx)
+        fa.synchronize()
+    hip = ctypes.CDLL("libamdhip64.so")
+    hip.hipMemcpyDtoD.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t]
+    for _ in range(args.ntt_reps):
+        if hip.hipMemcpyDtoD(y.ptr, x.ptr, 4 * n) != 0:
+            raise RuntimeError("hipMemcpyDtoD failed")
+    hip.hipDeviceSynchronize()
+    print(json.dumps({"ntt_only": True, "n": n, "reps": args.ntt_reps, "copy_bytes": 4 * n}), flush=True)
 
 
 def serving_leg(fa, args):

@@ -346,11 +346,14 @@ static void launch_rows(Context& c, bool lazy, int mode, const RowJob2& j, int l
   FPXG_HIP(hipGetLastError());
 }
 
-static int tile_log_cols(int S) { return std::min(4, 14 - S); }
+// Column tiles: 16 columns up to 2^9 levels, 8 for 2^10 and 2^11 (a 2^10 x 8
+// tile leaves room for four workgroups per CU), 4 for 2^12.
+static constexpr int col_tile_log(int S) { return S <= 9 ? 4 : (S <= 11 ? 3 : 2); }
+static int tile_log_cols(int S) { return col_tile_log(S); }
 
 template <int S, bool LAZY, bool INV>
 static void cols2_launch(Context& c, const ColJob2& j, const PassPlan& P, uint32_t njobs, const ColJob2* dj) {
-  constexpr int LOGC = S <= 10 ? 4 : 14 - S;
+  constexpr int LOGC = col_tile_log(S);
   using CS = ColShape<S, LOGC>;
   const size_t lds = (size_t)CS::words * 4;
   dim3 grid(1u << (P.b - LOGC), njobs), block(CS::T);
@@ -393,7 +396,7 @@ static bool rows2_attr() {
 }
 template <int S, bool LAZY, bool INV>
 static bool cols2_attr() {
-  constexpr int LOGC = S <= 10 ? 4 : 14 - S;
+  constexpr int LOGC = col_tile_log(S);
   return hipFuncSetAttribute((const void*)k_cols2<S, LOGC, LAZY, INV>, hipFuncAttributeMaxDynamicSharedMemorySize,
                              (int)(ColShape<S, LOGC>::words * 4)) == hipSuccess;
 }

@@ -118,24 +118,52 @@ __device__ __forceinline__ uint32_t group_lo(uint32_t gid) {
   constexpr int lo_bits = LOGB - s - r;
   return gid & ((1u << lo_bits) - 1);
 }
-// Number of groups a thread owns in phase P, and its g-th group id.
-template <int LOGB, int P>
+// Number of groups a thread owns in phase P, and its u-th group id.  Threads
+// of a unit are numbered g < GT; a wave holds WG consecutive g (64 / C of a
+// column tile, 64 of a long row, or the whole unit when it fits a wave).  In a
+// phase with NG > 1 groups per thread, the WG threads of a wave take WG * NG
+// consecutive group ids, so a wave's groups cover whole sub-blocks whenever
+// possible and the phase transitions around them stay inside the wave.
+template <int LOGB, int P, int LOGC = 0>
 struct PhaseGroups {
   static constexpr int r = Sched<LOGB>::r_of(P);
   static constexpr int NG = LOGB >= 4 ? (1 << (4 - r)) : 1;
-  __device__ static __forceinline__ uint32_t gid(uint32_t g, int u) { return (uint32_t)u * Sched<LOGB>::GT + g; }
+  static constexpr int GT = Sched<LOGB>::GT;
+  static constexpr int WG = (64 >> LOGC) < GT ? (64 >> LOGC) : GT;
+  __device__ static __forceinline__ uint32_t gid(uint32_t g, int u) {
+    if constexpr (NG == 1) return g;
+    else return (g / WG) * (uint32_t)(WG * NG) + (uint32_t)u * WG + (g % WG);
+  }
 };
 
+// Whether the exchange between phases P-1 and P stays inside each wave: true
+// when a wave's groups of phase P-1 cover whole 2^(LOGB - s(P-1)) sub-blocks
+// (then its phase-P groups are the same elements), or the unit fits a wave.
+template <int LOGB, int P, int LOGC>
+struct WaveLocal {
+  using G = PhaseGroups<LOGB, P - 1, LOGC>;
+  static constexpr int lb = LOGB - Sched<LOGB>::s_of(P - 1) - Sched<LOGB>::r_of(P - 1);
+  static constexpr bool value = (Sched<LOGB>::GT << LOGC) <= 64 || (G::WG * G::NG >= (1 << lb));
+};
+
+// Orders one wave's LDS accesses (a wave's LDS operations execute in order;
+// this only keeps the compiler from moving them across the exchange).
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 // Compute phase P on registers (forward or inverse).
-template <int LOGB, int P, bool INV, bool LAZY>
+template <int LOGB, int P, bool INV, bool LAZY, int LOGC = 0>
 __device__ __forceinline__ void phase_compute(uint32_t* x, uint32_t g, const uint32_t* planes, const Arith<LAZY>& A) {
   constexpr int r = Sched<LOGB>::r_of(P);
-  constexpr int NG = PhaseGroups<LOGB, P>::NG;
+  constexpr int NG = PhaseGroups<LOGB, P, LOGC>::NG;
   const uint32_t* plane = planes + plane_off(LOGB - Sched<LOGB>::s_of(P), r);
 #pragma unroll
   for (int u = 0; u < NG; ++u) {
     uint32_t tw[16];
-    load_tw<r>(tw, plane, group_lo<LOGB, P>(PhaseGroups<LOGB, P>::gid(g, u)));
+    load_tw<r>(tw, plane, group_lo<LOGB, P>(PhaseGroups<LOGB, P, LOGC>::gid(g, u)));
     if constexpr (INV) dit_block<r, LAZY>(x + u * (1 << r), tw, A);
     else dif_block<r, LAZY>(x + u * (1 << r), tw, A);
   }
@@ -170,10 +198,10 @@ template <int LOGB, int P, int LOGC>
 __device__ __forceinline__ void phase_to_lds(uint32_t* X, const uint32_t* x, uint32_t g, uint32_t c) {
   using LM = LdsMap<LOGB, P, LOGC>;
   constexpr int r = LM::r;
-  constexpr int NG = PhaseGroups<LOGB, P>::NG;
+  constexpr int NG = PhaseGroups<LOGB, P, LOGC>::NG;
 #pragma unroll
   for (int u = 0; u < NG; ++u) {
-    const uint32_t gid = PhaseGroups<LOGB, P>::gid(g, u);
+    const uint32_t gid = PhaseGroups<LOGB, P, LOGC>::gid(g, u);
     if constexpr (LM::affine) {
       uint32_t* b = X + LM::base(gid, c);
 #pragma unroll
@@ -188,10 +216,10 @@ template <int LOGB, int P, int LOGC>
 __device__ __forceinline__ void phase_from_lds(uint32_t* x, const uint32_t* X, uint32_t g, uint32_t c) {
   using LM = LdsMap<LOGB, P, LOGC>;
   constexpr int r = LM::r;
-  constexpr int NG = PhaseGroups<LOGB, P>::NG;
+  constexpr int NG = PhaseGroups<LOGB, P, LOGC>::NG;
 #pragma unroll
   for (int u = 0; u < NG; ++u) {
-    const uint32_t gid = PhaseGroups<LOGB, P>::gid(g, u);
+    const uint32_t gid = PhaseGroups<LOGB, P, LOGC>::gid(g, u);
     if constexpr (LM::affine) {
       const uint32_t* b = X + LM::base(gid, c);
 #pragma unroll
@@ -210,11 +238,18 @@ template <int LOGB, int P, int END, int LOGC, bool LAZY>
 __device__ __forceinline__ void fwd_phases(uint32_t* x, uint32_t* X, uint32_t g, uint32_t c, const uint32_t* planes,
                                            const Arith<LAZY>& A) {
   if constexpr (P < END) {
-    phase_to_lds<LOGB, P - 1, LOGC>(X, x, g, c);
-    __syncthreads();
-    phase_from_lds<LOGB, P, LOGC>(x, X, g, c);
-    __syncthreads();
-    phase_compute<LOGB, P, false, LAZY>(x, g, planes, A);
+    if constexpr (WaveLocal<LOGB, P, LOGC>::value) {
+      wave_sync();
+      phase_to_lds<LOGB, P - 1, LOGC>(X, x, g, c);
+      wave_sync();
+      phase_from_lds<LOGB, P, LOGC>(x, X, g, c);
+    } else {
+      if constexpr (P > 1) __syncthreads();  // earlier exchanges may still be reading
+      phase_to_lds<LOGB, P - 1, LOGC>(X, x, g, c);
+      __syncthreads();
+      phase_from_lds<LOGB, P, LOGC>(x, X, g, c);
+    }
+    phase_compute<LOGB, P, false, LAZY, LOGC>(x, g, planes, A);
     fwd_phases<LOGB, P + 1, END, LOGC, LAZY>(x, X, g, c, planes, A);
   }
 }
@@ -223,11 +258,18 @@ template <int LOGB, int P, int LOGC, bool LAZY>
 __device__ __forceinline__ void inv_phases(uint32_t* x, uint32_t* X, uint32_t g, uint32_t c, const uint32_t* planes,
                                            const Arith<LAZY>& A) {
   if constexpr (P >= 0) {
-    phase_to_lds<LOGB, P + 1, LOGC>(X, x, g, c);
-    __syncthreads();
-    phase_from_lds<LOGB, P, LOGC>(x, X, g, c);
-    __syncthreads();
-    phase_compute<LOGB, P, true, LAZY>(x, g, planes, A);
+    if constexpr (WaveLocal<LOGB, P + 1, LOGC>::value) {
+      wave_sync();
+      phase_to_lds<LOGB, P + 1, LOGC>(X, x, g, c);
+      wave_sync();
+      phase_from_lds<LOGB, P, LOGC>(x, X, g, c);
+    } else {
+      __syncthreads();  // a forward pass or an earlier exchange may still be reading
+      phase_to_lds<LOGB, P + 1, LOGC>(X, x, g, c);
+      __syncthreads();
+      phase_from_lds<LOGB, P, LOGC>(x, X, g, c);
+    }
+    phase_compute<LOGB, P, true, LAZY, LOGC>(x, g, planes, A);
     inv_phases<LOGB, P - 1, LOGC, LAZY>(x, X, g, c, planes, A);
   }
 }
@@ -414,23 +456,23 @@ __global__ void __launch_bounds__((ColShape<S, LOGC>::T)) k_cols2(ColJob2 j0, co
     constexpr int r0 = SC::r_of(0);
     const uint32_t last = J.in_count ? J.in_count - 1 : 0;
 #pragma unroll
-    for (int u = 0; u < PhaseGroups<S, 0>::NG; ++u)
+    for (int u = 0; u < PhaseGroups<S, 0, LOGC>::NG; ++u)
 #pragma unroll
       for (int k = 0; k < (1 << r0); ++k) {
-        const uint32_t e = elem<S, 0>(PhaseGroups<S, 0>::gid(g, u), k);
+        const uint32_t e = elem<S, 0>(PhaseGroups<S, 0, LOGC>::gid(g, u), k);
         const uint32_t gi = e * M + col;
         uint32_t v = J.in_count ? gload(J.src + min(gi, last)) : 0u;
         if (lift) v = A.mul(v, J.f.r2);
         x[u * (1 << r0) + k] = gi < J.in_count ? v : 0u;
       }
-    phase_compute<S, 0, false, LAZY>(x, g, J.f.planes_fwd, A);
+    phase_compute<S, 0, false, LAZY, LOGC>(x, g, J.f.planes_fwd, A);
     fwd_phases<S, 1, NPH, LOGC, LAZY>(x, X, g, c, J.f.planes_fwd, A);
     constexpr int rl = SC::r_of(PL);
 #pragma unroll
-    for (int u = 0; u < PhaseGroups<S, PL>::NG; ++u)
+    for (int u = 0; u < PhaseGroups<S, PL, LOGC>::NG; ++u)
 #pragma unroll
       for (int k = 0; k < (1 << rl); ++k) {
-        const uint32_t e = elem<S, PL>(PhaseGroups<S, PL>::gid(g, u), k);
+        const uint32_t e = elem<S, PL>(PhaseGroups<S, PL, LOGC>::gid(g, u), k);
         const uint32_t xx = (col * bitrev(e, S)) << shift;
         uint32_t v = A.mul(x[u * (1 << rl) + k], twiddle<LAZY>(A, J.f.tw_lo, J.f.tw_hi, J.f.split, xx));
         if (flags & R2_CANON) v = A.canon(v);
@@ -439,22 +481,22 @@ __global__ void __launch_bounds__((ColShape<S, LOGC>::T)) k_cols2(ColJob2 j0, co
   } else {
     constexpr int rl = SC::r_of(PL);
 #pragma unroll
-    for (int u = 0; u < PhaseGroups<S, PL>::NG; ++u)
+    for (int u = 0; u < PhaseGroups<S, PL, LOGC>::NG; ++u)
 #pragma unroll
       for (int k = 0; k < (1 << rl); ++k) {
-        const uint32_t e = elem<S, PL>(PhaseGroups<S, PL>::gid(g, u), k);
+        const uint32_t e = elem<S, PL>(PhaseGroups<S, PL, LOGC>::gid(g, u), k);
         const uint32_t xx = (col * bitrev(e, S)) << shift;
         const uint32_t tw = A.mul(twiddle<LAZY>(A, J.f.tw_lo_inv, J.f.tw_hi_inv, J.f.split, xx), J.inv_n);
         x[u * (1 << rl) + k] = A.mul(gload(J.src + e * M + col), tw);
       }
-    phase_compute<S, PL, true, LAZY>(x, g, J.f.planes_inv, A);
+    phase_compute<S, PL, true, LAZY, LOGC>(x, g, J.f.planes_inv, A);
     inv_phases<S, PL - 1, LOGC, LAZY>(x, X, g, c, J.f.planes_inv, A);
     constexpr int r0 = SC::r_of(0);
 #pragma unroll
-    for (int u = 0; u < PhaseGroups<S, 0>::NG; ++u)
+    for (int u = 0; u < PhaseGroups<S, 0, LOGC>::NG; ++u)
 #pragma unroll
       for (int k = 0; k < (1 << r0); ++k) {
-        const uint32_t e = elem<S, 0>(PhaseGroups<S, 0>::gid(g, u), k);
+        const uint32_t e = elem<S, 0>(PhaseGroups<S, 0, LOGC>::gid(g, u), k);
         const uint32_t gi = e * M + col;
         uint32_t v = A.canon(x[u * (1 << r0) + k]);
         if (flags & R2_NEG) v = v == 0 ? 0u : J.f.p - v;
