@@ -52,7 +52,9 @@ __device__ __forceinline__ void load_tw(uint32_t* tw, const uint32_t* plane, uin
   }
 }
 
-template <int r, bool LAZY>
+// LO0: the group's offset lo is 0 (last phase of a block), so every level's
+// first twiddle is w^0 = 1 and its products are skipped.
+template <int r, bool LAZY, bool LO0 = false>
 __device__ __forceinline__ void dif_block(uint32_t* x, const uint32_t* tw, const Arith<LAZY>& A) {
   int idx = 0;
 #pragma unroll
@@ -63,13 +65,14 @@ __device__ __forceinline__ void dif_block(uint32_t* x, const uint32_t* tw, const
       if (k & h) continue;
       const uint32_t a0 = x[k], a1 = x[k + h];
       x[k] = A.add(a0, a1);
-      x[k + h] = A.mul(a0 - a1 + A.R, tw[idx + (k & (h - 1))]);
+      if (LO0 && (k & (h - 1)) == 0) x[k + h] = A.sub(a0, a1);
+      else x[k + h] = A.mul(a0 - a1 + A.R, tw[idx + (k & (h - 1))]);
     }
     idx += h;
   }
 }
 
-template <int r, bool LAZY>
+template <int r, bool LAZY, bool LO0 = false>
 __device__ __forceinline__ void dit_block(uint32_t* x, const uint32_t* tw, const Arith<LAZY>& A) {
 #pragma unroll
   for (int q = r - 1; q >= 0; --q) {
@@ -78,7 +81,7 @@ __device__ __forceinline__ void dit_block(uint32_t* x, const uint32_t* tw, const
 #pragma unroll
     for (int k = 0; k < (1 << r); ++k) {
       if (k & h) continue;
-      const uint32_t t = A.mul(x[k + h], tw[idx + (k & (h - 1))]);
+      const uint32_t t = (LO0 && (k & (h - 1)) == 0) ? x[k + h] : A.mul(x[k + h], tw[idx + (k & (h - 1))]);
       const uint32_t a0 = x[k];
       x[k] = A.add(a0, t);
       x[k + h] = A.sub(a0, t);
@@ -164,8 +167,9 @@ __device__ __forceinline__ void phase_compute(uint32_t* x, uint32_t g, const uin
   for (int u = 0; u < NG; ++u) {
     uint32_t tw[16];
     load_tw<r>(tw, plane, group_lo<LOGB, P>(PhaseGroups<LOGB, P, LOGC>::gid(g, u)));
-    if constexpr (INV) dit_block<r, LAZY>(x + u * (1 << r), tw, A);
-    else dif_block<r, LAZY>(x + u * (1 << r), tw, A);
+    constexpr bool lo0 = LOGB - Sched<LOGB>::s_of(P) - r == 0;
+    if constexpr (INV) dit_block<r, LAZY, lo0>(x + u * (1 << r), tw, A);
+    else dif_block<r, LAZY, lo0>(x + u * (1 << r), tw, A);
   }
 }
 
@@ -274,6 +278,28 @@ __device__ __forceinline__ void inv_phases(uint32_t* x, uint32_t* X, uint32_t g,
   }
 }
 
+// The workgroup's job: jobs[blockIdx.y] when a job array is given, else the
+// by-value first kernel argument j0 (at offset 0 of the kernel arguments).
+// Both are read through the constant address space, so the job lands in
+// scalar registers with one round of scalar loads.
+template <class Job>
+__device__ __forceinline__ Job load_job(const Job& j0, const Job* jobs) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  (void)j0;
+  typedef __attribute__((address_space(4))) const uint32_t cword;
+  const cword* jp = jobs ? (const cword*)(uintptr_t)(jobs + blockIdx.y)
+                         : (const cword*)__builtin_amdgcn_kernarg_segment_ptr();
+  static_assert(sizeof(Job) % 4 == 0, "jobs are whole words");
+  Job J;
+  uint32_t* w = reinterpret_cast<uint32_t*>(&J);
+#pragma unroll
+  for (unsigned i = 0; i < sizeof(Job) / 4; ++i) w[i] = jp[i];
+  return J;
+#else
+  return jobs ? jobs[0] : j0;  // host pass only: never executed
+#endif
+}
+
 // ---------------------------------------------------------------------------
 // Row pass: U rows of 2^LOGM contiguous words per workgroup.
 // ---------------------------------------------------------------------------
@@ -310,7 +336,7 @@ __global__ void __launch_bounds__(RowShape<LOGM>::T) k_rows2(RowJob2 j0, const R
   using SC = Sched<LOGM>;
   constexpr int NPH = SC::nph;
   extern __shared__ uint32_t lds[];
-  const RowJob2 J = jobs ? jobs[blockIdx.y] : j0;
+  const RowJob2 J = load_job(j0, jobs);
   const Arith<LAZY> A(J.f);
   const uint32_t flags = __builtin_amdgcn_readfirstlane(J.flags);  // uniform: scalar branches
   const bool lift = (flags & R2_LIFT) != 0;
@@ -325,18 +351,33 @@ __global__ void __launch_bounds__(RowShape<LOGM>::T) k_rows2(RowJob2 j0, const R
 
   // Branch-free first-phase load: indices past in_count (or of an inactive
   // row) read a clamped valid word and are then replaced by zero.
+  // All loads are issued unconditionally (clamped index) so they overlap; the
+  // lift and the zero fill past in_count are applied afterwards.
   auto load_first = [&](const uint32_t* src, uint32_t in_count, uint32_t* dst_regs) {
     constexpr int r0 = SC::r_of(0);
-    const uint32_t last = in_count ? in_count - 1 : 0;
+    constexpr int NE = PhaseGroups<LOGM, 0>::NG * (1 << r0);
+    if (in_count == 0) {
+#pragma unroll
+      for (int i = 0; i < NE; ++i) dst_regs[i] = 0u;
+      return;
+    }
+    const uint32_t last = in_count - 1;
+#pragma unroll
+    for (int u = 0; u < PhaseGroups<LOGM, 0>::NG; ++u)
+#pragma unroll
+      for (int k = 0; k < (1 << r0); ++k)
+        dst_regs[u * (1 << r0) + k] =
+            gload(src + min(base + elem<LOGM, 0>(PhaseGroups<LOGM, 0>::gid(g, u), k), last));
+    if (lift) {
+#pragma unroll
+      for (int i = 0; i < NE; ++i) dst_regs[i] = A.mul(dst_regs[i], J.f.r2);
+    }
 #pragma unroll
     for (int u = 0; u < PhaseGroups<LOGM, 0>::NG; ++u)
 #pragma unroll
       for (int k = 0; k < (1 << r0); ++k) {
         const uint32_t gi = base + elem<LOGM, 0>(PhaseGroups<LOGM, 0>::gid(g, u), k);
-        const bool live = active && gi < in_count;
-        uint32_t v = in_count ? gload(src + min(gi, last)) : 0u;
-        if (lift) v = A.mul(v, J.f.r2);
-        dst_regs[u * (1 << r0) + k] = live ? v : 0u;
+        if (!(active && gi < in_count)) dst_regs[u * (1 << r0) + k] = 0u;
       }
   };
   // Forward transform of `src` ending with phase NPH-1's groups in regs.
@@ -350,6 +391,8 @@ __global__ void __launch_bounds__(RowShape<LOGM>::T) k_rows2(RowJob2 j0, const R
   constexpr int rl = SC::r_of(PL);
   constexpr int NGL = PhaseGroups<LOGM, PL>::NG;
 
+  // Canonical outputs: min(v, v - p) with p, or a no-op min(v, v) with 0.
+  const uint32_t canon_p = (flags & R2_CANON) ? J.f.p : 0u;
   if constexpr (MODE == R2M_FWD) {
     forward(J.src_a, J.in_a, x);
 #pragma unroll
@@ -357,9 +400,8 @@ __global__ void __launch_bounds__(RowShape<LOGM>::T) k_rows2(RowJob2 j0, const R
 #pragma unroll
       for (int k = 0; k < (1 << rl); ++k) {
         const uint32_t gi = base + elem<LOGM, PL>(PhaseGroups<LOGM, PL>::gid(g, u), k);
-        uint32_t v = x[u * (1 << rl) + k];
-        if (flags & R2_CANON) v = A.canon(v);
-        if (active) gstore(J.dst + gi, v);
+        const uint32_t v = x[u * (1 << rl) + k];
+        if (active) gstore(J.dst + gi, min(v, v - canon_p));
       }
     return;
   } else {
@@ -395,16 +437,22 @@ __global__ void __launch_bounds__(RowShape<LOGM>::T) k_rows2(RowJob2 j0, const R
     phase_compute<LOGM, PL, true, LAZY>(x, g, J.f.planes_inv, A);
     inv_phases<LOGM, PL - 1, 0, LAZY>(x, X, g, 0u, J.f.planes_inv, A);
     constexpr int r0 = SC::r_of(0);
+    constexpr int NE0 = PhaseGroups<LOGM, 0>::NG * (1 << r0);
+    if (flags & R2_SCALE) {
+#pragma unroll
+      for (int i = 0; i < NE0; ++i) x[i] = A.mul(x[i], J.inv_n);
+    }
+    if (flags & R2_NEG) {
+#pragma unroll
+      for (int i = 0; i < NE0; ++i) x[i] = x[i] == 0 ? 0u : A.R - x[i];
+    }
 #pragma unroll
     for (int u = 0; u < PhaseGroups<LOGM, 0>::NG; ++u)
 #pragma unroll
       for (int k = 0; k < (1 << r0); ++k) {
         const uint32_t gi = base + elem<LOGM, 0>(PhaseGroups<LOGM, 0>::gid(g, u), k);
-        uint32_t v = x[u * (1 << r0) + k];
-        if (flags & R2_SCALE) v = A.mul(v, J.inv_n);
-        if (flags & R2_NEG) v = v == 0 ? 0u : A.R - v;
-        if (flags & R2_CANON) v = A.canon(v);
-        if (active && gi >= J.out_lo && gi < J.out_hi) gstore(J.dst + (gi - J.out_shift), v);
+        const uint32_t v = x[u * (1 << r0) + k];
+        if (active && gi >= J.out_lo && gi < J.out_hi) gstore(J.dst + (gi - J.out_shift), min(v, v - canon_p));
       }
   }
 }
@@ -438,12 +486,13 @@ __global__ void __launch_bounds__((ColShape<S, LOGC>::T)) k_cols2(ColJob2 j0, co
   constexpr int NPH = SC::nph;
   constexpr int C = 1 << LOGC;
   extern __shared__ uint32_t X[];
-  const ColJob2 J = jobs ? jobs[blockIdx.y] : j0;
+  const ColJob2 J = load_job(j0, jobs);
   const Arith<LAZY> A(J.f);
   const uint32_t flags = __builtin_amdgcn_readfirstlane(J.flags);
   const bool lift = (flags & R2_LIFT) != 0;
   const int b = L - S;
   const uint32_t M = 1u << b;
+  const uint32_t canon_p = (flags & R2_CANON) ? J.f.p : 0u;
   const uint32_t tiles = M >> LOGC;
   uint32_t tile = blockIdx.x;
   if (tiles >= 64 && (tiles & 7) == 0) tile = (tile & 7) * (tiles >> 3) + (tile >> 3);
@@ -454,17 +503,28 @@ __global__ void __launch_bounds__((ColShape<S, LOGC>::T)) k_cols2(ColJob2 j0, co
   constexpr int PL = NPH - 1;
   if constexpr (!INV) {
     constexpr int r0 = SC::r_of(0);
-    const uint32_t last = J.in_count ? J.in_count - 1 : 0;
+    constexpr int NE0 = PhaseGroups<S, 0, LOGC>::NG * (1 << r0);
+    if (J.in_count == 0) {
 #pragma unroll
-    for (int u = 0; u < PhaseGroups<S, 0, LOGC>::NG; ++u)
+      for (int i = 0; i < NE0; ++i) x[i] = 0u;
+    } else {
+      // Unconditional (clamped) loads first so they overlap; lift and zero fill after.
+      const uint32_t last = J.in_count - 1;
 #pragma unroll
-      for (int k = 0; k < (1 << r0); ++k) {
-        const uint32_t e = elem<S, 0>(PhaseGroups<S, 0, LOGC>::gid(g, u), k);
-        const uint32_t gi = e * M + col;
-        uint32_t v = J.in_count ? gload(J.src + min(gi, last)) : 0u;
-        if (lift) v = A.mul(v, J.f.r2);
-        x[u * (1 << r0) + k] = gi < J.in_count ? v : 0u;
+      for (int u = 0; u < PhaseGroups<S, 0, LOGC>::NG; ++u)
+#pragma unroll
+        for (int k = 0; k < (1 << r0); ++k)
+          x[u * (1 << r0) + k] = gload(J.src + min(elem<S, 0>(PhaseGroups<S, 0, LOGC>::gid(g, u), k) * M + col, last));
+      if (lift) {
+#pragma unroll
+        for (int i = 0; i < NE0; ++i) x[i] = A.mul(x[i], J.f.r2);
       }
+#pragma unroll
+      for (int u = 0; u < PhaseGroups<S, 0, LOGC>::NG; ++u)
+#pragma unroll
+        for (int k = 0; k < (1 << r0); ++k)
+          if (elem<S, 0>(PhaseGroups<S, 0, LOGC>::gid(g, u), k) * M + col >= J.in_count) x[u * (1 << r0) + k] = 0u;
+    }
     phase_compute<S, 0, false, LAZY, LOGC>(x, g, J.f.planes_fwd, A);
     fwd_phases<S, 1, NPH, LOGC, LAZY>(x, X, g, c, J.f.planes_fwd, A);
     constexpr int rl = SC::r_of(PL);
@@ -474,12 +534,12 @@ __global__ void __launch_bounds__((ColShape<S, LOGC>::T)) k_cols2(ColJob2 j0, co
       for (int k = 0; k < (1 << rl); ++k) {
         const uint32_t e = elem<S, PL>(PhaseGroups<S, PL, LOGC>::gid(g, u), k);
         const uint32_t xx = (col * bitrev(e, S)) << shift;
-        uint32_t v = A.mul(x[u * (1 << rl) + k], twiddle<LAZY>(A, J.f.tw_lo, J.f.tw_hi, J.f.split, xx));
-        if (flags & R2_CANON) v = A.canon(v);
-        gstore(J.dst + e * M + col, v);
+        const uint32_t v = A.mul(x[u * (1 << rl) + k], twiddle<LAZY>(A, J.f.tw_lo, J.f.tw_hi, J.f.split, xx));
+        gstore(J.dst + e * M + col, min(v, v - canon_p));
       }
   } else {
     constexpr int rl = SC::r_of(PL);
+    const bool neg = (flags & R2_NEG) != 0;
 #pragma unroll
     for (int u = 0; u < PhaseGroups<S, PL, LOGC>::NG; ++u)
 #pragma unroll
@@ -498,9 +558,8 @@ __global__ void __launch_bounds__((ColShape<S, LOGC>::T)) k_cols2(ColJob2 j0, co
       for (int k = 0; k < (1 << r0); ++k) {
         const uint32_t e = elem<S, 0>(PhaseGroups<S, 0, LOGC>::gid(g, u), k);
         const uint32_t gi = e * M + col;
-        uint32_t v = A.canon(x[u * (1 << r0) + k]);
-        if (flags & R2_NEG) v = v == 0 ? 0u : J.f.p - v;
-        if (gi >= J.out_lo && gi < J.out_hi) gstore(J.dst + (gi - J.out_shift), v);
+        const uint32_t v = A.canon(x[u * (1 << r0) + k]);
+        if (gi >= J.out_lo && gi < J.out_hi) gstore(J.dst + (gi - J.out_shift), neg ? (v == 0 ? 0u : J.f.p - v) : v);
       }
   }
 }
