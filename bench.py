@@ -10,14 +10,19 @@ Inputs are resident in HBM before the timed region; `value` counts output
 coefficients per second of the whole step.
 
 Also reported on the same line:
-  roofline      2^24-point NTT passes (column pass / row pass) timed with HIP
-                events on the library stream; algorithmic bytes 8N per pass
-                (read + write N uint32), 16N per transform;
-  cpu_baseline  the reference library (compiled from the reference sources into
-                oracle/_ref) timed on this host, one thread, on the same step;
-                its outputs are compared limb for limb with the GPU's.
+  roofline        2^24-point NTT passes (column pass / row pass) timed with HIP
+                  events on the library stream; algorithmic bytes 8N per pass
+                  (read + write N uint32), 16N per transform; traffic from the
+                  FETCH_SIZE / WRITE_SIZE counter runs summarised in profiles/;
+  pcie_inclusive  the 2^23 conv through the host-buffer entry point;
+  batched         the uneven batch (8192 convs + 1024 inv/exp), merged vs per call;
+  serving         a Poisson request stream through the continuous-batching
+                  server: throughput, p50 / p99 latency;
+  cpu_baseline    the reference library (compiled from the reference sources into
+                  oracle/_ref) timed on this host, one thread, on a bounded
+                  sample of the step; its outputs are compared limb for limb.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-batch] [--no-serve]
 """
 import argparse
 import json
@@ -136,6 +141,16 @@ def main():
     ms_per_step = 1e3 * wall / args.steps
     value = output_coeffs() * args.steps / wall
 
+    # Host-buffer form of the same conv (fpxg_conv_u32): includes the PCIe copies
+    # of both operands and the product; reported beside `value`, never as it.
+    fa.conv(P, a, b)
+    t0 = time.perf_counter()
+    host_prod = fa.conv(P, a, b)
+    t_host = time.perf_counter() - t0
+    pcie = {"conv_2p23_host_buffers_ms": round(1e3 * t_host, 3),
+            "coeffs_per_s": (2 * N_CONV - 1) / t_host,
+            "bytes_over_pcie": int(4 * (2 * N_CONV + 2 * N_CONV - 1))}
+
     gpu_out = {"conv": out_conv.to_host(), "inv": out_s[0].to_host(), "log": out_s[1].to_host(),
                "exp": out_s[2].to_host(), "sqrt": out_s[3].to_host()}
     sq_host = dsq.to_host()
@@ -182,6 +197,7 @@ def main():
                    "modulus": P, "conv_len": N_CONV, "series_n": N_SERIES},
         "roofline": roofline,
         "cpu_baseline": None,
+        "pcie_inclusive": pcie | {"same_as_device_path": bool(np.array_equal(host_prod, gpu_out["conv"]))},
     }
 
     if not args.no_batch:
