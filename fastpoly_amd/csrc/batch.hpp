@@ -32,21 +32,27 @@ struct Chain {
   bool done() const { return pos >= recs.size(); }
 };
 
-struct LaunchKey {
-  int kind, a, b, c, d;
-  bool operator<(const LaunchKey& o) const {
-    return std::tie(kind, a, b, c, d) < std::tie(o.kind, o.a, o.b, o.c, o.d);
-  }
-};
+// Kernel shape of a record, packed: kind | a | b | c | d (d = row count of a
+// row pass or L of a column pass).  Rows: (logM, lazy, mode, rows); columns:
+// (S, lazy, inverse, L); recurrences by workgroup size; one key each for
+// elementwise and Garner records.
+typedef uint64_t LaunchKey;
+static LaunchKey pack_key(int kind, int a, int b, int c, uint32_t d) {
+  return ((uint64_t)kind << 56) | ((uint64_t)(a & 0xff) << 48) | ((uint64_t)(b & 0xff) << 40) |
+         ((uint64_t)(c & 0xff) << 32) | d;
+}
+static int key_kind(LaunchKey k) { return (int)(k >> 56); }
+static int key_a(LaunchKey k) { return (int)((k >> 48) & 0xff); }
+static int key_b(LaunchKey k) { return (int)((k >> 40) & 0xff); }
+static int key_c(LaunchKey k) { return (int)((k >> 32) & 0xff); }
+static uint32_t key_d(LaunchKey k) { return (uint32_t)k; }
 
-// Rows: (logM, lazy, mode, nrows); columns: (S, lazy, inverse, L); recurrences
-// by workgroup size; one key each for elementwise and Garner records.
 static LaunchKey key_of(const LaunchRec& r) {
   switch (r.kind) {
-    case LK_ROWS: return {LK_ROWS, r.k0, r.k1, r.k2, (int)r.row.nrows};
-    case LK_COLS: return {LK_COLS, r.k0, r.k1, r.k2, r.k3};
-    case LK_RECUR: return {LK_RECUR, r.rec.hi <= 128 ? 64 : 1024, 0, 0, 0};
-    default: return {r.kind, 0, 0, 0, 0};
+    case LK_ROWS: return pack_key(LK_ROWS, r.k0, r.k1, r.k2, r.row.nrows);
+    case LK_COLS: return pack_key(LK_COLS, r.k0, r.k1, r.k2, (uint32_t)r.k3);
+    case LK_RECUR: return pack_key(LK_RECUR, r.rec.hi <= 128 ? 0 : 1, 0, 0, 0);
+    default: return pack_key(r.kind, 0, 0, 0, 0);
   }
 }
 
@@ -54,68 +60,104 @@ struct Staged {
   LaunchKey key;
   size_t jobs = 0, chunks = 0;  // byte offsets into the upload
   uint32_t count = 0, nchunks = 0;
-  RowJob2 row0{};  // by-value kernel argument (unused when a job array is given)
-  ColJob2 col0{};
+  const LaunchRec* first = nullptr;  // by-value kernel argument (unused when a job array is given)
 };
 
 constexpr uint32_t kMaxJobsPerLaunch = 65535;  // gridDim.y
 
+// Job arrays and chunk tables of one upload, written straight into a pinned
+// staging slot sized in advance.
 struct Plan {
-  std::vector<uint8_t> bytes;
+  uint8_t* data = nullptr;
+  size_t size = 0, cap = 0;
   std::vector<Staged> launches;
-  size_t put(const void* p, size_t n) {
-    const size_t off = (bytes.size() + 255) & ~(size_t)255;
-    bytes.resize(off + n);
-    if (n) memcpy(bytes.data() + off, p, n);
+  size_t alloc(size_t n) {
+    const size_t off = (size + 255) & ~(size_t)255;
+    if (off + n > cap) throw Error(E_DEVICE, "batch staging overflow");
+    size = off + n;
     return off;
   }
-  template <class T>
-  size_t put_vec(const std::vector<T>& v) { return put(v.data(), v.size() * sizeof(T)); }
 };
 
-static void stage_group(Plan& plan, const LaunchKey& k, const std::vector<const LaunchRec*>& g) {
-  for (size_t s = 0; s < g.size(); s += kMaxJobsPerLaunch) {
-    const size_t e = std::min(g.size(), s + kMaxJobsPerLaunch);
+// Bytes stage_group will take for a group (same alignment rule as Plan::alloc).
+static size_t group_bytes(LaunchKey k, const std::pair<LaunchKey, const LaunchRec*>* g, size_t n, size_t at) {
+  const int kind = key_kind(k);
+  auto take = [&](size_t m) { at = ((at + 255) & ~(size_t)255) + m; };
+  for (size_t s = 0; s < n; s += kMaxJobsPerLaunch) {
+    const size_t cnt = std::min<size_t>(n - s, kMaxJobsPerLaunch);
+    switch (kind) {
+      case LK_ROWS: take(cnt * sizeof(RowJob2)); break;
+      case LK_COLS: take(cnt * sizeof(ColJob2)); break;
+      case LK_RECUR: take(cnt * sizeof(RecurRec)); break;
+      default: {
+        const bool ew = kind == LK_EW;
+        size_t nch = 0;
+        for (size_t i = 0; i < cnt; ++i) {
+          const uint32_t m = ew ? g[s + i].second->ew.n : g[s + i].second->gar.n;
+          nch += (m + kChunk - 1) / kChunk;
+        }
+        if (nch == 0) break;
+        take(cnt * (ew ? sizeof(EwRec) : sizeof(GarnerRec)));
+        take(nch * sizeof(uint2));
+      }
+    }
+  }
+  return at;
+}
+
+static void stage_group(Plan& plan, LaunchKey k, const std::pair<LaunchKey, const LaunchRec*>* g, size_t n) {
+  const int kind = key_kind(k);
+  for (size_t s = 0; s < n; s += kMaxJobsPerLaunch) {
+    const size_t cnt = std::min<size_t>(n - s, kMaxJobsPerLaunch);
     Staged st;
     st.key = k;
-    st.count = (uint32_t)(e - s);
-    switch (k.kind) {
+    st.count = (uint32_t)cnt;
+    st.first = g[s].second;
+    switch (kind) {
       case LK_ROWS: {
-        std::vector<RowJob2> v;
-        for (size_t i = s; i < e; ++i) v.push_back(g[i]->row);
-        st.row0 = v[0];
-        st.jobs = plan.put_vec(v);
+        st.jobs = plan.alloc(cnt * sizeof(RowJob2));
+        RowJob2* d = (RowJob2*)(plan.data + st.jobs);
+        for (size_t i = 0; i < cnt; ++i) d[i] = g[s + i].second->row;
         break;
       }
       case LK_COLS: {
-        std::vector<ColJob2> v;
-        for (size_t i = s; i < e; ++i) v.push_back(g[i]->col);
-        st.col0 = v[0];
-        st.jobs = plan.put_vec(v);
+        st.jobs = plan.alloc(cnt * sizeof(ColJob2));
+        ColJob2* d = (ColJob2*)(plan.data + st.jobs);
+        for (size_t i = 0; i < cnt; ++i) d[i] = g[s + i].second->col;
         break;
       }
       case LK_RECUR: {
-        std::vector<RecurRec> v;
-        for (size_t i = s; i < e; ++i) v.push_back(g[i]->rec);
-        st.jobs = plan.put_vec(v);
+        st.jobs = plan.alloc(cnt * sizeof(RecurRec));
+        RecurRec* d = (RecurRec*)(plan.data + st.jobs);
+        for (size_t i = 0; i < cnt; ++i) d[i] = g[s + i].second->rec;
         break;
       }
       case LK_EW:
       case LK_GARNER: {
-        std::vector<uint2> ch;
-        std::vector<EwRec> ve;
-        std::vector<GarnerRec> vg;
-        for (size_t i = s; i < e; ++i) {
-          const uint32_t n = k.kind == LK_EW ? g[i]->ew.n : g[i]->gar.n;
-          const uint32_t idx = (uint32_t)(i - s);
-          for (uint32_t o = 0; o < n; o += kChunk) ch.push_back(make_uint2(idx, o));
-          if (k.kind == LK_EW) ve.push_back(g[i]->ew);
-          else vg.push_back(g[i]->gar);
+        const bool ew = kind == LK_EW;
+        size_t nch = 0;
+        for (size_t i = 0; i < cnt; ++i) {
+          const uint32_t m = ew ? g[s + i].second->ew.n : g[s + i].second->gar.n;
+          nch += (m + kChunk - 1) / kChunk;
         }
-        if (ch.empty()) continue;
-        st.jobs = k.kind == LK_EW ? plan.put_vec(ve) : plan.put_vec(vg);
-        st.chunks = plan.put_vec(ch);
-        st.nchunks = (uint32_t)ch.size();
+        if (nch == 0) continue;
+        st.jobs = plan.alloc(cnt * (ew ? sizeof(EwRec) : sizeof(GarnerRec)));
+        st.chunks = plan.alloc(nch * sizeof(uint2));
+        uint2* ch = (uint2*)(plan.data + st.chunks);
+        size_t c = 0;
+        for (size_t i = 0; i < cnt; ++i) {
+          const LaunchRec& r = *g[s + i].second;
+          uint32_t m;
+          if (ew) {
+            ((EwRec*)(plan.data + st.jobs))[i] = r.ew;
+            m = r.ew.n;
+          } else {
+            ((GarnerRec*)(plan.data + st.jobs))[i] = r.gar;
+            m = r.gar.n;
+          }
+          for (uint32_t o = 0; o < m; o += kChunk) ch[c++] = make_uint2((uint32_t)i, o);
+        }
+        st.nchunks = (uint32_t)nch;
         break;
       }
       default:
@@ -126,14 +168,16 @@ static void stage_group(Plan& plan, const LaunchKey& k, const std::vector<const 
 }
 
 static void issue(Context& c, const Staged& s, const uint8_t* base) {
-  const LaunchKey& k = s.key;
-  switch (k.kind) {
+  const LaunchKey k = s.key;
+  switch (key_kind(k)) {
     case LK_ROWS:
-      launch_rows(c, k.b != 0, k.c, s.row0, k.a, (uint32_t)k.d, s.count, (const RowJob2*)(base + s.jobs));
+      launch_rows(c, key_b(k) != 0, key_c(k), s.first->row, key_a(k), key_d(k), s.count,
+                  (const RowJob2*)(base + s.jobs));
       return;
     case LK_COLS: {
-      const PassPlan P{k.d, k.d - k.a, k.a, tile_log_cols(k.a)};
-      launch_cols(c, k.b != 0, k.c != 0, s.col0, P, s.count, (const ColJob2*)(base + s.jobs));
+      const int S = key_a(k), L = (int)key_d(k);
+      const PassPlan P{L, L - S, S, tile_log_cols(S)};
+      launch_cols(c, key_b(k) != 0, key_c(k) != 0, s.first->col, P, s.count, (const ColJob2*)(base + s.jobs));
       return;
     }
     case LK_EW:
@@ -141,7 +185,7 @@ static void issue(Context& c, const Staged& s, const uint8_t* base) {
                          (const uint2*)(base + s.chunks), EwRec{});
       break;
     case LK_RECUR:
-      if (k.a == 64)
+      if (key_a(k) == 0)
         hipLaunchKernelGGL(k_recur<64>, dim3(s.count), dim3(64), 0, c.stream, (const RecurRec*)(base + s.jobs),
                            RecurRec{});
       else
@@ -156,43 +200,89 @@ static void issue(Context& c, const Staged& s, const uint8_t* base) {
   FPXG_HIP(hipGetLastError());
 }
 
-// Copies the job arrays to the device through a pinned staging slot.  Slots
-// rotate; a slot is rewritten only after its previous copy has completed.
-static void upload_plan(Context& c, void* dev, const std::vector<uint8_t>& bytes) {
+// Takes the next pinned staging slot with at least `bytes` capacity; a slot is
+// rewritten only after its previous copy has completed.
+static Context::Staging& take_slot(Context& c, size_t bytes) {
   Context::Staging& s = c.staging[c.staging_next];
   c.staging_next = (c.staging_next + 1) % Context::kStagingSlots;
   if (s.used) FPXG_HIP(hipEventSynchronize(s.ev));
-  if (s.cap < bytes.size()) {
+  s.used = false;
+  if (s.cap < bytes) {
     if (s.host) FPXG_HIP(hipHostFree(s.host));
     s.host = nullptr;
-    s.cap = std::max<size_t>(bytes.size() * 2, (size_t)1 << 20);
+    s.cap = std::max<size_t>(bytes + bytes / 2, (size_t)1 << 20);
     FPXG_HIP(hipHostMalloc((void**)&s.host, s.cap, hipHostMallocDefault));
   }
-  memcpy(s.host, bytes.data(), bytes.size());
-  FPXG_HIP(hipMemcpyAsync(dev, s.host, bytes.size(), hipMemcpyHostToDevice, c.stream));
-  FPXG_HIP(hipEventRecord(s.ev, c.stream));
-  s.used = true;
+  return s;
 }
 
 // Runs up to max_levels levels of the given chains.  Returns the launches issued.
 static size_t run_levels(Context& c, const std::vector<Chain*>& chains, size_t max_levels) {
-  Plan plan;
-  std::map<LaunchKey, std::vector<const LaunchRec*>> groups;
+  typedef std::pair<LaunchKey, const LaunchRec*> Head;
+  // Pass 1: every level's heads, sorted by shape; exact staging size.
+  std::vector<Head> heads;
+  std::vector<size_t> level_end;
+  size_t bytes = 0;
   for (size_t lvl = 0; lvl < max_levels; ++lvl) {
-    groups.clear();
+    const size_t start = heads.size();
     for (Chain* ch : chains)
       if (!ch->done()) {
         const LaunchRec& r = ch->recs[ch->pos++];
-        groups[key_of(r)].push_back(&r);
+        heads.emplace_back(key_of(r), &r);
       }
-    if (groups.empty()) break;
-    for (auto& kv : groups) stage_group(plan, kv.first, kv.second);
+    if (heads.size() == start) break;
+    std::stable_sort(heads.begin() + (ptrdiff_t)start, heads.end(),
+                     [](const Head& x, const Head& y) { return x.first < y.first; });
+    for (size_t i = start; i < heads.size();) {
+      size_t j = i + 1;
+      while (j < heads.size() && heads[j].first == heads[i].first) ++j;
+      bytes = group_bytes(heads[i].first, heads.data() + i, j - i, bytes);
+      i = j;
+    }
+    level_end.push_back(heads.size());
+  }
+  if (heads.empty() || bytes == 0) return 0;
+  // Pass 2: write the job arrays straight into a pinned slot, upload, launch.
+  Context::Staging& slot = take_slot(c, bytes);
+  Plan plan;
+  plan.data = slot.host;
+  plan.cap = slot.cap;
+  size_t start = 0;
+  for (size_t e : level_end) {
+    for (size_t i = start; i < e;) {
+      size_t j = i + 1;
+      while (j < e && heads[j].first == heads[i].first) ++j;
+      stage_group(plan, heads[i].first, heads.data() + i, j - i);
+      i = j;
+    }
+    start = e;
   }
   if (plan.launches.empty()) return 0;
-  DevBuf dev = c.buf(plan.bytes.size() / 4 + 1);
-  upload_plan(c, dev.get(), plan.bytes);
+  DevBuf dev = c.buf(plan.size / 4 + 1);
+  FPXG_HIP(hipMemcpyAsync(dev.get(), slot.host, plan.size, hipMemcpyHostToDevice, c.stream));
+  FPXG_HIP(hipEventRecord(slot.ev, c.stream));
+  slot.used = true;
   for (const Staged& s : plan.launches) issue(c, s, (const uint8_t*)dev.get());
   return plan.launches.size();
+}
+
+// Runs every level of the chains, uploading the job arrays of a few levels at
+// a time (2, 4, 8, then 16 levels per upload) so the device starts on the
+// first levels while the host stages the later ones.
+static size_t run_all(Context& c, const std::vector<Chain*>& chains) {
+  size_t launches = 0, step = 2;
+  for (;;) {
+    bool more = false;
+    for (const Chain* ch : chains)
+      if (!ch->done()) {
+        more = true;
+        break;
+      }
+    if (!more) break;
+    launches += run_levels(c, chains, step);
+    step = std::min<size_t>(step * 2, 16);
+  }
+  return launches;
 }
 
 // ---------------------------------------------------------------------------
@@ -207,8 +297,11 @@ struct Prepared {
   bool char2 = false; // sqrt in characteristic two (immediate path)
 };
 
-static void check_modulus_p(uint32_t p) {
-  if (p < 2 || p >= (1u << 31) || !is_prime_u32(p)) throw Error(E_UNSUPPORTED, "modulus must be a prime in [2, 2^31)");
+static void check_modulus_p(Context& c, uint32_t p) {
+  auto it = c.prime_cache.find(p);
+  const bool ok = it != c.prime_cache.end() ? it->second
+                                            : (c.prime_cache[p] = p >= 2 && p < (1u << 31) && is_prime_u32(p));
+  if (!ok) throw Error(E_UNSUPPORTED, "modulus must be a prime in [2, 2^31)");
 }
 
 static size_t conv_len_checked(size_t la, size_t lb, size_t n, size_t* pla, size_t* plb) {
@@ -226,7 +319,7 @@ static size_t conv_len_checked(size_t la, size_t lb, size_t n, size_t* pla, size
 static Prepared prepare(Context& c, fpxg_req& q) {
   Prepared r;
   q.out_len = 0;
-  check_modulus_p(q.p);
+  check_modulus_p(c, q.p);
   if (q.op == FPXG_CONV) {
     r.outc = conv_len_checked(q.la, q.lb, q.n, &r.xa, &r.xb);
     if (r.outc && (!q.a || !q.b || !q.out || q.out_cap < r.outc))
@@ -281,12 +374,18 @@ static void release_chains(Context& c, std::vector<Chain>& chains) {
 
 struct BatchStats {
   size_t launches = 0, waves = 0, records = 0;
+  double t_prepare = 0, t_scan = 0, t_record = 0, t_run = 0;  // host seconds
 };
+
+static double host_seconds() {
+  return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
 
 // One batch of device-pointer requests, executed level by level; large
 // batches run in waves so that recorded intermediates fit the arena.
 static BatchStats batch_dev(Context& c, fpxg_req* reqs, size_t k) {
   BatchStats bs;
+  double t0 = host_seconds();
   std::vector<Prepared> pr(k);
   std::vector<size_t> scan_idx(k, SIZE_MAX);
   std::vector<ScanRec> scans;
@@ -314,7 +413,10 @@ static BatchStats batch_dev(Context& c, fpxg_req* reqs, size_t k) {
         reqs[i].status = e.code;
       }
     }
+  bs.t_prepare = host_seconds() - t0;
+  t0 = host_seconds();
   const std::vector<ScanOut> so = scan_inputs(c, scans);
+  bs.t_scan = host_seconds() - t0;
 
   std::vector<Chain> wave;
   size_t base = c.arena->in_use();
@@ -326,7 +428,9 @@ static BatchStats batch_dev(Context& c, fpxg_req* reqs, size_t k) {
       ptrs.push_back(&ch);
       bs.records += ch.recs.size();
     }
-    bs.launches += run_levels(c, ptrs, SIZE_MAX);
+    const double tr = host_seconds();
+    bs.launches += run_all(c, ptrs);
+    bs.t_run += host_seconds() - tr;
     ++bs.waves;
     release_chains(c, wave);
     base = c.arena->in_use();
@@ -339,7 +443,9 @@ static BatchStats batch_dev(Context& c, fpxg_req* reqs, size_t k) {
     const ScanOut* s = scan_idx[i] == SIZE_MAX ? nullptr : &so[scan_idx[i]];
     for (int attempt = 0; attempt < 3; ++attempt) {
       wave.emplace_back();
+      const double tr = host_seconds();
       const int st = record_request(c, q, pr[i], s, wave.back());
+      bs.t_record += host_seconds() - tr;
       if (st == E_NOMEM && attempt < 2) {
         wave.pop_back();
         if (!wave.empty()) {

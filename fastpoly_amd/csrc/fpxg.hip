@@ -6,6 +6,7 @@
 #include "runtime.hpp"
 
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 
 namespace fpxg {
@@ -208,7 +209,7 @@ void* Arena::alloc(size_t bytes) {
   std::lock_guard<std::mutex> g(mu_);
   const int k = size_class(bytes);
   const size_t sz = (size_t)1 << k;
-  auto& fl = free_[k];
+  auto& fl = free_[(size_t)k];
   if (!fl.empty()) {
     void* p = fl.back();
     fl.pop_back();
@@ -218,12 +219,13 @@ void* Arena::alloc(size_t bytes) {
     in_use_ += sz;
     return p;
   }
-  for (auto& c : chunks_) {
+  if (!chunks_.empty()) {
+    Chunk& c = chunks_.back();
     const size_t off = (c.used + 255) & ~(size_t)255;
     if (off + sz <= c.size) {
       c.used = off + sz;
       void* p = c.base + off;
-      owner_[p] = k;
+      owner_[p] = (uint8_t)k;
       in_use_ += sz;
       return p;
     }
@@ -240,8 +242,12 @@ void* Arena::alloc(size_t bytes) {
   }
   ++sys_allocs_;
   reserved_ += take;
-  chunks_.push_back({base, take, sz});
-  owner_[base] = k;
+  if (take == sz && !chunks_.empty()) {
+    chunks_.insert(chunks_.end() - 1, {base, take, sz});  // keep carving the current chunk
+  } else {
+    chunks_.push_back({base, take, sz});
+  }
+  owner_[base] = (uint8_t)k;
   in_use_ += sz;
   return base;
 }
@@ -272,7 +278,7 @@ void Arena::release_idle() {
   if (in_use_ != 0) return;  // chunks are carved; only an idle arena can return them
   for (auto& c : chunks_) (void)hipFree(c.base);
   chunks_.clear();
-  free_.clear();
+  for (auto& fl : free_) fl.clear();
   owner_.clear();
   reserved_ = idle_bytes_ = idle_blocks_ = 0;
 }
@@ -454,7 +460,9 @@ HostField& Context::field(uint32_t p, uint32_t root) {
 PassPlan Context::plan(int L) const {
   auto it = plan_override.find(L);
   if (it != plan_override.end()) return it->second;
-  if (L <= single_log_max) return {L, L, 0, 0};
+  // Recorded schedules run merged with others, so a transform that fits one
+  // workgroup's LDS is one row pass there; a lone call splits it for parallelism.
+  if (L <= (rec ? row_log_max : single_log_max)) return {L, L, 0, 0};
   int S = std::max(L - row_log_max, std::min(L / 2, 10));
   if (S > 12) throw Error(E_SIZE, "transform too large for two passes");
   const int b = L - S;
@@ -583,6 +591,9 @@ static int crt_prime_count(uint32_t p, size_t terms) {
 }
 
 static GarnerConst garner_const(Context& c, const HostField& T, int primes, bool accumulate) {
+  const uint64_t key = ((uint64_t)T.p << 8) | ((uint64_t)primes << 1) | (accumulate ? 1u : 0u);
+  auto hit = c.garner_cache.find(key);
+  if (hit != c.garner_cache.end()) return hit->second;
   GarnerConst g{};
   const HostField* P[3];
   for (int k = 0; k < 3; ++k) {
@@ -601,6 +612,7 @@ static GarnerConst garner_const(Context& c, const HostField& T, int primes, bool
   g.primes = (uint32_t)primes;
   g.accumulate = accumulate;
   g.neg = 0;
+  c.garner_cache[key] = g;
   return g;
 }
 
@@ -979,7 +991,12 @@ int fpxg_batch_submit(fpxg_req* reqs, size_t count, uint64_t* ticket) {
   return guarded([&](Context& c) {
     if (!reqs && count) throw Error(E_ARG, "reqs is null");
     prune_tickets(c);
-    batch_dev(c, reqs, count);
+    const BatchStats bs = batch_dev(c, reqs, count);
+    static const bool trace = getenv("FPXG_TRACE_BATCH") != nullptr;
+    if (trace)
+      fprintf(stderr, "fpxg batch: %zu requests, %zu records, %zu launches, %zu waves; host ms prepare %.2f scan %.2f "
+              "record %.2f run %.2f\n", count, bs.records, bs.launches, bs.waves, 1e3 * bs.t_prepare,
+              1e3 * bs.t_scan, 1e3 * bs.t_record, 1e3 * bs.t_run);
     hipEvent_t ev;
     FPXG_HIP(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
     FPXG_HIP(hipEventRecord(ev, c.stream));

@@ -8,6 +8,8 @@
 
 #include <algorithm>
 #include <map>
+#include <array>
+#include <unordered_map>
 #include <memory>
 #include <mutex>
 #include <stdexcept>
@@ -106,9 +108,9 @@ class Arena {
   struct Chunk { char* base; size_t size, used; };
   size_t cap_;
   size_t reserved_ = 0, in_use_ = 0, hits_ = 0, sys_allocs_ = 0, idle_bytes_ = 0, idle_blocks_ = 0;
-  std::vector<Chunk> chunks_;
-  std::map<int, std::vector<void*>> free_;  // class -> blocks
-  std::map<void*, int> owner_;              // block -> class
+  std::vector<Chunk> chunks_;                  // blocks are carved from the last chunk only
+  std::array<std::vector<void*>, 64> free_;    // class -> idle blocks
+  std::unordered_map<void*, uint8_t> owner_;   // block -> class
   std::vector<void*>* defer_ = nullptr;
   mutable std::mutex mu_;
 };
@@ -154,40 +156,36 @@ enum { LK_ROWS = 0, LK_COLS = 1, LK_EW = 2, LK_RECUR = 3, LK_GARNER = 4 };
 struct LaunchRec {
   int kind;
   int k0, k1, k2, k3;  // shape: rows (logM, lazy, mode, -), cols (S, lazy, inv, L)
-  RowJob2 row;
-  ColJob2 col;
-  EwRec ew;
-  RecurRec rec;
-  GarnerRec gar;
+  union {
+    RowJob2 row;
+    ColJob2 col;
+    EwRec ew;
+    RecurRec rec;
+    GarnerRec gar;
+  };
+  LaunchRec() {}
 };
 
 struct Recorder {
   std::vector<LaunchRec> recs;
+  LaunchRec& next(int kind) {
+    recs.emplace_back();
+    LaunchRec& r = recs.back();
+    r.kind = kind;
+    r.k0 = r.k1 = r.k2 = r.k3 = 0;
+    return r;
+  }
   void push_rows(int logM, bool lazy, int mode, const RowJob2& j) {
-    LaunchRec r{};
-    r.kind = LK_ROWS; r.k0 = logM; r.k1 = lazy; r.k2 = mode; r.row = j;
-    recs.push_back(r);
+    LaunchRec& r = next(LK_ROWS);
+    r.k0 = logM; r.k1 = lazy; r.k2 = mode; r.row = j;
   }
   void push_cols(int S, bool lazy, bool inv, int L, const ColJob2& j) {
-    LaunchRec r{};
-    r.kind = LK_COLS; r.k0 = S; r.k1 = lazy; r.k2 = inv; r.k3 = L; r.col = j;
-    recs.push_back(r);
+    LaunchRec& r = next(LK_COLS);
+    r.k0 = S; r.k1 = lazy; r.k2 = inv; r.k3 = L; r.col = j;
   }
-  void push_ew(const EwRec& e) {
-    LaunchRec r{};
-    r.kind = LK_EW; r.ew = e;
-    recs.push_back(r);
-  }
-  void push_recur(const RecurRec& e) {
-    LaunchRec r{};
-    r.kind = LK_RECUR; r.rec = e;
-    recs.push_back(r);
-  }
-  void push_garner(const GarnerRec& g) {
-    LaunchRec r{};
-    r.kind = LK_GARNER; r.gar = g;
-    recs.push_back(r);
-  }
+  void push_ew(const EwRec& e) { next(LK_EW).ew = e; }
+  void push_recur(const RecurRec& e) { next(LK_RECUR).rec = e; }
+  void push_garner(const GarnerRec& g) { next(LK_GARNER).gar = g; }
 };
 
 struct Context {
@@ -210,6 +208,8 @@ struct Context {
   static constexpr int kStagingSlots = 4;
   Staging staging[kStagingSlots];
   int staging_next = 0;
+  std::unordered_map<uint64_t, GarnerConst> garner_cache;  // by (target p, primes, accumulate)
+  std::unordered_map<uint32_t, bool> prime_cache;          // modulus -> accepted
   // Outstanding batch tickets (completion events).
   std::map<uint64_t, hipEvent_t> tickets;
   uint64_t next_ticket = 1;
