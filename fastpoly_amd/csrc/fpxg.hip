@@ -430,6 +430,7 @@ static void context_init(Context& c) {
   c.lds_bytes = big ? (size_t)lds_words(1u << 14) * 4 : 65536;
   c.row_log_max = big ? 14 : 13;
   c.arena = std::make_unique<Arena>(g_arena_bytes);
+  c.graphs_enabled = getenv("FPXG_NO_GRAPHS") == nullptr;
 }
 
 Context& ctx() {
@@ -462,7 +463,7 @@ PassPlan Context::plan(int L) const {
   if (it != plan_override.end()) return it->second;
   // Recorded schedules run merged with others, so a transform that fits one
   // workgroup's LDS is one row pass there; a lone call splits it for parallelism.
-  if (L <= (rec ? row_log_max : single_log_max)) return {L, L, 0, 0};
+  if (L <= ((rec && rec_merged) ? row_log_max : single_log_max)) return {L, L, 0, 0};
   int S = std::max(L - row_log_max, std::min(L / 2, 10));
   if (S > 12) throw Error(E_SIZE, "transform too large for two passes");
   const int b = L - S;
@@ -673,6 +674,7 @@ void conv_dev(Context& c, const HostField& T, const uint32_t* a, size_t la, cons
 #include "series.hpp"
 #include "batch.hpp"
 #include "serve.hpp"
+#include "graphs.hpp"
 
 // ===========================================================================
 // C ABI
@@ -744,6 +746,7 @@ void fpxg_shutdown(void) {
   std::lock_guard<std::mutex> g(g_init_mu);
   if (!g_ctx) return;
   (void)hipStreamSynchronize(g_ctx->stream);
+  drop_graphs(*g_ctx);
   for (auto& kv : g_ctx->fields)
     if (kv.second->tables) (void)hipFree(kv.second->tables);
   for (auto& e : g_ctx->events) (void)hipEventDestroy(e);
@@ -780,6 +783,7 @@ int fpxg_arena_stats(fpxg_stats* out) {
 int fpxg_release_scratch(void) {
   return guarded([&](Context& c) {
     FPXG_HIP(hipStreamSynchronize(c.stream));
+    drop_graphs(c);
     c.arena->release_idle();
   });
 }
@@ -907,7 +911,7 @@ int fpxg_series_u32(int op, uint32_t p, const uint32_t* a, size_t la, size_t n, 
     const size_t use = std::min(la, n);
     Upload u(c, a, use);
     DevBuf o = c.buf(n);
-    series_dev(c, T, op, u.buf.get(), use, n, o.get());
+    series_call(c, T, op, u.buf.get(), use, n, o.get());
     FPXG_HIP(hipMemcpyAsync(out, o.get(), n * 4, hipMemcpyDeviceToHost, c.stream));
     FPXG_HIP(hipStreamSynchronize(c.stream));
   });
@@ -918,7 +922,7 @@ int fpxg_series_dev(int op, uint32_t p, const uint32_t* a, size_t la, size_t n, 
     check_modulus(p);
     if (n == 0) return;
     if (!out || (la && !a)) throw Error(E_ARG, "null pointer");
-    series_dev(c, c.field(p), op, a, std::min(la, n), n, out);
+    series_call(c, c.field(p), op, a, std::min(la, n), n, out);
   });
 }
 

@@ -330,8 +330,26 @@ enum : int { R2M_FWD = 1, R2M_INV = 2, R2M_CONV = 3, R2M_SPEC = 4, R2M_SPEC_ONLY
 // R2M_SPEC: forward a, a*spectrum(b), inverse
 // R2M_SPEC_ONLY: a (already a spectrum) * spectrum(b), inverse
 
+// Target waves per SIMD (register budget 512 / waves).  Eight waves (64
+// registers) lets two 1024-thread workgroups share a CU; the fused modes hold
+// two operands in registers and get a larger budget.
+#ifndef FPXG_ROW_WPE
+#define FPXG_ROW_WPE 8
+#endif
+#ifndef FPXG_ROW_WPE_FUSED
+#define FPXG_ROW_WPE_FUSED 5
+#endif
+#ifndef FPXG_COL_WPE
+#define FPXG_COL_WPE 8
+#endif
+template <int MODE>
+struct RowWaves {
+  static constexpr int value = (MODE == R2M_FWD || MODE == R2M_INV) ? FPXG_ROW_WPE : FPXG_ROW_WPE_FUSED;
+};
+
 template <int LOGM, bool LAZY, int MODE>
-__global__ void __launch_bounds__(RowShape<LOGM>::T) k_rows2(RowJob2 j0, const RowJob2* __restrict__ jobs) {
+__global__ void __launch_bounds__(RowShape<LOGM>::T) __attribute__((amdgpu_waves_per_eu(RowWaves<MODE>::value)))
+k_rows2(RowJob2 j0, const RowJob2* __restrict__ jobs) {
   using RS = RowShape<LOGM>;
   using SC = Sched<LOGM>;
   constexpr int NPH = SC::nph;
@@ -481,7 +499,8 @@ struct ColShape {
 };
 
 template <int S, int LOGC, bool LAZY, bool INV>
-__global__ void __launch_bounds__((ColShape<S, LOGC>::T)) k_cols2(ColJob2 j0, const ColJob2* __restrict__ jobs, int L) {
+__global__ void __launch_bounds__((ColShape<S, LOGC>::T)) __attribute__((amdgpu_waves_per_eu(FPXG_COL_WPE)))
+k_cols2(ColJob2 j0, const ColJob2* __restrict__ jobs, int L) {
   using SC = Sched<S>;
   constexpr int NPH = SC::nph;
   constexpr int C = 1 << LOGC;

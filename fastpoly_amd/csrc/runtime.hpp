@@ -198,6 +198,18 @@ struct Context {
   int cus = 256;
   std::map<int, PassPlan> plan_override;  // autotuned splits by log2 n
   Recorder* rec = nullptr;                // non-null: launches are recorded, not issued
+  bool rec_merged = true;                 // recorded for the merged executor (else for a graph of one call)
+  // Captured single-call schedules (hipGraph) with the arena blocks they use.
+  struct GraphEntry {
+    hipGraphExec_t exec = nullptr;
+    std::vector<void*> bufs;
+    size_t bytes = 0;
+    uint64_t stamp = 0;
+  };
+  std::map<std::array<uint64_t, 11>, GraphEntry> graphs;
+  uint64_t graph_clock = 0;
+  size_t graph_bytes = 0;
+  bool graphs_enabled = true;
   // Pinned host slots for job-array uploads of the merged executor.
   struct Staging {
     uint8_t* host = nullptr;

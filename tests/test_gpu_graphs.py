@@ -1,0 +1,81 @@
+"""GPU: series calls replayed from the graph cache give, on every replay, the
+same limbs as the eager path and the oracle, with other work between replays,
+new data under the same key, more keys than the cache holds, and after the
+cache is dropped."""
+import numpy as np
+import pytest
+
+import golden_inputs as gi
+
+pytestmark = pytest.mark.gpu
+
+P = 998244353
+Q = 1000000007
+
+
+@pytest.fixture(scope="module")
+def fa():
+    import fastpoly_amd
+    fastpoly_amd.init(0, 0)
+    return fastpoly_amd
+
+
+def _input(oracle, op, p, n, seed):
+    a = oracle.MT(seed).rnd(p, n)
+    if op == "exp":
+        a[0] = 0
+    elif op in ("log", "sqrt"):
+        a[0] = oracle.from_int(p, 1)
+    elif a[0] == 0:
+        a[0] = 1
+    return a
+
+
+@pytest.mark.parametrize("p", [P, Q])
+@pytest.mark.parametrize("op", ["inv", "log", "exp", "sqrt"])
+def test_replays_match_oracle(fa, oracle, p, op):
+    n = 3000
+    D = fa.DeviceArray
+    a = _input(oracle, op, p, n, 11)
+    want = oracle.series(op, p, a, n)
+    da, out = D.from_host(a), D(n)
+    x = D.from_host(_input(oracle, "inv", p, 500, 12))
+    y = D(1000)
+    for rep in range(4):
+        fa.series_dev(op, p, da, n, n, out)
+        assert np.array_equal(out.to_host(), want), (op, p, rep)
+        # eager work between replays, touching other buffers
+        fa.conv_dev(p, x, 500, x, 500, y)
+        fa.synchronize()
+
+
+def test_same_key_new_data(fa, oracle):
+    """New input data under the same pointers and scan signature replays the
+    cached graph, which must read the new data."""
+    D = fa.DeviceArray
+    for op, n in (("inv", 5000), ("exp", 4000), ("log", 3000)):
+        da, out = D(n), D(n)
+        for seed in (21, 22, 23):
+            a = _input(oracle, op, P, n, seed)
+            if op == "inv":
+                a[0] = 987654
+            assert fa.lib().fpxg_memcpy_h2d(da.ptr, a.ctypes.data, 4 * n) == 0
+            fa.series_dev(op, P, da, n, n, out)
+            assert np.array_equal(out.to_host(), oracle.series(op, P, a, n)), (op, seed)
+
+
+def test_many_keys_and_release(fa, oracle):
+    D = fa.DeviceArray
+    outs = []
+    for i in range(80):  # more keys than the cache keeps
+        n = 40 + 7 * i
+        a = _input(oracle, "inv", P, n, 100 + i)
+        da, out = D.from_host(a), D(n)
+        fa.series_dev("inv", P, da, n, n, out)
+        outs.append((a, n, da, out))
+    for a, n, da, out in outs:
+        assert np.array_equal(out.to_host(), oracle.series("inv", P, a, n)), n
+    fa.release_scratch_memory()
+    for a, n, da, out in outs[:5]:
+        fa.series_dev("inv", P, da, n, n, out)
+        assert np.array_equal(out.to_host(), oracle.series("inv", P, a, n)), n
