@@ -108,6 +108,9 @@ def main():
     if args.ntt_only:
         ntt_only(fa, args)
         return
+    t_tune = time.perf_counter()
+    tuned = fa.autotune(13, 24, 3)  # pass splits for single calls, untimed setup
+    t_tune = time.perf_counter() - t_tune
     rng = np.random.default_rng(20261203)
     a, b, inv_in, log_in, exp_in, d = make_inputs(rng)
     D = fa.DeviceArray
@@ -198,6 +201,9 @@ def main():
         "roofline": roofline,
         "cpu_baseline": None,
         "pcie_inclusive": pcie | {"same_as_device_path": bool(np.array_equal(host_prod, gpu_out["conv"]))},
+        "autotune": {"seconds": round(t_tune, 3),
+                     "col_levels_by_log2n": {str(k): v[0] for k, v in tuned.items()},
+                     "forward_us_by_log2n": {str(k): round(v[1], 2) for k, v in tuned.items()}},
     }
 
     if not args.no_batch:
@@ -357,11 +363,15 @@ def batched_leg(fa, args):
     def batched():
         fa.wait(fa.batch_submit(arr_b))
 
+    # Per call = one launch sequence per request, as a one-off call runs (no
+    # graph replay: 9216 distinct requests would only churn the graph cache).
+    fa.set_graphs(False)
     per_call()  # warm-up: fields, plans, arena
     batched()
     t0 = time.perf_counter()
     per_call()
     t_call = time.perf_counter() - t0
+    fa.set_graphs(True)
     times = []
     for _ in range(max(1, args.batch_reps)):
         t0 = time.perf_counter()

@@ -41,7 +41,7 @@ EXPORTED_SYMBOLS = [
     "fpxg_memcpy_d2h", "fpxg_synchronize", "fpxg_stream", "fpxg_event_record", "fpxg_event_elapsed",
     "fpxg_ntt_pass_dev", "fpxg_ntt_plan_info", "fpxg_batch_submit", "fpxg_wait", "fpxg_batch_u32",
     "fpxg_server_create", "fpxg_server_submit", "fpxg_server_wait", "fpxg_server_poll", "fpxg_server_now",
-    "fpxg_server_stats_get", "fpxg_server_destroy",
+    "fpxg_server_stats_get", "fpxg_server_destroy", "fpxg_autotune", "fpxg_set_option",
 ]
 
 
@@ -158,6 +158,9 @@ def lib():
         L.fpxg_server_now.restype = ctypes.c_double
         L.fpxg_server_now.argtypes = [ctypes.c_void_p]
         L.fpxg_server_stats_get.argtypes = [ctypes.c_void_p, ctypes.POINTER(ServerStats)]
+        L.fpxg_autotune.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_int),
+                                    ctypes.POINTER(ctypes.c_float)]
+        L.fpxg_set_option.argtypes = [ctypes.c_int, ctypes.c_int64]
         L.fpxg_server_destroy.restype = None
         L.fpxg_server_destroy.argtypes = [ctypes.c_void_p]
         _lib = L
@@ -548,3 +551,21 @@ class Server:
             self.close()
         except Exception:
             pass
+
+
+def autotune(min_log: int = 13, max_log: int = 24, reps: int = 5) -> dict:
+    """Picks the fastest pass split per transform size (fpxg_autotune);
+    returns {log2 n: (column levels, microseconds)}."""
+    k = max_log - min_log + 1
+    cols = (ctypes.c_int * k)()
+    us = (ctypes.c_float * k)()
+    _check(lib().fpxg_autotune(min_log, max_log, reps, cols, us), "autotune")
+    return {min_log + i: (cols[i], float(us[i])) for i in range(k)}
+
+
+OPT_GRAPHS = 0
+
+
+def set_graphs(enabled: bool) -> None:
+    """Replay repeated single series calls from captured hipGraphs (default on)."""
+    _check(lib().fpxg_set_option(OPT_GRAPHS, 1 if enabled else 0), "set_option")

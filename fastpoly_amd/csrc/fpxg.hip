@@ -459,8 +459,10 @@ HostField& Context::field(uint32_t p, uint32_t root) {
 // S = L - b column levels over rows of 2^b words with b <= row_log_max and
 // S <= 12, balanced so both passes keep enough workgroups.
 PassPlan Context::plan(int L) const {
-  auto it = plan_override.find(L);
-  if (it != plan_override.end()) return it->second;
+  if (!(rec && rec_merged)) {  // autotuned splits are measured on lone transforms
+    auto it = plan_override.find(L);
+    if (it != plan_override.end()) return it->second;
+  }
   // Recorded schedules run merged with others, so a transform that fits one
   // workgroup's LDS is one row pass there; a lone call splits it for parallelism.
   if (L <= ((rec && rec_merged) ? row_log_max : single_log_max)) return {L, L, 0, 0};
@@ -899,6 +901,62 @@ int fpxg_ntt_plan_info(uint32_t n, int* row_log, int* col_levels, int* tile_cols
     if (row_log) *row_log = P.b;
     if (col_levels) *col_levels = P.S;
     if (tile_cols_log) *tile_cols_log = P.logC;
+  });
+}
+
+// Autotuner: for each transform size, time the forward transform under every
+// admissible pass split (one row pass while it fits a workgroup; otherwise
+// S column levels with rows of 2^(L-S) words) and keep the fastest.
+int fpxg_autotune(int min_log, int max_log, int reps, int* chosen_col_levels, float* best_us) {
+  return guarded([&](Context& c) {
+    if (min_log < 1 || max_log > 26 || min_log > max_log || reps < 1) throw Error(E_ARG, "bad autotune range");
+    DevBuf buf = c.buf((size_t)1 << max_log);
+    fill(c, buf.get(), (size_t)1 << max_log, 1u);
+    for (int L = min_log; L <= max_log; ++L) {
+      const HostField& F = c.field(L <= 25 ? kCrtP[0] : kCrtP[1]);
+      std::vector<PassPlan> cand;
+      if (L <= c.row_log_max) cand.push_back({L, L, 0, 0});
+      for (int S = std::max(1, L - c.row_log_max); S <= std::min(12, L - 1); ++S)
+        if (S >= 4 && L - S >= 4) cand.push_back({L, L - S, S, tile_log_cols(S)});
+      if (cand.empty()) cand.push_back(c.plan(L));
+      float best = 1e30f;
+      PassPlan pick = cand[0];
+      for (const PassPlan& P : cand) {
+        c.plan_override[L] = P;
+        ntt_forward(c, F, buf.get(), buf.get(), 1u << L, L, false, false);
+        FPXG_HIP(hipEventRecord(c.events[14], c.stream));
+        for (int r = 0; r < reps; ++r) ntt_forward(c, F, buf.get(), buf.get(), 1u << L, L, false, false);
+        FPXG_HIP(hipEventRecord(c.events[15], c.stream));
+        FPXG_HIP(hipEventSynchronize(c.events[15]));
+        float ms = 0;
+        FPXG_HIP(hipEventElapsedTime(&ms, c.events[14], c.events[15]));
+        const float us = 1e3f * ms / (float)reps;
+        if (us < best) {
+          best = us;
+          pick = P;
+        }
+      }
+      c.plan_override[L] = pick;
+      if (chosen_col_levels) chosen_col_levels[L - min_log] = pick.S;
+      if (best_us) best_us[L - min_log] = best;
+    }
+  });
+}
+
+int fpxg_set_option(int option, int64_t value) {
+  return guarded([&](Context& c) {
+    switch (option) {
+      case FPXG_OPT_GRAPHS:
+        c.graphs_enabled = value != 0;
+        if (!c.graphs_enabled) {
+          FPXG_HIP(hipStreamSynchronize(c.stream));
+          drop_graphs(c);
+          c.graph_seen.clear();
+        }
+        return;
+      default:
+        throw Error(E_ARG, "unknown option");
+    }
   });
 }
 

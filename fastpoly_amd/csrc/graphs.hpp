@@ -2,13 +2,13 @@
 //
 // A Newton series call is a few hundred small launches whose shapes depend
 // only on (op, p, n, la) and the five words of the input scan.  The first call
-// with a given key records its schedule (the same recorder the merged executor
-// uses, with the one-call pass plans), captures it on the library stream into
-// a graph, and keeps the graph together with the arena blocks its kernels use;
-// a later call with the same key and the same input/output pointers replays
-// the graph (the device reads the current input, so new data with the same
-// scan gives its own result).  Least-recently-used graphs are dropped beyond 64
-// entries or 4 GiB of held blocks.
+// with a given key (those, plus the input and output pointers) runs eagerly.
+// The second records its schedule (the recorder of the merged executor, with
+// the one-call pass plans), captures it on the library stream into a graph and
+// keeps the graph with the arena blocks its kernels use; from then on the key
+// replays the graph (the device reads the current input, so new data with the
+// same scan signature gets its own result).  Least-recently-used graphs are
+// dropped beyond 64 entries or 4 GiB of held blocks.
 #pragma once
 
 #include <array>
@@ -119,6 +119,15 @@ static void series_call(Context& c, const HostField& T, int op, const uint32_t* 
                                      op == FPXG_SQRT ? (uint64_t)s.afirst : 0u};
   auto it = c.graphs.find(key);
   if (it == c.graphs.end()) {
+    // A key seen for the first time runs eagerly; capture pays off from the
+    // second call on (a loop over the same buffers).
+    uint64_t h = 1469598103934665603ull;
+    for (uint64_t v : key) h = (h ^ v) * 1099511628211ull;
+    if (c.graph_seen.insert(h).second) {
+      if (c.graph_seen.size() > 4096) c.graph_seen.clear();
+      series_emit(c, T, q, s);
+      return;
+    }
     Context::GraphEntry e;
     const size_t before = c.arena->in_use();
     std::vector<LaunchRec> recs = record_series(c, T, q, s, e.bufs);

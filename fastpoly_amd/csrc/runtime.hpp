@@ -10,6 +10,7 @@
 #include <map>
 #include <array>
 #include <unordered_map>
+#include <unordered_set>
 #include <memory>
 #include <mutex>
 #include <stdexcept>
@@ -210,6 +211,7 @@ struct Context {
   uint64_t graph_clock = 0;
   size_t graph_bytes = 0;
   bool graphs_enabled = true;
+  std::unordered_set<uint64_t> graph_seen;  // keys called once (hashes)
   // Pinned host slots for job-array uploads of the merged executor.
   struct Staging {
     uint8_t* host = nullptr;

@@ -93,6 +93,19 @@ int fpxg_ntt_pass_dev(uint32_t p, uint32_t root, uint32_t n, int pass, uint32_t*
  * levels *col_levels in tiles of 2^*tile_cols_log columns. */
 int fpxg_ntt_plan_info(uint32_t n, int* row_log, int* col_levels, int* tile_cols_log);
 
+/* Autotuner (no counterpart in the reference, whose stage split is fixed):
+ * for every size 2^min_log..2^max_log, times the forward transform under each
+ * admissible pass split (`reps` runs each) and keeps the fastest for single
+ * calls.  Optional outputs per size: the chosen column levels (0 = one row
+ * pass) and its time in microseconds. */
+int fpxg_autotune(int min_log, int max_log, int reps, int* chosen_col_levels, float* best_us);
+
+/* Library options (no counterpart in the reference).
+ *   FPXG_OPT_GRAPHS: 1 (default) replays repeated single series calls from
+ *   captured hipGraphs; 0 drops the cached graphs and launches every call. */
+enum { FPXG_OPT_GRAPHS = 0 };
+int fpxg_set_option(int option, int64_t value);
+
 /* Device memory and stream plumbing for the *_dev entry points. */
 void* fpxg_dev_alloc(size_t bytes);
 int fpxg_dev_free(void* ptr);

@@ -79,3 +79,21 @@ def test_many_keys_and_release(fa, oracle):
     for a, n, da, out in outs[:5]:
         fa.series_dev("inv", P, da, n, n, out)
         assert np.array_equal(out.to_host(), oracle.series("inv", P, a, n)), n
+
+
+def test_autotune_keeps_results_exact(fa, oracle):
+    """After the autotuner has picked pass splits, transforms and products
+    still give the oracle's limbs."""
+    plans = fa.autotune(10, 18, 2)
+    assert set(plans) == set(range(10, 19))
+    for lg, (cols, us) in plans.items():
+        assert 0 <= cols <= 12 and us > 0
+    for lg in (12, 13, 15, 17):
+        n = 1 << lg
+        x = oracle.MT(1 + n).rnd(167772161, n)
+        assert np.array_equal(fa.ntt_forward(167772161, x, root=3), oracle.ntt(167772161, x, root=3)), lg
+    a = oracle.MT(5).rnd(P, 70000)
+    b = oracle.MT(6).rnd(P, 50001)
+    assert np.array_equal(fa.conv(P, a, b), oracle.conv(P, a, b))
+    s = _input(oracle, "exp", P, 40000, 9)
+    assert np.array_equal(fa.exp(P, s, 40000), oracle.series("exp", P, s, 40000))
