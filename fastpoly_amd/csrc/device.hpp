@@ -62,6 +62,11 @@ struct Arith {
     if constexpr (LAZY) return min(a, a - p);
     return a;
   }
+  // From the inverse transforms' range [0, 4p) (lazy) to [0, 2p).
+  __device__ __forceinline__ uint32_t half(uint32_t a) const {
+    if constexpr (LAZY) return min(a, a - R);
+    return a;
+  }
 };
 
 // Full reduction for arbitrary odd p < 2^31 and t < p*2^32 (Garner, scalars):

@@ -81,10 +81,22 @@ __device__ __forceinline__ void dit_block(uint32_t* x, const uint32_t* tw, const
 #pragma unroll
     for (int k = 0; k < (1 << r); ++k) {
       if (k & h) continue;
-      const uint32_t t = (LO0 && (k & (h - 1)) == 0) ? x[k + h] : A.mul(x[k + h], tw[idx + (k & (h - 1))]);
-      const uint32_t a0 = x[k];
-      x[k] = A.add(a0, t);
-      x[k + h] = A.sub(a0, t);
+      if constexpr (LAZY) {
+        // Values live in [0, 4p) between levels (4p < 2^32): reduce the upper
+        // input to [0, 2p), take w * lower in [0, 2p) (the product accepts
+        // any factor below 2^32), and leave both outputs unreduced.
+        const uint32_t a0r = x[k];
+        const uint32_t a0 = min(a0r, a0r - A.R);
+        const uint32_t b = x[k + h];
+        const uint32_t t = (LO0 && (k & (h - 1)) == 0) ? min(b, b - A.R) : A.mul(b, tw[idx + (k & (h - 1))]);
+        x[k] = a0 + t;
+        x[k + h] = a0 - t + A.R;
+      } else {
+        const uint32_t t = (LO0 && (k & (h - 1)) == 0) ? x[k + h] : A.mul(x[k + h], tw[idx + (k & (h - 1))]);
+        const uint32_t a0 = x[k];
+        x[k] = A.add(a0, t);
+        x[k + h] = A.sub(a0, t);
+      }
     }
   }
 }
@@ -456,9 +468,13 @@ k_rows2(RowJob2 j0, const RowJob2* __restrict__ jobs) {
     inv_phases<LOGM, PL - 1, 0, LAZY>(x, X, g, 0u, J.f.planes_inv, A);
     constexpr int r0 = SC::r_of(0);
     constexpr int NE0 = PhaseGroups<LOGM, 0>::NG * (1 << r0);
+    // Inverse outputs are in [0, 4p) (lazy); the scale brings them to [0, 2p).
     if (flags & R2_SCALE) {
 #pragma unroll
       for (int i = 0; i < NE0; ++i) x[i] = A.mul(x[i], J.inv_n);
+    } else if (flags & (R2_NEG | R2_CANON)) {
+#pragma unroll
+      for (int i = 0; i < NE0; ++i) x[i] = A.half(x[i]);
     }
     if (flags & R2_NEG) {
 #pragma unroll
@@ -577,7 +593,7 @@ k_cols2(ColJob2 j0, const ColJob2* __restrict__ jobs, int L) {
       for (int k = 0; k < (1 << r0); ++k) {
         const uint32_t e = elem<S, 0>(PhaseGroups<S, 0, LOGC>::gid(g, u), k);
         const uint32_t gi = e * M + col;
-        const uint32_t v = A.canon(x[u * (1 << r0) + k]);
+        const uint32_t v = A.canon(A.half(x[u * (1 << r0) + k]));
         if (gi >= J.out_lo && gi < J.out_hi) gstore(J.dst + (gi - J.out_shift), neg ? (v == 0 ? 0u : J.f.p - v) : v);
       }
   }
