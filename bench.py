@@ -49,7 +49,7 @@ def parse():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--ntt-reps", type=int, default=50)
     ap.add_argument("--no-batch", action="store_true", help="skip the uneven-batch leg")
     ap.add_argument("--batch-convs", type=int, default=8192)
@@ -280,7 +280,12 @@ def serving_leg(fa, args):
             exact = False
             break
     span = last - first
+    cpu = None
+    if not args.no_cpu:
+        tuples = [("conv", r.p, r.a, r.b) if r.op == "conv" else (r.op, r.p, r.a, r.n) for r in reqs]
+        cpu = cpu_parallel_baseline(tuples, [g for g, _ in res], args.cpu_seconds)
     return {
+        "cpu_baseline": cpu,
         "workload": "Poisson stream: conv 50% / inv 20% / log 10% / exp 10% / sqrt 10%, lengths log-uniform "
                     "[2^10,2^20], moduli 998244353 70% / 1000000007 30%, host inputs",
         "requests": len(reqs), "offered_rate_per_s": args.serve_rate,
@@ -379,9 +384,14 @@ def batched_leg(fa, args):
         times.append(time.perf_counter() - t0)
     t_batch = min(times)
     st = [arr_b[i].status for i in range(k)]
-    exact = bool(np.array_equal(dout_call.to_host(), dout_batch.to_host())) and not any(st)
+    host_batch = dout_batch.to_host()
+    exact = bool(np.array_equal(dout_call.to_host(), host_batch)) and not any(st)
     for d in (din, dout_call, dout_batch):
         d.free()
+    cpu = None
+    if not args.no_cpu:
+        outs = [host_batch[offs_o[i]:offs_o[i] + sizes_out[i]] for i in range(k)]
+        cpu = cpu_parallel_baseline(reqs, outs, args.cpu_seconds)
     return {
         "workload": f"{args.batch_convs} convs (half mod 998244353, half mod 1000000007; lengths log-uniform "
                     f"[2^6,2^18]) + {args.batch_series} inv/exp (n log-uniform [2^8,2^16]), inputs in HBM",
@@ -390,7 +400,57 @@ def batched_leg(fa, args):
         "batched_coeffs_per_s": tot_out / t_batch, "per_call_coeffs_per_s": tot_out / t_call,
         "speedup_vs_per_call": round(t_call / t_batch, 3), "bit_exact_vs_per_call": exact,
         "batched_reps_s": [round(x, 5) for x in times], "setup_s": round(gen_s, 2),
+        "cpu_baseline": cpu,
     }
+
+
+def cpu_parallel_baseline(reqs, gpu_outs, seconds, max_threads=16):
+    """The reference library on this host's cores (one request per thread at a
+    time, up to 16 threads: the CPU share of one GPU), over a shuffled sample of
+    the same requests for about `seconds`; every CPU result is compared with the
+    GPU's.  Reports output limbs/s and the single-request CPU latency."""
+    import threading
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle as po
+
+    kind = "reference" if po.ref() is not None else "port"
+    conv = po.ref_conv if kind == "reference" else po.conv
+    series = po.ref_series if kind == "reference" else po.series
+    threads = max(1, min(max_threads, os.cpu_count() or 1))
+    order = list(np.random.default_rng(7).permutation(len(reqs)))
+    lock = threading.Lock()
+    it = iter(order)
+    done = []
+    t_start = time.perf_counter()
+    deadline = t_start + seconds
+
+    def worker():
+        while time.perf_counter() < deadline:
+            with lock:
+                i = next(it, None)
+            if i is None:
+                return
+            r = reqs[i]
+            t0 = time.perf_counter()
+            out = conv(r[1], r[2], r[3]) if r[0] == "conv" else series(r[0], r[1], r[2], r[3])
+            dt = time.perf_counter() - t0
+            g = gpu_outs[i]
+            ok = not isinstance(g, Exception) and np.array_equal(out, g)
+            with lock:
+                done.append((out.size, dt, ok))
+
+    ts = [threading.Thread(target=worker) for _ in range(threads)]
+    for t in ts:
+        t.start()
+    for t in ts:
+        t.join()
+    el = time.perf_counter() - t_start
+    lat = np.array([d[1] for d in done]) * 1e3 if done else np.zeros(1)
+    return {"value": sum(d[0] for d in done) / el, "unit": "coeffs/s", "cores": threads, "kind": kind,
+            "sample": f"{len(done)} of {len(reqs)} requests (shuffled), {el:.1f} s on {threads} threads",
+            "bit_exact_vs_gpu": bool(done) and all(d[2] for d in done),
+            "request_ms_p50": round(float(np.percentile(lat, 50)), 3),
+            "request_ms_p99": round(float(np.percentile(lat, 99)), 3)}
 
 
 def cpu_baseline(args, a, b, inv_in, log_in, exp_in, sq_host, gpu_out):

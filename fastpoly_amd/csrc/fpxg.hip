@@ -1159,6 +1159,12 @@ int fpxg_server_submit(fpxg_server* h, fpxg_req* req, uint64_t* id) {
   auto it = std::make_unique<SItem>();
   it->user = req;
   it->t_submit = S.now();
+  const int rc = S.stage(*it);  // this thread copies and scans the input
+  if (rc != OK) {
+    S.pinned.free(it->pin_in, it->pin_in_cls);
+    t_last_error = "invalid request";
+    return rc;
+  }
   {
     std::lock_guard<std::mutex> g(S.mu);
     if (!S.fatal.empty() || S.stopping) {
@@ -1178,14 +1184,19 @@ int fpxg_server_submit(fpxg_server* h, fpxg_req* req, uint64_t* id) {
 static int server_collect(fpxg_server* h, uint64_t id, fpxg_result* res, bool block) {
   if (!h) return E_ARG;
   Server& S = *h->s;
-  std::unique_lock<std::mutex> g(S.mu);
-  if (!S.outstanding.count(id)) return E_ARG;
-  if (block) S.cv_done.wait(g, [&] { return S.finished.count(id) > 0; });
-  auto f = S.finished.find(id);
-  if (f == S.finished.end()) return E_BUSY;
-  if (res) *res = f->second;
-  S.finished.erase(f);
-  S.outstanding.erase(id);
+  std::unique_ptr<SItem> it;
+  {
+    std::unique_lock<std::mutex> g(S.mu);
+    if (!S.outstanding.count(id)) return E_ARG;
+    if (block) S.cv_done.wait(g, [&] { return S.finished.count(id) > 0; });
+    auto f = S.finished.find(id);
+    if (f == S.finished.end()) return E_BUSY;
+    it = std::move(f->second);
+    S.finished.erase(f);
+    S.outstanding.erase(id);
+  }
+  S.finish(*it);  // result copy in this thread
+  if (res) *res = it->result;
   return OK;
 }
 

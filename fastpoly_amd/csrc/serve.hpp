@@ -32,6 +32,7 @@ class PinnedPool {
     int k = 12;
     while (((size_t)1 << k) < bytes) ++k;
     *cls = k;
+    std::lock_guard<std::mutex> g(mu_);
     auto& fl = free_[k];
     if (!fl.empty()) {
       void* p = fl.back();
@@ -44,12 +45,15 @@ class PinnedPool {
     return p;
   }
   void free(void* p, int cls) {
-    if (p) free_[cls].push_back(p);
+    if (!p) return;
+    std::lock_guard<std::mutex> g(mu_);
+    free_[cls].push_back(p);
   }
 
  private:
   std::map<int, std::vector<void*>> free_;
   std::vector<void*> all_;
+  std::mutex mu_;
 };
 
 struct SItem {
@@ -67,6 +71,8 @@ struct SItem {
   hipEvent_t ev = nullptr;
   double t_submit = 0, t_admit = 0, t_done = 0;
   int status = OK;
+  fpxg_result result{};
+  bool has_scan = false;  // input scanned at submit (series ops)
 };
 
 // Host scan of a series input (same fields as the device scan).
@@ -94,13 +100,16 @@ struct Server {
   std::mutex mu;
   std::condition_variable cv_work, cv_done;
   std::deque<std::unique_ptr<SItem>> incoming;
-  std::map<uint64_t, fpxg_result> finished;
+  std::map<uint64_t, std::unique_ptr<SItem>> finished;  // completed, result not yet collected
   uint64_t next_id = 1;
   bool stopping = false;
   fpxg_server_stats st{};  // submitted / completed under mu; the rest published by the scheduler
   std::set<uint64_t> outstanding;
   std::string fatal;
   fpxg_server_stats sched{};  // scheduler-thread counters
+  // Host-time breakdown, printed at destruction when FPXG_TRACE_SERVER is set.
+  double tr_admit = 0, tr_run = 0, tr_drain = 0, tr_idle = 0;
+  uint64_t tr_attempts = 0, tr_rollbacks = 0;
 
   // Scheduler-thread state.
   std::deque<std::unique_ptr<SItem>> waiting;
@@ -122,6 +131,9 @@ struct Server {
     cv_work.notify_all();
     th.join();
     for (hipEvent_t e : ev_pool) (void)hipEventDestroy(e);
+    if (getenv("FPXG_TRACE_SERVER"))
+      fprintf(stderr, "fpxg server: host s admit %.3f run %.3f drain %.3f idle %.3f; admit attempts %llu rollbacks %llu\n",
+              tr_admit, tr_run, tr_drain, tr_idle, (unsigned long long)tr_attempts, (unsigned long long)tr_rollbacks);
   }
 
   double now() const {
@@ -139,22 +151,60 @@ struct Server {
     return e;
   }
 
+  // The result (pin_out) is copied into the caller's buffer when collected.
   void complete(std::unique_ptr<SItem> it) {
     it->t_done = now();
-    fpxg_result r{};
+    fpxg_result& r = it->result;
     r.status = it->status;
     r.out_len = it->status == OK ? it->q.out_len : 0;
     r.submit_s = it->t_submit;
     r.admit_s = it->t_admit;
     r.done_s = it->t_done;
-    it->user->status = r.status;
-    it->user->out_len = r.out_len;
     {
       std::lock_guard<std::mutex> g(mu);
-      finished[it->id] = r;
+      const uint64_t id = it->id;
+      finished[id] = std::move(it);
       ++st.completed;
     }
     cv_done.notify_all();
+  }
+
+  // Caller's thread, at submit: the input goes into a pinned block and series
+  // inputs are scanned, so the scheduler only records and launches.
+  int stage(SItem& it) {
+    const fpxg_req& q = *it.user;
+    size_t xa = 0, xb = 0;
+    if (q.op == FPXG_CONV) {
+      if (q.la && q.lb && q.n) {
+        xa = std::min(q.la, q.n);
+        xb = std::min(q.lb, q.n);
+      }
+    } else {
+      xa = std::min(q.la, q.n);
+    }
+    if ((xa && !q.a) || (xb && !q.b)) return E_ARG;
+    if (xa + xb > ((size_t)1 << 33)) return E_SIZE;
+    if (xa + xb) {
+      it.pin_in = pinned.alloc((xa + xb) * 4, &it.pin_in_cls);
+      memcpy(it.pin_in, q.a, xa * 4);
+      if (xb) memcpy((uint32_t*)it.pin_in + xa, q.b, xb * 4);
+    }
+    if (q.op != FPXG_CONV) {
+      it.scan = host_scan(q.a, xa);
+      it.has_scan = true;
+    }
+    return OK;
+  }
+
+  // Caller's thread, at collect: result into the caller's buffer, blocks back to the pool.
+  void finish(SItem& it) {
+    if (it.result.status == OK && it.result.out_len && it.pin_out)
+      memcpy(it.user->out, it.pin_out, it.result.out_len * 4);
+    it.user->status = it.result.status;
+    it.user->out_len = it.result.out_len;
+    pinned.free(it.pin_in, it.pin_in_cls);
+    pinned.free(it.pin_out, it.pin_out_cls);
+    it.pin_in = it.pin_out = nullptr;
   }
 
   void release_device(SItem& it) {
@@ -191,8 +241,6 @@ struct Server {
       complete(std::move(it));
       return true;
     }
-    const uint32_t* ua = q.a;
-    const uint32_t* ub = q.b;
     q.a = it->din;
     q.b = q.op == FPXG_CONV ? it->din + pr.xa : nullptr;
     q.la = pr.xa;
@@ -200,10 +248,11 @@ struct Server {
     q.out = it->dout;
     q.out_cap = pr.outc;
     if (pr.char2) {  // characteristic two: immediate, with its own read-back
-      FPXG_HIP(hipMemcpyAsync(it->din, ua, pr.xa * 4, hipMemcpyHostToDevice, c.stream));
+      if (pr.xa) FPXG_HIP(hipMemcpyAsync(it->din, it->pin_in, pr.xa * 4, hipMemcpyHostToDevice, c.stream));
       try {
         series_dev(c, c.field(q.p), FPXG_SQRT, it->din, pr.xa, q.n, it->dout);
-        FPXG_HIP(hipMemcpyAsync(it->user->out, it->dout, pr.outc * 4, hipMemcpyDeviceToHost, c.stream));
+        it->pin_out = pinned.alloc(pr.outc * 4, &it->pin_out_cls);
+        FPXG_HIP(hipMemcpyAsync(it->pin_out, it->dout, pr.outc * 4, hipMemcpyDeviceToHost, c.stream));
         FPXG_HIP(hipStreamSynchronize(c.stream));
       } catch (const Error& e) {
         it->status = e.code;
@@ -213,7 +262,6 @@ struct Server {
       complete(std::move(it));
       return true;
     }
-    if (pr.scan) it->scan = host_scan(ua, pr.xa);
     const int rs = record_request(c, q, pr, pr.scan ? &it->scan : nullptr, it->chain);
     if (rs == E_NOMEM && !running.empty()) {
       release_device(*it);
@@ -232,10 +280,7 @@ struct Server {
       complete(std::move(it));
       return true;
     }
-    // Input upload through a pinned block (asynchronous, ordered before the schedule).
-    it->pin_in = pinned.alloc(std::max<size_t>(in_words, 1) * 4, &it->pin_in_cls);
-    memcpy(it->pin_in, ua, pr.xa * 4);
-    if (pr.xb) memcpy((uint32_t*)it->pin_in + pr.xa, ub, pr.xb * 4);
+    // Input upload from the pinned block staged at submit (ordered before the schedule).
     if (in_words) FPXG_HIP(hipMemcpyAsync(it->din, it->pin_in, in_words * 4, hipMemcpyHostToDevice, c.stream));
     it->t_admit = now();
     sched.peak_in_use_bytes = std::max(sched.peak_in_use_bytes, c.arena->in_use());
@@ -273,15 +318,20 @@ struct Server {
           inflight.pop_front();
         }
         // 1. Admission.
+        double t0 = host_seconds();
         while (!waiting.empty()) {
           std::unique_ptr<SItem> it = std::move(waiting.front());
           waiting.pop_front();
+          ++tr_attempts;
           if (!admit(it)) {
+            ++tr_rollbacks;
             waiting.push_front(std::move(it));
             break;
           }
           progressed = true;
         }
+        tr_admit += host_seconds() - t0;
+        t0 = host_seconds();
         sched.max_active = std::max<uint64_t>(sched.max_active, running.size());
         // 2. A few merged levels, at most two iterations ahead of the device.
         if (!running.empty() && inflight.size() < 2) {
@@ -294,6 +344,8 @@ struct Server {
           inflight.push_back(e);
           progressed = true;
         }
+        tr_run += host_seconds() - t0;
+        t0 = host_seconds();
         // 3. Finished schedules: download, release device blocks.
         for (size_t i = 0; i < running.size();) {
           SItem& it = *running[i];
@@ -314,25 +366,23 @@ struct Server {
         while (!draining.empty() && hipEventQuery(draining.front()->ev) == hipSuccess) {
           std::unique_ptr<SItem> it = std::move(draining.front());
           draining.pop_front();
-          memcpy(it->user->out, it->pin_out, it->pr.outc * 4);
-          pinned.free(it->pin_in, it->pin_in_cls);
-          pinned.free(it->pin_out, it->pin_out_cls);
           ev_pool.push_back(it->ev);
+          it->ev = nullptr;
           complete(std::move(it));
           progressed = true;
         }
+        tr_drain += host_seconds() - t0;
       } catch (const std::exception& e) {
         // A device failure ends the server: fail everything outstanding.
         std::lock_guard<std::mutex> g(mu);
         fatal = e.what();
         auto fail = [&](std::unique_ptr<SItem>& it) {
-          fpxg_result r{};
-          r.status = E_DEVICE;
-          r.submit_s = it->t_submit;
-          r.done_s = now();
-          it->user->status = E_DEVICE;
-          it->user->out_len = 0;
-          finished[it->id] = r;
+          it->result = fpxg_result{};
+          it->result.status = E_DEVICE;
+          it->result.submit_s = it->t_submit;
+          it->result.done_s = now();
+          const uint64_t id = it->id;
+          finished[id] = std::move(it);
         };
         for (auto& it : waiting) fail(it);
         for (auto& it : running) fail(it);
@@ -346,7 +396,11 @@ struct Server {
         cv_done.notify_all();
         return;
       }
-      if (!progressed) std::this_thread::sleep_for(std::chrono::microseconds(20));
+      if (!progressed) {
+        const double t0 = host_seconds();
+        std::this_thread::sleep_for(std::chrono::microseconds(20));
+        tr_idle += host_seconds() - t0;
+      }
     }
   }
 };

@@ -154,10 +154,11 @@ int fpxg_batch_u32(fpxg_req* reqs, size_t count);
  * submitted requests while the device arena stays within hbm_budget_bytes,
  * and runs every admitted request's schedule merged with the others', a few
  * levels (levels_per_iteration) per loop, so requests that arrive while others
- * run join their launches.  Requests carry HOST pointers; the caller keeps a,
- * b, out and the fpxg_req itself valid until the request completes, when
- * status and out_len are written into it and out holds the result.  Results
- * are bit-identical to the single calls.  Destroy servers before
+ * run join their launches.  Requests carry HOST pointers: fpxg_server_submit
+ * copies the inputs (a, b) before returning; the caller keeps out and the
+ * fpxg_req itself valid until the request is collected by fpxg_server_wait or
+ * fpxg_server_poll, which write status and out_len into it and the result into
+ * out.  Results are bit-identical to the single calls.  Destroy servers before
  * fpxg_shutdown.
  */
 typedef struct fpxg_server fpxg_server;
@@ -173,6 +174,7 @@ typedef struct {
 
 /* hbm_budget_bytes 0: half the arena; levels_per_iteration 0: 2. */
 fpxg_server* fpxg_server_create(size_t hbm_budget_bytes, int levels_per_iteration);
+/* FPXG_E_ARG for a null input with a nonzero length (nothing is queued). */
 int fpxg_server_submit(fpxg_server* s, fpxg_req* req, uint64_t* id);
 /* Blocks until request id completes; fills *res (may be null). */
 int fpxg_server_wait(fpxg_server* s, uint64_t id, fpxg_result* res);
