@@ -756,3 +756,172 @@ int fpo_sqrt_series(const fpo_field* f, const uint32_t* a, size_t la, size_t n, 
   free(c); free(sq); free(res);
   return rc;
 }
+
+/* ------------------------------------------------------------------------
+ * poly::pow (detail/poly/power.hpp:17-87): a^k mod x^n.
+ * ---------------------------------------------------------------------- */
+static int conv_trunc(const fpo_field* f, const uint32_t* a, size_t la, const uint32_t* b, size_t lb,
+                      size_t n, uint32_t** res, size_t* len) {
+  const size_t cap = (la && lb) ? (la + lb - 1 < n ? la + lb - 1 : n) : 0;
+  uint32_t* out = (uint32_t*)malloc((cap ? cap : 1) * 4);
+  if (!out) return FPO_E_NOMEM;
+  size_t got = 0;
+  const int rc = fpo_conv(f, a, la, b, lb, n, out, &got);
+  if (rc) { free(out); return rc; }
+  *res = out;
+  *len = got;
+  return FPO_OK;
+}
+
+int fpo_poly_pow(const fpo_field* f, const uint32_t* a, size_t la, uint64_t k, size_t n, uint32_t* out) {
+  if (n == 0) return FPO_OK;
+  memset(out, 0, n * 4);
+  if (k == 0) { out[0] = fpo_from_int(f, 1); return FPO_OK; }                       /* :20-24 */
+  if (k == 1) { memcpy(out, a, (la < n ? la : n) * 4); return FPO_OK; }             /* :25-29 */
+  size_t v = 0;
+  while (v < la && a[v] == 0) ++v;                                                  /* :30-32 */
+  if (v == la) return FPO_OK;
+  if (v != 0 && k > (n - 1) / v) return FPO_OK;                                     /* :34 */
+  const size_t sh = (size_t)(v * k), inner = n - sh;
+  size_t lu = la - v < inner ? la - v : inner;
+  const uint32_t* u = a + v;
+  while (lu > 0 && u[lu - 1] == 0) --lu;                                            /* trim, :39 */
+  if (lu == 1) { out[sh] = fpo_pow(f, u[0], k); return FPO_OK; }                   /* :40-44 */
+  if (k > 64 && lu == 2 && inner <= f->p) {                                        /* :45-60 */
+    const uint32_t e = (uint32_t)(k % f->p);
+    const size_t cnt = inner < (size_t)e + 1 ? inner : (size_t)e + 1;
+    uint32_t* den = 0;
+    int rc = inv_series(f, cnt - 1, &den);
+    if (rc) return rc;
+    out[sh] = fpo_pow(f, u[0], k);
+    const uint32_t ratio = fpo_mul(f, u[1], fpo_minv(f, u[0]));
+    uint32_t factor = fpo_mul(f, ratio, fpo_from_int(f, e));
+    for (size_t j = 1; j < cnt; ++j) {
+      out[sh + j] = fpo_mul(f, fpo_mul(f, out[sh + j - 1], factor), den[j]);
+      factor = fpo_sub(f, factor, ratio);
+    }
+    free(den);
+    return FPO_OK;
+  }
+  if (k <= 64 || inner > f->p) {                                                    /* :62-73 */
+    uint32_t* r = (uint32_t*)malloc(4);
+    uint32_t* base = (uint32_t*)malloc(lu * 4);
+    if (!r || !base) { free(r); free(base); return FPO_E_NOMEM; }
+    r[0] = fpo_from_int(f, 1);
+    size_t lr = 1, lb = lu;
+    memcpy(base, u, lu * 4);
+    for (uint64_t e = k; e; e >>= 1) {
+      if (e & 1) {
+        uint32_t* t = 0; size_t lt = 0;
+        int rc = conv_trunc(f, r, lr, base, lb, inner, &t, &lt);
+        if (rc) { free(r); free(base); return rc; }
+        free(r); r = t; lr = lt;
+      }
+      if (e > 1) {
+        uint32_t* t = 0; size_t lt = 0;
+        int rc = conv_trunc(f, base, lb, base, lb, inner, &t, &lt);
+        if (rc) { free(r); free(base); return rc; }
+        free(base); base = t; lb = lt;
+      }
+    }
+    memcpy(out + sh, r, (lr < inner ? lr : inner) * 4);
+    free(r); free(base);
+    return FPO_OK;
+  }
+  /* exp(k log(a / c)) * c^k (:75-86) */
+  const uint32_t c = u[0];
+  uint32_t* nrm = (uint32_t*)malloc(lu * 4);
+  uint32_t* l = (uint32_t*)malloc(inner * 4);
+  uint32_t* r = (uint32_t*)malloc(inner * 4);
+  int rc = (!nrm || !l || !r) ? FPO_E_NOMEM : FPO_OK;
+  if (!rc) {
+    const uint32_t ci = fpo_minv(f, c);
+    for (size_t i = 0; i < lu; ++i) nrm[i] = fpo_mul(f, u[i], ci);
+    rc = fpo_log(f, nrm, lu, inner, l);
+  }
+  if (!rc) {
+    const uint32_t kk = fpo_from_int(f, k);
+    for (size_t i = 0; i < inner; ++i) l[i] = fpo_mul(f, l[i], kk);
+    rc = fpo_exp(f, l, inner, inner, r);
+  }
+  if (!rc) {
+    const uint32_t ck = fpo_pow(f, c, k);
+    for (size_t i = 0; i < inner; ++i) out[sh + i] = fpo_mul(f, r[i], ck);
+  }
+  free(nrm); free(l); free(r);
+  return rc;
+}
+
+/* ------------------------------------------------------------------------
+ * poly::divmod (detail/poly/division.hpp:12-104): a = b q + r, deg r < deg b,
+ * both trimmed.  q holds at least la limbs, r at least la limbs.
+ * ---------------------------------------------------------------------- */
+int fpo_divmod(const fpo_field* f, const uint32_t* a, size_t la, const uint32_t* b, size_t lb,
+               uint32_t* q, size_t* qlen, uint32_t* r, size_t* rlen) {
+  size_t an = la, bn = lb;
+  while (an && a[an - 1] == 0) --an;
+  while (bn && b[bn - 1] == 0) --bn;
+  *qlen = *rlen = 0;
+  if (bn == 0) return FPO_E_DOMAIN;                                                 /* :17 */
+  if (an < bn) {                                                                    /* :18-22 */
+    memcpy(r, a, an * 4);
+    *rlen = an;
+    return FPO_OK;
+  }
+  if (bn == 1) {                                                                    /* :23-28 */
+    const uint32_t bi = fpo_minv(f, b[0]);
+    for (size_t i = 0; i < an; ++i) q[i] = fpo_mul(f, a[i], bi);
+    *qlen = an;
+    return FPO_OK;
+  }
+  const size_t dn = an - bn + 1, rn = bn - 1;
+  if (bn <= 16 || dn <= 32) {                                                       /* long division :29-52 */
+    uint32_t* w = (uint32_t*)malloc(an * 4);
+    if (!w) return FPO_E_NOMEM;
+    memcpy(w, a, an * 4);
+    const uint32_t li = fpo_minv(f, b[bn - 1]);
+    for (size_t i = dn; i-- > 0;) {
+      const uint32_t fac = fpo_mul(f, w[i + bn - 1], li);
+      q[i] = fac;
+      if (fac == 0) continue;
+      for (size_t j = 0; j < bn - 1; ++j) w[i + j] = fpo_sub(f, w[i + j], fpo_mul(f, fac, b[j]));
+    }
+    size_t lr = rn;
+    while (lr && w[lr - 1] == 0) --lr;
+    memcpy(r, w, lr * 4);
+    *qlen = dn;
+    *rlen = lr;
+    free(w);
+    return FPO_OK;
+  }
+  /* q = rev(rev(a) inv(rev(b)) mod x^dn) (:53-62) */
+  const size_t lrb = bn < dn ? bn : dn;
+  uint32_t* ra = (uint32_t*)malloc(dn * 4);
+  uint32_t* rb = (uint32_t*)malloc(lrb * 4);
+  uint32_t* ib = (uint32_t*)malloc(dn * 4);
+  int rc = (!ra || !rb || !ib) ? FPO_E_NOMEM : FPO_OK;
+  uint32_t* rq = 0;
+  size_t lq = 0;
+  if (!rc) {
+    for (size_t i = 0; i < dn; ++i) ra[i] = a[an - 1 - i];
+    for (size_t i = 0; i < lrb; ++i) rb[i] = b[bn - 1 - i];
+    rc = fpo_inv(f, rb, lrb, dn, ib);
+  }
+  if (!rc) rc = conv_trunc(f, ra, dn, ib, dn, dn, &rq, &lq);
+  if (!rc) {
+    for (size_t i = 0; i < dn; ++i) q[i] = (dn - 1 - i) < lq ? rq[dn - 1 - i] : 0;  /* resize, reverse */
+    *qlen = dn;
+  }
+  /* r = (a - b q) mod x^rn, trimmed (:63-75) */
+  uint32_t* pr = 0;
+  size_t lp = 0;
+  if (!rc) rc = conv_trunc(f, b, lb, q, dn, rn, &pr, &lp);
+  if (!rc) {
+    for (size_t i = 0; i < rn; ++i) r[i] = fpo_sub(f, a[i], i < lp ? pr[i] : 0);
+    size_t lr = rn;
+    while (lr && r[lr - 1] == 0) --lr;
+    *rlen = lr;
+  }
+  free(ra); free(rb); free(ib); free(rq); free(pr);
+  return rc;
+}

@@ -16,6 +16,8 @@
  *   conv / crt ....... published conv checksums (998244353, 1e9+7) + O(n^2)
  *                      integer products
  *   inv/log/exp/sqrt . published checksums + coefficient recurrences + identities
+ *   pow / divmod ..... published pow / div / mod / divmod checksums + products
+ *                      (a^j a^k = a^(j+k), b q + r = a with deg r < deg b)
  */
 #ifndef FPX_ORACLE_H
 #define FPX_ORACLE_H
@@ -97,6 +99,13 @@ int fpo_inv(const fpo_field* f, const uint32_t* a, size_t la, size_t n, uint32_t
 int fpo_log(const fpo_field* f, const uint32_t* a, size_t la, size_t n, uint32_t* out);
 int fpo_exp(const fpo_field* f, const uint32_t* a, size_t la, size_t n, uint32_t* out);
 int fpo_sqrt_series(const fpo_field* f, const uint32_t* a, size_t la, size_t n, uint32_t* out);
+
+/* poly::pow (detail/poly/power.hpp:17-87): a^k mod x^n, n limbs. */
+int fpo_poly_pow(const fpo_field* f, const uint32_t* a, size_t la, uint64_t k, size_t n, uint32_t* out);
+/* poly::divmod (detail/poly/division.hpp:12-104): a = b q + r, deg r < deg b,
+ * q and r trimmed; q and r must each hold la limbs.  FPO_E_DOMAIN for b == 0. */
+int fpo_divmod(const fpo_field* f, const uint32_t* a, size_t la, const uint32_t* b, size_t lb,
+               uint32_t* q, size_t* qlen, uint32_t* r, size_t* rlen);
 
 /* O(la*lb) product with exact integer accumulation (the independent pin). */
 int fpo_conv_naive(const fpo_field* f, const uint32_t* a, size_t la, const uint32_t* b,

@@ -87,6 +87,9 @@ def lib():
         L.fpo_crt_prime_count.argtypes = [ctypes.c_uint32, ctypes.c_size_t]
         for name in ("fpo_inv", "fpo_log", "fpo_exp", "fpo_sqrt_series"):
             getattr(L, name).argtypes = [FP, _u32p, ctypes.c_size_t, ctypes.c_size_t, _u32p]
+        L.fpo_poly_pow.argtypes = [FP, _u32p, ctypes.c_size_t, ctypes.c_uint64, ctypes.c_size_t, _u32p]
+        _szp = ctypes.POINTER(ctypes.c_size_t)
+        L.fpo_divmod.argtypes = [FP, _u32p, ctypes.c_size_t, _u32p, ctypes.c_size_t, _u32p, _szp, _u32p, _szp]
         _lib = L
     return _lib
 
@@ -224,6 +227,31 @@ def series(op: str, p: int, a, n: int) -> np.ndarray:
     if rc:
         raise OracleError(rc, op)
     return out[:n].copy()
+
+
+def poly_pow(p: int, a, k: int, n: int) -> np.ndarray:
+    """poly::pow(a, k, n) (power.hpp:17-87)."""
+    f = field(p)
+    a = _u32(a)
+    out = np.zeros(max(n, 1), dtype=np.uint32)
+    rc = lib().fpo_poly_pow(ctypes.byref(f), _ptr(a), a.size, k, n, _ptr(out))
+    if rc:
+        raise OracleError(rc, "pow")
+    return out[:n].copy()
+
+
+def divmod(p: int, a, b):
+    """poly::divmod(a, b) (division.hpp:12-104) -> (q, r), both trimmed."""
+    f = field(p)
+    a, b = _u32(a), _u32(b)
+    q = np.zeros(max(a.size, 1), dtype=np.uint32)
+    r = np.zeros(max(a.size, 1), dtype=np.uint32)
+    lq, lr = ctypes.c_size_t(0), ctypes.c_size_t(0)
+    rc = lib().fpo_divmod(ctypes.byref(f), _ptr(a), a.size, _ptr(b), b.size, _ptr(q), ctypes.byref(lq),
+                          _ptr(r), ctypes.byref(lr))
+    if rc:
+        raise OracleError(rc, "divmod")
+    return q[: lq.value].copy(), r[: lr.value].copy()
 
 
 # --------------------------------------------------------------------------

@@ -52,6 +52,30 @@ def bench_case(po, p, op, n):
     raise KeyError(op)
 
 
+def rotl64(x, r):
+    return ((x << r) | (x >> (64 - r))) & ((1 << 64) - 1)
+
+
+def division_case(po, p, n):
+    """div / mod / divmod bench inputs (bench/bench.cpp:255-263): a of n, b of max(1, n/2)."""
+    g = po.MT(4 + n)
+    a = g.rnd(p, n)
+    return a, g.rnd(p, max(1, n // 2))
+
+
+def division_checksum(po, op, q, r):
+    """checksum(q) ^ rotl(checksum(r), 17) with the operand div or mod does not return empty."""
+    empty = np.zeros(0, np.uint32)
+    if op == "div":
+        r = empty
+    if op == "mod":
+        q = empty
+    return format(po.checksum(q) ^ rotl64(po.checksum(r), 17), "016x")
+
+
+POW_K = 1000003  # bench/bench.cpp:372-373
+
+
 def series_input(po, p, n, random_a0=False):
     a = po.MT(3 + n).rnd(p, n)
     if random_a0:

@@ -326,3 +326,66 @@ def test_ref_workload_checksums(oracle, key):
     else:
         got = run_series_op(po, p, op, n)
     assert gi.hexsum(po, got) == want
+
+
+# ---------------------------------------------------------------- pow, divmod
+POW_MAX = 1 << 16
+
+
+@pytest.mark.parametrize("p,op,n,want", list(_golden_rows({"pow"}, POW_MAX)))
+def test_golden_pow(oracle, p, op, n, want):
+    a = gi.bench_case(oracle, p, "pow", n)
+    assert gi.hexsum(oracle, oracle.poly_pow(p, a, gi.POW_K, n)) == want
+
+
+@pytest.mark.parametrize("p,op,n,want", list(_golden_rows({"div", "mod", "divmod"}, POW_MAX)))
+def test_golden_division(oracle, p, op, n, want):
+    a, b = gi.division_case(oracle, p, n)
+    q, r = oracle.divmod(p, a, b)
+    assert gi.division_checksum(oracle, op, q, r) == want
+
+
+@pytest.mark.parametrize("p", [998244353, 1000000007, 100003, 3, 2])
+def test_pow_matches_repeated_products(oracle, p):
+    """a^k by k-1 naive products (every branch of power.hpp: shifts, a
+    binomial, binary powers, exp(k log))."""
+    rng = random.Random(p)
+    for trial in range(30):
+        n = rng.randint(1, 60)
+        la = rng.randint(0, 12)
+        a = np.array([rng.randrange(p) for _ in range(la)], dtype=np.uint32)
+        if la and trial % 3 == 0:
+            a[: rng.randint(0, la - 1)] = 0          # valuation
+        if la > 2 and trial % 5 == 0:
+            a[2:] = 0                                 # two-term polynomial
+        k = rng.choice([0, 1, 2, 3, 7, 64, 65, 200])
+        want = np.zeros(n, np.uint32)
+        want[0] = oracle.from_int(p, 1)
+        for _ in range(k):
+            prod = oracle.conv_naive(p, want, a, n) if la else np.zeros(n, np.uint32)
+            want = np.zeros(n, np.uint32)
+            want[: len(prod)] = prod[:n]
+        assert np.array_equal(oracle.poly_pow(p, a, k, n), want), (p, n, list(a), k)
+
+
+@pytest.mark.parametrize("p", [998244353, 1000000007, 100003, 2])
+def test_divmod_identity(oracle, p):
+    """b q + r = a with deg r < deg b, q and r trimmed, on every branch
+    (short divisor, long division, reversed-inverse quotient)."""
+    rng = random.Random(p + 1)
+    for la, lb in [(0, 1), (5, 9), (40, 1), (40, 3), (100, 17), (300, 100), (300, 250), (2000, 60), (5000, 2400)]:
+        a = np.array([rng.randrange(p) for _ in range(la)], dtype=np.uint32)
+        b = np.array([rng.randrange(p) for _ in range(lb)], dtype=np.uint32)
+        if lb and b[-1] == 0:
+            b[-1] = 1
+        q, r = oracle.divmod(p, a, b)
+        assert len(r) < lb and (len(r) == 0 or r[-1] != 0) and (len(q) == 0 or q[-1] != 0)
+        bq = oracle.conv_naive(p, b, q, la) if len(q) else np.zeros(0, np.uint32)
+        tot = np.zeros(max(la, 1), np.uint32)
+        tot[: len(bq)] = bq[:la] if la else bq[:0]
+        for i in range(len(r)):
+            tot[i] = (int(tot[i]) + int(r[i])) % p if p != 2 else int(tot[i]) ^ int(r[i])
+        at = a.copy()
+        assert np.array_equal(tot[:la], at), (p, la, lb)
+    with pytest.raises(oracle.OracleError):
+        oracle.divmod(p, np.ones(3, np.uint32), np.zeros(2, np.uint32))
