@@ -13,6 +13,7 @@ call needs the built library and a device, and raises otherwise.
 """
 from __future__ import annotations
 
+import builtins
 import ctypes
 import os
 
@@ -22,7 +23,7 @@ __all__ = [
     "DomainError", "NttSizeError", "DeviceError", "UnsupportedModulus", "lib", "lib_path",
     "conv", "inv", "log", "exp", "sqrt", "series", "ntt_forward", "ntt_inverse", "NttPlan",
     "release_scratch_memory", "scratch_memory_stats", "from_int", "to_int", "Mont", "DeviceArray",
-    "EXPORTED_SYMBOLS", "Request", "batch", "batch_submit", "wait", "Server",
+    "EXPORTED_SYMBOLS", "Request", "batch", "batch_submit", "wait", "Server", "pow", "divmod", "div", "mod",
 ]
 
 HERE = os.path.dirname(os.path.abspath(__file__))
@@ -42,6 +43,7 @@ EXPORTED_SYMBOLS = [
     "fpxg_ntt_pass_dev", "fpxg_ntt_plan_info", "fpxg_batch_submit", "fpxg_wait", "fpxg_batch_u32",
     "fpxg_server_create", "fpxg_server_submit", "fpxg_server_wait", "fpxg_server_poll", "fpxg_server_now",
     "fpxg_server_stats_get", "fpxg_server_destroy", "fpxg_autotune", "fpxg_set_option",
+    "fpxg_pow_u32", "fpxg_pow_dev", "fpxg_divmod_u32", "fpxg_divmod_dev",
 ]
 
 
@@ -161,6 +163,14 @@ def lib():
         L.fpxg_autotune.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_int),
                                     ctypes.POINTER(ctypes.c_float)]
         L.fpxg_set_option.argtypes = [ctypes.c_int, ctypes.c_int64]
+        L.fpxg_pow_u32.argtypes = [ctypes.c_uint32, _u32p, ctypes.c_size_t, ctypes.c_uint64, ctypes.c_size_t, _u32p]
+        L.fpxg_pow_dev.argtypes = [ctypes.c_uint32, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_uint64,
+                                   ctypes.c_size_t, ctypes.c_void_p]
+        L.fpxg_divmod_u32.argtypes = [ctypes.c_uint32, _u32p, ctypes.c_size_t, _u32p, ctypes.c_size_t, _u32p,
+                                      ctypes.c_size_t, _size_p, _u32p, ctypes.c_size_t, _size_p]
+        L.fpxg_divmod_dev.argtypes = [ctypes.c_uint32, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_void_p,
+                                      ctypes.c_size_t, ctypes.c_void_p, ctypes.c_size_t, _size_p,
+                                      ctypes.c_void_p, ctypes.c_size_t, _size_p]
         L.fpxg_server_destroy.restype = None
         L.fpxg_server_destroy.argtypes = [ctypes.c_void_p]
         _lib = L
@@ -203,7 +213,7 @@ def to_int(p: int, limbs) -> np.ndarray | int:
     if p == 2:
         conv = lambda v: int(v) & 1  # noqa: E731
     else:
-        rinv = pow(1 << 32, p - 2, p)
+        rinv = builtins.pow(1 << 32, p - 2, p)
         conv = lambda v: int(v) * rinv % p  # noqa: E731
     if np.isscalar(limbs):
         return conv(limbs)
@@ -569,3 +579,33 @@ OPT_GRAPHS = 0
 def set_graphs(enabled: bool) -> None:
     """Replay repeated single series calls from captured hipGraphs (default on)."""
     _check(lib().fpxg_set_option(OPT_GRAPHS, 1 if enabled else 0), "set_option")
+
+
+def pow(p: int, a, k: int, n: int) -> np.ndarray:  # noqa: A001 (the reference's name)
+    """poly::pow(a, k, n) (detail/poly/power.hpp:17-87)."""
+    a = _u32(a)
+    out = np.empty(max(n, 1), dtype=np.uint32)
+    _check(lib().fpxg_pow_u32(p, _ptr(a), a.size, int(k), n, _ptr(out)), "poly::pow")
+    return out[:n].copy()
+
+
+def divmod(p: int, a, b):  # noqa: A001
+    """poly::divmod(a, b) (detail/poly/division.hpp:12-104) -> (q, r), both trimmed."""
+    a, b = _u32(a), _u32(b)
+    cap = max(a.size, 1)
+    q = np.empty(cap, dtype=np.uint32)
+    r = np.empty(cap, dtype=np.uint32)
+    lq, lr = ctypes.c_size_t(0), ctypes.c_size_t(0)
+    _check(lib().fpxg_divmod_u32(p, _ptr(a), a.size, _ptr(b), b.size, _ptr(q), cap, ctypes.byref(lq), _ptr(r), cap,
+                                 ctypes.byref(lr)), "poly::divmod")
+    return q[: lq.value].copy(), r[: lr.value].copy()
+
+
+def div(p: int, a, b) -> np.ndarray:
+    """poly::div (division.hpp:86-87)."""
+    return divmod(p, a, b)[0]
+
+
+def mod(p: int, a, b) -> np.ndarray:
+    """poly::mod (division.hpp:89-100)."""
+    return divmod(p, a, b)[1]

@@ -10,7 +10,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 LIB = os.path.join(HERE, "libfpxg.so")
 SOURCES = [os.path.join(HERE, "csrc", "fpxg.hip")]
-DEPS = SOURCES + [os.path.join(HERE, "csrc", f) for f in ("device.hpp", "crt.hpp", "ntt2.hpp", "runtime.hpp", "series.hpp", "records.hpp", "ew.hpp", "batch.hpp", "serve.hpp", "graphs.hpp")] \
+DEPS = SOURCES + [os.path.join(HERE, "csrc", f) for f in ("device.hpp", "crt.hpp", "ntt2.hpp", "runtime.hpp", "series.hpp", "records.hpp", "ew.hpp", "batch.hpp", "serve.hpp", "graphs.hpp", "ops.hpp")] \
     + [os.path.join(ROOT, "include", "fpxg.h")]
 
 HIPCC_FLAGS = [

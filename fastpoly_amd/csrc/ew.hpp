@@ -114,6 +114,7 @@ __device__ __forceinline__ uint32_t ew_value(const EwRec& R, uint32_t i) {
     case EW_MUL_RECIP: return fs_mul(f, R.src[i], fs_inv(f, fs_from_int(f, R.c + i)));
     case EW_DERIV: return fs_mul(f, R.src[i + 1], fs_from_int(f, i + 1));
     case EW_INTEGRAL: return (i >= 1 && i - 1 < R.avail) ? fs_mul(f, R.src[i - 1], fs_inv(f, fs_from_int(f, i))) : 0u;
+    case EW_REVERSE: return R.src[R.n - 1 - i];
     default: return fs_sub(f, i < R.avail ? R.src[i] : 0u, R.src2[i]);
   }
 }

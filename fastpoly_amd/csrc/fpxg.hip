@@ -677,6 +677,7 @@ void conv_dev(Context& c, const HostField& T, const uint32_t* a, size_t la, cons
 #include "batch.hpp"
 #include "serve.hpp"
 #include "graphs.hpp"
+#include "ops.hpp"
 
 // ===========================================================================
 // C ABI
@@ -957,6 +958,58 @@ int fpxg_set_option(int option, int64_t value) {
       default:
         throw Error(E_ARG, "unknown option");
     }
+  });
+}
+
+int fpxg_pow_u32(uint32_t p, const uint32_t* a, size_t la, uint64_t k, size_t n, uint32_t* out) {
+  return guarded([&](Context& c) {
+    check_modulus(p);
+    if (n == 0) return;
+    if (!out || (la && !a)) throw Error(E_ARG, "null pointer");
+    const size_t use = std::min(la, n);
+    Upload u(c, a, use);
+    DevBuf o = c.buf(n);
+    pow_dev(c, c.field(p), u.buf.get(), use, k, n, o.get());
+    FPXG_HIP(hipMemcpyAsync(out, o.get(), n * 4, hipMemcpyDeviceToHost, c.stream));
+    FPXG_HIP(hipStreamSynchronize(c.stream));
+  });
+}
+
+int fpxg_pow_dev(uint32_t p, const uint32_t* a, size_t la, uint64_t k, size_t n, uint32_t* out) {
+  return guarded([&](Context& c) {
+    check_modulus(p);
+    if (n == 0) return;
+    if (!out || (la && !a)) throw Error(E_ARG, "null pointer");
+    pow_dev(c, c.field(p), a, la, k, n, out);
+  });
+}
+
+int fpxg_divmod_u32(uint32_t p, const uint32_t* a, size_t la, const uint32_t* b, size_t lb, uint32_t* q,
+                    size_t q_cap, size_t* q_len, uint32_t* r, size_t r_cap, size_t* r_len) {
+  return guarded([&](Context& c) {
+    check_modulus(p);
+    if (!q_len || !r_len || (la && !a) || (lb && !b)) throw Error(E_ARG, "null pointer");
+    *q_len = *r_len = 0;
+    Upload ua(c, a, la), ub(c, b, lb);
+    DevBuf dq = c.buf(std::max<size_t>(la, 1)), dr = c.buf(std::max<size_t>(la, 1));
+    size_t lq = 0, lr = 0;
+    divmod_dev(c, c.field(p), ua.buf.get(), la, ub.buf.get(), lb, dq.get(), std::max<size_t>(la, 1), &lq,
+               dr.get(), std::max<size_t>(la, 1), &lr);
+    if ((q && q_cap < lq) || (r && r_cap < lr)) throw Error(E_ARG, "output capacity too small");
+    if (q && lq) FPXG_HIP(hipMemcpyAsync(q, dq.get(), lq * 4, hipMemcpyDeviceToHost, c.stream));
+    if (r && lr) FPXG_HIP(hipMemcpyAsync(r, dr.get(), lr * 4, hipMemcpyDeviceToHost, c.stream));
+    FPXG_HIP(hipStreamSynchronize(c.stream));
+    *q_len = lq;
+    *r_len = lr;
+  });
+}
+
+int fpxg_divmod_dev(uint32_t p, const uint32_t* a, size_t la, const uint32_t* b, size_t lb, uint32_t* q,
+                    size_t q_cap, size_t* q_len, uint32_t* r, size_t r_cap, size_t* r_len) {
+  return guarded([&](Context& c) {
+    check_modulus(p);
+    if (!q_len || !r_len || (la && !a) || (lb && !b)) throw Error(E_ARG, "null pointer");
+    divmod_dev(c, c.field(p), a, la, b, lb, q, q_cap, q_len, r, r_cap, r_len);
   });
 }
 

@@ -33,7 +33,8 @@ enum {
   EW_MUL_RECIP = 3, // dst[i] = src[i] / (c + i)                    (exponential.hpp:194-195)
   EW_DERIV = 4,     // dst[i] = src[i+1] * (i+1)                    (calculus.hpp:17-39)
   EW_INTEGRAL = 5,  // dst[i] = i in [1, avail] ? src[i-1] / i : 0  (calculus.hpp:97-116)
-  EW_SUB_FROM = 6   // dst[i] = (i < avail ? src[i] : 0) - src2[i]  (square_root.hpp:93-94)
+  EW_SUB_FROM = 6,  // dst[i] = (i < avail ? src[i] : 0) - src2[i]  (square_root.hpp:93-94)
+  EW_REVERSE = 7    // dst[i] = src[n - 1 - i]                      (division.hpp:53-58)
 };
 struct EwRec {
   int op;

@@ -6,6 +6,7 @@
 //   fpx::poly::conv / inv / log / exp / sqrt   detail/poly/convolution.hpp:145,
 //                                              inverse.hpp:163, exponential.hpp:85/107,
 //                                              square_root.hpp:12
+//   fpx::poly::pow / divmod / div / mod        power.hpp:17, division.hpp:79-100
 //   fpx::Poly<M>                               include/fastpoly/poly.hpp:42-112
 //   fpx::NttPlan<M>::get / get_ref / forward...  include/fastpoly/ntt.hpp:130-192
 //   fpx::release_scratch_memory / scratch_memory_stats   memory.hpp:251-258
@@ -269,6 +270,34 @@ void trim(vec<M>& a) {
   while (!a.empty() && a.back().is_zero()) a.pop_back();
 }
 
+/// a^k mod x^n (detail/poly/power.hpp:17-87).
+template <class M>
+vec<M> pow(const vec<M>& a, uint64_t k, size_t n) {
+  if (n == 0) return {};
+  vec<M> out(n);
+  fpx::detail::raise_status(
+      fpxg_pow_u32(M::mod, fpx::detail::limbs(a), a.size(), k, n, fpx::detail::limbs(out)), "poly::pow");
+  return out;
+}
+
+/// Euclidean division a = b q + r, deg r < deg b, both trimmed (division.hpp:79-100).
+template <class M>
+std::pair<vec<M>, vec<M>> divmod(const vec<M>& a, const vec<M>& b) {
+  vec<M> q(std::max<size_t>(a.size(), 1)), r(std::max<size_t>(a.size(), 1));
+  size_t lq = 0, lr = 0;
+  fpx::detail::raise_status(fpxg_divmod_u32(M::mod, fpx::detail::limbs(a), a.size(), fpx::detail::limbs(b),
+                                            b.size(), fpx::detail::limbs(q), q.size(), &lq, fpx::detail::limbs(r),
+                                            r.size(), &lr),
+                            "poly::divmod");
+  q.resize(lq);
+  r.resize(lr);
+  return {std::move(q), std::move(r)};
+}
+template <class M>
+vec<M> div(const vec<M>& a, const vec<M>& b) { return divmod(a, b).first; }
+template <class M>
+vec<M> mod(const vec<M>& a, const vec<M>& b) { return divmod(a, b).second; }
+
 }  // namespace poly
 
 // ---------------------------------------------------------------------------
@@ -298,6 +327,13 @@ class Poly : public std::vector<M> {
   Poly log(size_t n) const { return poly::log<M>(*this, n); }
   Poly exp(size_t n) const { return poly::exp<M>(*this, n); }
   Poly sqrt(size_t n) const { return poly::sqrt<M>(*this, n); }
+  Poly pow(uint64_t k, size_t n) const { return poly::pow<M>(*this, k, n); }
+  std::pair<Poly, Poly> divmod(const base& b) const {
+    auto r = poly::divmod<M>(*this, b);
+    return {Poly(std::move(r.first)), Poly(std::move(r.second))};
+  }
+  Poly div(const base& b) const { return poly::div<M>(*this, b); }
+  Poly mod(const base& b) const { return poly::mod<M>(*this, b); }
   std::vector<uint32_t> to_ints() const {
     std::vector<uint32_t> r(this->size());
     for (size_t i = 0; i < this->size(); ++i) r[i] = (*this)[i].val();

@@ -74,15 +74,30 @@ int fpxg_conv_u32(uint32_t p, const uint32_t* a, size_t la, const uint32_t* b, s
  * square_root.hpp:12): n output limbs. */
 int fpxg_series_u32(int op, uint32_t p, const uint32_t* a, size_t la, size_t n, uint32_t* out);
 
+/* poly::pow(a, k, n) (detail/poly/power.hpp:17-87): n output limbs. */
+int fpxg_pow_u32(uint32_t p, const uint32_t* a, size_t la, uint64_t k, size_t n, uint32_t* out);
+
+/* poly::divmod(a, b) (detail/poly/division.hpp:12-104): a = b q + r with
+ * deg r < deg b, q and r trimmed; *q_len / *r_len receive their lengths.
+ * poly::div and poly::mod are the same call with r or q null.  Capacities of
+ * la limbs always suffice.  FPXG_E_DOMAIN when b is zero (division.hpp:17,
+ * :94). */
+int fpxg_divmod_u32(uint32_t p, const uint32_t* a, size_t la, const uint32_t* b, size_t lb,
+                    uint32_t* q, size_t q_cap, size_t* q_len, uint32_t* r, size_t r_cap, size_t* r_len);
+
 /* NttPlan<Mont<p, root>>::get(n) transforms in place (ntt.hpp:130-192);
  * root == 0 selects the smallest primitive root (modint.hpp:28-52). */
 int fpxg_ntt_u32(uint32_t p, uint32_t root, uint32_t n, int variant, size_t count, uint32_t* data);
 
-/* Device-resident forms of the three calls above (asynchronous). */
+/* Device-resident forms of conv, series, ntt, pow and divmod (asynchronous,
+ * except that divmod and pow read back a few scan words). */
 int fpxg_conv_dev(uint32_t p, const uint32_t* a, size_t la, const uint32_t* b, size_t lb,
                   size_t n, uint32_t* out, size_t out_cap, size_t* out_len);
 int fpxg_series_dev(int op, uint32_t p, const uint32_t* a, size_t la, size_t n, uint32_t* out);
 int fpxg_ntt_dev(uint32_t p, uint32_t root, uint32_t n, int variant, size_t count, uint32_t* data);
+int fpxg_pow_dev(uint32_t p, const uint32_t* a, size_t la, uint64_t k, size_t n, uint32_t* out);
+int fpxg_divmod_dev(uint32_t p, const uint32_t* a, size_t la, const uint32_t* b, size_t lb,
+                    uint32_t* q, size_t q_cap, size_t* q_len, uint32_t* r, size_t r_cap, size_t* r_len);
 
 /* One pass of a device-resident forward transform (0 = column pass with the
  * four-step twiddle, 1 = row pass; a single-pass size only has pass 1), for

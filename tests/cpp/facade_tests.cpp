@@ -3,6 +3,7 @@
 // inputs (std::mt19937_64, bench/bench.cpp:216-220) against its published
 // FNV-1a checksums (bench/bench.cpp:152-159).  Built into a shared object by
 // tests/cpp/build.sh; tests/test_facade.py calls fpx_facade_tests on the GPU.
+#include <bit>
 #include <cstdio>
 #include <cstring>
 #include <random>
@@ -72,6 +73,7 @@ void run_all() {
     check(checksum(e.exp(n)) == 0xff4d4e75dae0ae43ull, "exp 65536");
     const fpx::Poly<M> sq = a.mul(a, n);
     check(checksum(sq.sqrt(n)) == 0xd32ea1d3f38db84aull, "sqrt 65536");
+    check(checksum(a.pow(1000003, n)) == 0x6e2ecc1757af26c1ull, "pow(1000003) 65536");
     const auto one = a.mul(a.inv(n), n);
     bool unit = one[0] == M::from_int(1);
     for (size_t i = 1; i < one.size(); ++i) unit = unit && one[i].is_zero();
@@ -103,6 +105,16 @@ void run_all() {
     bool same = true;
     for (uint32_t i = 0; i < n; ++i) same = same && a[i] == in[i].raw_val();
     check(same, "NttPlan::inverse roundtrip");
+  }
+  {
+    const size_t n = 65536;
+    std::mt19937_64 rng(4 + n);
+    const fpx::Poly<M> a(rnd<M>(n, rng)), b(rnd<M>(n / 2, rng));
+    const auto qr = a.divmod(b);
+    const unsigned long long h = checksum(qr.first) ^ std::rotl(checksum(qr.second), 17);
+    check(h == 0x2762041dae10ec54ull, "divmod 65536 (bench checksum)");
+    check(checksum(a.div(b)) == checksum(qr.first) && checksum(a.mod(b)) == checksum(qr.second), "div / mod");
+    check(throws_domain([&] { a.divmod(std::vector<M>{M(), M()}); }), "divmod by zero -> domain_error");
   }
   check(throws_domain([] { fpx::poly::inv(std::vector<M>{M(), M(1)}, 4); }), "inv(a0 == 0) -> domain_error");
   check(throws_domain([] { fpx::poly::log(std::vector<M>{M(2)}, 4); }), "log(a0 != 1) -> domain_error");
