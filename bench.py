@@ -206,6 +206,7 @@ def main():
                      "forward_us_by_log2n": {str(k): round(v[1], 2) for k, v in tuned.items()}},
     }
 
+    result["pow_divmod"] = pow_divmod_leg(fa, args)
     if not args.no_batch:
         result["batched"] = batched_leg(fa, args)
     if not args.no_serve:
@@ -235,6 +236,56 @@ def ntt_only(fa, args):
             raise RuntimeError("hipMemcpyDtoD failed")
     hip.hipDeviceSynchronize()
     print(json.dumps({"ntt_only": True, "n": n, "reps": args.ntt_reps, "copy_bytes": 4 * n}), flush=True)
+
+
+def pow_divmod_leg(fa, args):
+    """poly::pow(a, 1000003, 2^20) and poly::divmod of 2^20 by 2^19 limbs mod
+    998244353 (the reference bench's inputs), inputs in HBM, against the
+    reference library on one host core with its outputs compared limb for limb."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle as po
+    n = N_SERIES
+    g = po.MT(3 + n)
+    a = g.rnd(P, n)
+    a[0] = po.from_int(P, 1)
+    g = po.MT(4 + n)
+    x = g.rnd(P, n)
+    y = g.rnd(P, n // 2)
+    D = fa.DeviceArray
+    da, dx, dy = D.from_host(a), D.from_host(x), D.from_host(y)
+    out, q, r = D(n), D(n), D(n)
+    reps = 5
+
+    def timed(fn):
+        fn()
+        fa.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            fn()
+        fa.synchronize()
+        return (time.perf_counter() - t0) / reps
+
+    t_pow = timed(lambda: fa.pow_dev(P, da, n, 1000003, n, out))
+    lens = []
+    t_div = timed(lambda: lens.append(fa.divmod_dev(P, dx, n, dy, n // 2, q, r)))
+    lq, lr = lens[-1]
+    res = {"workload": "pow(a, 1000003) at n = 2^20 and divmod of 2^20 by 2^19 limbs, mod 998244353",
+           "pow_ms": round(1e3 * t_pow, 3), "divmod_ms": round(1e3 * t_div, 3),
+           "pow_coeffs_per_s": n / t_pow, "divmod_coeffs_per_s": (lq + lr) / t_div}
+    if not args.no_cpu and po.ref() is not None and hasattr(po.ref(), "fpref_pow"):
+        t0 = time.perf_counter()
+        cp = po.ref_pow(P, a, 1000003, n)
+        t1 = time.perf_counter()
+        cq, cr = po.ref_divmod(P, x, y)
+        t2 = time.perf_counter()
+        res["cpu_baseline"] = {"pow_ms": round(1e3 * (t1 - t0), 1), "divmod_ms": round(1e3 * (t2 - t1), 1),
+                               "cores": 1, "kind": "reference",
+                               "bit_exact_vs_gpu": bool(np.array_equal(cp, out.to_host())
+                                                        and np.array_equal(cq, q.to_host(lq))
+                                                        and np.array_equal(cr, r.to_host(lr)))}
+    for d in (da, dx, dy, out, q, r):
+        d.free()
+    return res
 
 
 def serving_leg(fa, args):

@@ -609,3 +609,15 @@ def div(p: int, a, b) -> np.ndarray:
 def mod(p: int, a, b) -> np.ndarray:
     """poly::mod (division.hpp:89-100)."""
     return divmod(p, a, b)[1]
+
+
+def pow_dev(p: int, a: DeviceArray, la: int, k: int, n: int, out: DeviceArray) -> None:
+    _check(lib().fpxg_pow_dev(p, a.ptr, la, int(k), n, out.ptr), "pow_dev")
+
+
+def divmod_dev(p: int, a: DeviceArray, la: int, b: DeviceArray, lb: int, q: DeviceArray, r: DeviceArray):
+    """Device divmod; returns (len q, len r)."""
+    lq, lr = ctypes.c_size_t(0), ctypes.c_size_t(0)
+    _check(lib().fpxg_divmod_dev(p, a.ptr, la, b.ptr, lb, q.ptr, q.n, ctypes.byref(lq), r.ptr, r.n,
+                                 ctypes.byref(lr)), "divmod_dev")
+    return lq.value, lr.value

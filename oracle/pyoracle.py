@@ -301,6 +301,11 @@ def ref():
                                    ctypes.c_size_t, _u32p]
         L.fpref_ntt.argtypes = [ctypes.c_uint32, ctypes.c_int, ctypes.c_uint32, _u32p]
         L.fpref_msqrt.argtypes = [ctypes.c_uint32, ctypes.c_uint32, _u32p]
+        if hasattr(L, "fpref_pow"):
+            _szp = ctypes.POINTER(ctypes.c_size_t)
+            L.fpref_pow.argtypes = [ctypes.c_uint32, _u32p, ctypes.c_size_t, ctypes.c_uint64, ctypes.c_size_t, _u32p]
+            L.fpref_divmod.argtypes = [ctypes.c_uint32, _u32p, ctypes.c_size_t, _u32p, ctypes.c_size_t, _u32p, _szp,
+                                       _u32p, _szp]
         _ref = L
     return _ref
 
@@ -315,6 +320,28 @@ def ref_conv(p: int, a, b, n: int | None = None) -> np.ndarray:
     if rc:
         raise OracleError(rc, "ref_conv")
     return out[: got.value].copy()
+
+
+def ref_pow(p: int, a, k: int, n: int) -> np.ndarray:
+    L = ref()
+    a = _u32(a)
+    out = np.zeros(max(n, 1), dtype=np.uint32)
+    rc = L.fpref_pow(p, _ptr(a), a.size, k, n, _ptr(out))
+    if rc:
+        raise OracleError(rc, "ref_pow")
+    return out[:n].copy()
+
+
+def ref_divmod(p: int, a, b):
+    L = ref()
+    a, b = _u32(a), _u32(b)
+    q = np.zeros(max(a.size, 1), dtype=np.uint32)
+    r = np.zeros(max(a.size, 1), dtype=np.uint32)
+    lq, lr = ctypes.c_size_t(0), ctypes.c_size_t(0)
+    rc = L.fpref_divmod(p, _ptr(a), a.size, _ptr(b), b.size, _ptr(q), ctypes.byref(lq), _ptr(r), ctypes.byref(lr))
+    if rc:
+        raise OracleError(rc, "ref_divmod")
+    return q[: lq.value].copy(), r[: lr.value].copy()
 
 
 def ref_series(op: str, p: int, a, n: int) -> np.ndarray:

@@ -4,7 +4,8 @@
 // the "reference" CPU baseline of bench.py.  No reference source is copied;
 // this file only instantiates the public API of fastpoly.hpp
 // (detail/poly/convolution.hpp:145, inverse.hpp:163, exponential.hpp:85/107,
-// square_root.hpp:12, ntt.hpp:130-192) for the moduli the tests use.
+// square_root.hpp:12, power.hpp:17, division.hpp:79, ntt.hpp:130-192) for the
+// moduli the tests use.
 #include <cstdint>
 #include <cstring>
 #include <exception>
@@ -72,6 +73,26 @@ int series_t(int op, const uint32_t* a, size_t la, size_t n, uint32_t* out) {
 }
 
 template <class M>
+int pow_t(const uint32_t* a, size_t la, uint64_t k, size_t n, uint32_t* out) {
+  return guarded([&] {
+    auto r = fpx::poly::pow(load<M>(a, la), k, n);
+    store(r, out);
+  });
+}
+
+template <class M>
+int divmod_t(const uint32_t* a, size_t la, const uint32_t* b, size_t lb, uint32_t* q, size_t* lq, uint32_t* r,
+             size_t* lr) {
+  return guarded([&] {
+    auto qr = fpx::poly::divmod(load<M>(a, la), load<M>(b, lb));
+    store(qr.first, q);
+    store(qr.second, r);
+    *lq = qr.first.size();
+    *lr = qr.second.size();
+  });
+}
+
+template <class M>
 int msqrt_t(uint32_t limb, uint32_t* out) {
   M r;
   if (!M::raw(limb).sqrt(r)) return 1;
@@ -116,6 +137,15 @@ int fpref_conv(uint32_t p, const uint32_t* a, size_t la, const uint32_t* b, size
 
 int fpref_series(int op, uint32_t p, const uint32_t* a, size_t la, size_t n, uint32_t* out) {
   FPREF_DISPATCH(p, (series_t<M>(op, a, la, n, out)))
+}
+
+int fpref_pow(uint32_t p, const uint32_t* a, size_t la, uint64_t k, size_t n, uint32_t* out) {
+  FPREF_DISPATCH(p, (pow_t<M>(a, la, k, n, out)))
+}
+
+int fpref_divmod(uint32_t p, const uint32_t* a, size_t la, const uint32_t* b, size_t lb, uint32_t* q, size_t* lq,
+                 uint32_t* r, size_t* lr) {
+  FPREF_DISPATCH(p, (divmod_t<M>(a, la, b, lb, q, lq, r, lr)))
 }
 
 int fpref_msqrt(uint32_t p, uint32_t limb, uint32_t* out) {

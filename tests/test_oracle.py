@@ -389,3 +389,26 @@ def test_divmod_identity(oracle, p):
         assert np.array_equal(tot[:la], at), (p, la, lb)
     with pytest.raises(oracle.OracleError):
         oracle.divmod(p, np.ones(3, np.uint32), np.zeros(2, np.uint32))
+
+
+@pytest.mark.parametrize("p", [998244353, 1000000007, 100003, 2])
+def test_pow_divmod_match_reference_library(oracle, p):
+    """The restatement against the reference library itself (oracle/_ref)."""
+    if oracle.ref() is None or not hasattr(oracle.ref(), "fpref_pow"):
+        pytest.skip("reference library not built here")
+    rng = random.Random(p + 11)
+    for t in range(20):
+        n = rng.choice([1, 7, 100, 3000])
+        la = rng.choice([0, 1, 2, 5, 100, 4000])
+        a = np.array([rng.randrange(p) for _ in range(la)], np.uint32)
+        if la and t % 3 == 0:
+            a[: rng.randint(0, la - 1)] = 0
+        k = rng.choice([0, 1, 2, 5, 64, 65, 1000003, (1 << 64) - 1])
+        assert np.array_equal(oracle.poly_pow(p, a, k, n), oracle.ref_pow(p, a, k, n)), (p, n, la, k)
+    for la, lb in [(5, 9), (40, 3), (300, 100), (5000, 2400), (20000, 19990)]:
+        a = np.array([rng.randrange(p) for _ in range(la)], np.uint32)
+        b = np.array([rng.randrange(p) for _ in range(lb)], np.uint32)
+        b[-1] = 1
+        q1, r1 = oracle.divmod(p, a, b)
+        q2, r2 = oracle.ref_divmod(p, a, b)
+        assert np.array_equal(q1, q2) and np.array_equal(r1, r2), (p, la, lb)
