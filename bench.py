@@ -2,8 +2,9 @@
 """Benchmark of the fastpoly NTT convolution path on one MI355X.
 
 Workload (one "step"), mod 998244353 with synthetic uniform [0, p) limbs:
-  * conv of two 2^23-coefficient polynomials: product of 2^24 - 1 limbs through
-    three auxiliary-prime 2^24-point NTTs and Garner reconstruction;
+  * conv of two 2^23-coefficient polynomials: product of 2^24 - 1 limbs, one
+    transform size past 998244353's largest (2^23): four 2^23-point spectra of
+    the operands' halves, three block sums inverted and accumulated;
   * Newton series at n = 2^20: inv (random a0), log (a0 = 1), exp (b0 = 0),
     sqrt (c = d^2 mod x^n, d0 = 1).
 Inputs are resident in HBM before the timed region; `value` counts output
@@ -195,7 +196,7 @@ def main():
         "value": value, "unit": "coeffs/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": ms_per_step, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "u32", "data": "synthetic (uniform [0,p) Montgomery limbs, numpy PCG64 seed 20261203)",
-        "config": {"workload": "single large transforms mod 998244353: conv 2^23 x 2^23 (2^24-point aux NTTs) "
+        "config": {"workload": "single large transforms mod 998244353: conv 2^23 x 2^23 (split into 2^23-point native NTTs) "
                                "+ inv/log/exp/sqrt at n = 2^20",
                    "modulus": P, "conv_len": N_CONV, "series_n": N_SERIES},
         "roofline": roofline,

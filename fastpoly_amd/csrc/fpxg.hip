@@ -354,7 +354,7 @@ static void launch_rows(Context& c, bool lazy, int mode, const RowJob2& j, int l
 
 // Column tiles: 16 columns up to 2^9 levels, 8 for 2^10 and 2^11 (a 2^10 x 8
 // tile leaves room for four workgroups per CU), 4 for 2^12.
-static constexpr int col_tile_log(int S) { return S <= 9 ? 4 : (S <= 11 ? 3 : 2); }
+static constexpr int col_tile_log(int S) { return S <= 10 ? 4 : (S <= 11 ? 3 : 2); }
 static int tile_log_cols(int S) { return col_tile_log(S); }
 
 template <int S, bool LAZY, bool INV>
@@ -643,6 +643,60 @@ static bool native_ok(const HostField& T, size_t full) {
   return L <= std::min(T.maxlog, 26);
 }
 
+// Products over an NTT prime that exceed its largest transform 2^M (M =
+// min(v2(p - 1), 26)): both operands are cut into pieces of h = 2^(M-1)
+// coefficients, each piece is transformed once at 2^M points, and output block
+// k (2h - 1 coefficients at offset k*h) is the inverse of the spectral sum of
+// A_i * B_(k-i), accumulated into the output (neighbouring blocks overlap by
+// h - 1).  For 2^23 x 2^23 mod 998244353 that is seven 2^23-point transforms
+// against the three-prime route's nine 2^24-point ones (crt_convolution.hpp),
+// and the product is unique, so the limbs are the same.
+static bool split_ok(const HostField& T, size_t la, size_t lb) {
+  if (T.p == 2 || T.maxlog < 16) return false;
+  const size_t h = (size_t)1 << (std::min(T.maxlog, 26) - 1);
+  return ((la + h - 1) / h) * ((lb + h - 1) / h) <= 64;
+}
+
+static void split_conv(Context& c, const HostField& T, const uint32_t* a, size_t la, const uint32_t* b, size_t lb,
+                       uint32_t* out, size_t outc) {
+  const int M = std::min(T.maxlog, 26);
+  const size_t N = (size_t)1 << M, h = N / 2;
+  const size_t pa = (la + h - 1) / h, pb = (lb + h - 1) / h;
+  const bool same = a == b && la == lb;
+  std::vector<DevBuf> SA(pa), SB(same ? 0 : pb);
+  for (size_t i = 0; i < pa; ++i) {
+    SA[i] = c.buf(N);
+    ntt_forward(c, T, SA[i].get(), a + i * h, (uint32_t)std::min(h, la - i * h), M, false, true);
+  }
+  for (size_t j = 0; j < SB.size(); ++j) {
+    SB[j] = c.buf(N);
+    ntt_forward(c, T, SB[j].get(), b + j * h, (uint32_t)std::min(h, lb - j * h), M, false, true);
+  }
+  auto spec_b = [&](size_t j) -> const uint32_t* { return same ? SA[j].get() : SB[j].get(); };
+  fill(c, out, outc, 0);
+  DevBuf W = c.buf(N);
+  const PassPlan P = c.plan(M);
+  for (size_t k = 0; k + 1 < pa + pb && k * h < outc; ++k) {
+    const size_t lo = k * h, hi = std::min(outc, lo + 2 * h - 1);
+    const size_t i0 = k + 1 > pb ? k + 1 - pb : 0, i1 = std::min(k, pa - 1);
+    RowJob2 rj{};
+    rj.f = T.dev; rj.dst = W.get(); rj.nrows = 1u << P.S; rj.in_a = (uint32_t)N; rj.out_lo = 0; rj.out_hi = (uint32_t)N;
+    if (i0 == i1) {  // one pair: product and row inverse in one pass
+      rj.src_a = SA[i0].get(); rj.src_b = spec_b(k - i0);
+      launch_rows(c, T.lazy, R2M_SPEC_ONLY, rj, P.b, rj.nrows);
+    } else {
+      for (size_t i = i0; i <= i1; ++i)
+        emit_ew(c, T, EW_MULADD, W.get(), SA[i].get(), spec_b(k - i), N, 0, i == i0 ? 0u : 1u);
+      rj.src_a = W.get();
+      launch_rows(c, T.lazy, R2M_INV, rj, P.b, rj.nrows);
+    }
+    ColJob2 cj{};
+    cj.src = W.get(); cj.dst = out + lo; cj.out_lo = 0; cj.out_hi = (uint32_t)(hi - lo); cj.out_shift = 0;
+    cj.inv_n = inv_pow2(T, M); cj.flags = R2_ACC; cj.f = T.dev;
+    launch_cols(c, T.lazy, true, cj, P);
+  }
+}
+
 void conv_dev(Context& c, const HostField& T, const uint32_t* a, size_t la, const uint32_t* b, size_t lb,
               uint32_t* out, size_t outc) {
   if (la == 0 || lb == 0 || outc == 0) return;
@@ -651,6 +705,10 @@ void conv_dev(Context& c, const HostField& T, const uint32_t* a, size_t la, cons
   const size_t full = la + lb - 1;
   if (native_ok(T, full)) {
     conv_core(c, T, a, la, b, lb, out, outc, false);
+    return;
+  }
+  if (split_ok(T, la, lb)) {
+    split_conv(c, T, a, la, b, lb, out, outc);
     return;
   }
   if (la <= kCrtBlock && lb <= kCrtBlock) {

@@ -335,7 +335,8 @@ struct RowJob2 {
   uint32_t flags;
   FieldDev f;
 };
-enum : uint32_t { R2_LIFT = 1, R2_CANON = 2, R2_SCALE = 4, R2_NEG = 8 };
+enum : uint32_t { R2_LIFT = 1, R2_CANON = 2, R2_SCALE = 4, R2_NEG = 8, R2_ACC = 16 };
+// R2_ACC (column inverse only): dst[i] = dst[i] + result, both canonical.
 enum : int { R2M_FWD = 1, R2M_INV = 2, R2M_CONV = 3, R2M_SPEC = 4, R2M_SPEC_ONLY = 5 };
 // R2M_FWD:  forward a                      R2M_INV:       inverse a
 // R2M_CONV: forward a, forward b, a*b, inverse
@@ -502,7 +503,7 @@ struct ColJob2 {
   uint32_t in_count;
   uint32_t out_lo, out_hi, out_shift;
   uint32_t inv_n;
-  uint32_t flags;  // R2_LIFT, R2_CANON, R2_NEG
+  uint32_t flags;  // R2_LIFT, R2_CANON, R2_NEG, R2_ACC
   FieldDev f;
 };
 
@@ -587,6 +588,7 @@ k_cols2(ColJob2 j0, const ColJob2* __restrict__ jobs, int L) {
     phase_compute<S, PL, true, LAZY, LOGC>(x, g, J.f.planes_inv, A);
     inv_phases<S, PL - 1, LOGC, LAZY>(x, X, g, c, J.f.planes_inv, A);
     constexpr int r0 = SC::r_of(0);
+    const bool acc = (flags & R2_ACC) != 0;
 #pragma unroll
     for (int u = 0; u < PhaseGroups<S, 0, LOGC>::NG; ++u)
 #pragma unroll
@@ -594,7 +596,14 @@ k_cols2(ColJob2 j0, const ColJob2* __restrict__ jobs, int L) {
         const uint32_t e = elem<S, 0>(PhaseGroups<S, 0, LOGC>::gid(g, u), k);
         const uint32_t gi = e * M + col;
         const uint32_t v = A.canon(A.half(x[u * (1 << r0) + k]));
-        if (gi >= J.out_lo && gi < J.out_hi) gstore(J.dst + (gi - J.out_shift), neg ? (v == 0 ? 0u : J.f.p - v) : v);
+        if (gi >= J.out_lo && gi < J.out_hi) {
+          uint32_t w = neg ? (v == 0 ? 0u : J.f.p - v) : v;
+          if (acc) {
+            const uint32_t s = gload(J.dst + (gi - J.out_shift)) + w;
+            w = min(s, s - J.f.p);
+          }
+          gstore(J.dst + (gi - J.out_shift), w);
+        }
       }
   }
 }

@@ -34,7 +34,8 @@ enum {
   EW_DERIV = 4,     // dst[i] = src[i+1] * (i+1)                    (calculus.hpp:17-39)
   EW_INTEGRAL = 5,  // dst[i] = i in [1, avail] ? src[i-1] / i : 0  (calculus.hpp:97-116)
   EW_SUB_FROM = 6,  // dst[i] = (i < avail ? src[i] : 0) - src2[i]  (square_root.hpp:93-94)
-  EW_REVERSE = 7    // dst[i] = src[n - 1 - i]                      (division.hpp:53-58)
+  EW_REVERSE = 7,   // dst[i] = src[n - 1 - i]                      (division.hpp:53-58)
+  EW_MULADD = 8     // dst[i] = (c ? dst[i] : 0) + src[i] * src2[i] (spectra of split products)
 };
 struct EwRec {
   int op;

@@ -96,6 +96,21 @@ def test_batch_matches_oracle(fa, oracle):
         assert np.array_equal(g, w), (r.op, r.p, r.a.size)
 
 
+def test_batch_split_products(fa, oracle):
+    """Products past the prime's largest transform (one spectrum per piece,
+    spectral block sums, accumulated inverses) recorded into the merged executor."""
+    rng = np.random.default_rng(5)
+    reqs, wants = [], []
+    for p, la, lb in ((65537, 70000, 50000), (65537, 3 * 32768 + 7, 5), (786433, 300000, 200001),
+                      (P, 3000, 4000), (65537, 100, 200), (786433, 5, 262145)):
+        a = rng.integers(0, p, la, dtype=np.uint32)
+        b = rng.integers(0, p, lb, dtype=np.uint32)
+        reqs.append(fa.Request("conv", p, a, b))
+        wants.append(oracle.conv(p, a, b))
+    for r, g, w in zip(reqs, fa.batch(reqs), wants):
+        assert np.array_equal(g, w), (r.p, r.a.size, r.b.size)
+
+
 def test_batch_uneven_sample(fa):
     """A scaled-down draw of the uneven workload (log-uniform lengths, two moduli)."""
     from fastpoly_amd.workloads import uneven_batch
