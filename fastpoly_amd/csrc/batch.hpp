@@ -227,8 +227,11 @@ static size_t run_levels(Context& c, const std::vector<Chain*>& chains, size_t m
     const size_t start = heads.size();
     for (Chain* ch : chains)
       if (!ch->done()) {
-        const LaunchRec& r = ch->recs[ch->pos++];
-        heads.emplace_back(key_of(r), &r);
+        // The chain's next record, with the records that share its level.
+        do {
+          const LaunchRec& r = ch->recs[ch->pos++];
+          heads.emplace_back(key_of(r), &r);
+        } while (!ch->done() && ch->recs[ch->pos].par);
       }
     if (heads.size() == start) break;
     std::stable_sort(heads.begin() + (ptrdiff_t)start, heads.end(),

@@ -57,12 +57,25 @@ template <int TPB>
 __global__ void __launch_bounds__(TPB) k_recur(const RecurRec* __restrict__ recs, RecurRec r0) {
   __shared__ uint32_t nv[kRecMax];
   __shared__ uint32_t recip[kRecMax];
+  __shared__ uint32_t us[kRecMax], ws[kRecMax];
   __shared__ unsigned long long red[TPB / 64];
   __shared__ uint32_t v0s;
   const RecurRec R = recs ? recs[blockIdx.x] : r0;
   const FieldScalar f = R.f;
-  const uint32_t lo = R.lo, hi = R.hi, base = R.lo;
+  const uint32_t lo = R.lo, hi = R.hi;
   uint32_t* v = R.v;
+  // A seed (hi <= kRecMax): every index read is below hi, so u, w and the
+  // earlier values are staged in LDS once and each new coefficient costs no
+  // global round trip.  Longer recurrences read them from global memory.
+  const bool small = hi <= (uint32_t)kRecMax;
+  const uint32_t base = small ? 0u : R.lo;
+  if (small) {
+    for (uint32_t t = threadIdx.x; t < hi; t += TPB) {
+      us[t] = t < R.lu ? R.u[t] : 0u;
+      ws[t] = t < R.lw ? R.w[t] : 0u;
+      if (t < lo) nv[t] = v[t];
+    }
+  }
   if (R.kind == REC_EXP && threadIdx.x < hi - lo) {
     const uint32_t i = lo + threadIdx.x;
     recip[threadIdx.x] = i == 0 ? 0u : fs_inv(f, fs_from_int(f, i));
@@ -80,29 +93,89 @@ __global__ void __launch_bounds__(TPB) k_recur(const RecurRec* __restrict__ recs
   }
   __syncthreads();
   const uint32_t v0 = v0s;
+  if constexpr (TPB == 64) {
+    if (small) {
+      // Online form of a seed: lane i owns coefficient i and adds the term of
+      // each earlier coefficient as soon as it is known, so a new coefficient
+      // costs one finishing product, one lane broadcast and one product per
+      // lane, with no cross-lane reduction.
+      const uint32_t i = threadIdx.x;
+      const uint32_t k0 = lo == 0 ? 1u : lo;  // first coefficient computed here
+      // Contribution of v_k to lane i's sum (k < i).  SQRT adds the pair
+      // {k, i - k} once its larger member k is known (twice unless equal).
+      auto term = [&](uint32_t k, uint32_t vk) -> uint32_t {
+        const uint32_t d = i - k;
+        if (R.kind == REC_INV) return (d >= 1 && d + 1 <= R.lu) ? fs_mul(f, us[d], vk) : 0u;
+        if (R.kind == REC_EXP) return (d >= 1 && d <= R.lu) ? fs_mul(f, us[d - 1], vk) : 0u;
+        if (k == 0 || d == 0 || d > k) return 0u;
+        const uint32_t t = fs_mul(f, vk, nv[d]);
+        if (d == k) return t;
+        const uint32_t t2 = t + t;
+        return t2 >= f.p ? t2 - f.p : t2;
+      };
+      uint32_t acc = 0;
+      auto add = [&](uint32_t t) {
+        acc += t;
+        acc = acc >= f.p ? acc - f.p : acc;
+      };
+      if (i < hi)
+        for (uint32_t k = 0; k < k0 && k < i; ++k) add(term(k, nv[k]));
+      for (uint32_t k = k0; k < hi; ++k) {
+        uint32_t cand;
+        if (R.kind == REC_INV) cand = fs_mul(f, acc == 0 ? 0u : f.p - acc, v0);
+        else if (R.kind == REC_EXP) cand = fs_mul(f, acc, (i >= lo && i < hi) ? recip[i - lo] : 0u);
+        else cand = fs_mul(f, fs_sub(f, (i < hi && i < R.lw) ? ws[i] : 0u, acc), R.k);
+        const uint32_t vk = (uint32_t)__builtin_amdgcn_readlane((int)cand, (int)k);
+        if (i == k) nv[k] = vk;
+        if (i > k && i < hi) add(term(k, vk));
+      }
+      for (uint32_t t = k0 + i; t < hi; t += TPB) v[t] = nv[t];
+      return;
+    }
+  }
   auto val = [&](uint32_t idx) -> uint32_t { return idx < base ? v[idx] : nv[idx - base]; };
+  auto uat = [&](uint32_t idx) -> uint32_t { return small ? us[idx] : R.u[idx]; };
   for (uint32_t i = lo == 0 ? 1 : lo; i < hi; ++i) {
     uint32_t jmax;
     if (R.kind == REC_INV) jmax = min(i, R.lu ? R.lu - 1 : 0);
     else if (R.kind == REC_EXP) jmax = min(i, R.lu);
     else jmax = i - 1;
-    unsigned long long s = 0;
-    for (uint32_t j = 1 + threadIdx.x; j <= jmax; j += TPB) {
-      const uint32_t x = R.kind == REC_EXP ? R.u[j - 1] : (R.kind == REC_INV ? R.u[j] : val(j));
-      s += fs_mul(f, x, val(i - j));
+    uint32_t sum;
+    if constexpr (TPB == 64) {
+      // One wave: modular partial sums and a butterfly, 32-bit throughout.
+      uint32_t acc = 0;
+      for (uint32_t j = 1 + threadIdx.x; j <= jmax; j += TPB) {
+        const uint32_t x = R.kind == REC_EXP ? uat(j - 1) : (R.kind == REC_INV ? uat(j) : val(j));
+        acc += fs_mul(f, x, val(i - j));
+        acc = acc >= f.p ? acc - f.p : acc;
+      }
+#pragma unroll
+      for (int o = 32; o > 0; o >>= 1) {
+        acc += __shfl_xor(acc, o, 64);
+        acc = acc >= f.p ? acc - f.p : acc;
+      }
+      sum = acc;
+    } else {
+      unsigned long long s = 0;
+      for (uint32_t j = 1 + threadIdx.x; j <= jmax; j += TPB) {
+        const uint32_t x = R.kind == REC_EXP ? uat(j - 1) : (R.kind == REC_INV ? uat(j) : val(j));
+        s += fs_mul(f, x, val(i - j));
+      }
+      s = block_sum<TPB>(s, red);
+      sum = (uint32_t)(s % f.p);
     }
-    s = block_sum<TPB>(s, red);
     if (threadIdx.x == 0) {
-      const uint32_t sum = (uint32_t)(s % f.p);
       uint32_t r;
       if (R.kind == REC_INV) r = fs_mul(f, sum == 0 ? 0u : f.p - sum, v0);
       else if (R.kind == REC_EXP) r = fs_mul(f, sum, recip[i - lo]);
-      else r = fs_mul(f, fs_sub(f, i < R.lw ? R.w[i] : 0u, sum), R.k);
+      else r = fs_mul(f, fs_sub(f, i < R.lw ? (small ? ws[i] : R.w[i]) : 0u, sum), R.k);
       nv[i - base] = r;
-      v[i] = r;
     }
     __syncthreads();
   }
+  // The new values are written once at the end: no store is waited on per
+  // coefficient by the barriers above.
+  for (uint32_t i = (lo == 0 ? 1 : lo) + threadIdx.x; i < hi; i += TPB) v[i] = nv[i - base];
 }
 
 __device__ __forceinline__ uint32_t ew_value(const EwRec& R, uint32_t i) {

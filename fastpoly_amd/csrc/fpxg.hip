@@ -357,20 +357,24 @@ static void launch_rows(Context& c, bool lazy, int mode, const RowJob2& j, int l
 static constexpr int col_tile_log(int S) { return S <= 10 ? 4 : (S <= 11 ? 3 : 2); }
 static int tile_log_cols(int S) { return col_tile_log(S); }
 
+// njobs jobs from the device array dj, or (dj null) one job j, or two jobs j
+// and *j1 passed by value.
 template <int S, bool LAZY, bool INV>
-static void cols2_launch(Context& c, const ColJob2& j, const PassPlan& P, uint32_t njobs, const ColJob2* dj) {
+static void cols2_launch(Context& c, const ColJob2& j, const PassPlan& P, uint32_t njobs, const ColJob2* dj,
+                         const ColJob2* j1) {
   constexpr int LOGC = col_tile_log(S);
   using CS = ColShape<S, LOGC>;
   const size_t lds = (size_t)CS::words * 4;
   dim3 grid(1u << (P.b - LOGC), njobs), block(CS::T);
-  hipLaunchKernelGGL((k_cols2<S, LOGC, LAZY, INV>), grid, block, lds, c.stream, j, dj, P.L);
+  hipLaunchKernelGGL((k_cols2<S, LOGC, LAZY, INV>), grid, block, lds, c.stream, j, j1 ? *j1 : j, dj, P.L);
 }
 
 template <bool LAZY, bool INV>
-static void cols2_size(Context& c, const ColJob2& j, const PassPlan& P, uint32_t njobs, const ColJob2* dj) {
+static void cols2_size(Context& c, const ColJob2& j, const PassPlan& P, uint32_t njobs, const ColJob2* dj,
+                       const ColJob2* j1) {
   switch (P.S) {
 #define FPXG_C(S) \
-  case S: cols2_launch<S, LAZY, INV>(c, j, P, njobs, dj); break;
+  case S: cols2_launch<S, LAZY, INV>(c, j, P, njobs, dj, j1); break;
     FPXG_C(1) FPXG_C(2) FPXG_C(3) FPXG_C(4) FPXG_C(5) FPXG_C(6)
     FPXG_C(7) FPXG_C(8) FPXG_C(9) FPXG_C(10) FPXG_C(11) FPXG_C(12)
 #undef FPXG_C
@@ -379,19 +383,27 @@ static void cols2_size(Context& c, const ColJob2& j, const PassPlan& P, uint32_t
 }
 
 static void launch_cols(Context& c, bool lazy, bool inv, const ColJob2& j, const PassPlan& P, uint32_t njobs = 1,
-                        const ColJob2* dj = nullptr) {
+                        const ColJob2* dj = nullptr, const ColJob2* j1 = nullptr) {
   if (c.rec && !dj) {
     c.rec->push_cols(P.S, lazy, inv, P.L, j);
+    if (j1) c.rec->push_cols(P.S, lazy, inv, P.L, *j1, true);
     return;
   }
+  if (j1) njobs = 2;
   if (lazy) {
-    if (inv) cols2_size<true, true>(c, j, P, njobs, dj);
-    else cols2_size<true, false>(c, j, P, njobs, dj);
+    if (inv) cols2_size<true, true>(c, j, P, njobs, dj, j1);
+    else cols2_size<true, false>(c, j, P, njobs, dj, j1);
   } else {
-    if (inv) cols2_size<false, true>(c, j, P, njobs, dj);
-    else cols2_size<false, false>(c, j, P, njobs, dj);
+    if (inv) cols2_size<false, true>(c, j, P, njobs, dj, j1);
+    else cols2_size<false, false>(c, j, P, njobs, dj, j1);
   }
   FPXG_HIP(hipGetLastError());
+}
+// Two independent column passes of one shape in one launch (one recorded
+// record each, the second marked as sharing the first's launch level).
+static void launch_cols_pair(Context& c, bool lazy, bool inv, const ColJob2& j0, const ColJob2& j1,
+                             const PassPlan& P) {
+  launch_cols(c, lazy, inv, j0, P, 1, nullptr, &j1);
 }
 
 template <int LOGM, bool LAZY, int MODE>

@@ -135,7 +135,17 @@ static void series_call(Context& c, const HostField& T, int op, const uint32_t* 
     hipGraph_t g = nullptr;
     FPXG_HIP(hipStreamBeginCapture(c.stream, hipStreamCaptureModeThreadLocal));
     try {
-      for (const LaunchRec& r : recs) issue_one(c, r);
+      for (size_t i = 0; i < recs.size(); ++i) {
+        const LaunchRec& r = recs[i];
+        if (r.kind == LK_COLS && i + 1 < recs.size() && recs[i + 1].par && recs[i + 1].kind == LK_COLS &&
+            recs[i + 1].k0 == r.k0 && recs[i + 1].k1 == r.k1 && recs[i + 1].k2 == r.k2 && recs[i + 1].k3 == r.k3) {
+          const PassPlan P{r.k3, r.k3 - r.k0, r.k0, tile_log_cols(r.k0)};
+          launch_cols_pair(c, r.k1 != 0, r.k2 != 0, r.col, recs[i + 1].col, P);
+          ++i;
+          continue;
+        }
+        issue_one(c, r);
+      }
     } catch (...) {
       (void)hipStreamEndCapture(c.stream, &g);
       if (g) (void)hipGraphDestroy(g);

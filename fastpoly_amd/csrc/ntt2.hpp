@@ -291,16 +291,18 @@ __device__ __forceinline__ void inv_phases(uint32_t* x, uint32_t* X, uint32_t g,
 }
 
 // The workgroup's job: jobs[blockIdx.y] when a job array is given, else the
-// by-value first kernel argument j0 (at offset 0 of the kernel arguments).
-// Both are read through the constant address space, so the job lands in
-// scalar registers with one round of scalar loads.
+// by-value kernel argument number blockIdx.y (j0 at offset 0 of the kernel
+// arguments, and for column passes j1 right after it: a two-job launch needs no
+// device array).  Both are read through the constant address space, so the
+// job lands in scalar registers with one round of scalar loads.
 template <class Job>
 __device__ __forceinline__ Job load_job(const Job& j0, const Job* jobs) {
 #if defined(__HIP_DEVICE_COMPILE__)
   (void)j0;
   typedef __attribute__((address_space(4))) const uint32_t cword;
+  static_assert(sizeof(Job) % 8 == 0, "by-value jobs follow each other in the kernel arguments");
   const cword* jp = jobs ? (const cword*)(uintptr_t)(jobs + blockIdx.y)
-                         : (const cword*)__builtin_amdgcn_kernarg_segment_ptr();
+                         : (const cword*)__builtin_amdgcn_kernarg_segment_ptr() + blockIdx.y * (sizeof(Job) / 4);
   static_assert(sizeof(Job) % 4 == 0, "jobs are whole words");
   Job J;
   uint32_t* w = reinterpret_cast<uint32_t*>(&J);
@@ -517,7 +519,8 @@ struct ColShape {
 
 template <int S, int LOGC, bool LAZY, bool INV>
 __global__ void __launch_bounds__((ColShape<S, LOGC>::T)) __attribute__((amdgpu_waves_per_eu(FPXG_COL_WPE)))
-k_cols2(ColJob2 j0, const ColJob2* __restrict__ jobs, int L) {
+k_cols2(ColJob2 j0, ColJob2 j1, const ColJob2* __restrict__ jobs, int L) {
+  (void)j1;  // read by load_job for blockIdx.y == 1 of a two-job launch
   using SC = Sched<S>;
   constexpr int NPH = SC::nph;
   constexpr int C = 1 << LOGC;

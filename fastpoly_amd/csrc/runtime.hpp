@@ -157,6 +157,7 @@ enum { LK_ROWS = 0, LK_COLS = 1, LK_EW = 2, LK_RECUR = 3, LK_GARNER = 4 };
 struct LaunchRec {
   int kind;
   int k0, k1, k2, k3;  // shape: rows (logM, lazy, mode, -), cols (S, lazy, inv, L)
+  int par;             // independent of the previous record: may run in the same launch level
   union {
     RowJob2 row;
     ColJob2 col;
@@ -174,15 +175,16 @@ struct Recorder {
     LaunchRec& r = recs.back();
     r.kind = kind;
     r.k0 = r.k1 = r.k2 = r.k3 = 0;
+    r.par = 0;
     return r;
   }
   void push_rows(int logM, bool lazy, int mode, const RowJob2& j) {
     LaunchRec& r = next(LK_ROWS);
     r.k0 = logM; r.k1 = lazy; r.k2 = mode; r.row = j;
   }
-  void push_cols(int S, bool lazy, bool inv, int L, const ColJob2& j) {
+  void push_cols(int S, bool lazy, bool inv, int L, const ColJob2& j, bool par = false) {
     LaunchRec& r = next(LK_COLS);
-    r.k0 = S; r.k1 = lazy; r.k2 = inv; r.k3 = L; r.col = j;
+    r.k0 = S; r.k1 = lazy; r.k2 = inv; r.k3 = L; r.col = j; r.par = par;
   }
   void push_ew(const EwRec& e) { next(LK_EW).ew = e; }
   void push_recur(const RecurRec& e) { next(LK_RECUR).rec = e; }

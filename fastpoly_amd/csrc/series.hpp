@@ -105,6 +105,49 @@ struct Engine {
   }
 };
 
+// Native fields: out[i - lo] = (x * y)[i] for i in [lo, hi) of the 2^L-point
+// cyclic product of x[0, cx) and y[0, cy), both target limbs, with no stored
+// spectra: one launch for a single-row transform (two forwards, product and
+// inverse in one workgroup), three for the four-step ones (both column passes
+// in one launch, the fused row pass, the column inverse into the window).  A
+// Newton step's products are latency-bound chains of small launches, so fewer
+// launches beat reusing a spectrum.
+static void native_conv(Context& c, const HostField& T, int L, const uint32_t* x, size_t cx, const uint32_t* y,
+                        size_t cy, size_t lo, size_t hi, uint32_t* out, bool neg = false) {
+  const uint32_t N = 1u << L;
+  const PassPlan P = c.plan(L);
+  const uint32_t inv_n = inv_pow2(T, L);
+  const uint32_t negf = neg ? R2_NEG : 0u;
+  RowJob2 rj{};
+  rj.f = T.dev;
+  if (P.S == 0) {
+    rj.src_a = x; rj.src_b = y; rj.dst = out; rj.nrows = 1; rj.in_a = (uint32_t)cx; rj.in_b = (uint32_t)cy;
+    rj.out_lo = (uint32_t)lo; rj.out_hi = (uint32_t)hi; rj.out_shift = (uint32_t)lo; rj.inv_n = inv_n;
+    rj.flags = R2_SCALE | R2_CANON | negf;
+    launch_rows(c, T.lazy, R2M_CONV, rj, L, 1);
+    return;
+  }
+  const bool same = x == y && cx == cy;
+  DevBuf X = c.buf(N), Y;
+  ColJob2 jx{};
+  jx.src = x; jx.dst = X.get(); jx.in_count = (uint32_t)cx; jx.f = T.dev;
+  if (same) {
+    launch_cols(c, T.lazy, false, jx, P);
+  } else {
+    Y = c.buf(N);
+    ColJob2 jy = jx;
+    jy.src = y; jy.dst = Y.get(); jy.in_count = (uint32_t)cy;
+    launch_cols_pair(c, T.lazy, false, jx, jy, P);
+  }
+  rj.src_a = X.get(); rj.src_b = same ? X.get() : Y.get(); rj.dst = X.get(); rj.nrows = 1u << P.S;
+  rj.in_a = N; rj.in_b = N; rj.out_lo = 0; rj.out_hi = N;
+  launch_rows(c, T.lazy, R2M_CONV, rj, P.b, rj.nrows);
+  ColJob2 ij{};
+  ij.src = X.get(); ij.dst = out; ij.out_lo = (uint32_t)lo; ij.out_hi = (uint32_t)hi; ij.out_shift = (uint32_t)lo;
+  ij.inv_n = inv_n; ij.flags = negf; ij.f = T.dev;
+  launch_cols(c, T.lazy, true, ij, P);
+}
+
 // extend_inverse (inverse.hpp:18-65): c[0, k) is 1/a mod x^k; fill c[k, wanted).
 static void extend_inverse(Context& c, const HostField& T, const uint32_t* a, size_t la, uint32_t* cc, size_t k,
                            size_t wanted) {
@@ -114,9 +157,14 @@ static void extend_inverse(Context& c, const HostField& T, const uint32_t* a, si
     return;
   }
   Engine E(c, T, ceil_log2(2 * k));
+  DevBuf e = c.buf(hlen);
+  if (E.native) {
+    native_conv(c, T, E.L, a, std::min(la, wanted), cc, k, k, wanted, e.get(), true);
+    native_conv(c, T, E.L, e.get(), hlen, cc, k, 0, hlen, cc + k);
+    return;
+  }
   Spec Sc;
   E.forward(Sc, cc, k);
-  DevBuf e = c.buf(hlen);
   E.product(a, std::min(la, wanted), Sc, k, wanted, e.get(), true);
   E.product(e.get(), hlen, Sc, 0, hlen, cc + k);
 }
@@ -217,9 +265,16 @@ static void series_emit(Context& c, const HostField& T, const SeriesReq& q, cons
           csize = hlen;
         }
         Engine E(c, T, ceil_log2(2 * m));
+        DevBuf r = c.buf(hlen), t = c.buf(hlen);
+        if (E.native) {
+          native_conv(c, T, E.L, ap.get(), std::min(lap, m2 - 1), b, m, m - 1, m - 1 + hlen, r.get());
+          native_conv(c, T, E.L, r.get(), hlen, cbuf.get(), hlen, 0, hlen, t.get());
+          emit_ew(c, T, EW_MUL_RECIP, t.get(), t.get(), nullptr, hlen, 0, (uint32_t)m);
+          native_conv(c, T, E.L, t.get(), hlen, b, m, 0, hlen, b + m);
+          continue;
+        }
         Spec Sb, Sc;
         E.forward(Sb, b, m);
-        DevBuf r = c.buf(hlen), t = c.buf(hlen);
         E.product(ap.get(), std::min(lap, m2 - 1), Sb, m - 1, m - 1 + hlen, r.get());
         E.forward(Sc, cbuf.get(), hlen);
         E.product(r.get(), hlen, Sc, 0, hlen, t.get());
@@ -264,9 +319,17 @@ static void series_emit(Context& c, const HostField& T, const SeriesReq& q, cons
           csize = hlen;
         }
         Engine E(c, T, ceil_log2(2 * m));
+        DevBuf sq = c.buf(hlen), res = c.buf(hlen);
+        if (E.native) {
+          native_conv(c, T, E.L, b, m, b, m, m, m2, sq.get());
+          emit_ew(c, T, EW_SUB_FROM, res.get(), av + std::min(m, au), sq.get(), hlen,
+                  (uint32_t)(au > m ? au - m : 0), 0);
+          native_conv(c, T, E.L, res.get(), hlen, cbuf.get(), hlen, 0, hlen, b + m);
+          emit_ew(c, T, EW_SCALE, b + m, b + m, nullptr, hlen, 0, half_inv);
+          continue;
+        }
         Spec Sb, Sc;
         E.forward(Sb, b, m);
-        DevBuf sq = c.buf(hlen), res = c.buf(hlen);
         E.product(b, m, Sb, m, m2, sq.get());
         emit_ew(c, T, EW_SUB_FROM, res.get(), av + std::min(m, au), sq.get(), hlen,
                 (uint32_t)(au > m ? au - m : 0), 0);
