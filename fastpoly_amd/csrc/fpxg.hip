@@ -352,21 +352,35 @@ static void launch_rows(Context& c, bool lazy, int mode, const RowJob2& j, int l
   FPXG_HIP(hipGetLastError());
 }
 
-// Column tiles: 16 columns up to 2^9 levels, 8 for 2^10 and 2^11 (a 2^10 x 8
-// tile leaves room for four workgroups per CU), 4 for 2^12.
+// Column tiles: 16 columns up to 2^10 levels, 8 for 2^11, 4 for 2^12.  A
+// 2^10-level pass with fewer than 512 tiles of 16 uses tiles of 8, so a
+// mid-size transform still spreads over the CUs.
 static constexpr int col_tile_log(int S) { return S <= 10 ? 4 : (S <= 11 ? 3 : 2); }
-static int tile_log_cols(int S) { return col_tile_log(S); }
+static int tile_log_cols(int S, int L) {
+  if (S == 10 && L - S - 4 < 9) return 3;
+  return col_tile_log(S);
+}
 
 // njobs jobs from the device array dj, or (dj null) one job j, or two jobs j
 // and *j1 passed by value.
-template <int S, bool LAZY, bool INV>
-static void cols2_launch(Context& c, const ColJob2& j, const PassPlan& P, uint32_t njobs, const ColJob2* dj,
-                         const ColJob2* j1) {
-  constexpr int LOGC = col_tile_log(S);
+template <int S, int LOGC, bool LAZY, bool INV>
+static void cols2_launch_c(Context& c, const ColJob2& j, const PassPlan& P, uint32_t njobs, const ColJob2* dj,
+                           const ColJob2* j1) {
   using CS = ColShape<S, LOGC>;
   const size_t lds = (size_t)CS::words * 4;
   dim3 grid(1u << (P.b - LOGC), njobs), block(CS::T);
   hipLaunchKernelGGL((k_cols2<S, LOGC, LAZY, INV>), grid, block, lds, c.stream, j, j1 ? *j1 : j, dj, P.L);
+}
+template <int S, bool LAZY, bool INV>
+static void cols2_launch(Context& c, const ColJob2& j, const PassPlan& P, uint32_t njobs, const ColJob2* dj,
+                         const ColJob2* j1) {
+  if constexpr (S == 10) {
+    if (P.logC == 3) {
+      cols2_launch_c<S, 3, LAZY, INV>(c, j, P, njobs, dj, j1);
+      return;
+    }
+  }
+  cols2_launch_c<S, col_tile_log(S), LAZY, INV>(c, j, P, njobs, dj, j1);
 }
 
 template <bool LAZY, bool INV>
@@ -481,7 +495,7 @@ PassPlan Context::plan(int L) const {
   int S = std::max(L - row_log_max, std::min(L / 2, 10));
   if (S > 12) throw Error(E_SIZE, "transform too large for two passes");
   const int b = L - S;
-  return {L, b, S, tile_log_cols(S)};
+  return {L, b, S, tile_log_cols(S, L)};
 }
 
 static unsigned grid_for(size_t n) {
@@ -988,7 +1002,7 @@ int fpxg_autotune(int min_log, int max_log, int reps, int* chosen_col_levels, fl
       std::vector<PassPlan> cand;
       if (L <= c.row_log_max) cand.push_back({L, L, 0, 0});
       for (int S = std::max(1, L - c.row_log_max); S <= std::min(12, L - 1); ++S)
-        if (S >= 4 && L - S >= 4) cand.push_back({L, L - S, S, tile_log_cols(S)});
+        if (S >= 4 && L - S >= 4) cand.push_back({L, L - S, S, tile_log_cols(S, L)});
       if (cand.empty()) cand.push_back(c.plan(L));
       float best = 1e30f;
       PassPlan pick = cand[0];

@@ -21,7 +21,7 @@ static void issue_one(Context& c, const LaunchRec& r) {
       launch_rows(c, r.k1 != 0, r.k2, r.row, r.k0, r.row.nrows);
       return;
     case LK_COLS: {
-      const PassPlan P{r.k3, r.k3 - r.k0, r.k0, tile_log_cols(r.k0)};
+      const PassPlan P{r.k3, r.k3 - r.k0, r.k0, tile_log_cols(r.k0, r.k3)};
       launch_cols(c, r.k1 != 0, r.k2 != 0, r.col, P);
       return;
     }
@@ -139,7 +139,7 @@ static void series_call(Context& c, const HostField& T, int op, const uint32_t* 
         const LaunchRec& r = recs[i];
         if (r.kind == LK_COLS && i + 1 < recs.size() && recs[i + 1].par && recs[i + 1].kind == LK_COLS &&
             recs[i + 1].k0 == r.k0 && recs[i + 1].k1 == r.k1 && recs[i + 1].k2 == r.k2 && recs[i + 1].k3 == r.k3) {
-          const PassPlan P{r.k3, r.k3 - r.k0, r.k0, tile_log_cols(r.k0)};
+          const PassPlan P{r.k3, r.k3 - r.k0, r.k0, tile_log_cols(r.k0, r.k3)};
           launch_cols_pair(c, r.k1 != 0, r.k2 != 0, r.col, recs[i + 1].col, P);
           ++i;
           continue;

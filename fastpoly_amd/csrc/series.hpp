@@ -120,7 +120,9 @@ static void native_conv(Context& c, const HostField& T, int L, const uint32_t* x
   const uint32_t negf = neg ? R2_NEG : 0u;
   RowJob2 rj{};
   rj.f = T.dev;
-  if (P.S == 0) {
+  // Up to 2^14 points a product is one workgroup's launch even where a lone
+  // transform would be split over the CUs: one launch beats three here.
+  if (P.S == 0 || L <= c.row_log_max) {
     rj.src_a = x; rj.src_b = y; rj.dst = out; rj.nrows = 1; rj.in_a = (uint32_t)cx; rj.in_b = (uint32_t)cy;
     rj.out_lo = (uint32_t)lo; rj.out_hi = (uint32_t)hi; rj.out_shift = (uint32_t)lo; rj.inv_n = inv_n;
     rj.flags = R2_SCALE | R2_CANON | negf;
