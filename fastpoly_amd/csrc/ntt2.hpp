@@ -335,10 +335,24 @@ struct RowJob2 {
   uint32_t out_lo, out_hi, out_shift;
   uint32_t inv_n;            // flags & R2_SCALE: multiply by inv_n at the store
   uint32_t flags;
+  const uint32_t* epi;       // operand of R2_SUBFROM / R2_MULTAB, indexed by output position
+  uint32_t epi_avail, pad_;
   FieldDev f;
 };
-enum : uint32_t { R2_LIFT = 1, R2_CANON = 2, R2_SCALE = 4, R2_NEG = 8, R2_ACC = 16 };
+enum : uint32_t { R2_LIFT = 1, R2_CANON = 2, R2_SCALE = 4, R2_NEG = 8, R2_ACC = 16, R2_SUBFROM = 32, R2_MULTAB = 64 };
 // R2_ACC (column inverse only): dst[i] = dst[i] + result, both canonical.
+// Inverse-pass epilogues on the canonical result v at output position t (the
+// row pass needs R2_CANON with them): R2_SUBFROM v = (t < epi_avail ? epi[t]
+// : 0) - v; R2_MULTAB v = v * epi[t] (Montgomery limbs).
+__device__ __forceinline__ uint32_t out_epilogue(uint32_t flags, const uint32_t* epi, uint32_t avail, uint32_t t,
+                                                 uint32_t v, uint32_t p, uint32_t ninv) {
+  if (flags & R2_SUBFROM) {
+    const uint32_t s = t < avail ? gload(epi + t) : 0u;
+    v = s >= v ? s - v : s - v + p;
+  }
+  if (flags & R2_MULTAB) v = mulmod_full(v, gload(epi + t), p, ninv);
+  return v;
+}
 enum : int { R2M_FWD = 1, R2M_INV = 2, R2M_CONV = 3, R2M_SPEC = 4, R2M_SPEC_ONLY = 5 };
 // R2M_FWD:  forward a                      R2M_INV:       inverse a
 // R2M_CONV: forward a, forward b, a*b, inverse
@@ -489,7 +503,12 @@ k_rows2(RowJob2 j0, const RowJob2* __restrict__ jobs) {
       for (int k = 0; k < (1 << r0); ++k) {
         const uint32_t gi = base + elem<LOGM, 0>(PhaseGroups<LOGM, 0>::gid(g, u), k);
         const uint32_t v = x[u * (1 << r0) + k];
-        if (active && gi >= J.out_lo && gi < J.out_hi) gstore(J.dst + (gi - J.out_shift), min(v, v - canon_p));
+        if (active && gi >= J.out_lo && gi < J.out_hi) {
+          uint32_t w = min(v, v - canon_p);
+          if (flags & (R2_SUBFROM | R2_MULTAB))
+            w = out_epilogue(flags, J.epi, J.epi_avail, gi - J.out_shift, w, J.f.p, J.f.ninv);
+          gstore(J.dst + (gi - J.out_shift), w);
+        }
       }
   }
 }
@@ -505,7 +524,9 @@ struct ColJob2 {
   uint32_t in_count;
   uint32_t out_lo, out_hi, out_shift;
   uint32_t inv_n;
-  uint32_t flags;  // R2_LIFT, R2_CANON, R2_NEG, R2_ACC
+  uint32_t flags;  // R2_LIFT, R2_CANON, R2_NEG, R2_ACC, R2_SUBFROM, R2_MULTAB
+  const uint32_t* epi;
+  uint32_t epi_avail, pad_;
   FieldDev f;
 };
 
@@ -601,6 +622,8 @@ k_cols2(ColJob2 j0, ColJob2 j1, const ColJob2* __restrict__ jobs, int L) {
         const uint32_t v = A.canon(A.half(x[u * (1 << r0) + k]));
         if (gi >= J.out_lo && gi < J.out_hi) {
           uint32_t w = neg ? (v == 0 ? 0u : J.f.p - v) : v;
+          if (flags & (R2_SUBFROM | R2_MULTAB))
+            w = out_epilogue(flags, J.epi, J.epi_avail, gi - J.out_shift, w, J.f.p, J.f.ninv);
           if (acc) {
             const uint32_t s = gload(J.dst + (gi - J.out_shift)) + w;
             w = min(s, s - J.f.p);

@@ -35,7 +35,8 @@ enum {
   EW_INTEGRAL = 5,  // dst[i] = i in [1, avail] ? src[i-1] / i : 0  (calculus.hpp:97-116)
   EW_SUB_FROM = 6,  // dst[i] = (i < avail ? src[i] : 0) - src2[i]  (square_root.hpp:93-94)
   EW_REVERSE = 7,   // dst[i] = src[n - 1 - i]                      (division.hpp:53-58)
-  EW_MULADD = 8     // dst[i] = (c ? dst[i] : 0) + src[i] * src2[i] (spectra of split products)
+  EW_MULADD = 8,    // dst[i] = (c ? dst[i] : 0) + src[i] * src2[i] (spectra of split products)
+  EW_RECIP = 9      // dst[i] = 1 / (c + i), 0 for c + i == 0       (exponential.hpp:194-195 table)
 };
 struct EwRec {
   int op;

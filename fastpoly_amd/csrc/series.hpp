@@ -112,14 +112,27 @@ struct Engine {
 // in one launch, the fused row pass, the column inverse into the window).  A
 // Newton step's products are latency-bound chains of small launches, so fewer
 // launches beat reusing a spectrum.
+// Epilogue of a window product: the result is multiplied by `scale` (a
+// Montgomery limb, folded into 1/N; 0 for none), negated, then, with
+// epi_flags R2_SUBFROM or R2_MULTAB, combined with epi[t] at output position t
+// (see out_epilogue), which saves a Newton step its elementwise launch.
+struct Epi {
+  uint32_t scale = 0;
+  uint32_t flags = 0;
+  const uint32_t* src = nullptr;
+  uint32_t avail = 0;
+};
+
 static void native_conv(Context& c, const HostField& T, int L, const uint32_t* x, size_t cx, const uint32_t* y,
-                        size_t cy, size_t lo, size_t hi, uint32_t* out, bool neg = false) {
+                        size_t cy, size_t lo, size_t hi, uint32_t* out, bool neg = false, const Epi& ep = Epi()) {
   const uint32_t N = 1u << L;
   const PassPlan P = c.plan(L);
-  const uint32_t inv_n = inv_pow2(T, L);
-  const uint32_t negf = neg ? R2_NEG : 0u;
+  const uint32_t inv_n = ep.scale ? T.mul(inv_pow2(T, L), ep.scale) : inv_pow2(T, L);
+  const uint32_t negf = (neg ? R2_NEG : 0u) | ep.flags;
   RowJob2 rj{};
   rj.f = T.dev;
+  rj.epi = ep.src;
+  rj.epi_avail = ep.avail;
   // Up to 2^14 points a product is one workgroup's launch even where a lone
   // transform would be split over the CUs: one launch beats three here.
   if (P.S == 0 || L <= c.row_log_max) {
@@ -146,7 +159,7 @@ static void native_conv(Context& c, const HostField& T, int L, const uint32_t* x
   launch_rows(c, T.lazy, R2M_CONV, rj, P.b, rj.nrows);
   ColJob2 ij{};
   ij.src = X.get(); ij.dst = out; ij.out_lo = (uint32_t)lo; ij.out_hi = (uint32_t)hi; ij.out_shift = (uint32_t)lo;
-  ij.inv_n = inv_n; ij.flags = negf; ij.f = T.dev;
+  ij.inv_n = inv_n; ij.flags = negf; ij.epi = ep.src; ij.epi_avail = ep.avail; ij.f = T.dev;
   launch_cols(c, T.lazy, true, ij, P);
 }
 
@@ -256,6 +269,13 @@ static void series_emit(Context& c, const HostField& T, const SeriesReq& q, cons
       DevBuf cbuf = c.buf(n);
       recur(c, T, REC_INV, b, seed, nullptr, 0, cbuf.get(), 0, seed / 2, 0, 0);
       size_t csize = seed / 2;
+      // 1/i for the steps' t[i] /= m + i (exponential.hpp:194-195), applied in
+      // the product epilogue on native fields.
+      DevBuf rtab;
+      if (series_native(T, ceil_log2(2 * seed))) {
+        rtab = c.buf(n);
+        emit_ew(c, T, EW_RECIP, rtab.get(), nullptr, nullptr, n, 0, 0);
+      }
       for (size_t m = seed; m < n; m *= 2) {
         const size_t m2 = std::min(2 * m, n), hlen = m2 - m;
         if (hlen <= 8) {
@@ -268,10 +288,12 @@ static void series_emit(Context& c, const HostField& T, const SeriesReq& q, cons
         }
         Engine E(c, T, ceil_log2(2 * m));
         DevBuf r = c.buf(hlen), t = c.buf(hlen);
-        if (E.native) {
+        if (E.native && rtab.get()) {
           native_conv(c, T, E.L, ap.get(), std::min(lap, m2 - 1), b, m, m - 1, m - 1 + hlen, r.get());
-          native_conv(c, T, E.L, r.get(), hlen, cbuf.get(), hlen, 0, hlen, t.get());
-          emit_ew(c, T, EW_MUL_RECIP, t.get(), t.get(), nullptr, hlen, 0, (uint32_t)m);
+          Epi ep;
+          ep.flags = R2_MULTAB;
+          ep.src = rtab.get() + m;
+          native_conv(c, T, E.L, r.get(), hlen, cbuf.get(), hlen, 0, hlen, t.get(), false, ep);
           native_conv(c, T, E.L, t.get(), hlen, b, m, 0, hlen, b + m);
           continue;
         }
@@ -323,11 +345,14 @@ static void series_emit(Context& c, const HostField& T, const SeriesReq& q, cons
         Engine E(c, T, ceil_log2(2 * m));
         DevBuf sq = c.buf(hlen), res = c.buf(hlen);
         if (E.native) {
-          native_conv(c, T, E.L, b, m, b, m, m, m2, sq.get());
-          emit_ew(c, T, EW_SUB_FROM, res.get(), av + std::min(m, au), sq.get(), hlen,
-                  (uint32_t)(au > m ? au - m : 0), 0);
-          native_conv(c, T, E.L, res.get(), hlen, cbuf.get(), hlen, 0, hlen, b + m);
-          emit_ew(c, T, EW_SCALE, b + m, b + m, nullptr, hlen, 0, half_inv);
+          Epi sub;  // res = a[v + m + i] - (b*b)[m + i]  (square_root.hpp:93-94)
+          sub.flags = R2_SUBFROM;
+          sub.src = av + std::min(m, au);
+          sub.avail = (uint32_t)(au > m ? au - m : 0);
+          native_conv(c, T, E.L, b, m, b, m, m, m2, res.get(), false, sub);
+          Epi half;  // times 1/2 (square_root.hpp:96-97)
+          half.scale = half_inv;
+          native_conv(c, T, E.L, res.get(), hlen, cbuf.get(), hlen, 0, hlen, b + m, false, half);
           continue;
         }
         Spec Sb, Sc;
