@@ -84,6 +84,11 @@ __device__ __forceinline__ uint32_t mulmod_full(uint32_t a, uint32_t b, uint32_t
 __device__ __forceinline__ uint32_t pidx(uint32_t i) { return i + (i >> 5); }
 __host__ __device__ constexpr uint32_t lds_words(uint32_t n) { return n + (n >> 5) + 1; }
 
+__host__ __device__ constexpr uint32_t bitrev_c(uint32_t x, int bits) {
+  uint32_t r = 0;
+  for (int i = 0; i < bits; ++i) r |= ((x >> i) & 1u) << (bits - 1 - i);
+  return r;
+}
 __device__ __forceinline__ uint32_t bitrev(uint32_t x, int bits) {
   return bits == 0 ? 0u : (__builtin_bitreverse32(x) >> (32 - bits));
 }

@@ -587,28 +587,43 @@ k_cols2(ColJob2 j0, ColJob2 j1, const ColJob2* __restrict__ jobs, int L) {
     }
     phase_compute<S, 0, false, LAZY, LOGC>(x, g, J.f.planes_fwd, A);
     fwd_phases<S, 1, NPH, LOGC, LAZY>(x, X, g, c, J.f.planes_fwd, A);
+    // Four-step twiddle w^(col * bitrev_S(e)).  In the last phase e = gid *
+    // 2^rl + k, so the exponent is col * bitrev_(S-rl)(gid) + j * col *
+    // 2^(S-rl) with j = bitrev_rl(k): one table lookup for the group's first
+    // factor, one for the ratio D, then a running product per element.
     constexpr int rl = SC::r_of(PL);
+    const uint32_t D = twiddle<LAZY>(A, J.f.tw_lo, J.f.tw_hi, J.f.split, (col << (S - rl)) << shift);
 #pragma unroll
-    for (int u = 0; u < PhaseGroups<S, PL, LOGC>::NG; ++u)
+    for (int u = 0; u < PhaseGroups<S, PL, LOGC>::NG; ++u) {
+      const uint32_t gid = PhaseGroups<S, PL, LOGC>::gid(g, u);
+      uint32_t q = twiddle<LAZY>(A, J.f.tw_lo, J.f.tw_hi, J.f.split, (col * bitrev(gid << rl, S)) << shift);
 #pragma unroll
-      for (int k = 0; k < (1 << rl); ++k) {
-        const uint32_t e = elem<S, PL>(PhaseGroups<S, PL, LOGC>::gid(g, u), k);
-        const uint32_t xx = (col * bitrev(e, S)) << shift;
-        const uint32_t v = A.mul(x[u * (1 << rl) + k], twiddle<LAZY>(A, J.f.tw_lo, J.f.tw_hi, J.f.split, xx));
+      for (int j = 0; j < (1 << rl); ++j) {
+        const int k = (int)bitrev_c(j, rl);
+        const uint32_t e = elem<S, PL>(gid, k);
+        const uint32_t v = A.mul(x[u * (1 << rl) + k], q);
         gstore(J.dst + e * M + col, min(v, v - canon_p));
+        if (j + 1 < (1 << rl)) q = A.mul(q, D);
       }
+    }
   } else {
     constexpr int rl = SC::r_of(PL);
     const bool neg = (flags & R2_NEG) != 0;
+    // Inverse four-step twiddle times 1/N, by the same running product.
+    const uint32_t D = twiddle<LAZY>(A, J.f.tw_lo_inv, J.f.tw_hi_inv, J.f.split, (col << (S - rl)) << shift);
 #pragma unroll
-    for (int u = 0; u < PhaseGroups<S, PL, LOGC>::NG; ++u)
+    for (int u = 0; u < PhaseGroups<S, PL, LOGC>::NG; ++u) {
+      const uint32_t gid = PhaseGroups<S, PL, LOGC>::gid(g, u);
+      uint32_t q = A.mul(twiddle<LAZY>(A, J.f.tw_lo_inv, J.f.tw_hi_inv, J.f.split, (col * bitrev(gid << rl, S)) << shift),
+                         J.inv_n);
 #pragma unroll
-      for (int k = 0; k < (1 << rl); ++k) {
-        const uint32_t e = elem<S, PL>(PhaseGroups<S, PL, LOGC>::gid(g, u), k);
-        const uint32_t xx = (col * bitrev(e, S)) << shift;
-        const uint32_t tw = A.mul(twiddle<LAZY>(A, J.f.tw_lo_inv, J.f.tw_hi_inv, J.f.split, xx), J.inv_n);
-        x[u * (1 << rl) + k] = A.mul(gload(J.src + e * M + col), tw);
+      for (int j = 0; j < (1 << rl); ++j) {
+        const int k = (int)bitrev_c(j, rl);
+        const uint32_t e = elem<S, PL>(gid, k);
+        x[u * (1 << rl) + k] = A.mul(gload(J.src + e * M + col), q);
+        if (j + 1 < (1 << rl)) q = A.mul(q, D);
       }
+    }
     phase_compute<S, PL, true, LAZY, LOGC>(x, g, J.f.planes_inv, A);
     inv_phases<S, PL - 1, LOGC, LAZY>(x, X, g, c, J.f.planes_inv, A);
     constexpr int r0 = SC::r_of(0);

@@ -39,6 +39,23 @@ def test_ntt_forward_inverse_exact(fa, oracle, p):
         assert np.array_equal(back, x), (p, n)
 
 
+@pytest.mark.parametrize("p,lg", [(P, 23), (167772161, 24)])
+def test_ntt_largest_sizes(fa, oracle, p, lg):
+    """2^23 and the 2^24 transform the roofline times (column pass, then row
+    pass, in place through the pass entry point) against the oracle."""
+    n = 1 << lg
+    root = oracle.lib().fpo_primitive_root(p)
+    x = _rnd(oracle, p, n, 99 + lg)
+    want = oracle.ntt(p, x, root=root)
+    assert np.array_equal(fa.ntt_forward(p, x, root=root), want)
+    d = fa.DeviceArray.from_host(x)
+    fa.ntt_pass_dev(p, d, 0, root=root)
+    fa.ntt_pass_dev(p, d, 1, root=root)
+    assert np.array_equal(d.to_host(), want)
+    d.free()
+    assert np.array_equal(fa.ntt_inverse(p, want, root=root), x)
+
+
 def test_ntt_windows(fa, oracle):
     po = oracle
     for lg in (4, 10, 14, 15, 17):
