@@ -27,6 +27,17 @@ typedef __attribute__((address_space(1))) const u32x4 gu4_c;
 typedef __attribute__((address_space(1))) uint32_t gu32;
 
 __device__ __forceinline__ uint32_t gload(const uint32_t* p) { return *(gu32_c*)p; }
+
+// Raw buffer loads: the base lives in scalar registers and each load adds the
+// lane's 32-bit byte offset and a uniform byte offset, so a run of loads at
+// fixed distances costs no per-load 64-bit address arithmetic.  Offsets at or
+// past `bytes` read 0.
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t buf_rsrc(const void* p, uint32_t bytes) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), (short)0, (int)bytes, 0x00020000);
+}
+__device__ __forceinline__ uint32_t bload(__amdgpu_buffer_rsrc_t r, uint32_t lane_bytes, uint32_t uniform_bytes) {
+  return (uint32_t)__builtin_amdgcn_raw_buffer_load_b32(r, (int)lane_bytes, (int)uniform_bytes, 0);
+}
 __device__ __forceinline__ void gstore(uint32_t* p, uint32_t v) { *(gu32*)p = v; }
 
 // Twiddle planes.  plane(logB, r) starts at word plane_off(logB, r) and holds,
@@ -406,6 +417,20 @@ k_rows2(RowJob2 j0, const RowJob2* __restrict__ jobs) {
     if (in_count == 0) {
 #pragma unroll
       for (int i = 0; i < NE; ++i) dst_regs[i] = 0u;
+      return;
+    }
+    // The whole row is input: no clamps, no zero fill, and (radix-16 first
+    // phase) register k of thread g is word k * GT + g of the row, so each load
+    // is a uniform base plus the thread's 32-bit offset (no 64-bit address
+    // arithmetic per load).
+    if (RS::U == 1 && PhaseGroups<LOGM, 0>::NG == 1 && base + RS::M <= in_count) {
+      const __amdgpu_buffer_rsrc_t rs = buf_rsrc(src + base, RS::M * 4);
+#pragma unroll
+      for (int k = 0; k < (1 << r0); ++k) dst_regs[k] = bload(rs, g * 4, (uint32_t)k * RS::GT * 4);
+      if (lift) {
+#pragma unroll
+        for (int i = 0; i < NE; ++i) dst_regs[i] = A.mul(dst_regs[i], J.f.r2);
+      }
       return;
     }
     const uint32_t last = in_count - 1;
