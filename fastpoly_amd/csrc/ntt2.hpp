@@ -364,7 +364,11 @@ __device__ __forceinline__ uint32_t out_epilogue(uint32_t flags, const uint32_t*
   if (flags & R2_MULTAB) v = mulmod_full(v, gload(epi + t), p, ninv);
   return v;
 }
-enum : int { R2M_FWD = 1, R2M_INV = 2, R2M_CONV = 3, R2M_SPEC = 4, R2M_SPEC_ONLY = 5 };
+enum : int { R2M_FWD = 1, R2M_INV = 2, R2M_CONV = 3, R2M_SPEC = 4, R2M_SPEC_ONLY = 5, R2M_INVSTEP = 6 };
+// R2M_INVSTEP (one Newton step of the power-series inverse, inverse.hpp:18-65,
+// for a 2^LOGM = 2k point row): with c = b[0, k) and a = a[0, in_a),
+// e = (a * c)[k, k + out_hi) and out = (e * c)[0, out_hi) (negated with
+// R2_NEG); c's spectrum is computed once and e never leaves the registers.
 // R2M_FWD:  forward a                      R2M_INV:       inverse a
 // R2M_CONV: forward a, forward b, a*b, inverse
 // R2M_SPEC: forward a, a*spectrum(b), inverse
@@ -452,11 +456,15 @@ k_rows2(RowJob2 j0, const RowJob2* __restrict__ jobs) {
         if (!(active && gi < in_count)) dst_regs[u * (1 << r0) + k] = 0u;
       }
   };
-  // Forward transform of `src` ending with phase NPH-1's groups in regs.
-  auto forward = [&](const uint32_t* src, uint32_t in_count, uint32_t* regs) {
-    load_first(src, in_count, regs);
+  // Forward transform of registers holding phase 0's input, ending with phase
+  // NPH-1's groups in regs; `forward` loads the input first.
+  auto forward_regs = [&](uint32_t* regs) {
     phase_compute<LOGM, 0, false, LAZY>(regs, g, J.f.planes_fwd, A);
     fwd_phases<LOGM, 1, NPH, 0, LAZY>(regs, X, g, 0u, J.f.planes_fwd, A);
+  };
+  auto forward = [&](const uint32_t* src, uint32_t in_count, uint32_t* regs) {
+    load_first(src, in_count, regs);
+    forward_regs(regs);
   };
 
   constexpr int PL = NPH - 1;  // last forward phase = first inverse phase
@@ -482,6 +490,29 @@ k_rows2(RowJob2 j0, const RowJob2* __restrict__ jobs) {
       forward(J.src_b, J.in_b, y);
       __syncthreads();
       forward(J.src_a, J.in_a, x);
+#pragma unroll
+      for (int i = 0; i < NGL * (1 << rl); ++i) x[i] = A.mul(x[i], y[i]);
+    } else if constexpr (MODE == R2M_INVSTEP) {
+      static_assert(LOGM >= 4 && PhaseGroups<LOGM, 0>::NG == 1, "the half shift below needs a radix-16 first phase");
+      uint32_t y[16];
+      forward(J.src_b, J.in_b, y);  // c
+      __syncthreads();
+      forward(J.src_a, J.in_a, x);  // a
+#pragma unroll
+      for (int i = 0; i < NGL * (1 << rl); ++i) x[i] = A.mul(x[i], y[i]);
+      phase_compute<LOGM, PL, true, LAZY>(x, g, J.f.planes_inv, A);
+      inv_phases<LOGM, PL - 1, 0, LAZY>(x, X, g, 0u, J.f.planes_inv, A);
+      // Register k of thread g holds position k * GT + g: the upper half
+      // (k >= 8) is e's [0, k); keep e's first out_hi positions, zero the rest.
+#pragma unroll
+      for (int i = 0; i < 8; ++i) {
+        const uint32_t pos = (uint32_t)i * RowShape<LOGM>::GT + g;
+        const uint32_t v = A.mul(x[i + 8], J.inv_n);
+        x[i] = pos < J.out_hi ? v : 0u;
+        x[i + 8] = 0u;
+      }
+      __syncthreads();  // the exchanges above are done with LDS
+      forward_regs(x);
 #pragma unroll
       for (int i = 0; i < NGL * (1 << rl); ++i) x[i] = A.mul(x[i], y[i]);
     } else if constexpr (MODE == R2M_SPEC) {

@@ -172,6 +172,18 @@ static void extend_inverse(Context& c, const HostField& T, const uint32_t* a, si
     return;
   }
   Engine E(c, T, ceil_log2(2 * k));
+  if (E.native && E.L >= 6 && E.L <= c.row_log_max && (1u << E.L) == 2 * k) {
+    // Both products of the step in one workgroup (R2M_INVSTEP).
+    RowJob2 rj{};
+    rj.f = T.dev;
+    rj.src_a = a; rj.in_a = (uint32_t)std::min(la, wanted);
+    rj.src_b = cc; rj.in_b = (uint32_t)k;
+    rj.dst = cc + k; rj.nrows = 1; rj.out_lo = 0; rj.out_hi = (uint32_t)hlen; rj.out_shift = 0;
+    rj.inv_n = inv_pow2(T, E.L);
+    rj.flags = R2_SCALE | R2_CANON | R2_NEG;
+    launch_rows(c, T.lazy, R2M_INVSTEP, rj, E.L, 1);
+    return;
+  }
   DevBuf e = c.buf(hlen);
   if (E.native) {
     native_conv(c, T, E.L, a, std::min(la, wanted), cc, k, k, wanted, e.get(), true);
