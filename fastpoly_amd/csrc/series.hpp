@@ -133,9 +133,10 @@ static void native_conv(Context& c, const HostField& T, int L, const uint32_t* x
   rj.f = T.dev;
   rj.epi = ep.src;
   rj.epi_avail = ep.avail;
-  // Up to 2^14 points a product is one workgroup's launch even where a lone
-  // transform would be split over the CUs: one launch beats three here.
-  if (P.S == 0 || L <= c.row_log_max) {
+  // Up to 2^13 points a product is one workgroup's launch even where a lone
+  // transform would be split over the CUs: one launch beats three there (at
+  // 2^14 the three transforms in one workgroup take longer than three launches).
+  if (P.S == 0 || L <= std::min(c.row_log_max, 13)) {
     rj.src_a = x; rj.src_b = y; rj.dst = out; rj.nrows = 1; rj.in_a = (uint32_t)cx; rj.in_b = (uint32_t)cy;
     rj.out_lo = (uint32_t)lo; rj.out_hi = (uint32_t)hi; rj.out_shift = (uint32_t)lo; rj.inv_n = inv_n;
     rj.flags = R2_SCALE | R2_CANON | negf;
@@ -172,7 +173,7 @@ static void extend_inverse(Context& c, const HostField& T, const uint32_t* a, si
     return;
   }
   Engine E(c, T, ceil_log2(2 * k));
-  if (E.native && E.L >= 6 && E.L <= c.row_log_max && (1u << E.L) == 2 * k) {
+  if (E.native && E.L >= 6 && E.L <= std::min(c.row_log_max, 13) && (1u << E.L) == 2 * k) {
     // Both products of the step in one workgroup (R2M_INVSTEP).
     RowJob2 rj{};
     rj.f = T.dev;
