@@ -323,11 +323,15 @@ static void rows2_mode(Context& c, int mode, const RowJob2& j, uint32_t nrows, u
     case R2M_SPEC: rows2_launch<LOGM, LAZY, R2M_SPEC>(c, j, nrows, njobs, dj); break;
     case R2M_SPEC_ONLY: rows2_launch<LOGM, LAZY, R2M_SPEC_ONLY>(c, j, nrows, njobs, dj); break;
     case R2M_INVSTEP:
-      if constexpr (LOGM >= 6) {
-        rows2_launch<LOGM, LAZY, R2M_INVSTEP>(c, j, nrows, njobs, dj);
+    case R2M_EXPSTEP:
+    case R2M_SQRTSTEP:
+      if constexpr (LOGM >= 6 && LOGM <= 13) {
+        if (mode == R2M_INVSTEP) rows2_launch<LOGM, LAZY, R2M_INVSTEP>(c, j, nrows, njobs, dj);
+        else if (mode == R2M_EXPSTEP) rows2_launch<LOGM, LAZY, R2M_EXPSTEP>(c, j, nrows, njobs, dj);
+        else rows2_launch<LOGM, LAZY, R2M_SQRTSTEP>(c, j, nrows, njobs, dj);
         break;
       }
-      throw Error(E_DEVICE, "inverse-step row pass below 2^6 points");
+      throw Error(E_DEVICE, "Newton-step row pass outside 2^6 .. 2^13 points");
     default: throw Error(E_DEVICE, "unsupported row-pass mode");
   }
 }
@@ -451,8 +455,8 @@ static void context_init(Context& c) {
   // LDS per workgroup (gfx950: 160 KB per CU).  Without it, stop at 2^13.
   bool big = true;
 #define X(L, M) big = rows2_attr<L, true, M>() && rows2_attr<L, false, M>() && big;
-  X(13, R2M_FWD) X(13, R2M_INV) X(13, R2M_CONV) X(13, R2M_SPEC) X(13, R2M_SPEC_ONLY) X(13, R2M_INVSTEP)
-  X(14, R2M_FWD) X(14, R2M_INV) X(14, R2M_CONV) X(14, R2M_SPEC) X(14, R2M_SPEC_ONLY) X(14, R2M_INVSTEP)
+  X(13, R2M_FWD) X(13, R2M_INV) X(13, R2M_CONV) X(13, R2M_SPEC) X(13, R2M_SPEC_ONLY)
+  X(14, R2M_FWD) X(14, R2M_INV) X(14, R2M_CONV) X(14, R2M_SPEC) X(14, R2M_SPEC_ONLY)
 #undef X
 #define X(S) big = cols2_attr<S, true, false>() && cols2_attr<S, true, true>() && cols2_attr<S, false, false>() && \
                    cols2_attr<S, false, true>() && big;

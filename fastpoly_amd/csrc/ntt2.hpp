@@ -347,7 +347,8 @@ struct RowJob2 {
   uint32_t inv_n;            // flags & R2_SCALE: multiply by inv_n at the store
   uint32_t flags;
   const uint32_t* epi;       // operand of R2_SUBFROM / R2_MULTAB, indexed by output position
-  uint32_t epi_avail, pad_;
+  uint32_t epi_avail, aux;   // aux: R2M_EXPSTEP's window shift
+  const uint32_t* src_c;     // third operand of the Newton-step modes
   FieldDev f;
 };
 enum : uint32_t { R2_LIFT = 1, R2_CANON = 2, R2_SCALE = 4, R2_NEG = 8, R2_ACC = 16, R2_SUBFROM = 32, R2_MULTAB = 64 };
@@ -364,11 +365,19 @@ __device__ __forceinline__ uint32_t out_epilogue(uint32_t flags, const uint32_t*
   if (flags & R2_MULTAB) v = mulmod_full(v, gload(epi + t), p, ninv);
   return v;
 }
-enum : int { R2M_FWD = 1, R2M_INV = 2, R2M_CONV = 3, R2M_SPEC = 4, R2M_SPEC_ONLY = 5, R2M_INVSTEP = 6 };
+enum : int { R2M_FWD = 1, R2M_INV = 2, R2M_CONV = 3, R2M_SPEC = 4, R2M_SPEC_ONLY = 5, R2M_INVSTEP = 6,
+             R2M_EXPSTEP = 7, R2M_SQRTSTEP = 8 };
 // R2M_INVSTEP (one Newton step of the power-series inverse, inverse.hpp:18-65,
 // for a 2^LOGM = 2k point row): with c = b[0, k) and a = a[0, in_a),
 // e = (a * c)[k, k + out_hi) and out = (e * c)[0, out_hi) (negated with
 // R2_NEG); c's spectrum is computed once and e never leaves the registers.
+// R2M_EXPSTEP (one doubling m -> m + h of exp, exponential.hpp:176-200, for a
+// 2^LOGM = 2m point row, h = out_hi): b = src_b[0, in_b = m), a' = src_a[0,
+// in_a), c = src_c[0, h): r = (a' * b)[aux = m - 1, aux + h), t = (r * c)[0, h)
+// times epi[i] (1 / (m + i)), out = (t * b)[0, h).
+// R2M_SQRTSTEP (square_root.hpp:86-100, 2^LOGM = 2m): b = src_b[0, m), c =
+// src_c[0, h): res = (i < epi_avail ? epi[i] : 0) - (b * b)[m + i], out =
+// (res * c)[0, h) * aux (aux = 1/2 as a Montgomery limb).
 // R2M_FWD:  forward a                      R2M_INV:       inverse a
 // R2M_CONV: forward a, forward b, a*b, inverse
 // R2M_SPEC: forward a, a*spectrum(b), inverse
@@ -388,7 +397,9 @@ enum : int { R2M_FWD = 1, R2M_INV = 2, R2M_CONV = 3, R2M_SPEC = 4, R2M_SPEC_ONLY
 #endif
 template <int MODE>
 struct RowWaves {
-  static constexpr int value = (MODE == R2M_FWD || MODE == R2M_INV) ? FPXG_ROW_WPE : FPXG_ROW_WPE_FUSED;
+  static constexpr int value = (MODE == R2M_FWD || MODE == R2M_INV) ? FPXG_ROW_WPE
+                               : (MODE == R2M_INVSTEP || MODE == R2M_EXPSTEP || MODE == R2M_SQRTSTEP) ? 2
+                                                                                             : FPXG_ROW_WPE_FUSED;
 };
 
 template <int LOGM, bool LAZY, int MODE>
@@ -515,6 +526,74 @@ k_rows2(RowJob2 j0, const RowJob2* __restrict__ jobs) {
       forward_regs(x);
 #pragma unroll
       for (int i = 0; i < NGL * (1 << rl); ++i) x[i] = A.mul(x[i], y[i]);
+    } else if constexpr (MODE == R2M_EXPSTEP || MODE == R2M_SQRTSTEP) {
+      static_assert(LOGM >= 4 && PhaseGroups<LOGM, 0>::NG == 1, "positions k * GT + g need a radix-16 first phase");
+      constexpr uint32_t GT = RowShape<LOGM>::GT;
+      const uint32_t h = J.out_hi;
+      uint32_t yb[16], yc[16];
+      forward(J.src_b, J.in_b, yb);  // b[0, m)
+      __syncthreads();
+      forward(J.src_c, h, yc);  // c[0, h)
+      __syncthreads();
+      // Inverse of regs into phase 0's layout (register k of thread g holds
+      // position k * GT + g), times 1/N: values in [0, 2p).
+      auto inverse_scaled = [&](uint32_t* r) {
+        phase_compute<LOGM, PL, true, LAZY>(r, g, J.f.planes_inv, A);
+        inv_phases<LOGM, PL - 1, 0, LAZY>(r, X, g, 0u, J.f.planes_inv, A);
+#pragma unroll
+        for (int i = 0; i < 16; ++i) r[i] = A.mul(r[i], J.inv_n);
+      };
+      if constexpr (MODE == R2M_EXPSTEP) {
+        forward(J.src_a, J.in_a, x);  // a'
+#pragma unroll
+        for (int i = 0; i < 16; ++i) x[i] = A.mul(x[i], yb[i]);
+        inverse_scaled(x);
+        // r = window [aux, aux + h): a shift by aux positions through LDS
+        // (aux + h <= 2^LOGM - 1, inside the row's words).
+        __syncthreads();
+#pragma unroll
+        for (int i = 0; i < 16; ++i) X[pidx((uint32_t)i * GT + g)] = x[i];
+        __syncthreads();
+#pragma unroll
+        for (int i = 0; i < 16; ++i) {
+          const uint32_t pos = (uint32_t)i * GT + g;
+          x[i] = pos < h ? X[pidx(pos + J.aux)] : 0u;
+        }
+        __syncthreads();
+        forward_regs(x);
+#pragma unroll
+        for (int i = 0; i < 16; ++i) x[i] = A.mul(x[i], yc[i]);
+        inverse_scaled(x);
+        // t = (r * c)[0, h) / (m + i)
+#pragma unroll
+        for (int i = 0; i < 16; ++i) {
+          const uint32_t pos = (uint32_t)i * GT + g;
+          x[i] = pos < h ? A.mul(x[i], gload(J.epi + pos)) : 0u;
+        }
+        __syncthreads();
+        forward_regs(x);
+#pragma unroll
+        for (int i = 0; i < 16; ++i) x[i] = A.mul(x[i], yb[i]);
+      } else {
+#pragma unroll
+        for (int i = 0; i < 16; ++i) x[i] = A.mul(yb[i], yb[i]);
+        inverse_scaled(x);
+        // (b * b)[m + i] is the upper half, registers 8..15; res = a - it.
+        const uint32_t p = J.f.p;
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+          const uint32_t pos = (uint32_t)i * GT + g;
+          const uint32_t v = A.canon(x[i + 8]);
+          const uint32_t s = pos < J.epi_avail ? gload(J.epi + pos) : 0u;
+          const uint32_t r = s >= v ? s - v : s - v + p;
+          x[i] = pos < h ? r : 0u;
+          x[i + 8] = 0u;
+        }
+        __syncthreads();
+        forward_regs(x);
+#pragma unroll
+        for (int i = 0; i < 16; ++i) x[i] = A.mul(A.mul(x[i], yc[i]), J.aux);  // times aux = 1/2
+      }
     } else if constexpr (MODE == R2M_SPEC) {
       forward(J.src_a, J.in_a, x);
 #pragma unroll

@@ -300,6 +300,19 @@ static void series_emit(Context& c, const HostField& T, const SeriesReq& q, cons
           csize = hlen;
         }
         Engine E(c, T, ceil_log2(2 * m));
+        if (E.native && rtab.get() && E.L >= 6 && E.L <= std::min(c.row_log_max, 13) && (1u << E.L) == 2 * m) {
+          // The step's three products in one workgroup (R2M_EXPSTEP).
+          RowJob2 rj{};
+          rj.f = T.dev;
+          rj.src_a = ap.get(); rj.in_a = (uint32_t)std::min(lap, m2 - 1);
+          rj.src_b = b; rj.in_b = (uint32_t)m;
+          rj.src_c = cbuf.get(); rj.aux = (uint32_t)(m - 1); rj.epi = rtab.get() + m;
+          rj.dst = b + m; rj.nrows = 1; rj.out_lo = 0; rj.out_hi = (uint32_t)hlen; rj.out_shift = 0;
+          rj.inv_n = inv_pow2(T, E.L);
+          rj.flags = R2_SCALE | R2_CANON;
+          launch_rows(c, T.lazy, R2M_EXPSTEP, rj, E.L, 1);
+          continue;
+        }
         DevBuf r = c.buf(hlen), t = c.buf(hlen);
         if (E.native && rtab.get()) {
           native_conv(c, T, E.L, ap.get(), std::min(lap, m2 - 1), b, m, m - 1, m - 1 + hlen, r.get());
@@ -356,6 +369,19 @@ static void series_emit(Context& c, const HostField& T, const SeriesReq& q, cons
           csize = hlen;
         }
         Engine E(c, T, ceil_log2(2 * m));
+        if (E.native && E.L >= 6 && E.L <= std::min(c.row_log_max, 13) && (1u << E.L) == 2 * m) {
+          // The step's two products in one workgroup (R2M_SQRTSTEP).
+          RowJob2 rj{};
+          rj.f = T.dev;
+          rj.src_b = b; rj.in_b = (uint32_t)m;
+          rj.src_c = cbuf.get(); rj.aux = half_inv;
+          rj.epi = av + std::min(m, au); rj.epi_avail = (uint32_t)(au > m ? au - m : 0);
+          rj.dst = b + m; rj.nrows = 1; rj.out_lo = 0; rj.out_hi = (uint32_t)hlen; rj.out_shift = 0;
+          rj.inv_n = inv_pow2(T, E.L);
+          rj.flags = R2_SCALE | R2_CANON;
+          launch_rows(c, T.lazy, R2M_SQRTSTEP, rj, E.L, 1);
+          continue;
+        }
         DevBuf sq = c.buf(hlen), res = c.buf(hlen);
         if (E.native) {
           Epi sub;  // res = a[v + m + i] - (b*b)[m + i]  (square_root.hpp:93-94)
