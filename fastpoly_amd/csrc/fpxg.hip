@@ -310,6 +310,16 @@ template <int LOGM, bool LAZY, int MODE>
 static void rows2_launch(Context& c, const RowJob2& j, uint32_t nrows, uint32_t njobs, const RowJob2* dj) {
   using RS = RowShape<LOGM>;
   const size_t lds = (size_t)RS::U * RS::lds_words_per_row * 4;
+  // Many whole rows of 2^14 words, forward: the persistent prefetching pass.
+  if constexpr (LOGM == kDmaRowLog && MODE == R2M_FWD) {
+    if (!dj && njobs == 1 && c.row_dma && !(j.flags & R2_LIFT) && nrows >= 2u * (uint32_t)c.cus &&
+        (size_t)j.in_a >= ((size_t)nrows << LOGM)) {
+      RowJob2 jj = j;
+      jj.nrows = nrows;
+      hipLaunchKernelGGL((k_rows_fwd_dma<LAZY>), dim3((uint32_t)c.cus), dim3(RS::T), 0, c.stream, jj);
+      return;
+    }
+  }
   dim3 grid((nrows + RS::U - 1) / RS::U, njobs), block(RS::T);
   hipLaunchKernelGGL((k_rows2<LOGM, LAZY, MODE>), grid, block, lds, c.stream, j, dj);
 }

@@ -649,6 +649,116 @@ k_rows2(RowJob2 j0, const RowJob2* __restrict__ jobs) {
 }
 
 // ---------------------------------------------------------------------------
+// Persistent forward row pass for many rows of 2^14 words (the second pass of
+// large transforms).  One 1024-thread workgroup per CU walks rows blockIdx.x,
+// + gridDim.x, ...; while it transforms a row, the next row streams into a
+// second LDS buffer by global_load_lds_dwordx4 (no registers), and every
+// phase's twiddles stay in registers for all rows.  The only wait for that
+// prefetch is the vmcnt(0) right before the row's stores: by then the
+// prefetch (issued after the first phase) and the previous row's stores
+// (issued a whole row earlier) are done, so the wait is short and relies on
+// no ordering between loads and stores.  Barriers inside use LDS-only waits
+// (s_waitcnt lgkmcnt(0) + s_barrier), not __syncthreads, whose workgroup
+// fence would wait for the prefetch.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void lds_barrier() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_s_waitcnt(0xC07F);  // lgkmcnt(0); vmcnt and expcnt left at their maxima
+  __builtin_amdgcn_s_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+__device__ __forceinline__ void wait_vmem() { __builtin_amdgcn_s_waitcnt(0x0F70); }  // vmcnt(0)
+
+template <int LOGB, int P>
+__device__ __forceinline__ void load_phase_tw(uint32_t* T, const uint32_t* planes, uint32_t g) {
+  constexpr int r = Sched<LOGB>::r_of(P);
+  const uint32_t* plane = planes + plane_off(LOGB - Sched<LOGB>::s_of(P), r);
+#pragma unroll
+  for (int u = 0; u < PhaseGroups<LOGB, P>::NG; ++u)
+    load_tw<r>(T + u * (1 << r), plane, group_lo<LOGB, P>(PhaseGroups<LOGB, P>::gid(g, u)));
+}
+template <int LOGB, int P, bool LAZY>
+__device__ __forceinline__ void compute_with(uint32_t* x, const uint32_t* T, const Arith<LAZY>& A) {
+  constexpr int r = Sched<LOGB>::r_of(P);
+  constexpr bool lo0 = LOGB - Sched<LOGB>::s_of(P) - r == 0;
+#pragma unroll
+  for (int u = 0; u < PhaseGroups<LOGB, P>::NG; ++u) dif_block<r, LAZY, lo0>(x + u * (1 << r), T + u * (1 << r), A);
+}
+
+constexpr int kDmaRowLog = 14;
+
+template <bool LAZY>
+__global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(4)))
+k_rows_fwd_dma(RowJob2 j0) {
+  constexpr int LOGM = kDmaRowLog;
+  using SC = Sched<LOGM>;
+  static_assert(SC::nph == 4 && SC::r_of(0) == 4 && SC::r_of(1) == 2 && SC::r_of(2) == 4 && SC::r_of(3) == 4,
+                "phase shape of a 2^14 row");
+  static_assert(!WaveLocal<LOGM, 1, 0>::value && WaveLocal<LOGM, 2, 0>::value && WaveLocal<LOGM, 3, 0>::value,
+                "one workgroup-wide exchange, two inside waves");
+  static_assert(RowShape<LOGM>::T == 1024 && RowShape<LOGM>::U == 1, "one row a workgroup");
+  __shared__ uint32_t LA[1u << LOGM];          // prefetched row, unpadded
+  __shared__ uint32_t LE[lds_words(1u << LOGM)];  // phase exchanges, padded
+  const RowJob2 J = load_job(j0, (const RowJob2*)nullptr);
+  const Arith<LAZY> A(J.f);
+  const uint32_t g = threadIdx.x;
+  const uint32_t wv = __builtin_amdgcn_readfirstlane(g >> 6), ln = g & 63;
+  const uint32_t canon_p = (J.flags & R2_CANON) ? J.f.p : 0u;
+  // Row r into LA: 64 chunks of 1 KiB (64 lanes x 16 bytes), four per wave.
+  auto prefetch = [&](uint32_t r) {
+    const uint32_t* src = J.src_a + ((size_t)r << LOGM);
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const uint32_t chunk = wv * 4 + i;
+      __builtin_amdgcn_global_load_lds((__attribute__((address_space(1))) void*)(src + chunk * 256 + ln * 4),
+                                       (__attribute__((address_space(3))) void*)(LA + chunk * 256), 16, 0, 0);
+    }
+  };
+  uint32_t T0[16], T1[16], T2[16], T3[16];
+  load_phase_tw<LOGM, 0>(T0, J.f.planes_fwd, g);
+  load_phase_tw<LOGM, 1>(T1, J.f.planes_fwd, g);
+  load_phase_tw<LOGM, 2>(T2, J.f.planes_fwd, g);
+  load_phase_tw<LOGM, 3>(T3, J.f.planes_fwd, g);
+  uint32_t row = blockIdx.x;
+  prefetch(row);
+  wait_vmem();
+  for (;;) {
+    lds_barrier();  // every wave's part of this row has landed (each waited before its stores)
+    uint32_t x[16];
+#pragma unroll
+    for (int k = 0; k < 16; ++k) x[k] = LA[elem<LOGM, 0>(g, k)];
+    compute_with<LOGM, 0, LAZY>(x, T0, A);
+    const uint32_t next = row + gridDim.x;
+    lds_barrier();  // LA consumed by every wave; the previous row's LE reads are done
+    // Always issued (the last row re-reads a valid row) so the code stays straight-line.
+    prefetch(min(next, J.nrows - 1));
+    phase_to_lds<LOGM, 0, 0>(LE, x, g, 0);
+    lds_barrier();
+    phase_from_lds<LOGM, 1, 0>(x, LE, g, 0);
+    compute_with<LOGM, 1, LAZY>(x, T1, A);
+    wave_sync();
+    phase_to_lds<LOGM, 1, 0>(LE, x, g, 0);
+    wave_sync();
+    phase_from_lds<LOGM, 2, 0>(x, LE, g, 0);
+    compute_with<LOGM, 2, LAZY>(x, T2, A);
+    wave_sync();
+    phase_to_lds<LOGM, 2, 0>(LE, x, g, 0);
+    wave_sync();
+    phase_from_lds<LOGM, 3, 0>(x, LE, g, 0);
+    compute_with<LOGM, 3, LAZY>(x, T3, A);
+    wait_vmem();  // the prefetch of the next row and this wave's previous stores
+#pragma unroll
+    for (int k = 0; k < 16; ++k) {
+      const uint32_t v = x[k];
+      gstore(J.dst + ((size_t)row << LOGM) + elem<LOGM, 3>(g, k), min(v, v - canon_p));
+    }
+    row = next;
+    if (row >= J.nrows) break;
+  }
+  wait_vmem();  // the redundant last prefetch lands before the workgroup's LDS is released
+}
+
+// ---------------------------------------------------------------------------
 // Column pass: a tile of C = 2^LOGC adjacent columns x 2^S rows of the
 // 2^S x M row-major view, with the four-step twiddle w_N^(c * bitrev_S(row))
 // applied after the forward levels (before the inverse ones).

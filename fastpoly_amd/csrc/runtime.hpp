@@ -197,6 +197,7 @@ struct Context {
   hipEvent_t events[16] = {};
   size_t lds_bytes = 65536;  // usable dynamic LDS per workgroup
   int row_log_max = 14;      // largest row pass
+  bool row_dma = true;       // persistent prefetching forward pass for many 2^14-word rows
   int single_log_max = 12;   // largest transform done as one row pass
   int cus = 256;
   std::map<int, PassPlan> plan_override;  // autotuned splits by log2 n
