@@ -125,12 +125,15 @@ def main():
     out_s = [D(N_SERIES) for _ in range(4)]
     fa.synchronize()
 
+    ops = (("conv", lambda: fa.conv_dev(P, da, N_CONV, db, N_CONV, out_conv)),
+           ("inv", lambda: fa.series_dev("inv", P, dinv, N_SERIES, N_SERIES, out_s[0])),
+           ("log", lambda: fa.series_dev("log", P, dlog, N_SERIES, N_SERIES, out_s[1])),
+           ("exp", lambda: fa.series_dev("exp", P, dexp, N_SERIES, N_SERIES, out_s[2])),
+           ("sqrt", lambda: fa.series_dev("sqrt", P, dsq, N_SERIES, N_SERIES, out_s[3])))
+
     def step():
-        fa.conv_dev(P, da, N_CONV, db, N_CONV, out_conv)
-        fa.series_dev("inv", P, dinv, N_SERIES, N_SERIES, out_s[0])
-        fa.series_dev("log", P, dlog, N_SERIES, N_SERIES, out_s[1])
-        fa.series_dev("exp", P, dexp, N_SERIES, N_SERIES, out_s[2])
-        fa.series_dev("sqrt", P, dsq, N_SERIES, N_SERIES, out_s[3])
+        for _, fn in ops:
+            fn()
 
     for _ in range(args.warmup):
         step()
@@ -145,6 +148,16 @@ def main():
     wall = t1 - t0
     ms_per_step = 1e3 * wall / args.steps
     value = output_coeffs() * args.steps / wall
+    # Per-operation split of the step (after the timed region; HIP events on
+    # the library stream, 3 back-to-back calls of each operation).
+    by_op = {}
+    for name, fn in ops:
+        fa.synchronize()
+        fa.event_record(2)
+        for _ in range(3):
+            fn()
+        fa.event_record(3)
+        by_op[name] = round(fa.event_elapsed_ms(2, 3) / 3, 4)
 
     # Host-buffer form of the same conv (fpxg_conv_u32): includes the PCIe copies
     # of both operands and the product; reported beside `value`, never as it.
@@ -168,11 +181,14 @@ def main():
         "config": {"workload": "single large transforms mod 998244353: conv 2^23 x 2^23 (split into 2^23-point native NTTs) "
                                "+ inv/log/exp/sqrt at n = 2^20",
                    "modulus": P, "conv_len": N_CONV, "series_n": N_SERIES},
+        "step_ms_by_op": by_op,
         "roofline": roofline,
         "cpu_baseline": None,
         "pcie_inclusive": pcie | {"same_as_device_path": bool(np.array_equal(host_prod, gpu_out["conv"]))},
         "autotune": {"seconds": round(t_tune, 3),
                      "col_levels_by_log2n": {str(k): v[0] for k, v in tuned.items()},
+                     "product_col_levels_by_log2n": {str(k): fa.product_plan_info(1 << k)["col_levels"]
+                                                     for k in tuned if k >= 13},
                      "forward_us_by_log2n": {str(k): round(v[1], 2) for k, v in tuned.items()}},
     }
 

@@ -43,7 +43,7 @@ EXPORTED_SYMBOLS = [
     "fpxg_ntt_pass_dev", "fpxg_ntt_plan_info", "fpxg_batch_submit", "fpxg_wait", "fpxg_batch_u32",
     "fpxg_server_create", "fpxg_server_submit", "fpxg_server_wait", "fpxg_server_poll", "fpxg_server_now",
     "fpxg_server_stats_get", "fpxg_server_destroy", "fpxg_autotune", "fpxg_set_option",
-    "fpxg_pow_u32", "fpxg_pow_dev", "fpxg_divmod_u32", "fpxg_divmod_dev",
+    "fpxg_pow_u32", "fpxg_pow_dev", "fpxg_divmod_u32", "fpxg_divmod_dev", "fpxg_product_plan_info",
 ]
 
 
@@ -141,6 +141,7 @@ def lib():
                                         ctypes.c_void_p]
         _ip = ctypes.POINTER(ctypes.c_int)
         L.fpxg_ntt_plan_info.argtypes = [ctypes.c_uint32, _ip, _ip, _ip]
+        L.fpxg_product_plan_info.argtypes = [ctypes.c_uint32, _ip, _ip]
         L.fpxg_dev_alloc.restype = ctypes.c_void_p
         L.fpxg_dev_alloc.argtypes = [ctypes.c_size_t]
         L.fpxg_dev_free.argtypes = [ctypes.c_void_p]
@@ -412,6 +413,13 @@ def ntt_plan_info(n: int) -> dict:
     r, c, t = ctypes.c_int(0), ctypes.c_int(0), ctypes.c_int(0)
     _check(lib().fpxg_ntt_plan_info(n, ctypes.byref(r), ctypes.byref(c), ctypes.byref(t)), "plan_info")
     return {"row_log": r.value, "col_levels": c.value, "tile_cols_log": t.value}
+
+
+def product_plan_info(n: int) -> dict:
+    """Split of an n-point fused product (column levels, row words)."""
+    r, c = ctypes.c_int(0), ctypes.c_int(0)
+    _check(lib().fpxg_product_plan_info(n, ctypes.byref(r), ctypes.byref(c)), "product_plan_info")
+    return {"row_log": r.value, "col_levels": c.value}
 
 
 # ---------------------------------------------------------------------------

@@ -306,26 +306,30 @@ static int g_device = 0;
 
 // ---- launch dispatch over the compile-time shapes of ntt2.hpp ----
 
+// A row launch: njobs jobs from the device array dj, or (dj null) the njobs
+// by-value jobs j[0 .. njobs).
 template <int LOGM, bool LAZY, int MODE>
-static void rows2_launch(Context& c, const RowJob2& j, uint32_t nrows, uint32_t njobs, const RowJob2* dj) {
+static void rows2_launch(Context& c, const RowJob2* j, uint32_t nrows, uint32_t njobs, const RowJob2* dj) {
   using RS = RowShape<LOGM>;
   const size_t lds = (size_t)RS::U * RS::lds_words_per_row * 4;
   // Many whole rows of 2^14 words, forward: the persistent prefetching pass.
   if constexpr (LOGM == kDmaRowLog && MODE == R2M_FWD) {
-    if (!dj && njobs == 1 && c.row_dma && !(j.flags & R2_LIFT) && nrows >= 2u * (uint32_t)c.cus &&
-        (size_t)j.in_a >= ((size_t)nrows << LOGM)) {
-      RowJob2 jj = j;
+    if (!dj && njobs == 1 && c.row_dma && !(j[0].flags & R2_LIFT) && nrows >= 2u * (uint32_t)c.cus &&
+        (size_t)j[0].in_a >= ((size_t)nrows << LOGM)) {
+      RowJob2 jj = j[0];
       jj.nrows = nrows;
       hipLaunchKernelGGL((k_rows_fwd_dma<LAZY>), dim3((uint32_t)c.cus), dim3(RS::T), 0, c.stream, jj);
       return;
     }
   }
+  JobPack<RowJob2, kRowPack> pack;
+  for (int i = 0; i < kRowPack; ++i) pack.j[i] = j[(!dj && (uint32_t)i < njobs) ? i : 0];
   dim3 grid((nrows + RS::U - 1) / RS::U, njobs), block(RS::T);
-  hipLaunchKernelGGL((k_rows2<LOGM, LAZY, MODE>), grid, block, lds, c.stream, j, dj);
+  hipLaunchKernelGGL((k_rows2<LOGM, LAZY, MODE>), grid, block, lds, c.stream, pack, dj);
 }
 
 template <int LOGM, bool LAZY>
-static void rows2_mode(Context& c, int mode, const RowJob2& j, uint32_t nrows, uint32_t njobs, const RowJob2* dj) {
+static void rows2_mode(Context& c, int mode, const RowJob2* j, uint32_t nrows, uint32_t njobs, const RowJob2* dj) {
   switch (mode) {
     case R2M_FWD: rows2_launch<LOGM, LAZY, R2M_FWD>(c, j, nrows, njobs, dj); break;
     case R2M_INV: rows2_launch<LOGM, LAZY, R2M_INV>(c, j, nrows, njobs, dj); break;
@@ -347,7 +351,7 @@ static void rows2_mode(Context& c, int mode, const RowJob2& j, uint32_t nrows, u
 }
 
 template <bool LAZY>
-static void rows2_size(Context& c, int logM, int mode, const RowJob2& j, uint32_t nrows, uint32_t njobs,
+static void rows2_size(Context& c, int logM, int mode, const RowJob2* j, uint32_t nrows, uint32_t njobs,
                        const RowJob2* dj) {
   switch (logM) {
 #define FPXG_R(L) \
@@ -367,8 +371,26 @@ static void launch_rows(Context& c, bool lazy, int mode, const RowJob2& j, int l
     c.rec->push_rows(logM, lazy, mode, jj);
     return;
   }
-  if (lazy) rows2_size<true>(c, logM, mode, j, nrows, njobs, dj);
-  else rows2_size<false>(c, logM, mode, j, nrows, njobs, dj);
+  if (lazy) rows2_size<true>(c, logM, mode, &j, nrows, njobs, dj);
+  else rows2_size<false>(c, logM, mode, &j, nrows, njobs, dj);
+  FPXG_HIP(hipGetLastError());
+}
+// k <= kRowPack independent row passes of one shape (nrows rows of 2^logM
+// each) in one launch; recorded as k records sharing one launch level.
+static void launch_rows_group(Context& c, bool lazy, int mode, const RowJob2* js, int k, int logM,
+                              uint32_t nrows) {
+  if (k < 1 || k > kRowPack) throw Error(E_DEVICE, "row launch group size");
+  RowJob2 jj[kRowPack];
+  for (int i = 0; i < k; ++i) {
+    jj[i] = js[i];
+    jj[i].nrows = nrows;
+  }
+  if (c.rec) {
+    for (int i = 0; i < k; ++i) c.rec->push_rows(logM, lazy, mode, jj[i], i > 0);
+    return;
+  }
+  if (lazy) rows2_size<true>(c, logM, mode, jj, nrows, (uint32_t)k, nullptr);
+  else rows2_size<false>(c, logM, mode, jj, nrows, (uint32_t)k, nullptr);
   FPXG_HIP(hipGetLastError());
 }
 
@@ -381,34 +403,33 @@ static int tile_log_cols(int S, int L) {
   return col_tile_log(S);
 }
 
-// njobs jobs from the device array dj, or (dj null) one job j, or two jobs j
-// and *j1 passed by value.
+// njobs jobs from the device array dj, or (dj null) the njobs by-value jobs
+// j[0 .. njobs).
 template <int S, int LOGC, bool LAZY, bool INV>
-static void cols2_launch_c(Context& c, const ColJob2& j, const PassPlan& P, uint32_t njobs, const ColJob2* dj,
-                           const ColJob2* j1) {
+static void cols2_launch_c(Context& c, const ColJob2* j, const PassPlan& P, uint32_t njobs, const ColJob2* dj) {
   using CS = ColShape<S, LOGC>;
   const size_t lds = (size_t)CS::words * 4;
+  JobPack<ColJob2, kColPack> pack;
+  for (int i = 0; i < kColPack; ++i) pack.j[i] = j[(!dj && (uint32_t)i < njobs) ? i : 0];
   dim3 grid(1u << (P.b - LOGC), njobs), block(CS::T);
-  hipLaunchKernelGGL((k_cols2<S, LOGC, LAZY, INV>), grid, block, lds, c.stream, j, j1 ? *j1 : j, dj, P.L);
+  hipLaunchKernelGGL((k_cols2<S, LOGC, LAZY, INV>), grid, block, lds, c.stream, pack, dj, P.L);
 }
 template <int S, bool LAZY, bool INV>
-static void cols2_launch(Context& c, const ColJob2& j, const PassPlan& P, uint32_t njobs, const ColJob2* dj,
-                         const ColJob2* j1) {
+static void cols2_launch(Context& c, const ColJob2* j, const PassPlan& P, uint32_t njobs, const ColJob2* dj) {
   if constexpr (S == 10) {
     if (P.logC == 3) {
-      cols2_launch_c<S, 3, LAZY, INV>(c, j, P, njobs, dj, j1);
+      cols2_launch_c<S, 3, LAZY, INV>(c, j, P, njobs, dj);
       return;
     }
   }
-  cols2_launch_c<S, col_tile_log(S), LAZY, INV>(c, j, P, njobs, dj, j1);
+  cols2_launch_c<S, col_tile_log(S), LAZY, INV>(c, j, P, njobs, dj);
 }
 
 template <bool LAZY, bool INV>
-static void cols2_size(Context& c, const ColJob2& j, const PassPlan& P, uint32_t njobs, const ColJob2* dj,
-                       const ColJob2* j1) {
+static void cols2_size(Context& c, const ColJob2* j, const PassPlan& P, uint32_t njobs, const ColJob2* dj) {
   switch (P.S) {
 #define FPXG_C(S) \
-  case S: cols2_launch<S, LAZY, INV>(c, j, P, njobs, dj, j1); break;
+  case S: cols2_launch<S, LAZY, INV>(c, j, P, njobs, dj); break;
     FPXG_C(1) FPXG_C(2) FPXG_C(3) FPXG_C(4) FPXG_C(5) FPXG_C(6)
     FPXG_C(7) FPXG_C(8) FPXG_C(9) FPXG_C(10) FPXG_C(11) FPXG_C(12)
 #undef FPXG_C
@@ -416,28 +437,34 @@ static void cols2_size(Context& c, const ColJob2& j, const PassPlan& P, uint32_t
   }
 }
 
-static void launch_cols(Context& c, bool lazy, bool inv, const ColJob2& j, const PassPlan& P, uint32_t njobs = 1,
-                        const ColJob2* dj = nullptr, const ColJob2* j1 = nullptr) {
+// njobs jobs from the device array dj, or (dj null) the njobs by-value jobs
+// j[0 .. njobs); recorded as one record per job, all but the first marked as
+// sharing the first's launch level.
+static void launch_cols_n(Context& c, bool lazy, bool inv, const ColJob2* j, const PassPlan& P, uint32_t njobs,
+                          const ColJob2* dj) {
+  if (!dj && (njobs < 1 || njobs > (uint32_t)kColPack)) throw Error(E_DEVICE, "column launch group size");
   if (c.rec && !dj) {
-    c.rec->push_cols(P.S, lazy, inv, P.L, j);
-    if (j1) c.rec->push_cols(P.S, lazy, inv, P.L, *j1, true);
+    for (uint32_t i = 0; i < njobs; ++i) c.rec->push_cols(P.S, lazy, inv, P.L, j[i], i > 0);
     return;
   }
-  if (j1) njobs = 2;
   if (lazy) {
-    if (inv) cols2_size<true, true>(c, j, P, njobs, dj, j1);
-    else cols2_size<true, false>(c, j, P, njobs, dj, j1);
+    if (inv) cols2_size<true, true>(c, j, P, njobs, dj);
+    else cols2_size<true, false>(c, j, P, njobs, dj);
   } else {
-    if (inv) cols2_size<false, true>(c, j, P, njobs, dj, j1);
-    else cols2_size<false, false>(c, j, P, njobs, dj, j1);
+    if (inv) cols2_size<false, true>(c, j, P, njobs, dj);
+    else cols2_size<false, false>(c, j, P, njobs, dj);
   }
   FPXG_HIP(hipGetLastError());
 }
-// Two independent column passes of one shape in one launch (one recorded
-// record each, the second marked as sharing the first's launch level).
+static void launch_cols(Context& c, bool lazy, bool inv, const ColJob2& j, const PassPlan& P, uint32_t njobs = 1,
+                        const ColJob2* dj = nullptr) {
+  launch_cols_n(c, lazy, inv, &j, P, njobs, dj);
+}
+// Two independent column passes of one shape in one launch.
 static void launch_cols_pair(Context& c, bool lazy, bool inv, const ColJob2& j0, const ColJob2& j1,
                              const PassPlan& P) {
-  launch_cols(c, lazy, inv, j0, P, 1, nullptr, &j1);
+  const ColJob2 js[2] = {j0, j1};
+  launch_cols_n(c, lazy, inv, js, P, 2, nullptr);
 }
 
 template <int LOGM, bool LAZY, int MODE>
@@ -509,6 +536,24 @@ PassPlan Context::plan(int L) const {
     auto it = plan_override.find(L);
     if (it != plan_override.end()) return it->second;
   }
+  return plan_base(L);
+}
+
+// Split of a fused product (both forwards, the pointwise product and the
+// inverse; three transforms per row in the row pass), which needs its own
+// choice: the forward's best split can leave the product's row pass with a
+// few long rows.  A product with no stored spectrum is free to use it.
+// Untuned, a lone product up to 2^13 points is one workgroup's launch.
+PassPlan Context::conv_plan(int L) const {
+  if (!(rec && rec_merged)) {
+    auto it = conv_override.find(L);
+    if (it != conv_override.end()) return it->second;
+    if (L <= std::min(row_log_max, 13)) return {L, L, 0, 0};
+  }
+  return plan_base(L);
+}
+
+PassPlan Context::plan_base(int L) const {
   // Recorded schedules run merged with others, so a transform that fits one
   // workgroup's LDS is one row pass there; a lone call splits it for parallelism.
   if (L <= ((rec && rec_merged) ? row_log_max : single_log_max)) return {L, L, 0, 0};
@@ -598,33 +643,100 @@ void ntt_inverse(Context& c, const HostField& F, uint32_t* dst, const uint32_t* 
 
 // Product window [0, outc) of the cyclic-free product over one NTT field F;
 // with lift the inputs are raw limbs of another field (CRT auxiliary use).
-static void conv_core(Context& c, const HostField& F, const uint32_t* a, size_t la, const uint32_t* b, size_t lb,
-                      uint32_t* out, size_t outc, bool lift) {
+// The same product computed in k <= 3 fields at once (the auxiliary primes of
+// a CRT product: one launch per pass carries every field's jobs).  out[i] gets
+// the window [0, outc) in field F[i]; lift reads a and b as limbs of another
+// field (times r2).  All fields must share the lazy mode.
+static void conv_core_fields(Context& c, const HostField* const* F, int k, const uint32_t* a, size_t la,
+                             const uint32_t* b, size_t lb, uint32_t* const* out, size_t outc, bool lift) {
   const int L = std::max(1, ceil_log2(la + lb - 1));
   const uint32_t N = 1u << L;
-  const PassPlan P = c.plan(L);
-  const uint32_t inv_n = inv_pow2(F, L);
-  RowJob2 rj{};
-  rj.f = F.dev;
+  const PassPlan P = c.conv_plan(L);
+  const bool lazy = F[0]->lazy;
+  for (int i = 1; i < k; ++i)
+    if (F[i]->lazy != lazy) throw Error(E_DEVICE, "fields of one launch differ in lazy mode");
+  RowJob2 rj[kRowPack];
+  for (int i = 0; i < k; ++i) {
+    rj[i] = RowJob2{};
+    rj[i].f = F[i]->dev;
+    rj[i].inv_n = inv_pow2(*F[i], L);
+  }
   if (P.S == 0) {
-    rj.src_a = a; rj.src_b = b; rj.dst = out; rj.nrows = 1; rj.in_a = (uint32_t)la; rj.in_b = (uint32_t)lb;
-    rj.out_lo = 0; rj.out_hi = (uint32_t)outc; rj.inv_n = inv_n;
-    rj.flags = R2_SCALE | R2_CANON | (lift ? R2_LIFT : 0);
-    launch_rows(c, F.lazy, R2M_CONV, rj, L, 1);
+    for (int i = 0; i < k; ++i) {
+      rj[i].src_a = a; rj[i].src_b = b; rj[i].dst = out[i]; rj[i].in_a = (uint32_t)la; rj[i].in_b = (uint32_t)lb;
+      rj[i].out_lo = 0; rj[i].out_hi = (uint32_t)outc;
+      rj[i].flags = R2_SCALE | R2_CANON | (lift ? R2_LIFT : 0);
+    }
+    launch_rows_group(c, lazy, R2M_CONV, rj, k, L, 1);
     return;
   }
-  DevBuf A = c.buf(N), B = c.buf(N);
-  ColJob2 cj{};
-  cj.src = a; cj.dst = A.get(); cj.in_count = (uint32_t)la; cj.flags = lift ? R2_LIFT : 0; cj.f = F.dev;
-  launch_cols(c, F.lazy, false, cj, P);
-  cj.src = b; cj.dst = B.get(); cj.in_count = (uint32_t)lb;
-  launch_cols(c, F.lazy, false, cj, P);
-  rj.src_a = A.get(); rj.src_b = B.get(); rj.dst = A.get(); rj.nrows = 1u << P.S; rj.in_a = N; rj.in_b = N;
-  rj.out_lo = 0; rj.out_hi = N;
-  launch_rows(c, F.lazy, R2M_CONV, rj, P.b, rj.nrows);
-  ColJob2 ij{};
-  ij.src = A.get(); ij.dst = out; ij.out_lo = 0; ij.out_hi = (uint32_t)outc; ij.inv_n = inv_n; ij.f = F.dev;
-  launch_cols(c, F.lazy, true, ij, P);
+  DevBuf A[kRowPack], B[kRowPack];
+  ColJob2 cj[2 * kRowPack];
+  for (int i = 0; i < k; ++i) {
+    A[i] = c.buf(N);
+    B[i] = c.buf(N);
+    ColJob2& ja = cj[2 * i];
+    ja = ColJob2{};
+    ja.src = a; ja.dst = A[i].get(); ja.in_count = (uint32_t)la; ja.flags = lift ? R2_LIFT : 0; ja.f = F[i]->dev;
+    ColJob2& jb = cj[2 * i + 1];
+    jb = ja;
+    jb.src = b; jb.dst = B[i].get(); jb.in_count = (uint32_t)lb;
+  }
+  launch_cols_n(c, lazy, false, cj, P, (uint32_t)(2 * k), nullptr);
+  for (int i = 0; i < k; ++i) {
+    rj[i].src_a = A[i].get(); rj[i].src_b = B[i].get(); rj[i].dst = A[i].get(); rj[i].in_a = N; rj[i].in_b = N;
+    rj[i].out_lo = 0; rj[i].out_hi = N;
+  }
+  launch_rows_group(c, lazy, R2M_CONV, rj, k, P.b, 1u << P.S);
+  ColJob2 ij[kRowPack];
+  for (int i = 0; i < k; ++i) {
+    ij[i] = ColJob2{};
+    ij[i].src = A[i].get(); ij[i].dst = out[i]; ij[i].out_lo = 0; ij[i].out_hi = (uint32_t)outc;
+    ij[i].inv_n = inv_pow2(*F[i], L); ij[i].f = F[i]->dev;
+  }
+  launch_cols_n(c, lazy, true, ij, P, (uint32_t)k, nullptr);
+}
+
+static void conv_core(Context& c, const HostField& F, const uint32_t* a, size_t la, const uint32_t* b, size_t lb,
+                      uint32_t* out, size_t outc, bool lift) {
+  const HostField* Fs[1] = {&F};
+  uint32_t* outs[1] = {out};
+  conv_core_fields(c, Fs, 1, a, la, b, lb, outs, outc, lift);
+}
+
+// Forward transforms of one input into k <= 3 fields (same lazy mode), one
+// launch per pass for all of them.
+static void ntt_forward_fields(Context& c, const HostField* const* F, int k, uint32_t* const* dst,
+                               const uint32_t* src, uint32_t in_count, int L, bool lift) {
+  const PassPlan P = c.plan(L);
+  const uint32_t N = 1u << L;
+  const bool lazy = F[0]->lazy;
+  for (int i = 1; i < k; ++i)
+    if (F[i]->lazy != lazy) throw Error(E_DEVICE, "fields of one launch differ in lazy mode");
+  RowJob2 rj[kRowPack];
+  for (int i = 0; i < k; ++i) {
+    rj[i] = RowJob2{};
+    rj[i].f = F[i]->dev;
+    rj[i].out_lo = 0; rj[i].out_hi = N;
+  }
+  if (P.S == 0) {
+    for (int i = 0; i < k; ++i) {
+      rj[i].src_a = src; rj[i].dst = dst[i]; rj[i].in_a = in_count; rj[i].flags = lift ? R2_LIFT : 0;
+    }
+    launch_rows_group(c, lazy, R2M_FWD, rj, k, L, 1);
+    return;
+  }
+  ColJob2 cj[kRowPack];
+  for (int i = 0; i < k; ++i) {
+    cj[i] = ColJob2{};
+    cj[i].src = src; cj[i].dst = dst[i]; cj[i].in_count = in_count; cj[i].flags = lift ? R2_LIFT : 0;
+    cj[i].f = F[i]->dev;
+  }
+  launch_cols_n(c, lazy, false, cj, P, (uint32_t)k, nullptr);
+  for (int i = 0; i < k; ++i) {
+    rj[i].src_a = dst[i]; rj[i].dst = dst[i]; rj[i].in_a = N;
+  }
+  launch_rows_group(c, lazy, R2M_FWD, rj, k, P.b, 1u << P.S);
 }
 
 static const uint32_t kCrtP[3] = {167772161u, 469762049u, 754974721u};  // crt_convolution.hpp:14-19
@@ -669,10 +781,14 @@ static void crt_window(Context& c, const HostField& T, const uint32_t* a, size_t
                        uint32_t* out, size_t outc, bool accumulate) {
   const int primes = crt_prime_count(T.p, std::min(la, lb));
   DevBuf res[3];
+  const HostField* Fs[3];
+  uint32_t* outs[3];
   for (int k = 0; k < primes; ++k) {
     res[k] = c.buf(outc);
-    conv_core(c, c.field(kCrtP[k]), a, la, b, lb, res[k].get(), outc, true);
+    Fs[k] = &c.field(kCrtP[k]);
+    outs[k] = res[k].get();
   }
+  conv_core_fields(c, Fs, primes, a, la, b, lb, outs, outc, true);
   GarnerRec g{};
   g.c1 = res[0].get();
   g.c2 = primes > 1 ? res[1].get() : res[0].get();
@@ -1045,6 +1161,46 @@ int fpxg_autotune(int min_log, int max_log, int reps, int* chosen_col_levels, fl
       if (chosen_col_levels) chosen_col_levels[L - min_log] = pick.S;
       if (best_us) best_us[L - min_log] = best;
     }
+    // Fused products (two half-length operands, the whole cyclic window):
+    // one workgroup's launch while the size fits it, else every admissible split.
+    DevBuf out = c.buf((size_t)1 << max_log);
+    for (int L = std::max(min_log, 13); L <= max_log; ++L) {
+      const HostField& F = c.field(L <= 25 ? kCrtP[0] : kCrtP[1]);
+      const size_t half = (size_t)1 << (L - 1);
+      std::vector<PassPlan> cand;
+      if (L <= c.row_log_max) cand.push_back({L, L, 0, 0});
+      for (int S = std::max(1, L - c.row_log_max); S <= std::min(12, L - 1); ++S)
+        if (S >= 4 && L - S >= 4) cand.push_back({L, L - S, S, tile_log_cols(S, L)});
+      if (cand.empty()) continue;
+      float best = 1e30f;
+      PassPlan pick = cand[0];
+      for (const PassPlan& P : cand) {
+        c.conv_override[L] = P;
+        auto run = [&] { native_conv(c, F, L, buf.get(), half, buf.get() + half, half, 0, 2 * half, out.get()); };
+        run();
+        FPXG_HIP(hipEventRecord(c.events[14], c.stream));
+        for (int r = 0; r < reps; ++r) run();
+        FPXG_HIP(hipEventRecord(c.events[15], c.stream));
+        FPXG_HIP(hipEventSynchronize(c.events[15]));
+        float ms = 0;
+        FPXG_HIP(hipEventElapsedTime(&ms, c.events[14], c.events[15]));
+        const float us = 1e3f * ms / (float)reps;
+        if (us < best) {
+          best = us;
+          pick = P;
+        }
+      }
+      c.conv_override[L] = pick;
+    }
+  });
+}
+
+int fpxg_product_plan_info(uint32_t n, int* row_log, int* col_levels) {
+  return guarded([&](Context& c) {
+    if (n < 2 || (n & (n - 1))) throw Error(E_SIZE, "NTT size must be a power of two");
+    const PassPlan P = c.conv_plan(ceil_log2(n));
+    if (row_log) *row_log = P.b;
+    if (col_levels) *col_levels = P.S;
   });
 }
 

@@ -45,6 +45,29 @@ static void issue_one(Context& c, const LaunchRec& r) {
   FPXG_HIP(hipGetLastError());
 }
 
+// Records of one launch level that one launch can carry: the same pass shape
+// (and for row passes the same row count), each after the first marked par.
+static bool same_launch(const LaunchRec& a, const LaunchRec& b) {
+  if (!b.par || a.kind != b.kind || a.k0 != b.k0 || a.k1 != b.k1 || a.k2 != b.k2 || a.k3 != b.k3) return false;
+  if (a.kind == LK_COLS) return true;
+  return a.kind == LK_ROWS && a.row.nrows == b.row.nrows;
+}
+
+// Issues recs[0 .. n) as one launch (n > 1 records of same_launch shape).
+static void issue_group(Context& c, const LaunchRec* recs, size_t n) {
+  const LaunchRec& r = recs[0];
+  if (r.kind == LK_COLS) {
+    ColJob2 js[kColPack];
+    for (size_t i = 0; i < n; ++i) js[i] = recs[i].col;
+    const PassPlan P{r.k3, r.k3 - r.k0, r.k0, tile_log_cols(r.k0, r.k3)};
+    launch_cols_n(c, r.k1 != 0, r.k2 != 0, js, P, (uint32_t)n, nullptr);
+  } else {
+    RowJob2 js[kRowPack];
+    for (size_t i = 0; i < n; ++i) js[i] = recs[i].row;
+    launch_rows_group(c, r.k1 != 0, r.k2, js, (int)n, r.k0, r.row.nrows);
+  }
+}
+
 static void drop_graph(Context& c, Context::GraphEntry& e) {
   if (e.exec) (void)hipGraphExecDestroy(e.exec);
   e.exec = nullptr;
@@ -135,16 +158,14 @@ static void series_call(Context& c, const HostField& T, int op, const uint32_t* 
     hipGraph_t g = nullptr;
     FPXG_HIP(hipStreamBeginCapture(c.stream, hipStreamCaptureModeThreadLocal));
     try {
-      for (size_t i = 0; i < recs.size(); ++i) {
+      for (size_t i = 0; i < recs.size();) {
         const LaunchRec& r = recs[i];
-        if (r.kind == LK_COLS && i + 1 < recs.size() && recs[i + 1].par && recs[i + 1].kind == LK_COLS &&
-            recs[i + 1].k0 == r.k0 && recs[i + 1].k1 == r.k1 && recs[i + 1].k2 == r.k2 && recs[i + 1].k3 == r.k3) {
-          const PassPlan P{r.k3, r.k3 - r.k0, r.k0, tile_log_cols(r.k0, r.k3)};
-          launch_cols_pair(c, r.k1 != 0, r.k2 != 0, r.col, recs[i + 1].col, P);
-          ++i;
-          continue;
-        }
-        issue_one(c, r);
+        const size_t cap = r.kind == LK_COLS ? (size_t)kColPack : r.kind == LK_ROWS ? (size_t)kRowPack : 1;
+        size_t e = i + 1;
+        while (e < recs.size() && e - i < cap && same_launch(r, recs[e])) ++e;
+        if (e - i > 1) issue_group(c, recs.data() + i, e - i);
+        else issue_one(c, r);
+        i = e;
       }
     } catch (...) {
       (void)hipStreamEndCapture(c.stream, &g);

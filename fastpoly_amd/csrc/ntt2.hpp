@@ -301,11 +301,18 @@ __device__ __forceinline__ void inv_phases(uint32_t* x, uint32_t* X, uint32_t g,
   }
 }
 
+// Up to K jobs passed by value at the start of the kernel arguments: a launch
+// of a few independent jobs of one shape (both operands of a product, the
+// three auxiliary primes of a CRT product) needs no device job array.
+template <class Job, int K>
+struct JobPack {
+  Job j[K];
+};
+
 // The workgroup's job: jobs[blockIdx.y] when a job array is given, else the
-// by-value kernel argument number blockIdx.y (j0 at offset 0 of the kernel
-// arguments, and for column passes j1 right after it: a two-job launch needs no
-// device array).  Both are read through the constant address space, so the
-// job lands in scalar registers with one round of scalar loads.
+// by-value job number blockIdx.y of the JobPack at offset 0 of the kernel
+// arguments.  Both are read through the constant address space, so the job
+// lands in scalar registers with one round of scalar loads.
 template <class Job>
 __device__ __forceinline__ Job load_job(const Job& j0, const Job* jobs) {
 #if defined(__HIP_DEVICE_COMPILE__)
@@ -351,6 +358,8 @@ struct RowJob2 {
   const uint32_t* src_c;     // third operand of the Newton-step modes
   FieldDev f;
 };
+// By-value jobs per row launch: one for each of the three auxiliary primes.
+constexpr int kRowPack = 3;
 enum : uint32_t { R2_LIFT = 1, R2_CANON = 2, R2_SCALE = 4, R2_NEG = 8, R2_ACC = 16, R2_SUBFROM = 32, R2_MULTAB = 64 };
 // R2_ACC (column inverse only): dst[i] = dst[i] + result, both canonical.
 // Inverse-pass epilogues on the canonical result v at output position t (the
@@ -404,12 +413,12 @@ struct RowWaves {
 
 template <int LOGM, bool LAZY, int MODE>
 __global__ void __launch_bounds__(RowShape<LOGM>::T) __attribute__((amdgpu_waves_per_eu(RowWaves<MODE>::value)))
-k_rows2(RowJob2 j0, const RowJob2* __restrict__ jobs) {
+k_rows2(JobPack<RowJob2, kRowPack> jp, const RowJob2* __restrict__ jobs) {
   using RS = RowShape<LOGM>;
   using SC = Sched<LOGM>;
   constexpr int NPH = SC::nph;
   extern __shared__ uint32_t lds[];
-  const RowJob2 J = load_job(j0, jobs);
+  const RowJob2 J = load_job(jp.j[0], jobs);
   const Arith<LAZY> A(J.f);
   const uint32_t flags = __builtin_amdgcn_readfirstlane(J.flags);  // uniform: scalar branches
   const bool lift = (flags & R2_LIFT) != 0;
@@ -775,6 +784,10 @@ struct ColJob2 {
   FieldDev f;
 };
 
+// By-value jobs per column launch: both operands of a product for each of
+// the three auxiliary primes.
+constexpr int kColPack = 6;
+
 template <int S, int LOGC>
 struct ColShape {
   static constexpr int GT = Sched<S>::GT;
@@ -785,13 +798,12 @@ struct ColShape {
 
 template <int S, int LOGC, bool LAZY, bool INV>
 __global__ void __launch_bounds__((ColShape<S, LOGC>::T)) __attribute__((amdgpu_waves_per_eu(FPXG_COL_WPE)))
-k_cols2(ColJob2 j0, ColJob2 j1, const ColJob2* __restrict__ jobs, int L) {
-  (void)j1;  // read by load_job for blockIdx.y == 1 of a two-job launch
+k_cols2(JobPack<ColJob2, kColPack> jp, const ColJob2* __restrict__ jobs, int L) {
   using SC = Sched<S>;
   constexpr int NPH = SC::nph;
   constexpr int C = 1 << LOGC;
   extern __shared__ uint32_t X[];
-  const ColJob2 J = load_job(j0, jobs);
+  const ColJob2 J = load_job(jp.j[0], jobs);
   const Arith<LAZY> A(J.f);
   const uint32_t flags = __builtin_amdgcn_readfirstlane(J.flags);
   const bool lift = (flags & R2_LIFT) != 0;

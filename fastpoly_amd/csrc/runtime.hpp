@@ -178,9 +178,9 @@ struct Recorder {
     r.par = 0;
     return r;
   }
-  void push_rows(int logM, bool lazy, int mode, const RowJob2& j) {
+  void push_rows(int logM, bool lazy, int mode, const RowJob2& j, bool par = false) {
     LaunchRec& r = next(LK_ROWS);
-    r.k0 = logM; r.k1 = lazy; r.k2 = mode; r.row = j;
+    r.k0 = logM; r.k1 = lazy; r.k2 = mode; r.row = j; r.par = par;
   }
   void push_cols(int S, bool lazy, bool inv, int L, const ColJob2& j, bool par = false) {
     LaunchRec& r = next(LK_COLS);
@@ -201,6 +201,7 @@ struct Context {
   int single_log_max = 12;   // largest transform done as one row pass
   int cus = 256;
   std::map<int, PassPlan> plan_override;  // autotuned splits by log2 n
+  std::map<int, PassPlan> conv_override;  // autotuned splits of fused products by log2 n
   Recorder* rec = nullptr;                // non-null: launches are recorded, not issued
   bool rec_merged = true;                 // recorded for the merged executor (else for a graph of one call)
   // Captured single-call schedules (hipGraph) with the arena blocks they use.
@@ -235,6 +236,8 @@ struct Context {
   std::recursive_mutex mu;
 
   PassPlan plan(int L) const;
+  PassPlan conv_plan(int L) const;
+  PassPlan plan_base(int L) const;
   HostField& field(uint32_t p, uint32_t root = 0);
   DevBuf buf(size_t words) { return DevBuf(arena.get(), words); }
 };

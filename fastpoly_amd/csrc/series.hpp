@@ -52,10 +52,12 @@ struct Engine {
   void forward(Spec& S, const uint32_t* src, size_t count) {
     const uint32_t N = 1u << L;
     S.L = L;
+    uint32_t* dst[3];
     for (int i = 0; i < k; ++i) {
       S.b[i] = c.buf(N);
-      ntt_forward(c, *F[i], S.b[i].get(), src, (uint32_t)count, L, !native, false);
+      dst[i] = S.b[i].get();
     }
+    ntt_forward_fields(c, F, k, dst, src, (uint32_t)count, L, !native);
   }
   // out[i - lo] = (x * y)[i] for i in [lo, hi) of the 2^L-point cyclic product,
   // x given as target limbs (count of them), y as a spectrum; neg negates.
@@ -63,37 +65,43 @@ struct Engine {
                bool neg = false) {
     const uint32_t N = 1u << L;
     const PassPlan P = c.plan(L);
-    DevBuf res[3];
+    const bool lazy = F[0]->lazy;
+    DevBuf res[3], W[3];
+    uint32_t* dst[3];
     const uint32_t negf = (neg && native) ? R2_NEG : 0u;
+    RowJob2 rj[kRowPack];
     for (int i = 0; i < k; ++i) {
-      const HostField& Fi = *F[i];
-      uint32_t* dst = out;
-      if (!native) {
-        res[i] = c.buf(hi - lo);
-        dst = res[i].get();
+      if (!native) res[i] = c.buf(hi - lo);
+      dst[i] = native ? out : res[i].get();
+      rj[i] = RowJob2{};
+      rj[i].f = F[i]->dev;
+    }
+    // Every field's pass in one launch (the three auxiliary primes share the
+    // lazy mode and the plan).
+    if (P.S == 0) {
+      for (int i = 0; i < k; ++i) {
+        rj[i].src_a = x; rj[i].src_b = Y.b[i].get(); rj[i].dst = dst[i]; rj[i].in_a = (uint32_t)count;
+        rj[i].out_lo = (uint32_t)lo; rj[i].out_hi = (uint32_t)hi; rj[i].out_shift = (uint32_t)lo;
+        rj[i].inv_n = inv_pow2(*F[i], L);
+        rj[i].flags = R2_SCALE | R2_CANON | negf | (native ? 0u : R2_LIFT);
       }
-      const uint32_t inv_n = inv_pow2(Fi, L);
-      RowJob2 rj{};
-      rj.f = Fi.dev;
-      if (P.S == 0) {
-        rj.src_a = x; rj.src_b = Y.b[i].get(); rj.dst = dst; rj.nrows = 1; rj.in_a = (uint32_t)count;
-        rj.out_lo = (uint32_t)lo; rj.out_hi = (uint32_t)hi; rj.out_shift = (uint32_t)lo; rj.inv_n = inv_n;
-        rj.flags = R2_SCALE | R2_CANON | negf | (native ? 0u : R2_LIFT);
-        launch_rows(c, Fi.lazy, R2M_SPEC, rj, L, 1);
-      } else {
-        DevBuf W = c.buf(N);
-        ColJob2 cj{};
-        cj.src = x; cj.dst = W.get(); cj.in_count = (uint32_t)count; cj.flags = native ? 0u : R2_LIFT;
-        cj.f = Fi.dev;
-        launch_cols(c, Fi.lazy, false, cj, P);
-        rj.src_a = W.get(); rj.src_b = Y.b[i].get(); rj.dst = W.get(); rj.nrows = 1u << P.S; rj.in_a = N;
-        rj.out_lo = 0; rj.out_hi = N;
-        launch_rows(c, Fi.lazy, R2M_SPEC, rj, P.b, rj.nrows);
-        ColJob2 ij{};
-        ij.src = W.get(); ij.dst = dst; ij.out_lo = (uint32_t)lo; ij.out_hi = (uint32_t)hi;
-        ij.out_shift = (uint32_t)lo; ij.inv_n = inv_n; ij.flags = negf; ij.f = Fi.dev;
-        launch_cols(c, Fi.lazy, true, ij, P);
+      launch_rows_group(c, lazy, R2M_SPEC, rj, k, L, 1);
+    } else {
+      ColJob2 cj[kRowPack], ij[kRowPack];
+      for (int i = 0; i < k; ++i) {
+        W[i] = c.buf(N);
+        cj[i] = ColJob2{};
+        cj[i].src = x; cj[i].dst = W[i].get(); cj[i].in_count = (uint32_t)count; cj[i].flags = native ? 0u : R2_LIFT;
+        cj[i].f = F[i]->dev;
+        rj[i].src_a = W[i].get(); rj[i].src_b = Y.b[i].get(); rj[i].dst = W[i].get(); rj[i].in_a = N;
+        rj[i].out_lo = 0; rj[i].out_hi = N;
+        ij[i] = ColJob2{};
+        ij[i].src = W[i].get(); ij[i].dst = dst[i]; ij[i].out_lo = (uint32_t)lo; ij[i].out_hi = (uint32_t)hi;
+        ij[i].out_shift = (uint32_t)lo; ij[i].inv_n = inv_pow2(*F[i], L); ij[i].flags = negf; ij[i].f = F[i]->dev;
       }
+      launch_cols_n(c, lazy, false, cj, P, (uint32_t)k, nullptr);
+      launch_rows_group(c, lazy, R2M_SPEC, rj, k, P.b, 1u << P.S);
+      launch_cols_n(c, lazy, true, ij, P, (uint32_t)k, nullptr);
     }
     if (!native) {
       GarnerRec g{};
@@ -126,17 +134,14 @@ struct Epi {
 static void native_conv(Context& c, const HostField& T, int L, const uint32_t* x, size_t cx, const uint32_t* y,
                         size_t cy, size_t lo, size_t hi, uint32_t* out, bool neg = false, const Epi& ep = Epi()) {
   const uint32_t N = 1u << L;
-  const PassPlan P = c.plan(L);
+  const PassPlan P = c.conv_plan(L);
   const uint32_t inv_n = ep.scale ? T.mul(inv_pow2(T, L), ep.scale) : inv_pow2(T, L);
   const uint32_t negf = (neg ? R2_NEG : 0u) | ep.flags;
   RowJob2 rj{};
   rj.f = T.dev;
   rj.epi = ep.src;
   rj.epi_avail = ep.avail;
-  // Up to 2^13 points a product is one workgroup's launch even where a lone
-  // transform would be split over the CUs: one launch beats three there (at
-  // 2^14 the three transforms in one workgroup take longer than three launches).
-  if (P.S == 0 || L <= std::min(c.row_log_max, 13)) {
+  if (P.S == 0) {
     rj.src_a = x; rj.src_b = y; rj.dst = out; rj.nrows = 1; rj.in_a = (uint32_t)cx; rj.in_b = (uint32_t)cy;
     rj.out_lo = (uint32_t)lo; rj.out_hi = (uint32_t)hi; rj.out_shift = (uint32_t)lo; rj.inv_n = inv_n;
     rj.flags = R2_SCALE | R2_CANON | negf;

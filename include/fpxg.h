@@ -111,9 +111,14 @@ int fpxg_ntt_plan_info(uint32_t n, int* row_log, int* col_levels, int* tile_cols
 /* Autotuner (no counterpart in the reference, whose stage split is fixed):
  * for every size 2^min_log..2^max_log, times the forward transform under each
  * admissible pass split (`reps` runs each) and keeps the fastest for single
- * calls.  Optional outputs per size: the chosen column levels (0 = one row
+ * calls; then, from 2^13 up, does the same for the fused product (two
+ * forwards, pointwise product, inverse) used by conv and the Newton steps.
+ * Optional outputs per size: the forward's chosen column levels (0 = one row
  * pass) and its time in microseconds. */
 int fpxg_autotune(int min_log, int max_log, int reps, int* chosen_col_levels, float* best_us);
+/* The split chosen for an n-point fused product (no counterpart in the
+ * reference): rows of 2^*row_log words after *col_levels column levels. */
+int fpxg_product_plan_info(uint32_t n, int* row_log, int* col_levels);
 
 /* Library options (no counterpart in the reference).
  *   FPXG_OPT_GRAPHS: 1 (default) replays repeated single series calls from

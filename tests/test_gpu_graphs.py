@@ -92,8 +92,16 @@ def test_autotune_keeps_results_exact(fa, oracle):
         n = 1 << lg
         x = oracle.MT(1 + n).rnd(167772161, n)
         assert np.array_equal(fa.ntt_forward(167772161, x, root=3), oracle.ntt(167772161, x, root=3)), lg
+    # the fused products' own splits (tuned from 2^13 up)
+    for lg in range(13, 19):
+        pp = fa.product_plan_info(1 << lg)
+        assert pp["row_log"] + pp["col_levels"] == lg and pp["row_log"] <= 14, (lg, pp)
+        assert pp["col_levels"] == 0 or pp["col_levels"] >= 4, (lg, pp)
     a = oracle.MT(5).rnd(P, 70000)
     b = oracle.MT(6).rnd(P, 50001)
     assert np.array_equal(fa.conv(P, a, b), oracle.conv(P, a, b))
-    s = _input(oracle, "exp", P, 40000, 9)
-    assert np.array_equal(fa.exp(P, s, 40000), oracle.series("exp", P, s, 40000))
+    for la, lb in ((3000, 2000), (5000, 4000), (9000, 9000), (40000, 30000)):
+        assert np.array_equal(fa.conv(P, a[:la], b[:lb]), oracle.conv(P, a[:la], b[:lb])), (la, lb)
+    for op in ("exp", "inv", "sqrt", "log"):
+        s = _input(oracle, op, P, 40000, 9)
+        assert np.array_equal(fa.series(op, P, s, 40000), oracle.series(op, P, s, 40000)), op
