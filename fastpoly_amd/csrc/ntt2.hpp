@@ -432,29 +432,27 @@ k_rows2(JobPack<RowJob2, kRowPack> jp, const RowJob2* __restrict__ jobs) {
   uint32_t x[16];
 
   // Branch-free first-phase load: indices past in_count (or of an inactive
-  // row) read a clamped valid word and are then replaced by zero.
-  // All loads are issued unconditionally (clamped index) so they overlap; the
-  // lift and the zero fill past in_count are applied afterwards.
-  auto load_first = [&](const uint32_t* src, uint32_t in_count, uint32_t* dst_regs) {
-    constexpr int r0 = SC::r_of(0);
-    constexpr int NE = PhaseGroups<LOGM, 0>::NG * (1 << r0);
-    if (in_count == 0) {
-#pragma unroll
-      for (int i = 0; i < NE; ++i) dst_regs[i] = 0u;
-      return;
-    }
-    // The whole row is input: no clamps, no zero fill, and (radix-16 first
-    // phase) register k of thread g is word k * GT + g of the row, so each load
-    // is a uniform base plus the thread's 32-bit offset (no 64-bit address
-    // arithmetic per load).
-    if (RS::U == 1 && PhaseGroups<LOGM, 0>::NG == 1 && base + RS::M <= in_count) {
+  // row) read a clamped valid word and are then replaced by zero.  load_issue
+  // only issues the loads (every index clamped, so they overlap); load_fix
+  // waits for them and applies the lift and the zero fill past in_count.  A
+  // mode with several operands issues all of them before its first transform,
+  // so they cost one memory round trip together.
+  //
+  // The whole row is input: no clamps, no zero fill, and (radix-16 first
+  // phase) register k of thread g is word k * GT + g of the row, so each load
+  // is a uniform base plus the thread's 32-bit offset (no 64-bit address
+  // arithmetic per load).
+  constexpr int r0 = SC::r_of(0);
+  constexpr int NE0 = PhaseGroups<LOGM, 0>::NG * (1 << r0);
+  auto whole_row = [&](uint32_t in_count) {
+    return RS::U == 1 && PhaseGroups<LOGM, 0>::NG == 1 && base + RS::M <= in_count;
+  };
+  auto load_issue = [&](const uint32_t* src, uint32_t in_count, uint32_t* dst_regs) {
+    if (in_count == 0) return;
+    if (whole_row(in_count)) {
       const __amdgpu_buffer_rsrc_t rs = buf_rsrc(src + base, RS::M * 4);
 #pragma unroll
       for (int k = 0; k < (1 << r0); ++k) dst_regs[k] = bload(rs, g * 4, (uint32_t)k * RS::GT * 4);
-      if (lift) {
-#pragma unroll
-        for (int i = 0; i < NE; ++i) dst_regs[i] = A.mul(dst_regs[i], J.f.r2);
-      }
       return;
     }
     const uint32_t last = in_count - 1;
@@ -464,10 +462,18 @@ k_rows2(JobPack<RowJob2, kRowPack> jp, const RowJob2* __restrict__ jobs) {
       for (int k = 0; k < (1 << r0); ++k)
         dst_regs[u * (1 << r0) + k] =
             gload(src + min(base + elem<LOGM, 0>(PhaseGroups<LOGM, 0>::gid(g, u), k), last));
+  };
+  auto load_fix = [&](uint32_t in_count, uint32_t* dst_regs) {
+    if (in_count == 0) {
+#pragma unroll
+      for (int i = 0; i < NE0; ++i) dst_regs[i] = 0u;
+      return;
+    }
     if (lift) {
 #pragma unroll
-      for (int i = 0; i < NE; ++i) dst_regs[i] = A.mul(dst_regs[i], J.f.r2);
+      for (int i = 0; i < NE0; ++i) dst_regs[i] = A.mul(dst_regs[i], J.f.r2);
     }
+    if (whole_row(in_count)) return;
 #pragma unroll
     for (int u = 0; u < PhaseGroups<LOGM, 0>::NG; ++u)
 #pragma unroll
@@ -475,6 +481,10 @@ k_rows2(JobPack<RowJob2, kRowPack> jp, const RowJob2* __restrict__ jobs) {
         const uint32_t gi = base + elem<LOGM, 0>(PhaseGroups<LOGM, 0>::gid(g, u), k);
         if (!(active && gi < in_count)) dst_regs[u * (1 << r0) + k] = 0u;
       }
+  };
+  auto load_first = [&](const uint32_t* src, uint32_t in_count, uint32_t* dst_regs) {
+    load_issue(src, in_count, dst_regs);
+    load_fix(in_count, dst_regs);
   };
   // Forward transform of registers holding phase 0's input, ending with phase
   // NPH-1's groups in regs; `forward` loads the input first.
@@ -507,17 +517,25 @@ k_rows2(JobPack<RowJob2, kRowPack> jp, const RowJob2* __restrict__ jobs) {
   } else {
     if constexpr (MODE == R2M_CONV) {
       uint32_t y[16];
-      forward(J.src_b, J.in_b, y);
+      load_issue(J.src_b, J.in_b, y);
+      load_issue(J.src_a, J.in_a, x);
+      load_fix(J.in_b, y);
+      forward_regs(y);
       __syncthreads();
-      forward(J.src_a, J.in_a, x);
+      load_fix(J.in_a, x);
+      forward_regs(x);
 #pragma unroll
       for (int i = 0; i < NGL * (1 << rl); ++i) x[i] = A.mul(x[i], y[i]);
     } else if constexpr (MODE == R2M_INVSTEP) {
       static_assert(LOGM >= 4 && PhaseGroups<LOGM, 0>::NG == 1, "the half shift below needs a radix-16 first phase");
       uint32_t y[16];
-      forward(J.src_b, J.in_b, y);  // c
+      load_issue(J.src_b, J.in_b, y);  // c
+      load_issue(J.src_a, J.in_a, x);  // a
+      load_fix(J.in_b, y);
+      forward_regs(y);
       __syncthreads();
-      forward(J.src_a, J.in_a, x);  // a
+      load_fix(J.in_a, x);
+      forward_regs(x);
 #pragma unroll
       for (int i = 0; i < NGL * (1 << rl); ++i) x[i] = A.mul(x[i], y[i]);
       phase_compute<LOGM, PL, true, LAZY>(x, g, J.f.planes_inv, A);
@@ -539,10 +557,27 @@ k_rows2(JobPack<RowJob2, kRowPack> jp, const RowJob2* __restrict__ jobs) {
       static_assert(LOGM >= 4 && PhaseGroups<LOGM, 0>::NG == 1, "positions k * GT + g need a radix-16 first phase");
       constexpr uint32_t GT = RowShape<LOGM>::GT;
       const uint32_t h = J.out_hi;
-      uint32_t yb[16], yc[16];
-      forward(J.src_b, J.in_b, yb);  // b[0, m)
+      uint32_t yb[16], yc[16], ep[16];
+      load_issue(J.src_b, J.in_b, yb);  // b[0, m)
+      load_issue(J.src_c, h, yc);       // c[0, h)
+      if constexpr (MODE == R2M_EXPSTEP) load_issue(J.src_a, J.in_a, x);  // a'
+      // The epilogue operand at positions i * GT + g (EXPSTEP: 1/(m + pos)
+      // for pos < h; SQRTSTEP: a[v + m + pos] for pos < epi_avail), clamped.
+      {
+        const uint32_t ecount = MODE == R2M_EXPSTEP ? h : J.epi_avail;
+        if (ecount) {
+#pragma unroll
+          for (int i = 0; i < 16; ++i) ep[i] = gload(J.epi + min((uint32_t)i * GT + g, ecount - 1));
+        } else {
+#pragma unroll
+          for (int i = 0; i < 16; ++i) ep[i] = 0u;
+        }
+      }
+      load_fix(J.in_b, yb);
+      forward_regs(yb);
       __syncthreads();
-      forward(J.src_c, h, yc);  // c[0, h)
+      load_fix(h, yc);
+      forward_regs(yc);
       __syncthreads();
       // Inverse of regs into phase 0's layout (register k of thread g holds
       // position k * GT + g), times 1/N: values in [0, 2p).
@@ -553,7 +588,8 @@ k_rows2(JobPack<RowJob2, kRowPack> jp, const RowJob2* __restrict__ jobs) {
         for (int i = 0; i < 16; ++i) r[i] = A.mul(r[i], J.inv_n);
       };
       if constexpr (MODE == R2M_EXPSTEP) {
-        forward(J.src_a, J.in_a, x);  // a'
+        load_fix(J.in_a, x);
+        forward_regs(x);
 #pragma unroll
         for (int i = 0; i < 16; ++i) x[i] = A.mul(x[i], yb[i]);
         inverse_scaled(x);
@@ -577,7 +613,7 @@ k_rows2(JobPack<RowJob2, kRowPack> jp, const RowJob2* __restrict__ jobs) {
 #pragma unroll
         for (int i = 0; i < 16; ++i) {
           const uint32_t pos = (uint32_t)i * GT + g;
-          x[i] = pos < h ? A.mul(x[i], gload(J.epi + pos)) : 0u;
+          x[i] = pos < h ? A.mul(x[i], ep[i]) : 0u;
         }
         __syncthreads();
         forward_regs(x);
@@ -593,7 +629,7 @@ k_rows2(JobPack<RowJob2, kRowPack> jp, const RowJob2* __restrict__ jobs) {
         for (int i = 0; i < 8; ++i) {
           const uint32_t pos = (uint32_t)i * GT + g;
           const uint32_t v = A.canon(x[i + 8]);
-          const uint32_t s = pos < J.epi_avail ? gload(J.epi + pos) : 0u;
+          const uint32_t s = pos < J.epi_avail ? ep[i] : 0u;
           const uint32_t r = s >= v ? s - v : s - v + p;
           x[i] = pos < h ? r : 0u;
           x[i + 8] = 0u;
@@ -604,15 +640,20 @@ k_rows2(JobPack<RowJob2, kRowPack> jp, const RowJob2* __restrict__ jobs) {
         for (int i = 0; i < 16; ++i) x[i] = A.mul(A.mul(x[i], yc[i]), J.aux);  // times aux = 1/2
       }
     } else if constexpr (MODE == R2M_SPEC) {
-      forward(J.src_a, J.in_a, x);
+      // The spectrum (phase PL's grouping) is loaded with the operand.
+      uint32_t w[16];
+      load_issue(J.src_a, J.in_a, x);
 #pragma unroll
       for (int u = 0; u < NGL; ++u)
 #pragma unroll
         for (int k = 0; k < (1 << rl); ++k) {
           const uint32_t gi = base + elem<LOGM, PL>(PhaseGroups<LOGM, PL>::gid(g, u), k);
-          const uint32_t w = gload(J.src_b + (active ? gi : 0u));
-          x[u * (1 << rl) + k] = A.mul(x[u * (1 << rl) + k], w);
+          w[u * (1 << rl) + k] = gload(J.src_b + (active ? gi : 0u));
         }
+      load_fix(J.in_a, x);
+      forward_regs(x);
+#pragma unroll
+      for (int i = 0; i < NGL * (1 << rl); ++i) x[i] = A.mul(x[i], w[i]);
     } else {  // R2M_INV / R2M_SPEC_ONLY: load the spectrum in phase PL's grouping
 #pragma unroll
       for (int u = 0; u < NGL; ++u)
@@ -627,8 +668,6 @@ k_rows2(JobPack<RowJob2, kRowPack> jp, const RowJob2* __restrict__ jobs) {
     // Inverse: phase PL on registers, then PL-1 .. 0 through LDS.
     phase_compute<LOGM, PL, true, LAZY>(x, g, J.f.planes_inv, A);
     inv_phases<LOGM, PL - 1, 0, LAZY>(x, X, g, 0u, J.f.planes_inv, A);
-    constexpr int r0 = SC::r_of(0);
-    constexpr int NE0 = PhaseGroups<LOGM, 0>::NG * (1 << r0);
     // Inverse outputs are in [0, 4p) (lazy); the scale brings them to [0, 2p).
     if (flags & R2_SCALE) {
 #pragma unroll
@@ -821,17 +860,49 @@ k_cols2(JobPack<ColJob2, kColPack> jp, const ColJob2* __restrict__ jobs, int L) 
   if constexpr (!INV) {
     constexpr int r0 = SC::r_of(0);
     constexpr int NE0 = PhaseGroups<S, 0, LOGC>::NG * (1 << r0);
-    if (J.in_count == 0) {
+    // Four-step twiddle w^(col * bitrev_S(e)).  In the last phase e = gid *
+    // 2^rl + k, so the exponent is col * bitrev_(S-rl)(gid) + j * col *
+    // 2^(S-rl) with j = bitrev_rl(k): one table lookup for each group's first
+    // factor, one for the ratio D, then a running product per element.  The
+    // table words (two per factor: w^x = hi[x >> split] * lo[x & mask]) are
+    // loaded first, ahead of the data, and multiplied while the data is in
+    // flight, so the factors cost no memory round trip after the last phase
+    // (with at most four groups in the last phase; more would spill).
+    constexpr int rl = SC::r_of(PL);
+    constexpr int NGL = PhaseGroups<S, PL, LOGC>::NG;
+    constexpr bool kEarly = NGL <= 4;
+    const uint32_t xd = (col << (S - rl)) << shift;
+    auto xq = [&](int u) { return (col * bitrev(PhaseGroups<S, PL, LOGC>::gid(g, u) << rl, S)) << shift; };
+    uint32_t D = 0, qf[NGL], ql[NGL], dh = 0, dl = 0;
+    if constexpr (kEarly) {
+      const uint32_t tmask = (1u << J.f.split) - 1;
+      dh = gload(J.f.tw_hi + (xd >> J.f.split));
+      dl = gload(J.f.tw_lo + (xd & tmask));
 #pragma unroll
-      for (int i = 0; i < NE0; ++i) x[i] = 0u;
-    } else {
-      // Unconditional (clamped) loads first so they overlap; lift and zero fill after.
+      for (int u = 0; u < NGL; ++u) {
+        qf[u] = gload(J.f.tw_hi + (xq(u) >> J.f.split));
+        ql[u] = gload(J.f.tw_lo + (xq(u) & tmask));
+      }
+    }
+    // Unconditional (clamped) data loads so they overlap; the factors are
+    // formed while they are in flight; lift and zero fill after.
+    if (J.in_count != 0) {
       const uint32_t last = J.in_count - 1;
 #pragma unroll
       for (int u = 0; u < PhaseGroups<S, 0, LOGC>::NG; ++u)
 #pragma unroll
         for (int k = 0; k < (1 << r0); ++k)
           x[u * (1 << r0) + k] = gload(J.src + min(elem<S, 0>(PhaseGroups<S, 0, LOGC>::gid(g, u), k) * M + col, last));
+    }
+    if constexpr (kEarly) {
+      D = A.mul(dh, dl);
+#pragma unroll
+      for (int u = 0; u < NGL; ++u) qf[u] = A.mul(qf[u], ql[u]);
+    }
+    if (J.in_count == 0) {
+#pragma unroll
+      for (int i = 0; i < NE0; ++i) x[i] = 0u;
+    } else {
       if (lift) {
 #pragma unroll
         for (int i = 0; i < NE0; ++i) x[i] = A.mul(x[i], J.f.r2);
@@ -844,16 +915,15 @@ k_cols2(JobPack<ColJob2, kColPack> jp, const ColJob2* __restrict__ jobs, int L) 
     }
     phase_compute<S, 0, false, LAZY, LOGC>(x, g, J.f.planes_fwd, A);
     fwd_phases<S, 1, NPH, LOGC, LAZY>(x, X, g, c, J.f.planes_fwd, A);
-    // Four-step twiddle w^(col * bitrev_S(e)).  In the last phase e = gid *
-    // 2^rl + k, so the exponent is col * bitrev_(S-rl)(gid) + j * col *
-    // 2^(S-rl) with j = bitrev_rl(k): one table lookup for the group's first
-    // factor, one for the ratio D, then a running product per element.
-    constexpr int rl = SC::r_of(PL);
-    const uint32_t D = twiddle<LAZY>(A, J.f.tw_lo, J.f.tw_hi, J.f.split, (col << (S - rl)) << shift);
+    if constexpr (!kEarly) {
+      D = twiddle<LAZY>(A, J.f.tw_lo, J.f.tw_hi, J.f.split, xd);
 #pragma unroll
-    for (int u = 0; u < PhaseGroups<S, PL, LOGC>::NG; ++u) {
+      for (int u = 0; u < NGL; ++u) qf[u] = twiddle<LAZY>(A, J.f.tw_lo, J.f.tw_hi, J.f.split, xq(u));
+    }
+#pragma unroll
+    for (int u = 0; u < NGL; ++u) {
       const uint32_t gid = PhaseGroups<S, PL, LOGC>::gid(g, u);
-      uint32_t q = twiddle<LAZY>(A, J.f.tw_lo, J.f.tw_hi, J.f.split, (col * bitrev(gid << rl, S)) << shift);
+      uint32_t q = qf[u];
 #pragma unroll
       for (int j = 0; j < (1 << rl); ++j) {
         const int k = (int)bitrev_c(j, rl);
