@@ -237,6 +237,7 @@ struct Context {
 
   PassPlan plan(int L) const;
   PassPlan conv_plan(int L) const;
+  PassPlan conv_plan_base(int L) const;
   PassPlan plan_base(int L) const;
   HostField& field(uint32_t p, uint32_t root = 0);
   DevBuf buf(size_t words) { return DevBuf(arena.get(), words); }
