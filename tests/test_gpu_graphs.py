@@ -49,6 +49,26 @@ def test_replays_match_oracle(fa, oracle, p, op):
         fa.synchronize()
 
 
+@pytest.mark.parametrize("op", ["exp", "sqrt"])
+def test_large_step_replays(fa, oracle, op):
+    """Newton steps of 2^14 points and more (split products): eager first
+    call, captured second call, replays with new data under the same key,
+    then the same without graphs."""
+    n = 70000
+    D = fa.DeviceArray
+    da, out = D(n), D(n)
+    try:
+        for graphs in (True, False):
+            fa.set_graphs(graphs)
+            for seed in (31, 32, 33, 34):
+                a = _input(oracle, op, P, n, seed)
+                assert fa.lib().fpxg_memcpy_h2d(da.ptr, a.ctypes.data, 4 * n) == 0
+                fa.series_dev(op, P, da, n, n, out)
+                assert np.array_equal(out.to_host(), oracle.series(op, P, a, n)), (op, graphs, seed)
+    finally:
+        fa.set_graphs(True)
+
+
 def test_same_key_new_data(fa, oracle):
     """New input data under the same pointers and scan signature replays the
     cached graph, which must read the new data."""
