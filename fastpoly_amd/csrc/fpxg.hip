@@ -322,6 +322,13 @@ static void rows2_launch(Context& c, const RowJob2* j, uint32_t nrows, uint32_t 
       return;
     }
   }
+  // A lone small Newton step: its window products computed directly.
+  if constexpr ((MODE == R2M_INVSTEP || MODE == R2M_EXPSTEP || MODE == R2M_SQRTSTEP) && LOGM >= 6 && LOGM <= 9) {
+    if (!dj && njobs == 1) {
+      hipLaunchKernelGGL((k_step_direct<MODE, LOGM - 1>), dim3(1), dim3(1u << LOGM), 0, c.stream, j[0]);
+      return;
+    }
+  }
   JobPack<RowJob2, kRowPack> pack;
   for (int i = 0; i < kRowPack; ++i) pack.j[i] = j[(!dj && (uint32_t)i < njobs) ? i : 0];
   dim3 grid((nrows + RS::U - 1) / RS::U, njobs), block(RS::T);

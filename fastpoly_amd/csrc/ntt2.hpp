@@ -807,6 +807,94 @@ k_rows_fwd_dma(RowJob2 j0) {
 }
 
 // ---------------------------------------------------------------------------
+// Newton steps of a lone call with m = 2^LM <= 256 (transforms of 2^6..2^9
+// points): the step's window products, computed directly.  Every window the
+// NTT step modes read is free of cyclic wraparound, so each is a plain middle
+// product out[o] = sum_j X[j] * Y[o + off - j]; at these sizes 2m threads
+// (two per output, even and odd j) finish one in a few hundred instructions,
+// where the one-workgroup NTT step is a chain of radix-16 blocks on a few
+// waves.  Same job fields and results as R2M_INVSTEP / R2M_EXPSTEP /
+// R2M_SQRTSTEP (which merged batches keep using):
+//   INVSTEP : e = (a c)[k + i], out[i] = -(e c)[i]        (inverse.hpp:18-65)
+//   EXPSTEP : r = (a' b)[m - 1 + i], t = (r c)[i] / (m + i),
+//             out[i] = (t b)[i]                           (exponential.hpp:172-204)
+//   SQRTSTEP: res = a[v + m + i] - (b b)[m + i],
+//             out[i] = (res c)[i] / 2                     (square_root.hpp:86-100)
+// for i < out_hi.  Values are canonical limbs; two products are summed
+// before each reduction (2 p^2 < p 2^32 for p < 2^31).
+// ---------------------------------------------------------------------------
+template <int MODE, int LM>
+__global__ void __launch_bounds__(2 << LM) k_step_direct(RowJob2 J) {
+  constexpr uint32_t M = 1u << LM;
+  __shared__ uint32_t A[2 * M], B[M], Cc[M], W[M];
+  const uint32_t t = threadIdx.x, p = J.f.p, ninv = J.f.ninv;
+  const uint32_t h = J.out_hi;  // outputs, <= M
+  const uint32_t o = t >> 1, par = t & 1;
+  auto addm = [&](uint32_t x, uint32_t y) {
+    const uint32_t z = x + y;
+    return z >= p ? z - p : z;
+  };
+  // sum_j X[j] Y[o + off - j] over 0 <= j < nx, 0 <= o + off - j < ny: this
+  // thread's half (j of its parity); the pair of threads adds both halves.
+  auto mid = [&](const uint32_t* X, uint32_t nx, const uint32_t* Y, uint32_t ny, uint32_t off) -> uint32_t {
+    uint32_t acc = 0;
+    const uint32_t s = o + off;
+    if (o < h && nx && ny) {
+      const uint32_t jlo = s >= ny ? s - (ny - 1) : 0u;
+      const uint32_t jhi = min(nx - 1, s);  // inclusive
+      uint32_t j = jlo + ((jlo & 1) != par ? 1u : 0u);
+      for (; j + 2 <= jhi; j += 4)
+        acc = addm(acc, redc_full((uint64_t)X[j] * Y[s - j] + (uint64_t)X[j + 2] * Y[s - j - 2], p, ninv));
+      if (j <= jhi) acc = addm(acc, redc_full((uint64_t)X[j] * Y[s - j], p, ninv));
+    }
+    return addm(acc, (uint32_t)__shfl_xor((int)acc, 1, 64));
+  };
+  auto stage = [&](uint32_t* dst, const uint32_t* src, uint32_t count, uint32_t cap) {
+    for (uint32_t i = t; i < cap; i += 2 * M) dst[i] = i < count ? src[i] : 0u;
+  };
+  if constexpr (MODE == R2M_INVSTEP) {
+    const uint32_t k = J.in_b;
+    stage(A, J.src_a, min(J.in_a, 2 * M), 2 * M);
+    stage(B, J.src_b, k, M);
+    __syncthreads();
+    const uint32_t e = mid(A, min(J.in_a, 2 * M), B, k, k);
+    if (par == 0 && o < h) W[o] = e;
+    __syncthreads();
+    const uint32_t v = mid(W, h, B, k, 0);
+    if (par == 0 && o < h) J.dst[o] = v == 0 ? 0u : p - v;
+  } else if constexpr (MODE == R2M_EXPSTEP) {
+    const uint32_t m = J.in_b;
+    stage(A, J.src_a, min(J.in_a, 2 * M), 2 * M);
+    stage(B, J.src_b, m, M);
+    stage(Cc, J.src_c, h, M);
+    __syncthreads();
+    const uint32_t r = mid(A, min(J.in_a, 2 * M), B, m, J.aux);
+    if (par == 0 && o < h) W[o] = r;
+    __syncthreads();
+    uint32_t tt = mid(W, h, Cc, h, 0);
+    __syncthreads();  // W is rewritten below
+    if (par == 0 && o < h) W[o] = mulmod_full(tt, J.epi[o], p, ninv);
+    __syncthreads();
+    const uint32_t v = mid(W, h, B, m, 0);
+    if (par == 0 && o < h) J.dst[o] = v;
+  } else {
+    static_assert(MODE == R2M_SQRTSTEP, "direct Newton steps: INVSTEP, EXPSTEP, SQRTSTEP");
+    const uint32_t m = J.in_b;
+    stage(B, J.src_b, m, M);
+    stage(Cc, J.src_c, h, M);
+    __syncthreads();
+    const uint32_t sq = mid(B, m, B, m, m);
+    if (par == 0 && o < h) {
+      const uint32_t a = o < J.epi_avail ? J.epi[o] : 0u;
+      W[o] = a >= sq ? a - sq : a - sq + p;
+    }
+    __syncthreads();
+    const uint32_t v = mid(W, h, Cc, h, 0);
+    if (par == 0 && o < h) J.dst[o] = mulmod_full(v, J.aux, p, ninv);
+  }
+}
+
+// ---------------------------------------------------------------------------
 // Column pass: a tile of C = 2^LOGC adjacent columns x 2^S rows of the
 // 2^S x M row-major view, with the four-step twiddle w_N^(c * bitrev_S(row))
 // applied after the forward levels (before the inverse ones).
