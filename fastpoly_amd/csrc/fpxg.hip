@@ -322,17 +322,17 @@ static void rows2_launch(Context& c, const RowJob2* j, uint32_t nrows, uint32_t 
       return;
     }
   }
-  // A lone small Newton step: its window products computed directly.
-  if constexpr ((MODE == R2M_INVSTEP || MODE == R2M_EXPSTEP || MODE == R2M_SQRTSTEP) && LOGM >= 6 && LOGM <= 9) {
-    if (!dj && njobs == 1) {
-      hipLaunchKernelGGL((k_step_direct<MODE, LOGM - 1>), dim3(1), dim3(1u << LOGM), 0, c.stream, j[0]);
-      return;
-    }
+  if constexpr ((MODE == R2M_INVSTEP || MODE == R2M_EXPSTEP || MODE == R2M_SQRTSTEP) && LOGM <= 9) {
+    // Small Newton steps: their window products computed directly (one job
+    // by value, or a device job array).
+    if (!dj && njobs != 1) throw Error(E_DEVICE, "direct Newton steps take one job by value");
+    hipLaunchKernelGGL((k_step_direct<MODE, LOGM - 1>), dim3(1, njobs), dim3(1u << LOGM), 0, c.stream, j[0], dj);
+  } else {
+    JobPack<RowJob2, kRowPack> pack;
+    for (int i = 0; i < kRowPack; ++i) pack.j[i] = j[(!dj && (uint32_t)i < njobs) ? i : 0];
+    dim3 grid((nrows + RS::U - 1) / RS::U, njobs), block(RS::T);
+    hipLaunchKernelGGL((k_rows2<LOGM, LAZY, MODE>), grid, block, lds, c.stream, pack, dj);
   }
-  JobPack<RowJob2, kRowPack> pack;
-  for (int i = 0; i < kRowPack; ++i) pack.j[i] = j[(!dj && (uint32_t)i < njobs) ? i : 0];
-  dim3 grid((nrows + RS::U - 1) / RS::U, njobs), block(RS::T);
-  hipLaunchKernelGGL((k_rows2<LOGM, LAZY, MODE>), grid, block, lds, c.stream, pack, dj);
 }
 
 template <int LOGM, bool LAZY>

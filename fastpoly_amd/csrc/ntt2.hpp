@@ -807,25 +807,28 @@ k_rows_fwd_dma(RowJob2 j0) {
 }
 
 // ---------------------------------------------------------------------------
-// Newton steps of a lone call with m = 2^LM <= 256 (transforms of 2^6..2^9
-// points): the step's window products, computed directly.  Every window the
+// Newton steps with m = 2^LM <= 256 (transforms of 2^6..2^9 points): the
+// step's window products, computed directly.  Every window the
 // NTT step modes read is free of cyclic wraparound, so each is a plain middle
 // product out[o] = sum_j X[j] * Y[o + off - j]; at these sizes 2m threads
 // (two per output, even and odd j) finish one in a few hundred instructions,
 // where the one-workgroup NTT step is a chain of radix-16 blocks on a few
-// waves.  Same job fields and results as R2M_INVSTEP / R2M_EXPSTEP /
-// R2M_SQRTSTEP (which merged batches keep using):
+// waves.  Same job fields and results as the R2M_INVSTEP / R2M_EXPSTEP /
+// R2M_SQRTSTEP row modes, which serve the larger steps:
 //   INVSTEP : e = (a c)[k + i], out[i] = -(e c)[i]        (inverse.hpp:18-65)
 //   EXPSTEP : r = (a' b)[m - 1 + i], t = (r c)[i] / (m + i),
 //             out[i] = (t b)[i]                           (exponential.hpp:172-204)
 //   SQRTSTEP: res = a[v + m + i] - (b b)[m + i],
 //             out[i] = (res c)[i] / 2                     (square_root.hpp:86-100)
 // for i < out_hi.  Values are canonical limbs; two products are summed
-// before each reduction (2 p^2 < p 2^32 for p < 2^31).
+// before each reduction (2 p^2 < p 2^32 for p < 2^31).  Only Montgomery
+// arithmetic mod p is used, so any odd prime p < 2^31 is served, NTT prime or
+// not.  One workgroup a job: the job by value, or jobs[blockIdx.y].
 // ---------------------------------------------------------------------------
 template <int MODE, int LM>
-__global__ void __launch_bounds__(2 << LM) k_step_direct(RowJob2 J) {
+__global__ void __launch_bounds__(2 << LM) k_step_direct(RowJob2 j0, const RowJob2* __restrict__ jobs) {
   constexpr uint32_t M = 1u << LM;
+  const RowJob2 J = load_job(j0, jobs);
   __shared__ uint32_t A[2 * M], B[M], Cc[M], W[M];
   const uint32_t t = threadIdx.x, p = J.f.p, ninv = J.f.ninv;
   const uint32_t h = J.out_hi;  // outputs, <= M

@@ -169,6 +169,14 @@ static void native_conv(Context& c, const HostField& T, int L, const uint32_t* x
   launch_cols(c, T.lazy, true, ij, P);
 }
 
+// A full doubling step of 2^L = 2m points that one launch serves: the direct
+// window products up to 2^9 points (any odd prime), the one-workgroup NTT
+// step modes up to 2^13 (NTT primes).
+static bool one_launch_step(const Context& c, const HostField& T, const Engine& E, size_t m) {
+  if (E.L < 6 || ((size_t)1 << E.L) != 2 * m || T.p == 2) return false;
+  return E.L <= 9 || (E.native && E.L <= std::min(c.row_log_max, 13));
+}
+
 // extend_inverse (inverse.hpp:18-65): c[0, k) is 1/a mod x^k; fill c[k, wanted).
 static void extend_inverse(Context& c, const HostField& T, const uint32_t* a, size_t la, uint32_t* cc, size_t k,
                            size_t wanted) {
@@ -178,7 +186,7 @@ static void extend_inverse(Context& c, const HostField& T, const uint32_t* a, si
     return;
   }
   Engine E(c, T, ceil_log2(2 * k));
-  if (E.native && E.L >= 6 && E.L <= std::min(c.row_log_max, 13) && (1u << E.L) == 2 * k) {
+  if (one_launch_step(c, T, E, k)) {
     // Both products of the step in one workgroup (R2M_INVSTEP).
     RowJob2 rj{};
     rj.f = T.dev;
@@ -290,9 +298,14 @@ static void series_emit(Context& c, const HostField& T, const SeriesReq& q, cons
       // 1/i for the steps' t[i] /= m + i (exponential.hpp:194-195), applied in
       // the product epilogue on native fields.
       DevBuf rtab;
+      // (Other fields: only the direct steps, up to 2m = 512, read it.)
       if (series_native(T, ceil_log2(2 * seed))) {
         rtab = c.buf(n);
         emit_ew(c, T, EW_RECIP, rtab.get(), nullptr, nullptr, n, 0, 0);
+      } else if (T.p != 2) {
+        const size_t nr = std::min<size_t>(n, 512);
+        rtab = c.buf(nr);
+        emit_ew(c, T, EW_RECIP, rtab.get(), nullptr, nullptr, nr, 0, 0);
       }
       for (size_t m = seed; m < n; m *= 2) {
         const size_t m2 = std::min(2 * m, n), hlen = m2 - m;
@@ -305,7 +318,7 @@ static void series_emit(Context& c, const HostField& T, const SeriesReq& q, cons
           csize = hlen;
         }
         Engine E(c, T, ceil_log2(2 * m));
-        if (E.native && rtab.get() && E.L >= 6 && E.L <= std::min(c.row_log_max, 13) && (1u << E.L) == 2 * m) {
+        if (rtab.get() && one_launch_step(c, T, E, m)) {
           // The step's three products in one workgroup (R2M_EXPSTEP).
           RowJob2 rj{};
           rj.f = T.dev;
@@ -374,7 +387,7 @@ static void series_emit(Context& c, const HostField& T, const SeriesReq& q, cons
           csize = hlen;
         }
         Engine E(c, T, ceil_log2(2 * m));
-        if (E.native && E.L >= 6 && E.L <= std::min(c.row_log_max, 13) && (1u << E.L) == 2 * m) {
+        if (one_launch_step(c, T, E, m)) {
           // The step's two products in one workgroup (R2M_SQRTSTEP).
           RowJob2 rj{};
           rj.f = T.dev;
