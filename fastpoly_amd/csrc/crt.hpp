@@ -78,9 +78,11 @@ constexpr uint32_t kChunk = 1024;
 
 __global__ void __launch_bounds__(256) k_garner_batch(const GarnerRec* __restrict__ recs, const uint2* __restrict__ chunks) {
   const uint2 ch = chunks[blockIdx.x];
-  const GarnerRec R = recs[ch.x];
-  const uint32_t end = min(R.n, ch.y + kChunk);
-  for (uint32_t i = ch.y + threadIdx.x; i < end; i += blockDim.x) garner_one(R.c1, R.c2, R.c3, R.out, i, R.g);
+  GarnerRec R = recs[ch.x];
+  make_uniform(R);
+  const uint32_t start = __builtin_amdgcn_readfirstlane(ch.y);
+  const uint32_t end = min(R.n, start + kChunk);
+  for (uint32_t i = start + threadIdx.x; i < end; i += blockDim.x) garner_one(R.c1, R.c2, R.c3, R.out, i, R.g);
 }
 
 }  // namespace fpxg

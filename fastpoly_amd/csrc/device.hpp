@@ -80,6 +80,16 @@ __device__ __forceinline__ uint32_t mulmod_full(uint32_t a, uint32_t b, uint32_t
   return redc_full((uint64_t)a * b, p, ninv);
 }
 
+// A record that is the same for the whole workgroup, word by word into
+// scalar registers: tests on its fields become scalar branches.
+template <class Rec>
+__device__ __forceinline__ void make_uniform(Rec& r) {
+  static_assert(sizeof(Rec) % 4 == 0, "whole words");
+  uint32_t* w = reinterpret_cast<uint32_t*>(&r);
+#pragma unroll
+  for (unsigned i = 0; i < sizeof(Rec) / 4; ++i) w[i] = __builtin_amdgcn_readfirstlane(w[i]);
+}
+
 // Padded LDS index: one spare word every 32 breaks power-of-two strides.
 __device__ __forceinline__ uint32_t pidx(uint32_t i) { return i + (i >> 5); }
 __host__ __device__ constexpr uint32_t lds_words(uint32_t n) { return n + (n >> 5) + 1; }

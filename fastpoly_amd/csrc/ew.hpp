@@ -60,7 +60,11 @@ __global__ void __launch_bounds__(TPB) k_recur(const RecurRec* __restrict__ recs
   __shared__ uint32_t us[kRecMax], ws[kRecMax];
   __shared__ unsigned long long red[TPB / 64];
   __shared__ uint32_t v0s;
-  const RecurRec R = recs ? recs[blockIdx.x] : r0;
+  RecurRec R = recs ? recs[blockIdx.x] : r0;
+  // The record is the workgroup's: its words are made wave-uniform, so the
+  // kind and field tests below are scalar branches, not exec-mask juggling
+  // inside the serial loops.
+  make_uniform(R);
   const FieldScalar f = R.f;
   const uint32_t lo = R.lo, hi = R.hi;
   uint32_t* v = R.v;
@@ -209,6 +213,8 @@ __global__ void __launch_bounds__(256) k_ew(const EwRec* __restrict__ recs, cons
     R = recs[ch.x];
     start = ch.y;
   }
+  make_uniform(R);
+  start = __builtin_amdgcn_readfirstlane(start);
   const uint32_t end = min(R.n, start + kChunk);
   for (uint32_t i = start + threadIdx.x; i < end; i += blockDim.x) R.dst[i] = ew_value(R, i);
 }
