@@ -951,49 +951,17 @@ k_cols2(JobPack<ColJob2, kColPack> jp, const ColJob2* __restrict__ jobs, int L) 
   if constexpr (!INV) {
     constexpr int r0 = SC::r_of(0);
     constexpr int NE0 = PhaseGroups<S, 0, LOGC>::NG * (1 << r0);
-    // Four-step twiddle w^(col * bitrev_S(e)).  In the last phase e = gid *
-    // 2^rl + k, so the exponent is col * bitrev_(S-rl)(gid) + j * col *
-    // 2^(S-rl) with j = bitrev_rl(k): one table lookup for each group's first
-    // factor, one for the ratio D, then a running product per element.  The
-    // table words (two per factor: w^x = hi[x >> split] * lo[x & mask]) are
-    // loaded first, ahead of the data, and multiplied while the data is in
-    // flight, so the factors cost no memory round trip after the last phase
-    // (with at most four groups in the last phase; more would spill).
-    constexpr int rl = SC::r_of(PL);
-    constexpr int NGL = PhaseGroups<S, PL, LOGC>::NG;
-    constexpr bool kEarly = NGL <= 4;
-    const uint32_t xd = (col << (S - rl)) << shift;
-    auto xq = [&](int u) { return (col * bitrev(PhaseGroups<S, PL, LOGC>::gid(g, u) << rl, S)) << shift; };
-    uint32_t D = 0, qf[NGL], ql[NGL], dh = 0, dl = 0;
-    if constexpr (kEarly) {
-      const uint32_t tmask = (1u << J.f.split) - 1;
-      dh = gload(J.f.tw_hi + (xd >> J.f.split));
-      dl = gload(J.f.tw_lo + (xd & tmask));
+    if (J.in_count == 0) {
 #pragma unroll
-      for (int u = 0; u < NGL; ++u) {
-        qf[u] = gload(J.f.tw_hi + (xq(u) >> J.f.split));
-        ql[u] = gload(J.f.tw_lo + (xq(u) & tmask));
-      }
-    }
-    // Unconditional (clamped) data loads so they overlap; the factors are
-    // formed while they are in flight; lift and zero fill after.
-    if (J.in_count != 0) {
+      for (int i = 0; i < NE0; ++i) x[i] = 0u;
+    } else {
+      // Unconditional (clamped) loads first so they overlap; lift and zero fill after.
       const uint32_t last = J.in_count - 1;
 #pragma unroll
       for (int u = 0; u < PhaseGroups<S, 0, LOGC>::NG; ++u)
 #pragma unroll
         for (int k = 0; k < (1 << r0); ++k)
           x[u * (1 << r0) + k] = gload(J.src + min(elem<S, 0>(PhaseGroups<S, 0, LOGC>::gid(g, u), k) * M + col, last));
-    }
-    if constexpr (kEarly) {
-      D = A.mul(dh, dl);
-#pragma unroll
-      for (int u = 0; u < NGL; ++u) qf[u] = A.mul(qf[u], ql[u]);
-    }
-    if (J.in_count == 0) {
-#pragma unroll
-      for (int i = 0; i < NE0; ++i) x[i] = 0u;
-    } else {
       if (lift) {
 #pragma unroll
         for (int i = 0; i < NE0; ++i) x[i] = A.mul(x[i], J.f.r2);
@@ -1006,15 +974,16 @@ k_cols2(JobPack<ColJob2, kColPack> jp, const ColJob2* __restrict__ jobs, int L) 
     }
     phase_compute<S, 0, false, LAZY, LOGC>(x, g, J.f.planes_fwd, A);
     fwd_phases<S, 1, NPH, LOGC, LAZY>(x, X, g, c, J.f.planes_fwd, A);
-    if constexpr (!kEarly) {
-      D = twiddle<LAZY>(A, J.f.tw_lo, J.f.tw_hi, J.f.split, xd);
+    // Four-step twiddle w^(col * bitrev_S(e)).  In the last phase e = gid *
+    // 2^rl + k, so the exponent is col * bitrev_(S-rl)(gid) + j * col *
+    // 2^(S-rl) with j = bitrev_rl(k): one table lookup for the group's first
+    // factor, one for the ratio D, then a running product per element.
+    constexpr int rl = SC::r_of(PL);
+    const uint32_t D = twiddle<LAZY>(A, J.f.tw_lo, J.f.tw_hi, J.f.split, (col << (S - rl)) << shift);
 #pragma unroll
-      for (int u = 0; u < NGL; ++u) qf[u] = twiddle<LAZY>(A, J.f.tw_lo, J.f.tw_hi, J.f.split, xq(u));
-    }
-#pragma unroll
-    for (int u = 0; u < NGL; ++u) {
+    for (int u = 0; u < PhaseGroups<S, PL, LOGC>::NG; ++u) {
       const uint32_t gid = PhaseGroups<S, PL, LOGC>::gid(g, u);
-      uint32_t q = qf[u];
+      uint32_t q = twiddle<LAZY>(A, J.f.tw_lo, J.f.tw_hi, J.f.split, (col * bitrev(gid << rl, S)) << shift);
 #pragma unroll
       for (int j = 0; j < (1 << rl); ++j) {
         const int k = (int)bitrev_c(j, rl);
