@@ -219,6 +219,82 @@ __global__ void __launch_bounds__(256) k_ew(const EwRec* __restrict__ recs, cons
   for (uint32_t i = start + threadIdx.x; i < end; i += blockDim.x) R.dst[i] = ew_value(R, i);
 }
 
+// ---------------------------------------------------------------------------
+// Direct product with a short operand (convolution.hpp:33-95: the reference's
+// tiled direct product for a short side <= 40): out[o] = sum_j s[j] l[o - j]
+// for o < outc, with ns <= kDirectShort (256).  A workgroup owns kDirectTile
+// consecutive outputs; the long operand's window [base - (ns - 1), base +
+// tile) and the short operand are staged in LDS (zeros outside the inputs),
+// and each thread slides a four-word window down the long operand (one LDS
+// read per term for four outputs).  Two products are summed before each
+// Montgomery reduction (2 p^2 < p 2^32), so any odd prime p < 2^31 works:
+// no transform and no auxiliary primes.
+// ---------------------------------------------------------------------------
+constexpr uint32_t kDirectShort = 256;
+constexpr uint32_t kDirectTile = 1024;
+
+__global__ void __launch_bounds__(256) k_conv_direct(const uint32_t* __restrict__ sh, uint32_t ns,
+                                                     const uint32_t* __restrict__ lg, uint32_t nl,
+                                                     uint32_t* __restrict__ out, uint32_t outc, uint32_t p,
+                                                     uint32_t ninv) {
+  __shared__ uint32_t S[kDirectShort];
+  __shared__ uint32_t W[kDirectTile + kDirectShort];
+  const uint32_t t = threadIdx.x;
+  const uint32_t base = blockIdx.x * kDirectTile;
+  const int64_t w0 = (int64_t)base - (int64_t)(ns - 1);  // long-operand index of W[0]
+  for (uint32_t i = t; i < ns; i += 256) S[i] = sh[i];
+  for (uint32_t i = t; i < kDirectTile + ns - 1; i += 256) {
+    const int64_t k = w0 + (int64_t)i;
+    W[i] = (k >= 0 && k < (int64_t)nl) ? lg[k] : 0u;
+  }
+  __syncthreads();
+  // Outputs o = base + 4t + q; term j reads W[4t + q - j + ns - 1].
+  uint32_t w[4];
+  const uint32_t top = 4 * t + ns - 1;
+#pragma unroll
+  for (int q = 0; q < 4; ++q) w[q] = W[top + q];
+  uint64_t a64[4] = {0, 0, 0, 0};
+  uint32_t acc[4] = {0, 0, 0, 0};
+  auto fold = [&]() {
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const uint32_t r = redc_full(a64[q], p, ninv);
+      const uint32_t z = acc[q] + r;
+      acc[q] = z >= p ? z - p : z;
+      a64[q] = 0;
+    }
+  };
+  // Two terms an iteration, their LDS reads issued together: term j uses the
+  // window w, term j + 1 the window shifted by one (n1 = W[top - j - 1]).
+  uint32_t j = 0;
+  for (; j + 1 < ns; j += 2) {
+    const uint64_t s0 = S[j], s1 = S[j + 1];
+    const uint32_t n1 = W[top - j - 1];
+    const uint32_t n2 = j + 2 < ns ? W[top - j - 2] : 0u;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) a64[q] += s0 * w[q];
+    a64[0] += s1 * n1;
+#pragma unroll
+    for (int q = 1; q < 4; ++q) a64[q] += s1 * w[q - 1];
+    fold();
+    w[3] = w[1];
+    w[2] = w[0];
+    w[1] = n1;
+    w[0] = n2;
+  }
+  if (j < ns) {
+    const uint64_t s0 = S[j];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) a64[q] += s0 * w[q];
+  }
+  fold();
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const uint32_t o = base + 4 * t + q;
+    if (o < outc) out[o] = acc[q];
+  }
+}
+
 // Characteristic two (square_root.hpp:15-25): (sum b_i x^i)^2 = sum b_i x^(2i).
 // out[i/2] = a[i] for even i; any nonzero odd coefficient sets *bad.
 __global__ void k_frobenius_sqrt(const uint32_t* a, size_t cnt, uint32_t* out, unsigned int* bad) {

@@ -876,6 +876,22 @@ void conv_dev(Context& c, const HostField& T, const uint32_t* a, size_t la, cons
   la = std::min(la, outc);
   lb = std::min(lb, outc);
   const size_t full = la + lb - 1;
+  // A short side: direct product (lone calls; recorded schedules keep the
+  // transform path, whose launches merge across requests).  Measured
+  // crossover against the transforms (long side 2^20): a short side of ~150
+  // over an NTT prime, ~350 over the auxiliary primes.
+  const size_t direct_max = native_ok(T, full) || split_ok(T, la, lb) ? 128 : kDirectShort;
+  if (!c.rec && T.p != 2 && std::min(la, lb) <= direct_max && outc <= ((size_t)1 << 31)) {
+    const bool a_short = la <= lb;
+    const uint32_t* sh = a_short ? a : b;
+    const uint32_t* lg = a_short ? b : a;
+    const uint32_t ns = (uint32_t)std::min(la, lb), nl = (uint32_t)std::max(la, lb);
+    const unsigned blocks = (unsigned)((outc + kDirectTile - 1) / kDirectTile);
+    hipLaunchKernelGGL(k_conv_direct, dim3(blocks), dim3(256), 0, c.stream, sh, ns, lg, nl, out, (uint32_t)outc,
+                       T.p, T.ninv);
+    FPXG_HIP(hipGetLastError());
+    return;
+  }
   if (native_ok(T, full)) {
     conv_core(c, T, a, la, b, lb, out, outc, false);
     return;

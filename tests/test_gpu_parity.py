@@ -117,6 +117,25 @@ def test_conv_random_shapes(fa, oracle, p):
     assert fa.conv(p, [], x).size == 0 and fa.conv(p, x, x, 0).size == 0
 
 
+@pytest.mark.parametrize("p", [P, Q, 1224736769, 3])
+def test_conv_short_side_direct(fa, oracle, p):
+    """Products with a short side (at most 128 coefficients over an NTT prime,
+    256 otherwise) are computed directly (no transform): both orders, both
+    boundaries, tile edges, a long side of 2^20, truncation inside and past
+    the long side."""
+    rng = random.Random(p + 9)
+    for ls, ll in ((1, 1), (1, 1025), (2, 3), (40, 1023), (63, 1024), (64, 1025), (65, 5000), (64, 64),
+                   (128, 3000), (129, 3000), (256, 5000), (257, 5000), (200, 200), (17, 1 << 20),
+                   (64, 300001)):
+        a = _rnd(oracle, p, ls, rng.randrange(1 << 30))
+        b = _rnd(oracle, p, ll, rng.randrange(1 << 30))
+        want = oracle.conv(p, a, b)
+        assert np.array_equal(fa.conv(p, a, b), want), (p, ls, ll)
+        assert np.array_equal(fa.conv(p, b, a), want), (p, ll, ls)
+        for n in (1, ll // 2 + 1, ll + 3):
+            assert np.array_equal(fa.conv(p, a, b, n), want[:n]), (p, ls, ll, n)
+
+
 @pytest.mark.parametrize("p,shapes", [
     # 65537 = 2^16 + 1: largest transform 2^16, pieces of 2^15
     (65537, [(70000, 50000), (3 * 32768 + 7, 5), (65537, 65537), (40000, 130000), (1 << 16, 1 << 16),
