@@ -295,6 +295,31 @@ __global__ void __launch_bounds__(256) k_conv_direct(const uint32_t* __restrict_
   }
 }
 
+// Tail repair (convolution.hpp:102-137): out[0, min(outc, N)) holds the
+// N-point cyclic product of a and b, whose true length is N + tail (tail <=
+// 40, la, lb <= N).  Thread i < tail computes the high coefficient h = (a
+// b)[N + i] directly (tail - i terms), stores it at N + i and removes its
+// wraparound from position i.
+__global__ void __launch_bounds__(64) k_tail_fix(const uint32_t* __restrict__ a, uint32_t la,
+                                                 const uint32_t* __restrict__ b, uint32_t lb,
+                                                 uint32_t* __restrict__ out, uint32_t outc, uint32_t N,
+                                                 uint32_t tail, uint32_t p, uint32_t ninv) {
+  const uint32_t i = threadIdx.x;
+  if (i >= tail) return;
+  const uint32_t s = N + i;  // a_j b_(s - j) with s - j < lb and j < la
+  uint32_t h = 0;
+  for (uint32_t j = s - (lb - 1); j < la; ++j) {
+    const uint32_t t = mulmod_full(a[j], b[s - j], p, ninv);
+    const uint32_t z = h + t;
+    h = z >= p ? z - p : z;
+  }
+  if (s < outc) out[s] = h;
+  if (i < outc) {
+    const uint32_t v = out[i];
+    out[i] = v >= h ? v - h : v - h + p;
+  }
+}
+
 // Characteristic two (square_root.hpp:15-25): (sum b_i x^i)^2 = sum b_i x^(2i).
 // out[i/2] = a[i] for even i; any nonzero odd coefficient sets *bad.
 __global__ void k_frobenius_sqrt(const uint32_t* a, size_t cnt, uint32_t* out, unsigned int* bad) {

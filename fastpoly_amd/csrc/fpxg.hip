@@ -659,8 +659,10 @@ void ntt_inverse(Context& c, const HostField& F, uint32_t* dst, const uint32_t* 
 // the window [0, outc) in field F[i]; lift reads a and b as limbs of another
 // field (times r2).  All fields must share the lazy mode.
 static void conv_core_fields(Context& c, const HostField* const* F, int k, const uint32_t* a, size_t la,
-                             const uint32_t* b, size_t lb, uint32_t* const* out, size_t outc, bool lift) {
-  const int L = std::max(1, ceil_log2(la + lb - 1));
+                             const uint32_t* b, size_t lb, uint32_t* const* out, size_t outc, bool lift,
+                             int L_cyclic = 0) {
+  // L_cyclic: the 2^L_cyclic-point cyclic product (outc <= 2^L_cyclic), else the whole product.
+  const int L = L_cyclic ? L_cyclic : std::max(1, ceil_log2(la + lb - 1));
   const uint32_t N = 1u << L;
   const PassPlan P = c.conv_plan(L);
   const bool lazy = F[0]->lazy;
@@ -709,10 +711,10 @@ static void conv_core_fields(Context& c, const HostField* const* F, int k, const
 }
 
 static void conv_core(Context& c, const HostField& F, const uint32_t* a, size_t la, const uint32_t* b, size_t lb,
-                      uint32_t* out, size_t outc, bool lift) {
+                      uint32_t* out, size_t outc, bool lift, int L_cyclic = 0) {
   const HostField* Fs[1] = {&F};
   uint32_t* outs[1] = {out};
-  conv_core_fields(c, Fs, 1, a, la, b, lb, outs, outc, lift);
+  conv_core_fields(c, Fs, 1, a, la, b, lb, outs, outc, lift, L_cyclic);
 }
 
 // Forward transforms of one input into k <= 3 fields (same lazy mode), one
@@ -891,6 +893,19 @@ void conv_dev(Context& c, const HostField& T, const uint32_t* a, size_t la, cons
                        T.p, T.ninv);
     FPXG_HIP(hipGetLastError());
     return;
+  }
+  // A product a few coefficients past a power of two N: the N-point cyclic
+  // product plus the high tail computed directly (convolution.hpp:102-137).
+  if (!c.rec && T.p != 2 && full > 64) {
+    const int Lf = ceil_log2(full) - 1;
+    const size_t N = (size_t)1 << Lf, tail = full - N;
+    if (tail >= 1 && tail <= 40 && la <= N && lb <= N && native_ok(T, N)) {
+      conv_core(c, T, a, la, b, lb, out, std::min(outc, N), false, Lf);
+      hipLaunchKernelGGL(k_tail_fix, dim3(1), dim3(64), 0, c.stream, a, (uint32_t)la, b, (uint32_t)lb, out,
+                         (uint32_t)outc, (uint32_t)N, (uint32_t)tail, T.p, T.ninv);
+      FPXG_HIP(hipGetLastError());
+      return;
+    }
   }
   if (native_ok(T, full)) {
     conv_core(c, T, a, la, b, lb, out, outc, false);

@@ -136,6 +136,30 @@ def test_conv_short_side_direct(fa, oracle, p):
             assert np.array_equal(fa.conv(p, a, b, n), want[:n]), (p, ls, ll, n)
 
 
+@pytest.mark.parametrize("p", [P, 167772161, 65537])
+def test_conv_tail_repair(fa, oracle, p):
+    """Products a few coefficients past a power of two N: the N-point cyclic
+    product plus the directly computed high tail (tails 1, 2, 39, 40, 41 at
+    several N, operands up to N each, truncation below N, inside the tail and
+    past it, squaring)."""
+    rng = random.Random(p + 13)
+    for N in (256, 4096, 1 << 15):
+        for tail in (1, 2, 39, 40, 41):
+            full = N + tail
+            for la in (N, full // 2 + 1, full - 129):
+                lb = full + 1 - la
+                if la > N or lb > N or min(la, lb) <= 128:
+                    continue
+                a = _rnd(oracle, p, la, rng.randrange(1 << 30))
+                b = _rnd(oracle, p, lb, rng.randrange(1 << 30))
+                want = oracle.conv(p, a, b)
+                assert np.array_equal(fa.conv(p, a, b), want), (p, N, tail, la)
+                for n in (N - 5, N + tail // 2 + 1, full + 7):
+                    assert np.array_equal(fa.conv(p, a, b, n), want[:n]), (p, N, tail, la, n)
+        x = _rnd(oracle, p, N // 2 + 10, 3)
+        assert np.array_equal(fa.conv(p, x, x), oracle.conv(p, x, x)), (p, N)
+
+
 @pytest.mark.parametrize("p,shapes", [
     # 65537 = 2^16 + 1: largest transform 2^16, pieces of 2^15
     (65537, [(70000, 50000), (3 * 32768 + 7, 5), (65537, 65537), (40000, 130000), (1 << 16, 1 << 16),
