@@ -660,8 +660,9 @@ void ntt_inverse(Context& c, const HostField& F, uint32_t* dst, const uint32_t* 
 // field (times r2).  All fields must share the lazy mode.
 static void conv_core_fields(Context& c, const HostField* const* F, int k, const uint32_t* a, size_t la,
                              const uint32_t* b, size_t lb, uint32_t* const* out, size_t outc, bool lift,
-                             int L_cyclic = 0) {
-  // L_cyclic: the 2^L_cyclic-point cyclic product (outc <= 2^L_cyclic), else the whole product.
+                             int L_cyclic = 0, size_t lo = 0) {
+  // L_cyclic: the 2^L_cyclic-point cyclic product, whose window [lo, lo +
+  // outc) is written (lo + outc <= 2^L_cyclic); else the whole product from 0.
   const int L = L_cyclic ? L_cyclic : std::max(1, ceil_log2(la + lb - 1));
   const uint32_t N = 1u << L;
   const PassPlan P = c.conv_plan(L);
@@ -677,7 +678,7 @@ static void conv_core_fields(Context& c, const HostField* const* F, int k, const
   if (P.S == 0) {
     for (int i = 0; i < k; ++i) {
       rj[i].src_a = a; rj[i].src_b = b; rj[i].dst = out[i]; rj[i].in_a = (uint32_t)la; rj[i].in_b = (uint32_t)lb;
-      rj[i].out_lo = 0; rj[i].out_hi = (uint32_t)outc;
+      rj[i].out_lo = (uint32_t)lo; rj[i].out_hi = (uint32_t)(lo + outc); rj[i].out_shift = (uint32_t)lo;
       rj[i].flags = R2_SCALE | R2_CANON | (lift ? R2_LIFT : 0);
     }
     launch_rows_group(c, lazy, R2M_CONV, rj, k, L, 1);
@@ -704,8 +705,8 @@ static void conv_core_fields(Context& c, const HostField* const* F, int k, const
   ColJob2 ij[kRowPack];
   for (int i = 0; i < k; ++i) {
     ij[i] = ColJob2{};
-    ij[i].src = A[i].get(); ij[i].dst = out[i]; ij[i].out_lo = 0; ij[i].out_hi = (uint32_t)outc;
-    ij[i].inv_n = inv_pow2(*F[i], L); ij[i].f = F[i]->dev;
+    ij[i].src = A[i].get(); ij[i].dst = out[i]; ij[i].out_lo = (uint32_t)lo; ij[i].out_hi = (uint32_t)(lo + outc);
+    ij[i].out_shift = (uint32_t)lo; ij[i].inv_n = inv_pow2(*F[i], L); ij[i].f = F[i]->dev;
   }
   launch_cols_n(c, lazy, true, ij, P, (uint32_t)k, nullptr);
 }

@@ -177,6 +177,31 @@ static bool one_launch_step(const Context& c, const HostField& T, const Engine& 
   return E.L <= 9 || (E.native && E.L <= std::min(c.row_log_max, 13));
 }
 
+// Fields without their own transform: out[i - lo] = (x * y)[i] for i in
+// [lo, hi) of the 2^L-point cyclic product through the three auxiliary primes,
+// each pass one launch for all three (no stored spectra), then Garner (negated
+// on request).  Up to 2^kCrtFusedLog points a Newton step's products are
+// latency-bound chains of launches, where this beats the stored-spectra
+// `Engine` (fewer launches, a little more transform work).
+constexpr int kCrtFusedLog = 16;
+static void crt_conv(Context& c, const HostField& T, int L, const uint32_t* x, size_t cx, const uint32_t* y,
+                     size_t cy, size_t lo, size_t hi, uint32_t* out, bool neg = false) {
+  DevBuf res[3];
+  const HostField* Fs[3];
+  uint32_t* outs[3];
+  for (int i = 0; i < 3; ++i) {
+    res[i] = c.buf(hi - lo);
+    Fs[i] = &c.field(kCrtP[i]);
+    outs[i] = res[i].get();
+  }
+  conv_core_fields(c, Fs, 3, x, cx, y, cy, outs, hi - lo, true, L, lo);
+  GarnerRec g{};
+  g.c1 = res[0].get(); g.c2 = res[1].get(); g.c3 = res[2].get(); g.out = out; g.n = (uint32_t)(hi - lo);
+  g.g = garner_const(c, T, 3, false);
+  g.g.neg = neg;
+  emit_garner(c, g);
+}
+
 // extend_inverse (inverse.hpp:18-65): c[0, k) is 1/a mod x^k; fill c[k, wanted).
 static void extend_inverse(Context& c, const HostField& T, const uint32_t* a, size_t la, uint32_t* cc, size_t k,
                            size_t wanted) {
@@ -202,6 +227,11 @@ static void extend_inverse(Context& c, const HostField& T, const uint32_t* a, si
   if (E.native) {
     native_conv(c, T, E.L, a, std::min(la, wanted), cc, k, k, wanted, e.get(), true);
     native_conv(c, T, E.L, e.get(), hlen, cc, k, 0, hlen, cc + k);
+    return;
+  }
+  if (E.L <= kCrtFusedLog) {
+    crt_conv(c, T, E.L, a, std::min(la, wanted), cc, k, k, wanted, e.get(), true);
+    crt_conv(c, T, E.L, e.get(), hlen, cc, k, 0, hlen, cc + k);
     return;
   }
   Spec Sc;
@@ -341,6 +371,13 @@ static void series_emit(Context& c, const HostField& T, const SeriesReq& q, cons
           native_conv(c, T, E.L, t.get(), hlen, b, m, 0, hlen, b + m);
           continue;
         }
+        if (E.L <= kCrtFusedLog) {
+          crt_conv(c, T, E.L, ap.get(), std::min(lap, m2 - 1), b, m, m - 1, m - 1 + hlen, r.get());
+          crt_conv(c, T, E.L, r.get(), hlen, cbuf.get(), hlen, 0, hlen, t.get());
+          emit_ew(c, T, EW_MUL_RECIP, t.get(), t.get(), nullptr, hlen, 0, (uint32_t)m);
+          crt_conv(c, T, E.L, t.get(), hlen, b, m, 0, hlen, b + m);
+          continue;
+        }
         Spec Sb, Sc;
         E.forward(Sb, b, m);
         E.product(ap.get(), std::min(lap, m2 - 1), Sb, m - 1, m - 1 + hlen, r.get());
@@ -410,6 +447,14 @@ static void series_emit(Context& c, const HostField& T, const SeriesReq& q, cons
           Epi half;  // times 1/2 (square_root.hpp:96-97)
           half.scale = half_inv;
           native_conv(c, T, E.L, res.get(), hlen, cbuf.get(), hlen, 0, hlen, b + m, false, half);
+          continue;
+        }
+        if (E.L <= kCrtFusedLog) {
+          crt_conv(c, T, E.L, b, m, b, m, m, m2, sq.get());
+          emit_ew(c, T, EW_SUB_FROM, res.get(), av + std::min(m, au), sq.get(), hlen,
+                  (uint32_t)(au > m ? au - m : 0), 0);
+          crt_conv(c, T, E.L, res.get(), hlen, cbuf.get(), hlen, 0, hlen, b + m);
+          emit_ew(c, T, EW_SCALE, b + m, b + m, nullptr, hlen, 0, half_inv);
           continue;
         }
         Spec Sb, Sc;
