@@ -718,41 +718,6 @@ static void conv_core(Context& c, const HostField& F, const uint32_t* a, size_t 
   conv_core_fields(c, Fs, 1, a, la, b, lb, outs, outc, lift, L_cyclic);
 }
 
-// Forward transforms of one input into k <= 3 fields (same lazy mode), one
-// launch per pass for all of them.
-static void ntt_forward_fields(Context& c, const HostField* const* F, int k, uint32_t* const* dst,
-                               const uint32_t* src, uint32_t in_count, int L, bool lift) {
-  const PassPlan P = c.plan(L);
-  const uint32_t N = 1u << L;
-  const bool lazy = F[0]->lazy;
-  for (int i = 1; i < k; ++i)
-    if (F[i]->lazy != lazy) throw Error(E_DEVICE, "fields of one launch differ in lazy mode");
-  RowJob2 rj[kRowPack];
-  for (int i = 0; i < k; ++i) {
-    rj[i] = RowJob2{};
-    rj[i].f = F[i]->dev;
-    rj[i].out_lo = 0; rj[i].out_hi = N;
-  }
-  if (P.S == 0) {
-    for (int i = 0; i < k; ++i) {
-      rj[i].src_a = src; rj[i].dst = dst[i]; rj[i].in_a = in_count; rj[i].flags = lift ? R2_LIFT : 0;
-    }
-    launch_rows_group(c, lazy, R2M_FWD, rj, k, L, 1);
-    return;
-  }
-  ColJob2 cj[kRowPack];
-  for (int i = 0; i < k; ++i) {
-    cj[i] = ColJob2{};
-    cj[i].src = src; cj[i].dst = dst[i]; cj[i].in_count = in_count; cj[i].flags = lift ? R2_LIFT : 0;
-    cj[i].f = F[i]->dev;
-  }
-  launch_cols_n(c, lazy, false, cj, P, (uint32_t)k, nullptr);
-  for (int i = 0; i < k; ++i) {
-    rj[i].src_a = dst[i]; rj[i].dst = dst[i]; rj[i].in_a = N;
-  }
-  launch_rows_group(c, lazy, R2M_FWD, rj, k, P.b, 1u << P.S);
-}
-
 static const uint32_t kCrtP[3] = {167772161u, 469762049u, 754974721u};  // crt_convolution.hpp:14-19
 static const size_t kCrtBlock = (size_t)1 << 23;                          // crt_convolution.hpp:19
 
