@@ -365,11 +365,25 @@ __global__ void __launch_bounds__(256) k_scan_part(const ScanRec* __restrict__ r
   const uint32_t lo = (uint32_t)min((uint64_t)blockIdx.x * per, (uint64_t)R.n);
   const uint32_t hi = (uint32_t)min((uint64_t)lo + per, (uint64_t)R.n);
   uint32_t m1 = R.n, m2 = R.n, last = 0;
-  for (uint32_t i = lo + threadIdx.x; i < hi; i += blockDim.x)
-    if (R.a[i] != 0) {
-      if (i < m1) { m2 = m1; m1 = i; } else if (i < m2) { m2 = i; }
-      last = i + 1;
+  // Eight words a thread a round, all loads issued before any test (one
+  // memory round trip a round); indices rise within a thread.
+  constexpr uint32_t K = 8;
+  for (uint32_t c0 = lo; c0 < hi; c0 += 256 * K) {
+    uint32_t v[K];
+#pragma unroll
+    for (uint32_t k = 0; k < K; ++k) {
+      const uint32_t i = c0 + threadIdx.x + k * 256;
+      v[k] = i < hi ? R.a[i] : 0u;
     }
+#pragma unroll
+    for (uint32_t k = 0; k < K; ++k) {
+      const uint32_t i = c0 + threadIdx.x + k * 256;
+      if (v[k] != 0) {
+        if (i < m1) { m2 = m1; m1 = i; } else if (i < m2) { m2 = i; }
+        last = i + 1;
+      }
+    }
+  }
   m1s[threadIdx.x] = m1; m2s[threadIdx.x] = m2; lss[threadIdx.x] = last;
   __syncthreads();
   for (int st = 128; st > 0; st >>= 1) {
@@ -470,8 +484,8 @@ static std::vector<ScanOut> scan_inputs(Context& c, const std::vector<ScanRec>& 
   if (recs.empty()) return out;
   uint32_t maxn = 0;
   for (const ScanRec& r : recs) maxn = std::max(maxn, r.n);
-  // Slices of >= 8192 words, at most 256 per record.
-  const uint32_t B = std::max<uint32_t>(1, std::min<uint32_t>(256, (maxn + 8191) / 8192));
+  // Slices of >= 4096 words, at most 256 per record.
+  const uint32_t B = std::max<uint32_t>(1, std::min<uint32_t>(256, (maxn + 4095) / 4096));
   DevBuf dr = c.buf(recs.size() * sizeof(ScanRec) / 4 + 1);
   DevBuf dout = c.buf(recs.size() * sizeof(ScanOut) / 4 + 1);
   DevBuf dpart = c.buf(recs.size() * B * sizeof(ScanPart) / 4 + 1);
