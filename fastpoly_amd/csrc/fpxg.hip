@@ -322,7 +322,10 @@ static void rows2_launch(Context& c, const RowJob2* j, uint32_t nrows, uint32_t 
       return;
     }
   }
-  if constexpr ((MODE == R2M_INVSTEP || MODE == R2M_EXPSTEP || MODE == R2M_SQRTSTEP) && LOGM <= 9) {
+  if constexpr (MODE == R2M_NINV || MODE == R2M_NEXP || MODE == R2M_NSQRT) {
+    if (!dj && njobs != 1) throw Error(E_DEVICE, "small-series launches take one job by value");
+    hipLaunchKernelGGL((k_newton_small<MODE>), dim3(1, njobs), dim3(kSmallSeries), 0, c.stream, j[0], dj);
+  } else if constexpr ((MODE == R2M_INVSTEP || MODE == R2M_EXPSTEP || MODE == R2M_SQRTSTEP) && LOGM <= 9) {
     // Small Newton steps: their window products computed directly (one job
     // by value, or a device job array).
     if (!dj && njobs != 1) throw Error(E_DEVICE, "direct Newton steps take one job by value");
@@ -353,6 +356,16 @@ static void rows2_mode(Context& c, int mode, const RowJob2* j, uint32_t nrows, u
         break;
       }
       throw Error(E_DEVICE, "Newton-step row pass outside 2^6 .. 2^13 points");
+    case R2M_NINV:
+    case R2M_NEXP:
+    case R2M_NSQRT:
+      if constexpr (LOGM == kSmallSeriesLog) {
+        if (mode == R2M_NINV) rows2_launch<LOGM, LAZY, R2M_NINV>(c, j, nrows, njobs, dj);
+        else if (mode == R2M_NEXP) rows2_launch<LOGM, LAZY, R2M_NEXP>(c, j, nrows, njobs, dj);
+        else rows2_launch<LOGM, LAZY, R2M_NSQRT>(c, j, nrows, njobs, dj);
+        break;
+      }
+      throw Error(E_DEVICE, "small-series launch recorded with the wrong size key");
     default: throw Error(E_DEVICE, "unsupported row-pass mode");
   }
 }

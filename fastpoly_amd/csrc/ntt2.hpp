@@ -375,7 +375,9 @@ __device__ __forceinline__ uint32_t out_epilogue(uint32_t flags, const uint32_t*
   return v;
 }
 enum : int { R2M_FWD = 1, R2M_INV = 2, R2M_CONV = 3, R2M_SPEC = 4, R2M_SPEC_ONLY = 5, R2M_INVSTEP = 6,
-             R2M_EXPSTEP = 7, R2M_SQRTSTEP = 8 };
+             R2M_EXPSTEP = 7, R2M_SQRTSTEP = 8, R2M_NINV = 9, R2M_NEXP = 10, R2M_NSQRT = 11 };
+// R2M_NINV / R2M_NEXP / R2M_NSQRT: the first coefficients of a series in one
+// workgroup (k_newton_small), recorded and merged like the row modes.
 // R2M_INVSTEP (one Newton step of the power-series inverse, inverse.hpp:18-65,
 // for a 2^LOGM = 2k point row): with c = b[0, k) and a = a[0, in_a),
 // e = (a * c)[k, k + out_hi) and out = (e * c)[0, out_hi) (negated with
@@ -894,6 +896,142 @@ __global__ void __launch_bounds__(2 << LM) k_step_direct(RowJob2 j0, const RowJo
     __syncthreads();
     const uint32_t v = mid(W, h, Cc, h, 0);
     if (par == 0 && o < h) J.dst[o] = mulmod_full(v, J.aux, p, ninv);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// The first N <= 512 coefficients of a Newton series in one workgroup, from
+// the constant term up: doublings m -> w = min(2m, N) whose products are the
+// same direct middle products as k_step_direct, every operand in LDS.  It
+// stands in for the reference's 32-coefficient seed recurrences and first
+// doubling steps (inverse.hpp:68-77 and 18-65, exponential.hpp:158-204,
+// square_root.hpp:46-100): every output coefficient is unique, so the limbs
+// are the same.  Job fields (out_hi = N):
+//   R2M_NINV : c = 1/a mod x^N; src_a = a (in_a words, a[0] != 0), dst = c.
+//   R2M_NEXP : b = exp(a) mod x^N; src_a = a' (in_a words), epi = 1/i table,
+//              dst = b; src_c (an output here) <- c = 1/b mod x^(N/2).
+//   R2M_NSQRT: b = sqrt(a) mod x^N with b[0] = aux; src_a = a (in_a words),
+//              inv_n = 1/aux, dst = b; src_c (an output) <- c = 1/b mod x^(N/2).
+// Each doubling keeps c = 1/b to the precision of the next step's half
+// (extend_c, the inverse step with b in place of a).  512 threads, two an
+// output of each product.
+// ---------------------------------------------------------------------------
+constexpr uint32_t kSmallSeries = 512;
+constexpr int kSmallSeriesLog = 9;  // the size key its launches are recorded under
+
+// sum_j X[j] Y[o + off - j] over 0 <= j < nx, 0 <= o + off - j < ny, for an
+// output o < h: the half over j of parity `par`, added to the partner lane's.
+__device__ __forceinline__ uint32_t mid_product(const uint32_t* X, uint32_t nx, const uint32_t* Y, uint32_t ny,
+                                                uint32_t off, uint32_t o, uint32_t par, uint32_t h, uint32_t p,
+                                                uint32_t ninv) {
+  uint32_t acc = 0;
+  const uint32_t s = o + off;
+  if (o < h && nx && ny) {
+    const uint32_t jlo = s >= ny ? s - (ny - 1) : 0u;
+    const uint32_t jhi = min(nx - 1, s);  // inclusive
+    uint32_t j = jlo + ((jlo & 1) != par ? 1u : 0u);
+    for (; j + 2 <= jhi; j += 4) {
+      const uint32_t r = redc_full((uint64_t)X[j] * Y[s - j] + (uint64_t)X[j + 2] * Y[s - j - 2], p, ninv);
+      const uint32_t z = acc + r;
+      acc = z >= p ? z - p : z;
+    }
+    if (j <= jhi) {
+      const uint32_t z = acc + redc_full((uint64_t)X[j] * Y[s - j], p, ninv);
+      acc = z >= p ? z - p : z;
+    }
+  }
+  const uint32_t z = acc + (uint32_t)__shfl_xor((int)acc, 1, 64);
+  return z >= p ? z - p : z;
+}
+
+template <int MODE>
+__global__ void __launch_bounds__(kSmallSeries) k_newton_small(RowJob2 j0, const RowJob2* __restrict__ jobs) {
+  constexpr uint32_t NM = kSmallSeries;
+  const RowJob2 J = load_job(j0, jobs);
+  __shared__ uint32_t A[NM], B[NM], C[NM], E[NM / 2];
+  const uint32_t t = threadIdx.x, p = J.f.p, ninv = J.f.ninv;
+  const uint32_t N = min(J.out_hi, NM), na = min(J.in_a, N);
+  const uint32_t o = t >> 1, par = t & 1;
+  for (uint32_t i = t; i < NM; i += NM) {
+    A[i] = i < na ? J.src_a[i] : 0u;
+    B[i] = 0u;
+    C[i] = 0u;
+  }
+  if (t == 0) {
+    if constexpr (MODE == R2M_NINV) {
+      // 1/a[0] = a[0]^(p-2) (Montgomery limbs), read from the device: the
+      // schedule stays independent of the input's values.
+      uint32_t r = J.f.one, x = na ? J.src_a[0] : 0u;
+      for (uint32_t e = p - 2; e; e >>= 1) {
+        if (e & 1) r = mulmod_full(r, x, p, ninv);
+        x = mulmod_full(x, x, p, ninv);
+      }
+      C[0] = r;
+    } else if constexpr (MODE == R2M_NEXP) {
+      B[0] = J.f.one;
+      C[0] = J.f.one;
+    } else {
+      B[0] = J.aux;
+      C[0] = J.inv_n;
+    }
+  }
+  __syncthreads();
+  auto mp = [&](const uint32_t* X, uint32_t nx, const uint32_t* Y, uint32_t ny, uint32_t off, uint32_t h) {
+    return mid_product(X, nx, Y, ny, off, o, par, h, p, ninv);
+  };
+  // c[k, w) from c[0, k) = 1/x mod x^k, x = X[0, nx): e = (x c)[k + i],
+  // c[k + i] = -(e c)[i] (inverse.hpp:18-65).  Reads C[0, k) only, writes C[k, w).
+  auto extend_c = [&](const uint32_t* X, uint32_t nx, uint32_t k, uint32_t w) {
+    const uint32_t hh = w - k;
+    const uint32_t e = mp(X, min(nx, w), C, k, k, hh);
+    if (par == 0 && o < hh) E[o] = e;
+    __syncthreads();
+    const uint32_t v = mp(E, hh, C, k, 0, hh);
+    if (par == 0 && o < hh) C[k + o] = v == 0 ? 0u : p - v;
+    __syncthreads();
+  };
+  uint32_t cp = 1;  // precision of c
+  for (uint32_t m = 1; m < N; m *= 2) {
+    const uint32_t w = min(2 * m, N), h = w - m;
+    if constexpr (MODE == R2M_NINV) {
+      extend_c(A, na, m, w);
+    } else {
+      if (cp < h) {
+        extend_c(B, m, cp, h);
+        cp = h;
+      }
+      if constexpr (MODE == R2M_NEXP) {
+        // r = (a' b)[m - 1 + i]; t = (r c)[i] / (m + i); b[m + i] = (t b)[i]
+        const uint32_t r = mp(A, min(na, w - 1), B, m, m - 1, h);
+        if (par == 0 && o < h) E[o] = r;
+        __syncthreads();
+        const uint32_t rc = mp(E, h, C, h, 0, h);
+        __syncthreads();  // E is rewritten below
+        if (par == 0 && o < h) E[o] = mulmod_full(rc, J.epi[m + o], p, ninv);
+        __syncthreads();
+        const uint32_t v = mp(E, h, B, m, 0, h);
+        if (par == 0 && o < h) B[m + o] = v;
+      } else {
+        // res = a[m + i] - (b b)[m + i]; b[m + i] = (res c)[i] / 2
+        const uint32_t sq = mp(B, m, B, m, m, h);
+        if (par == 0 && o < h) {
+          const uint32_t av = A[m + o];
+          E[o] = av >= sq ? av - sq : av - sq + p;
+        }
+        __syncthreads();
+        const uint32_t v = mp(E, h, C, h, 0, h);
+        const uint32_t half = mulmod_full((p + 1) / 2, J.f.r2, p, ninv);
+        if (par == 0 && o < h) B[m + o] = mulmod_full(v, half, p, ninv);
+      }
+      __syncthreads();
+    }
+  }
+  if constexpr (MODE == R2M_NINV) {
+    for (uint32_t i = t; i < N; i += NM) J.dst[i] = C[i];
+  } else {
+    uint32_t* c_out = const_cast<uint32_t*>(J.src_c);
+    for (uint32_t i = t; i < N; i += NM) J.dst[i] = B[i];
+    for (uint32_t i = t; i < cp; i += NM) c_out[i] = C[i];
   }
 }
 

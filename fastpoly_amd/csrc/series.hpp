@@ -156,10 +156,27 @@ static void extend_inverse(Context& c, const HostField& T, const uint32_t* a, si
   crt_conv(c, T, E.L, e.get(), hlen, cc, k, 0, hlen, cc + k);
 }
 
+// The first min(n, 512) coefficients of a small-series mode in one launch
+// (k_newton_small); the caller fills the mode's fields.
+static void small_series(Context& c, const HostField& T, int mode, RowJob2& rj, size_t n0) {
+  rj.f = T.dev;
+  rj.nrows = 1;
+  rj.out_hi = (uint32_t)n0;
+  launch_rows(c, T.lazy, mode, rj, kSmallSeriesLog, 1);
+}
+
 static void inv_newton(Context& c, const HostField& T, const uint32_t* a, size_t la, size_t n, uint32_t* out) {
-  const size_t seed = std::min<size_t>(n, 32);
-  recur(c, T, REC_INV, a, la, nullptr, 0, out, 0, seed, 0, 0);
-  for (size_t m = seed; m < n; m <<= 1) extend_inverse(c, T, a, la, out, m, std::min(2 * m, n));
+  size_t m0;
+  if (T.p != 2) {
+    m0 = std::min<size_t>(n, kSmallSeries);
+    RowJob2 rj{};
+    rj.src_a = a; rj.in_a = (uint32_t)std::min(la, m0); rj.dst = out;
+    small_series(c, T, R2M_NINV, rj, m0);
+  } else {
+    m0 = std::min<size_t>(n, 32);
+    recur(c, T, REC_INV, a, la, nullptr, 0, out, 0, m0, 0, 0);
+  }
+  for (size_t m = m0; m < n; m <<= 1) extend_inverse(c, T, a, la, out, m, std::min(2 * m, n));
 }
 
 // One series call: its input and output (device pointers).
@@ -234,26 +251,30 @@ static void series_emit(Context& c, const HostField& T, const SeriesReq& q, cons
       }
       DevBuf ap = c.buf(lap);
       emit_ew(c, T, EW_DERIV, ap.get(), a, nullptr, lap, 0, 0);
-      const size_t seed = std::min<size_t>(n, 32);
       uint32_t* b = out;
-      recur(c, T, REC_EXP, ap.get(), lap, nullptr, 0, b, 0, seed, one, 0);
-      if (n == seed) return;
-      DevBuf cbuf = c.buf(n);
-      recur(c, T, REC_INV, b, seed, nullptr, 0, cbuf.get(), 0, seed / 2, 0, 0);
-      size_t csize = seed / 2;
-      // 1/i for the steps' t[i] /= m + i (exponential.hpp:194-195), applied in
-      // the product epilogue on native fields.
-      DevBuf rtab;
-      // (Other fields: only the direct steps, up to 2m = 512, read it.)
-      if (series_native(T, ceil_log2(2 * seed))) {
-        rtab = c.buf(n);
-        emit_ew(c, T, EW_RECIP, rtab.get(), nullptr, nullptr, n, 0, 0);
-      } else if (T.p != 2) {
-        const size_t nr = std::min<size_t>(n, 512);
-        rtab = c.buf(nr);
-        emit_ew(c, T, EW_RECIP, rtab.get(), nullptr, nullptr, nr, 0, 0);
+      if (T.p == 2) {  // n <= 2 (n - 1 < p): the recurrence alone
+        recur(c, T, REC_EXP, ap.get(), lap, nullptr, 0, b, 0, n, one, 0);
+        return;
       }
-      for (size_t m = seed; m < n; m *= 2) {
+      // 1/i for the steps' t[i] /= m + i (exponential.hpp:194-195), read by the
+      // small-series kernel and, on native fields, by the product epilogues.
+      // (Other fields read it only below 512.)
+      DevBuf rtab;
+      const size_t nr = series_native(T, 6) ? n : std::min<size_t>(n, kSmallSeries);
+      rtab = c.buf(nr);
+      emit_ew(c, T, EW_RECIP, rtab.get(), nullptr, nullptr, nr, 0, 0);
+      // The first min(n, 512) coefficients of b, and c = 1/b to half of that.
+      const size_t n0 = std::min<size_t>(n, kSmallSeries);
+      DevBuf cbuf = c.buf(n);
+      {
+        RowJob2 rj{};
+        rj.src_a = ap.get(); rj.in_a = (uint32_t)std::min(lap, n0 - 1); rj.epi = rtab.get();
+        rj.dst = b; rj.src_c = cbuf.get();
+        small_series(c, T, R2M_NEXP, rj, n0);
+      }
+      if (n == n0) return;
+      size_t csize = n0 / 2;
+      for (size_t m = n0; m < n; m *= 2) {
         const size_t m2 = std::min(2 * m, n), hlen = m2 - m;
         if (hlen <= 8) {
           recur(c, T, REC_EXP, ap.get(), lap, nullptr, 0, b, m, m2, one, 0);
@@ -312,14 +333,19 @@ static void series_emit(Context& c, const HostField& T, const SeriesReq& q, cons
       }
       const uint32_t half_inv = T.from_int(((uint64_t)T.p + 1) / 2);
       const uint32_t tri = T.mul(T.inv(root0), half_inv);
-      const size_t seed = std::min<size_t>(inner, 32);
       const uint32_t* av = a + v;
-      recur(c, T, REC_SQRT, nullptr, 0, av, au, b, 0, seed, root0, tri);
-      if (inner == seed) return;
+      // The first min(inner, 512) coefficients of b, and c = 1/b to half of that.
+      const size_t n0 = std::min<size_t>(inner, kSmallSeries);
       DevBuf cbuf = c.buf(inner);
-      recur(c, T, REC_INV, b, seed, nullptr, 0, cbuf.get(), 0, seed / 2, 0, 0);
-      size_t csize = seed / 2;
-      for (size_t m = seed; m < inner; m *= 2) {
+      {
+        RowJob2 rj{};
+        rj.src_a = av; rj.in_a = (uint32_t)std::min(au, n0); rj.aux = root0; rj.inv_n = T.inv(root0);
+        rj.dst = b; rj.src_c = cbuf.get();
+        small_series(c, T, R2M_NSQRT, rj, n0);
+      }
+      if (inner == n0) return;
+      size_t csize = n0 / 2;
+      for (size_t m = n0; m < inner; m *= 2) {
         const size_t m2 = std::min(2 * m, inner), hlen = m2 - m;
         if (hlen <= 8) {
           recur(c, T, REC_SQRT, nullptr, 0, av, au, b, m, m2, root0, tri);

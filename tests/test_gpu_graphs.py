@@ -84,6 +84,21 @@ def test_same_key_new_data(fa, oracle):
             assert np.array_equal(out.to_host(), oracle.series(op, P, a, n)), (op, seed)
 
 
+def test_same_key_new_constant_term(fa, oracle):
+    """The replay key knows only whether a[0] is 0, 1 or other; a replayed
+    inverse with another a[0] must still use that a[0] (regression: a schedule
+    that baked 1/a[0] in on the host)."""
+    D = fa.DeviceArray
+    for n in (40, 3000, 70000):
+        da, out = D(n), D(n)
+        for seed, a0 in ((41, 12345), (42, 987654), (43, 5), (44, 998244352)):
+            a = _input(oracle, "inv", P, n, seed)
+            a[0] = oracle.from_int(P, a0)
+            assert fa.lib().fpxg_memcpy_h2d(da.ptr, a.ctypes.data, 4 * n) == 0
+            fa.series_dev("inv", P, da, n, n, out)
+            assert np.array_equal(out.to_host(), oracle.series("inv", P, a, n)), (n, a0)
+
+
 def test_many_keys_and_release(fa, oracle):
     D = fa.DeviceArray
     outs = []
