@@ -582,11 +582,18 @@ def autotune(min_log: int = 13, max_log: int = 24, reps: int = 5) -> dict:
 
 
 OPT_GRAPHS = 0
+OPT_ROW_DMA = 1
 
 
 def set_graphs(enabled: bool) -> None:
     """Replay repeated single series calls from captured hipGraphs (default on)."""
     _check(lib().fpxg_set_option(OPT_GRAPHS, 1 if enabled else 0), "set_option")
+
+
+def set_row_dma(enabled: bool) -> None:
+    """Forward row passes over many 2^14-word rows: one persistent prefetching
+    workgroup per CU (default) or one workgroup per row (same results)."""
+    _check(lib().fpxg_set_option(OPT_ROW_DMA, 1 if enabled else 0), "set_option")
 
 
 def pow(p: int, a, k: int, n: int) -> np.ndarray:  # noqa: A001 (the reference's name)

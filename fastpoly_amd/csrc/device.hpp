@@ -69,6 +69,24 @@ struct Arith {
   }
 };
 
+// Signed residues for the forward transforms of primes p < 2^28 (8p < 2^31).
+// A register holds the bit pattern of an int32 x with |x| < b*p standing for
+// x mod p; sums and differences are one instruction each.  mul is a signed
+// REDC: for |a| * w < 2^31 * p (w >= 0) the product a*w (v_mad_i64_i32),
+// m = lo * ninv and a*w + m*p (v_mad_i64_i32 again, m read as signed) is a
+// multiple of 2^32 below 2^32 * p in magnitude, so its high word u has
+// |u| < p and u = a*w*2^-32 mod p.  The same three instructions as the
+// unsigned REDC, but differences need no bias and sums no reduction.
+struct ArithS {
+  uint32_t p, ninv, one;
+  __device__ __forceinline__ explicit ArithS(const FieldDev& f) : p(f.p), ninv(f.ninv), one(f.one) {}
+  __device__ __forceinline__ uint32_t mul(uint32_t a, uint32_t w) const {
+    const int64_t t = (int64_t)(int32_t)a * (int64_t)(int32_t)w;
+    const uint32_t m = (uint32_t)t * ninv;
+    return (uint32_t)((uint64_t)(t + (int64_t)(int32_t)m * (int64_t)(int32_t)p) >> 32);
+  }
+};
+
 // Full reduction for arbitrary odd p < 2^31 and t < p*2^32 (Garner, scalars):
 // (t + m*p) < 2^64, the quotient is < 2p, one conditional subtraction.
 __device__ __forceinline__ uint32_t redc_full(uint64_t t, uint32_t p, uint32_t ninv) {

@@ -306,6 +306,9 @@ static int g_device = 0;
 
 // ---- launch dispatch over the compile-time shapes of ntt2.hpp ----
 
+// Forward passes on signed residues (ArithS) need 8p < 2^31.
+static bool signed_ok(uint32_t p) { return p < (1u << 28); }
+
 // A row launch: njobs jobs from the device array dj, or (dj null) the njobs
 // by-value jobs j[0 .. njobs).
 template <int LOGM, bool LAZY, int MODE>
@@ -313,12 +316,15 @@ static void rows2_launch(Context& c, const RowJob2* j, uint32_t nrows, uint32_t 
   using RS = RowShape<LOGM>;
   const size_t lds = (size_t)RS::U * RS::lds_words_per_row * 4;
   // Many whole rows of 2^14 words, forward: the persistent prefetching pass.
-  if constexpr (LOGM == kDmaRowLog && MODE == R2M_FWD) {
+  if constexpr (LOGM == kDmaRowLog && (MODE == R2M_FWD || MODE == R2M_FWDS)) {
+    // 16-byte row loads (LDS DMA) and stores need 16-byte aligned rows.
     if (!dj && njobs == 1 && c.row_dma && !(j[0].flags & R2_LIFT) && nrows >= 2u * (uint32_t)c.cus &&
-        (size_t)j[0].in_a >= ((size_t)nrows << LOGM)) {
+        (size_t)j[0].in_a >= ((size_t)nrows << LOGM) && ((uintptr_t)j[0].src_a & 15) == 0 &&
+        ((uintptr_t)j[0].dst & 15) == 0) {
       RowJob2 jj = j[0];
       jj.nrows = nrows;
-      hipLaunchKernelGGL((k_rows_fwd_dma<LAZY>), dim3((uint32_t)c.cus), dim3(RS::T), 0, c.stream, jj);
+      hipLaunchKernelGGL((k_rows_fwd_dma<LAZY, MODE == R2M_FWDS>), dim3((uint32_t)c.cus), dim3(RS::T), 0, c.stream,
+                         jj);
       return;
     }
   }
@@ -341,7 +347,18 @@ static void rows2_launch(Context& c, const RowJob2* j, uint32_t nrows, uint32_t 
 template <int LOGM, bool LAZY>
 static void rows2_mode(Context& c, int mode, const RowJob2* j, uint32_t nrows, uint32_t njobs, const RowJob2* dj) {
   switch (mode) {
-    case R2M_FWD: rows2_launch<LOGM, LAZY, R2M_FWD>(c, j, nrows, njobs, dj); break;
+    case R2M_FWD:
+      if constexpr (LAZY) {
+        // Signed residues when every by-value job's prime allows them.
+        bool sgn = !dj;
+        for (uint32_t i = 0; sgn && i < njobs; ++i) sgn = signed_ok(j[i].f.p);
+        if (sgn) {
+          rows2_launch<LOGM, LAZY, R2M_FWDS>(c, j, nrows, njobs, dj);
+          break;
+        }
+      }
+      rows2_launch<LOGM, LAZY, R2M_FWD>(c, j, nrows, njobs, dj);
+      break;
     case R2M_INV: rows2_launch<LOGM, LAZY, R2M_INV>(c, j, nrows, njobs, dj); break;
     case R2M_CONV: rows2_launch<LOGM, LAZY, R2M_CONV>(c, j, nrows, njobs, dj); break;
     case R2M_SPEC: rows2_launch<LOGM, LAZY, R2M_SPEC>(c, j, nrows, njobs, dj); break;
@@ -425,31 +442,31 @@ static int tile_log_cols(int S, int L) {
 
 // njobs jobs from the device array dj, or (dj null) the njobs by-value jobs
 // j[0 .. njobs).
-template <int S, int LOGC, bool LAZY, bool INV>
+template <int S, int LOGC, bool LAZY, bool INV, bool SGN>
 static void cols2_launch_c(Context& c, const ColJob2* j, const PassPlan& P, uint32_t njobs, const ColJob2* dj) {
   using CS = ColShape<S, LOGC>;
   const size_t lds = (size_t)CS::words * 4;
   JobPack<ColJob2, kColPack> pack;
   for (int i = 0; i < kColPack; ++i) pack.j[i] = j[(!dj && (uint32_t)i < njobs) ? i : 0];
   dim3 grid(1u << (P.b - LOGC), njobs), block(CS::T);
-  hipLaunchKernelGGL((k_cols2<S, LOGC, LAZY, INV>), grid, block, lds, c.stream, pack, dj, P.L);
+  hipLaunchKernelGGL((k_cols2<S, LOGC, LAZY, INV, SGN>), grid, block, lds, c.stream, pack, dj, P.L);
 }
-template <int S, bool LAZY, bool INV>
+template <int S, bool LAZY, bool INV, bool SGN>
 static void cols2_launch(Context& c, const ColJob2* j, const PassPlan& P, uint32_t njobs, const ColJob2* dj) {
   if constexpr (S == 10) {
     if (P.logC == 3) {
-      cols2_launch_c<S, 3, LAZY, INV>(c, j, P, njobs, dj);
+      cols2_launch_c<S, 3, LAZY, INV, SGN>(c, j, P, njobs, dj);
       return;
     }
   }
-  cols2_launch_c<S, col_tile_log(S), LAZY, INV>(c, j, P, njobs, dj);
+  cols2_launch_c<S, col_tile_log(S), LAZY, INV, SGN>(c, j, P, njobs, dj);
 }
 
-template <bool LAZY, bool INV>
+template <bool LAZY, bool INV, bool SGN = false>
 static void cols2_size(Context& c, const ColJob2* j, const PassPlan& P, uint32_t njobs, const ColJob2* dj) {
   switch (P.S) {
 #define FPXG_C(S) \
-  case S: cols2_launch<S, LAZY, INV>(c, j, P, njobs, dj); break;
+  case S: cols2_launch<S, LAZY, INV, SGN>(c, j, P, njobs, dj); break;
     FPXG_C(1) FPXG_C(2) FPXG_C(3) FPXG_C(4) FPXG_C(5) FPXG_C(6)
     FPXG_C(7) FPXG_C(8) FPXG_C(9) FPXG_C(10) FPXG_C(11) FPXG_C(12)
 #undef FPXG_C
@@ -467,8 +484,13 @@ static void launch_cols_n(Context& c, bool lazy, bool inv, const ColJob2* j, con
     for (uint32_t i = 0; i < njobs; ++i) c.rec->push_cols(P.S, lazy, inv, P.L, j[i], i > 0);
     return;
   }
+  // Signed residues when every by-value job's prime allows them (a device
+  // job array may mix primes).
+  bool sgn = lazy && !inv && !dj;
+  for (uint32_t i = 0; sgn && i < njobs; ++i) sgn = signed_ok(j[i].f.p);
   if (lazy) {
     if (inv) cols2_size<true, true>(c, j, P, njobs, dj);
+    else if (sgn) cols2_size<true, false, true>(c, j, P, njobs, dj);
     else cols2_size<true, false>(c, j, P, njobs, dj);
   } else {
     if (inv) cols2_size<false, true>(c, j, P, njobs, dj);
@@ -493,10 +515,10 @@ static bool rows2_attr() {
   return hipFuncSetAttribute((const void*)k_rows2<LOGM, LAZY, MODE>, hipFuncAttributeMaxDynamicSharedMemorySize,
                              (int)(RS::U * RS::lds_words_per_row * 4)) == hipSuccess;
 }
-template <int S, bool LAZY, bool INV>
+template <int S, bool LAZY, bool INV, bool SGN = false>
 static bool cols2_attr() {
   constexpr int LOGC = col_tile_log(S);
-  return hipFuncSetAttribute((const void*)k_cols2<S, LOGC, LAZY, INV>, hipFuncAttributeMaxDynamicSharedMemorySize,
+  return hipFuncSetAttribute((const void*)k_cols2<S, LOGC, LAZY, INV, SGN>, hipFuncAttributeMaxDynamicSharedMemorySize,
                              (int)(ColShape<S, LOGC>::words * 4)) == hipSuccess;
 }
 
@@ -515,8 +537,9 @@ static void context_init(Context& c) {
   X(13, R2M_FWD) X(13, R2M_INV) X(13, R2M_CONV) X(13, R2M_SPEC) X(13, R2M_SPEC_ONLY)
   X(14, R2M_FWD) X(14, R2M_INV) X(14, R2M_CONV) X(14, R2M_SPEC) X(14, R2M_SPEC_ONLY)
 #undef X
+  big = rows2_attr<13, true, R2M_FWDS>() && rows2_attr<14, true, R2M_FWDS>() && big;
 #define X(S) big = cols2_attr<S, true, false>() && cols2_attr<S, true, true>() && cols2_attr<S, false, false>() && \
-                   cols2_attr<S, false, true>() && big;
+                   cols2_attr<S, false, true>() && cols2_attr<S, true, false, true>() && big;
   X(9) X(10) X(11) X(12)
 #undef X
   (void)hipGetLastError();
@@ -1257,6 +1280,13 @@ int fpxg_set_option(int option, int64_t value) {
           drop_graphs(c);
           c.graph_seen.clear();
         }
+        return;
+      case FPXG_OPT_ROW_DMA:
+        // Captured graphs hold the old choice: drop them.
+        FPXG_HIP(hipStreamSynchronize(c.stream));
+        c.row_dma = value != 0;
+        drop_graphs(c);
+        c.graph_seen.clear();
         return;
       default:
         throw Error(E_ARG, "unknown option");

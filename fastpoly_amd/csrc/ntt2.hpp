@@ -83,6 +83,60 @@ __device__ __forceinline__ void dif_block(uint32_t* x, const uint32_t* tw, const
   }
 }
 
+// dif_block on signed residues (ArithS, p < 2^28).  Every register's bound b
+// (|x| < b*p, b in {1, 2, 4, 8}) is a compile-time constant after unrolling:
+// inputs have b = 2 (any value in [0, 2p)), a sum or difference has 2b, a
+// product 1.  A pair with b = 8 is first brought to b = 1 by a product with
+// the Montgomery one, so no sum or difference reaches 16p > 2^31 and every
+// product's factor stays below 2^31.  A butterfly is 5 instructions (add,
+// subtract, product) against dif_block's 8.  Outputs have b <= BOUT.
+template <int r, bool LO0, int BOUT>
+__device__ __forceinline__ void dif_block_s(uint32_t* x, const uint32_t* tw, const ArithS& S) {
+  int bd[1 << r];
+#pragma unroll
+  for (int k = 0; k < (1 << r); ++k) bd[k] = 2;
+  int idx = 0;
+#pragma unroll
+  for (int q = 0; q < r; ++q) {
+    const int h = 1 << (r - q - 1);
+#pragma unroll
+    for (int k = 0; k < (1 << r); ++k) {
+      if (k & h) continue;
+      int b = bd[k];  // == bd[k + h]: both come through the same sums and products
+      uint32_t a0 = x[k], a1 = x[k + h];
+      if (b > 4) {
+        a0 = S.mul(a0, S.one);
+        a1 = S.mul(a1, S.one);
+        b = 1;
+      }
+      x[k] = a0 + a1;
+      if (LO0 && (k & (h - 1)) == 0) {
+        x[k + h] = a0 - a1;
+        bd[k + h] = 2 * b;
+      } else {
+        x[k + h] = S.mul(a0 - a1, tw[idx + (k & (h - 1))]);
+        bd[k + h] = 1;
+      }
+      bd[k] = 2 * b;
+    }
+    idx += h;
+  }
+#pragma unroll
+  for (int k = 0; k < (1 << r); ++k)
+    if (bd[k] > BOUT) x[k] = S.mul(x[k], S.one);
+}
+
+// One forward group with either arithmetic: unsigned lazy/canonical limbs or
+// signed residues (outputs bounded by BOUT * p).
+template <int r, bool LO0, int BOUT, bool LAZY>
+__device__ __forceinline__ void dif_any(uint32_t* x, const uint32_t* tw, const Arith<LAZY>& A) {
+  dif_block<r, LAZY, LO0>(x, tw, A);
+}
+template <int r, bool LO0, int BOUT, bool LAZY>
+__device__ __forceinline__ void dif_any(uint32_t* x, const uint32_t* tw, const ArithS& S) {
+  dif_block_s<r, LO0, BOUT>(x, tw, S);
+}
+
 template <int r, bool LAZY, bool LO0 = false>
 __device__ __forceinline__ void dit_block(uint32_t* x, const uint32_t* tw, const Arith<LAZY>& A) {
 #pragma unroll
@@ -180,9 +234,10 @@ __device__ __forceinline__ void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// Compute phase P on registers (forward or inverse).
-template <int LOGB, int P, bool INV, bool LAZY, int LOGC = 0>
-__device__ __forceinline__ void phase_compute(uint32_t* x, uint32_t g, const uint32_t* planes, const Arith<LAZY>& A) {
+// Compute phase P on registers (forward or inverse).  AR is Arith<LAZY>, or
+// ArithS for a forward phase on signed residues with outputs below BOUT * p.
+template <int LOGB, int P, bool INV, bool LAZY, int LOGC = 0, int BOUT = 2, class AR = Arith<LAZY>>
+__device__ __forceinline__ void phase_compute(uint32_t* x, uint32_t g, const uint32_t* planes, const AR& A) {
   constexpr int r = Sched<LOGB>::r_of(P);
   constexpr int NG = PhaseGroups<LOGB, P, LOGC>::NG;
   const uint32_t* plane = planes + plane_off(LOGB - Sched<LOGB>::s_of(P), r);
@@ -192,7 +247,7 @@ __device__ __forceinline__ void phase_compute(uint32_t* x, uint32_t g, const uin
     load_tw<r>(tw, plane, group_lo<LOGB, P>(PhaseGroups<LOGB, P, LOGC>::gid(g, u)));
     constexpr bool lo0 = LOGB - Sched<LOGB>::s_of(P) - r == 0;
     if constexpr (INV) dit_block<r, LAZY, lo0>(x + u * (1 << r), tw, A);
-    else dif_block<r, LAZY, lo0>(x + u * (1 << r), tw, A);
+    else dif_any<r, lo0, BOUT, LAZY>(x + u * (1 << r), tw, A);
   }
 }
 
@@ -260,10 +315,12 @@ __device__ __forceinline__ void phase_from_lds(uint32_t* x, const uint32_t* X, u
 
 // Phases [P, END) of a forward transform whose registers already hold phase
 // P-1's output (P >= 1): each step writes LDS, syncs, reads phase P's groups
-// and computes.  All threads of the workgroup must call it.
-template <int LOGB, int P, int END, int LOGC, bool LAZY>
+// and computes.  All threads of the workgroup must call it.  With signed
+// residues (AR = ArithS) phases hand over values below 2p in magnitude and
+// the last one's outputs are below BLAST * p.
+template <int LOGB, int P, int END, int LOGC, bool LAZY, int BLAST = 2, class AR = Arith<LAZY>>
 __device__ __forceinline__ void fwd_phases(uint32_t* x, uint32_t* X, uint32_t g, uint32_t c, const uint32_t* planes,
-                                           const Arith<LAZY>& A) {
+                                           const AR& A) {
   if constexpr (P < END) {
     if constexpr (WaveLocal<LOGB, P, LOGC>::value) {
       wave_sync();
@@ -276,8 +333,8 @@ __device__ __forceinline__ void fwd_phases(uint32_t* x, uint32_t* X, uint32_t g,
       __syncthreads();
       phase_from_lds<LOGB, P, LOGC>(x, X, g, c);
     }
-    phase_compute<LOGB, P, false, LAZY, LOGC>(x, g, planes, A);
-    fwd_phases<LOGB, P + 1, END, LOGC, LAZY>(x, X, g, c, planes, A);
+    phase_compute<LOGB, P, false, LAZY, LOGC, P + 1 == END ? BLAST : 2>(x, g, planes, A);
+    fwd_phases<LOGB, P + 1, END, LOGC, LAZY, BLAST>(x, X, g, c, planes, A);
   }
 }
 // Inverse phases P, P-1, ..., 0 where registers hold phase P+1's inverse output.
@@ -375,7 +432,9 @@ __device__ __forceinline__ uint32_t out_epilogue(uint32_t flags, const uint32_t*
   return v;
 }
 enum : int { R2M_FWD = 1, R2M_INV = 2, R2M_CONV = 3, R2M_SPEC = 4, R2M_SPEC_ONLY = 5, R2M_INVSTEP = 6,
-             R2M_EXPSTEP = 7, R2M_SQRTSTEP = 8, R2M_NINV = 9, R2M_NEXP = 10, R2M_NSQRT = 11 };
+             R2M_EXPSTEP = 7, R2M_SQRTSTEP = 8, R2M_NINV = 9, R2M_NEXP = 10, R2M_NSQRT = 11, R2M_FWDS = 12 };
+// R2M_FWDS: R2M_FWD on signed residues (ArithS, p < 2^28), chosen at launch
+// time for a recorded R2M_FWD; never recorded itself.
 // R2M_NINV / R2M_NEXP / R2M_NSQRT: the first coefficients of a series in one
 // workgroup (k_newton_small), recorded and merged like the row modes.
 // R2M_INVSTEP (one Newton step of the power-series inverse, inverse.hpp:18-65,
@@ -408,7 +467,7 @@ enum : int { R2M_FWD = 1, R2M_INV = 2, R2M_CONV = 3, R2M_SPEC = 4, R2M_SPEC_ONLY
 #endif
 template <int MODE>
 struct RowWaves {
-  static constexpr int value = (MODE == R2M_FWD || MODE == R2M_INV) ? FPXG_ROW_WPE
+  static constexpr int value = (MODE == R2M_FWD || MODE == R2M_FWDS || MODE == R2M_INV) ? FPXG_ROW_WPE
                                : (MODE == R2M_INVSTEP || MODE == R2M_EXPSTEP || MODE == R2M_SQRTSTEP) ? 2
                                                                                              : FPXG_ROW_WPE_FUSED;
 };
@@ -490,9 +549,17 @@ k_rows2(JobPack<RowJob2, kRowPack> jp, const RowJob2* __restrict__ jobs) {
   };
   // Forward transform of registers holding phase 0's input, ending with phase
   // NPH-1's groups in regs; `forward` loads the input first.
+  static_assert(MODE != R2M_FWDS || LAZY, "signed residues: lazy primes only");
+  const ArithS Sg(J.f);
   auto forward_regs = [&](uint32_t* regs) {
-    phase_compute<LOGM, 0, false, LAZY>(regs, g, J.f.planes_fwd, A);
-    fwd_phases<LOGM, 1, NPH, 0, LAZY>(regs, X, g, 0u, J.f.planes_fwd, A);
+    if constexpr (MODE == R2M_FWDS) {
+      // outputs below 4p in magnitude; the store brings them to [0, 2p)
+      phase_compute<LOGM, 0, false, LAZY, 0, NPH == 1 ? 4 : 2>(regs, g, J.f.planes_fwd, Sg);
+      fwd_phases<LOGM, 1, NPH, 0, LAZY, 4>(regs, X, g, 0u, J.f.planes_fwd, Sg);
+    } else {
+      phase_compute<LOGM, 0, false, LAZY>(regs, g, J.f.planes_fwd, A);
+      fwd_phases<LOGM, 1, NPH, 0, LAZY>(regs, X, g, 0u, J.f.planes_fwd, A);
+    }
   };
   auto forward = [&](const uint32_t* src, uint32_t in_count, uint32_t* regs) {
     load_first(src, in_count, regs);
@@ -505,14 +572,14 @@ k_rows2(JobPack<RowJob2, kRowPack> jp, const RowJob2* __restrict__ jobs) {
 
   // Canonical outputs: min(v, v - p) with p, or a no-op min(v, v) with 0.
   const uint32_t canon_p = (flags & R2_CANON) ? J.f.p : 0u;
-  if constexpr (MODE == R2M_FWD) {
+  if constexpr (MODE == R2M_FWD || MODE == R2M_FWDS) {
     forward(J.src_a, J.in_a, x);
 #pragma unroll
     for (int u = 0; u < NGL; ++u)
 #pragma unroll
       for (int k = 0; k < (1 << rl); ++k) {
         const uint32_t gi = base + elem<LOGM, PL>(PhaseGroups<LOGM, PL>::gid(g, u), k);
-        const uint32_t v = x[u * (1 << rl) + k];
+        const uint32_t v = MODE == R2M_FWDS ? Sg.mul(x[u * (1 << rl) + k], Sg.one) + Sg.p : x[u * (1 << rl) + k];
         if (active) gstore(J.dst + gi, min(v, v - canon_p));
       }
     return;
@@ -727,20 +794,23 @@ __device__ __forceinline__ void load_phase_tw(uint32_t* T, const uint32_t* plane
   for (int u = 0; u < PhaseGroups<LOGB, P>::NG; ++u)
     load_tw<r>(T + u * (1 << r), plane, group_lo<LOGB, P>(PhaseGroups<LOGB, P>::gid(g, u)));
 }
-template <int LOGB, int P, bool LAZY>
-__device__ __forceinline__ void compute_with(uint32_t* x, const uint32_t* T, const Arith<LAZY>& A) {
+template <int LOGB, int P, bool LAZY, int BOUT = 2, class AR = Arith<LAZY>>
+__device__ __forceinline__ void compute_with(uint32_t* x, const uint32_t* T, const AR& A) {
   constexpr int r = Sched<LOGB>::r_of(P);
   constexpr bool lo0 = LOGB - Sched<LOGB>::s_of(P) - r == 0;
 #pragma unroll
-  for (int u = 0; u < PhaseGroups<LOGB, P>::NG; ++u) dif_block<r, LAZY, lo0>(x + u * (1 << r), T + u * (1 << r), A);
+  for (int u = 0; u < PhaseGroups<LOGB, P>::NG; ++u) dif_any<r, lo0, BOUT, LAZY>(x + u * (1 << r), T + u * (1 << r), A);
 }
 
 constexpr int kDmaRowLog = 14;
 
-template <bool LAZY>
+// SGN: signed residues (ArithS) for p < 2^28; the row's values are converted
+// back to [0, 2p) by one product with the Montgomery one and + p.
+template <bool LAZY, bool SGN>
 __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(4)))
 k_rows_fwd_dma(RowJob2 j0) {
   constexpr int LOGM = kDmaRowLog;
+  static_assert(!SGN || LAZY, "signed residues: lazy primes only");
   using SC = Sched<LOGM>;
   static_assert(SC::nph == 4 && SC::r_of(0) == 4 && SC::r_of(1) == 2 && SC::r_of(2) == 4 && SC::r_of(3) == 4,
                 "phase shape of a 2^14 row");
@@ -751,6 +821,12 @@ k_rows_fwd_dma(RowJob2 j0) {
   __shared__ uint32_t LE[lds_words(1u << LOGM)];  // phase exchanges, padded
   const RowJob2 J = load_job(j0, (const RowJob2*)nullptr);
   const Arith<LAZY> A(J.f);
+  const ArithS Sg(J.f);
+  auto comp = [&](auto ph, uint32_t* x, const uint32_t* T) {
+    constexpr int P = decltype(ph)::value;
+    if constexpr (SGN) compute_with<LOGM, P, LAZY, P == 3 ? 4 : 2>(x, T, Sg);
+    else compute_with<LOGM, P, LAZY>(x, T, A);
+  };
   const uint32_t g = threadIdx.x;
   const uint32_t wv = __builtin_amdgcn_readfirstlane(g >> 6), ln = g & 63;
   const uint32_t canon_p = (J.flags & R2_CANON) ? J.f.p : 0u;
@@ -777,7 +853,7 @@ k_rows_fwd_dma(RowJob2 j0) {
     uint32_t x[16];
 #pragma unroll
     for (int k = 0; k < 16; ++k) x[k] = LA[elem<LOGM, 0>(g, k)];
-    compute_with<LOGM, 0, LAZY>(x, T0, A);
+    comp(std::integral_constant<int, 0>(), x, T0);
     const uint32_t next = row + gridDim.x;
     lds_barrier();  // LA consumed by every wave; the previous row's LE reads are done
     // Always issued (the last row re-reads a valid row) so the code stays straight-line.
@@ -785,22 +861,34 @@ k_rows_fwd_dma(RowJob2 j0) {
     phase_to_lds<LOGM, 0, 0>(LE, x, g, 0);
     lds_barrier();
     phase_from_lds<LOGM, 1, 0>(x, LE, g, 0);
-    compute_with<LOGM, 1, LAZY>(x, T1, A);
+    comp(std::integral_constant<int, 1>(), x, T1);
     wave_sync();
     phase_to_lds<LOGM, 1, 0>(LE, x, g, 0);
     wave_sync();
     phase_from_lds<LOGM, 2, 0>(x, LE, g, 0);
-    compute_with<LOGM, 2, LAZY>(x, T2, A);
+    comp(std::integral_constant<int, 2>(), x, T2);
     wave_sync();
     phase_to_lds<LOGM, 2, 0>(LE, x, g, 0);
     wave_sync();
     phase_from_lds<LOGM, 3, 0>(x, LE, g, 0);
-    compute_with<LOGM, 3, LAZY>(x, T3, A);
-    wait_vmem();  // the prefetch of the next row and this wave's previous stores
+    comp(std::integral_constant<int, 3>(), x, T3);
+    // Coalesced stores: the wave's outputs are the 1024 consecutive words
+    // elem = g * 16 + k of its own phase-3 slice of LE (the 2 -> 3 exchange
+    // stays inside the wave), so they go back through that slice and leave
+    // as 16 stores of 64 consecutive words.
+    static_assert(LOGM - SC::s_of(3) - SC::r_of(3) == 0, "last phase groups are contiguous");
+    wave_sync();
 #pragma unroll
     for (int k = 0; k < 16; ++k) {
-      const uint32_t v = x[k];
-      gstore(J.dst + ((size_t)row << LOGM) + elem<LOGM, 3>(g, k), min(v, v - canon_p));
+      const uint32_t v = SGN ? Sg.mul(x[k], Sg.one) + Sg.p : x[k];
+      LE[pidx(elem<LOGM, 3>(g, k))] = min(v, v - canon_p);
+    }
+    wave_sync();
+    wait_vmem();  // the prefetch of the next row and this wave's previous stores
+#pragma unroll
+    for (int j = 0; j < 16; ++j) {
+      const uint32_t e = wv * 1024 + j * 64 + ln;
+      gstore(J.dst + ((size_t)row << LOGM) + e, LE[pidx(e)]);
     }
     row = next;
     if (row >= J.nrows) break;
@@ -1064,10 +1152,14 @@ struct ColShape {
   static constexpr uint32_t words = lds_words((1u << S) << LOGC);
 };
 
-template <int S, int LOGC, bool LAZY, bool INV>
+// SGN (forward, p < 2^28): the phases on signed residues (ArithS); the
+// four-step twiddle product (factor < 2p, values below 4p in magnitude)
+// brings them back, + p gives the usual [0, 2p).
+template <int S, int LOGC, bool LAZY, bool INV, bool SGN = false>
 __global__ void __launch_bounds__((ColShape<S, LOGC>::T)) __attribute__((amdgpu_waves_per_eu(FPXG_COL_WPE)))
 k_cols2(JobPack<ColJob2, kColPack> jp, const ColJob2* __restrict__ jobs, int L) {
   using SC = Sched<S>;
+  static_assert(!SGN || (LAZY && !INV), "signed residues: lazy forward passes only");
   constexpr int NPH = SC::nph;
   constexpr int C = 1 << LOGC;
   extern __shared__ uint32_t X[];
@@ -1110,8 +1202,14 @@ k_cols2(JobPack<ColJob2, kColPack> jp, const ColJob2* __restrict__ jobs, int L) 
         for (int k = 0; k < (1 << r0); ++k)
           if (elem<S, 0>(PhaseGroups<S, 0, LOGC>::gid(g, u), k) * M + col >= J.in_count) x[u * (1 << r0) + k] = 0u;
     }
-    phase_compute<S, 0, false, LAZY, LOGC>(x, g, J.f.planes_fwd, A);
-    fwd_phases<S, 1, NPH, LOGC, LAZY>(x, X, g, c, J.f.planes_fwd, A);
+    const ArithS Sg(J.f);
+    if constexpr (SGN) {
+      phase_compute<S, 0, false, LAZY, LOGC, NPH == 1 ? 4 : 2>(x, g, J.f.planes_fwd, Sg);
+      fwd_phases<S, 1, NPH, LOGC, LAZY, 4>(x, X, g, c, J.f.planes_fwd, Sg);
+    } else {
+      phase_compute<S, 0, false, LAZY, LOGC>(x, g, J.f.planes_fwd, A);
+      fwd_phases<S, 1, NPH, LOGC, LAZY>(x, X, g, c, J.f.planes_fwd, A);
+    }
     // Four-step twiddle w^(col * bitrev_S(e)).  In the last phase e = gid *
     // 2^rl + k, so the exponent is col * bitrev_(S-rl)(gid) + j * col *
     // 2^(S-rl) with j = bitrev_rl(k): one table lookup for the group's first
@@ -1126,7 +1224,7 @@ k_cols2(JobPack<ColJob2, kColPack> jp, const ColJob2* __restrict__ jobs, int L) 
       for (int j = 0; j < (1 << rl); ++j) {
         const int k = (int)bitrev_c(j, rl);
         const uint32_t e = elem<S, PL>(gid, k);
-        const uint32_t v = A.mul(x[u * (1 << rl) + k], q);
+        const uint32_t v = SGN ? Sg.mul(x[u * (1 << rl) + k], q) + Sg.p : A.mul(x[u * (1 << rl) + k], q);
         gstore(J.dst + e * M + col, min(v, v - canon_p));
         if (j + 1 < (1 << rl)) q = A.mul(q, D);
       }

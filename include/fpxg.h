@@ -122,8 +122,11 @@ int fpxg_product_plan_info(uint32_t n, int* row_log, int* col_levels);
 
 /* Library options (no counterpart in the reference).
  *   FPXG_OPT_GRAPHS: 1 (default) replays repeated single series calls from
- *   captured hipGraphs; 0 drops the cached graphs and launches every call. */
-enum { FPXG_OPT_GRAPHS = 0 };
+ *   captured hipGraphs; 0 drops the cached graphs and launches every call.
+ *   FPXG_OPT_ROW_DMA: 1 (default) runs the forward row pass over many
+ *   2^14-word rows as one persistent prefetching workgroup per CU; 0 runs one
+ *   workgroup per row.  Results are identical either way. */
+enum { FPXG_OPT_GRAPHS = 0, FPXG_OPT_ROW_DMA = 1 };
 int fpxg_set_option(int option, int64_t value);
 
 /* Device memory and stream plumbing for the *_dev entry points. */

@@ -56,6 +56,38 @@ def test_ntt_largest_sizes(fa, oracle, p, lg):
     assert np.array_equal(fa.ntt_inverse(p, want, root=root), x)
 
 
+@pytest.mark.parametrize("p,lg", [(P, 23), (167772161, 24)])
+def test_ntt_row_kernel_option(fa, oracle, p, lg):
+    """The per-row forward kernel (persistent row pass switched off) gives
+    the same limbs as the oracle at the largest sizes."""
+    n = 1 << lg
+    root = oracle.lib().fpo_primitive_root(p)
+    x = _rnd(oracle, p, n, 7 + lg)
+    want = oracle.ntt(p, x, root=root)
+    fa.set_row_dma(False)
+    try:
+        assert np.array_equal(fa.ntt_forward(p, x, root=root), want)
+    finally:
+        fa.set_row_dma(True)
+    assert np.array_equal(fa.ntt_forward(p, x, root=root), want)
+
+
+@pytest.mark.parametrize("p,lg", [(167772161, 16), (167772161, 20), (167772161, 24), (469762049, 20)])
+def test_ntt_extreme_inputs(fa, oracle, p, lg):
+    """Inputs at the top of the range (p - 1 everywhere, p - 1 / 0
+    alternating, p - 1 on a random half) give the forward passes'
+    intermediate sums and differences their largest magnitudes."""
+    n = 1 << lg
+    root = oracle.lib().fpo_primitive_root(p)
+    rng = np.random.default_rng(lg)
+    pats = [np.full(n, p - 1, dtype=np.uint32),
+            np.tile(np.array([p - 1, 0], dtype=np.uint32), n // 2),
+            np.where(rng.random(n) < 0.5, p - 1, 0).astype(np.uint32)]
+    for i, x in enumerate(pats):
+        want = oracle.ntt(p, x, root=root)
+        assert np.array_equal(fa.ntt_forward(p, x, root=root), want), (p, lg, i)
+
+
 def test_ntt_windows(fa, oracle):
     po = oracle
     for lg in (4, 10, 14, 15, 17):
