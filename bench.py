@@ -57,7 +57,7 @@ def parse():
     ap.add_argument("--no-batch", action="store_true", help="skip the uneven-batch leg")
     ap.add_argument("--batch-convs", type=int, default=8192)
     ap.add_argument("--batch-series", type=int, default=1024)
-    ap.add_argument("--batch-reps", type=int, default=3)
+    ap.add_argument("--batch-reps", type=int, default=5)
     ap.add_argument("--no-serve", action="store_true", help="skip the serving leg")
     ap.add_argument("--ntt-only", action="store_true",
                     help="only the 2^24 transform passes and a 64 MiB device copy (for counter runs)")

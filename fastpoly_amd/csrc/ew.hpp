@@ -203,6 +203,35 @@ __device__ __forceinline__ uint32_t ew_value(const EwRec& R, uint32_t i) {
   }
 }
 
+// The ops built on 1 / (c + i) (EW_RECIP, EW_MUL_RECIP; EW_INTEGRAL with
+// c = 0) over four consecutive elements with one Fermat inverse between them
+// (Montgomery's batch inversion: prefix products, one inverse, products back
+// down).  A factor c + i = 0 mod p is left out of the products and gives 0,
+// as fs_inv(0) does.
+__device__ __forceinline__ void ew_recip4(const EwRec& R, uint32_t i0, uint32_t end) {
+  const FieldScalar& f = R.f;
+  const uint32_t c0 = R.op == EW_INTEGRAL ? 0u : R.c;
+  uint32_t x[4], pre[4], acc = f.one;
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    x[j] = i0 + j < end ? fs_from_int(f, c0 + i0 + j) : 0u;
+    pre[j] = acc;
+    if (x[j]) acc = fs_mul(f, acc, x[j]);
+  }
+  uint32_t inv = fs_inv(f, acc);
+#pragma unroll
+  for (int j = 3; j >= 0; --j) {
+    const uint32_t i = i0 + j;
+    const uint32_t r = x[j] ? fs_mul(f, inv, pre[j]) : 0u;
+    if (x[j]) inv = fs_mul(f, inv, x[j]);
+    if (i >= end) continue;
+    uint32_t v = r;
+    if (R.op == EW_MUL_RECIP) v = fs_mul(f, R.src[i], r);
+    else if (R.op == EW_INTEGRAL) v = (i >= 1 && i - 1 < R.avail) ? fs_mul(f, R.src[i - 1], r) : 0u;
+    R.dst[i] = v;
+  }
+}
+
 // Elementwise records; one launch serves any number of them through a chunk
 // table of (record, first element) pairs, kChunk elements per chunk.
 __global__ void __launch_bounds__(256) k_ew(const EwRec* __restrict__ recs, const uint2* __restrict__ chunks, EwRec r0) {
@@ -216,6 +245,10 @@ __global__ void __launch_bounds__(256) k_ew(const EwRec* __restrict__ recs, cons
   make_uniform(R);
   start = __builtin_amdgcn_readfirstlane(start);
   const uint32_t end = min(R.n, start + kChunk);
+  if (R.op == EW_RECIP || R.op == EW_MUL_RECIP || R.op == EW_INTEGRAL) {  // uniform branch
+    for (uint32_t i0 = start + 4 * threadIdx.x; i0 < end; i0 += 4 * blockDim.x) ew_recip4(R, i0, end);
+    return;
+  }
   for (uint32_t i = start + threadIdx.x; i < end; i += blockDim.x) R.dst[i] = ew_value(R, i);
 }
 

@@ -1233,7 +1233,7 @@ int fpxg_autotune(int min_log, int max_log, int reps, int* chosen_col_levels, fl
       if (best_us) best_us[L - min_log] = best;
     }
     // Fused products (two half-length operands, the whole cyclic window):
-    // one workgroup's launch while the size fits it, else every admissible split.
+    // one workgroup's launch up to 2^13 points, else every admissible split.
     DevBuf out = c.buf((size_t)1 << max_log);
     for (int L = std::max(min_log, 13); L <= max_log; ++L) {
       const HostField& F = c.field(L <= 25 ? kCrtP[0] : kCrtP[1]);
@@ -1243,9 +1243,13 @@ int fpxg_autotune(int min_log, int max_log, int reps, int* chosen_col_levels, fl
       // Splits within two column levels of the default (and one workgroup
       // while the size fits it): the product's passes are short, latency-bound
       // launches, where far-off splits only win by timing noise.
+      // One workgroup only where it is also the default (<= 2^13): the eager
+      // timing here charges each launch its host gap, which graph replay (the
+      // series calls) does not pay, and a one-workgroup 2^14 product occupies
+      // a single CU for ~25 us against ~17 us split.
       const int s_ref = def.S ? def.S : std::max(1, L / 2);
       std::vector<PassPlan> cand;
-      if (L <= c.row_log_max) cand.push_back({L, L, 0, 0});
+      if (L <= std::min(c.row_log_max, 13)) cand.push_back({L, L, 0, 0});
       for (int S = std::max(1, L - c.row_log_max); S <= std::min(12, L - 1); ++S)
         if (S >= 4 && L - S >= 4 && std::abs(S - s_ref) <= 2) cand.push_back({L, L - S, S, tile_log_cols(S, L)});
       if (cand.empty()) continue;

@@ -132,6 +132,8 @@ def test_autotune_keeps_results_exact(fa, oracle):
         pp = fa.product_plan_info(1 << lg)
         assert pp["row_log"] + pp["col_levels"] == lg and pp["row_log"] <= 14, (lg, pp)
         assert pp["col_levels"] == 0 or pp["col_levels"] >= 4, (lg, pp)
+        # one workgroup's product (no column levels) only up to 2^13 points
+        assert lg <= 13 or pp["col_levels"] >= 4, (lg, pp)
     a = oracle.MT(5).rnd(P, 70000)
     b = oracle.MT(6).rnd(P, 50001)
     assert np.array_equal(fa.conv(P, a, b), oracle.conv(P, a, b))
