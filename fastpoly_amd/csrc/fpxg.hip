@@ -330,7 +330,7 @@ static void rows2_launch(Context& c, const RowJob2* j, uint32_t nrows, uint32_t 
   }
   if constexpr (MODE == R2M_NINV || MODE == R2M_NEXP || MODE == R2M_NSQRT) {
     if (!dj && njobs != 1) throw Error(E_DEVICE, "small-series launches take one job by value");
-    hipLaunchKernelGGL((k_newton_small<MODE>), dim3(1, njobs), dim3(kSmallSeries), 0, c.stream, j[0], dj);
+    hipLaunchKernelGGL((k_newton_small<MODE>), dim3(1, njobs), dim3(kSmallThreads), 0, c.stream, j[0], dj);
   } else if constexpr ((MODE == R2M_INVSTEP || MODE == R2M_EXPSTEP || MODE == R2M_SQRTSTEP) && LOGM <= 9) {
     // Small Newton steps: their window products computed directly (one job
     // by value, or a device job array).

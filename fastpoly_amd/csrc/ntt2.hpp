@@ -1001,25 +1001,30 @@ __global__ void __launch_bounds__(2 << LM) k_step_direct(RowJob2 j0, const RowJo
 //   R2M_NSQRT: b = sqrt(a) mod x^N with b[0] = aux; src_a = a (in_a words),
 //              inv_n = 1/aux, dst = b; src_c (an output) <- c = 1/b mod x^(N/2).
 // Each doubling keeps c = 1/b to the precision of the next step's half
-// (extend_c, the inverse step with b in place of a).  512 threads, two an
-// output of each product.
+// (extend_c, the inverse step with b in place of a).  kSmallThreads =
+// 1024 threads, four an output of each product (each product has at most
+// N/2 outputs).
 // ---------------------------------------------------------------------------
 constexpr uint32_t kSmallSeries = 512;
 constexpr int kSmallSeriesLog = 9;  // the size key its launches are recorded under
+constexpr uint32_t kSmallTPO = 4;   // threads an output
+constexpr uint32_t kSmallThreads = kSmallSeries / 2 * kSmallTPO;
 
 // sum_j X[j] Y[o + off - j] over 0 <= j < nx, 0 <= o + off - j < ny, for an
-// output o < h: the half over j of parity `par`, added to the partner lane's.
+// output o < h: the terms with j = par mod kSmallTPO, added to those of the
+// output's other kSmallTPO - 1 lanes (adjacent lanes of one wave).
 __device__ __forceinline__ uint32_t mid_product(const uint32_t* X, uint32_t nx, const uint32_t* Y, uint32_t ny,
                                                 uint32_t off, uint32_t o, uint32_t par, uint32_t h, uint32_t p,
                                                 uint32_t ninv) {
+  constexpr uint32_t D = kSmallTPO;
   uint32_t acc = 0;
   const uint32_t s = o + off;
   if (o < h && nx && ny) {
     const uint32_t jlo = s >= ny ? s - (ny - 1) : 0u;
     const uint32_t jhi = min(nx - 1, s);  // inclusive
-    uint32_t j = jlo + ((jlo & 1) != par ? 1u : 0u);
-    for (; j + 2 <= jhi; j += 4) {
-      const uint32_t r = redc_full((uint64_t)X[j] * Y[s - j] + (uint64_t)X[j + 2] * Y[s - j - 2], p, ninv);
+    uint32_t j = jlo + ((par - jlo) & (D - 1));  // the first j >= jlo with j = par mod D
+    for (; j + D <= jhi; j += 2 * D) {
+      const uint32_t r = redc_full((uint64_t)X[j] * Y[s - j] + (uint64_t)X[j + D] * Y[s - j - D], p, ninv);
       const uint32_t z = acc + r;
       acc = z >= p ? z - p : z;
     }
@@ -1028,18 +1033,22 @@ __device__ __forceinline__ uint32_t mid_product(const uint32_t* X, uint32_t nx, 
       acc = z >= p ? z - p : z;
     }
   }
-  const uint32_t z = acc + (uint32_t)__shfl_xor((int)acc, 1, 64);
-  return z >= p ? z - p : z;
+#pragma unroll
+  for (uint32_t d = 1; d < D; d <<= 1) {
+    const uint32_t z = acc + (uint32_t)__shfl_xor((int)acc, (int)d, 64);
+    acc = z >= p ? z - p : z;
+  }
+  return acc;
 }
 
 template <int MODE>
-__global__ void __launch_bounds__(kSmallSeries) k_newton_small(RowJob2 j0, const RowJob2* __restrict__ jobs) {
+__global__ void __launch_bounds__(kSmallThreads) k_newton_small(RowJob2 j0, const RowJob2* __restrict__ jobs) {
   constexpr uint32_t NM = kSmallSeries;
   const RowJob2 J = load_job(j0, jobs);
   __shared__ uint32_t A[NM], B[NM], C[NM], E[NM / 2];
   const uint32_t t = threadIdx.x, p = J.f.p, ninv = J.f.ninv;
   const uint32_t N = min(J.out_hi, NM), na = min(J.in_a, N);
-  const uint32_t o = t >> 1, par = t & 1;
+  const uint32_t o = t / kSmallTPO, par = t % kSmallTPO;
   for (uint32_t i = t; i < NM; i += NM) {
     A[i] = i < na ? J.src_a[i] : 0u;
     B[i] = 0u;
