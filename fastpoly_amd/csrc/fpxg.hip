@@ -335,7 +335,8 @@ static void rows2_launch(Context& c, const RowJob2* j, uint32_t nrows, uint32_t 
     // Small Newton steps: their window products computed directly (one job
     // by value, or a device job array).
     if (!dj && njobs != 1) throw Error(E_DEVICE, "direct Newton steps take one job by value");
-    hipLaunchKernelGGL((k_step_direct<MODE, LOGM - 1>), dim3(1, njobs), dim3(1u << LOGM), 0, c.stream, j[0], dj);
+    hipLaunchKernelGGL((k_step_direct<MODE, LOGM - 1>), dim3(1, njobs), dim3(kMidTPO << (LOGM - 1)), 0, c.stream,
+                       j[0], dj);
   } else {
     JobPack<RowJob2, kRowPack> pack;
     for (int i = 0; i < kRowPack; ++i) pack.j[i] = j[(!dj && (uint32_t)i < njobs) ? i : 0];

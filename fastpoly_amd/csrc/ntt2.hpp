@@ -896,12 +896,45 @@ k_rows_fwd_dma(RowJob2 j0) {
   wait_vmem();  // the redundant last prefetch lands before the workgroup's LDS is released
 }
 
+constexpr uint32_t kMidTPO = 4;  // threads an output of a direct middle product
+
+// sum_j X[j] Y[o + off - j] over 0 <= j < nx, 0 <= o + off - j < ny, for an
+// output o < h: the terms with j = par mod kMidTPO, added to those of the
+// output's other kMidTPO - 1 lanes (adjacent lanes of one wave).
+__device__ __forceinline__ uint32_t mid_product(const uint32_t* X, uint32_t nx, const uint32_t* Y, uint32_t ny,
+                                                uint32_t off, uint32_t o, uint32_t par, uint32_t h, uint32_t p,
+                                                uint32_t ninv) {
+  constexpr uint32_t D = kMidTPO;
+  uint32_t acc = 0;
+  const uint32_t s = o + off;
+  if (o < h && nx && ny) {
+    const uint32_t jlo = s >= ny ? s - (ny - 1) : 0u;
+    const uint32_t jhi = min(nx - 1, s);  // inclusive
+    uint32_t j = jlo + ((par - jlo) & (D - 1));  // the first j >= jlo with j = par mod D
+    for (; j + D <= jhi; j += 2 * D) {
+      const uint32_t r = redc_full((uint64_t)X[j] * Y[s - j] + (uint64_t)X[j + D] * Y[s - j - D], p, ninv);
+      const uint32_t z = acc + r;
+      acc = z >= p ? z - p : z;
+    }
+    if (j <= jhi) {
+      const uint32_t z = acc + redc_full((uint64_t)X[j] * Y[s - j], p, ninv);
+      acc = z >= p ? z - p : z;
+    }
+  }
+#pragma unroll
+  for (uint32_t d = 1; d < D; d <<= 1) {
+    const uint32_t z = acc + (uint32_t)__shfl_xor((int)acc, (int)d, 64);
+    acc = z >= p ? z - p : z;
+  }
+  return acc;
+}
+
 // ---------------------------------------------------------------------------
 // Newton steps with m = 2^LM <= 256 (transforms of 2^6..2^9 points): the
 // step's window products, computed directly.  Every window the
 // NTT step modes read is free of cyclic wraparound, so each is a plain middle
-// product out[o] = sum_j X[j] * Y[o + off - j]; at these sizes 2m threads
-// (two per output, even and odd j) finish one in a few hundred instructions,
+// product out[o] = sum_j X[j] * Y[o + off - j]; at these sizes 4m threads
+// (four per output, j split mod 4) finish one in a few hundred instructions,
 // where the one-workgroup NTT step is a chain of radix-16 blocks on a few
 // waves.  Same job fields and results as the R2M_INVSTEP / R2M_EXPSTEP /
 // R2M_SQRTSTEP row modes, which serve the larger steps:
@@ -916,34 +949,18 @@ k_rows_fwd_dma(RowJob2 j0) {
 // not.  One workgroup a job: the job by value, or jobs[blockIdx.y].
 // ---------------------------------------------------------------------------
 template <int MODE, int LM>
-__global__ void __launch_bounds__(2 << LM) k_step_direct(RowJob2 j0, const RowJob2* __restrict__ jobs) {
+__global__ void __launch_bounds__(kMidTPO << LM) k_step_direct(RowJob2 j0, const RowJob2* __restrict__ jobs) {
   constexpr uint32_t M = 1u << LM;
   const RowJob2 J = load_job(j0, jobs);
   __shared__ uint32_t A[2 * M], B[M], Cc[M], W[M];
   const uint32_t t = threadIdx.x, p = J.f.p, ninv = J.f.ninv;
   const uint32_t h = J.out_hi;  // outputs, <= M
-  const uint32_t o = t >> 1, par = t & 1;
-  auto addm = [&](uint32_t x, uint32_t y) {
-    const uint32_t z = x + y;
-    return z >= p ? z - p : z;
-  };
-  // sum_j X[j] Y[o + off - j] over 0 <= j < nx, 0 <= o + off - j < ny: this
-  // thread's half (j of its parity); the pair of threads adds both halves.
+  const uint32_t o = t / kMidTPO, par = t % kMidTPO;
   auto mid = [&](const uint32_t* X, uint32_t nx, const uint32_t* Y, uint32_t ny, uint32_t off) -> uint32_t {
-    uint32_t acc = 0;
-    const uint32_t s = o + off;
-    if (o < h && nx && ny) {
-      const uint32_t jlo = s >= ny ? s - (ny - 1) : 0u;
-      const uint32_t jhi = min(nx - 1, s);  // inclusive
-      uint32_t j = jlo + ((jlo & 1) != par ? 1u : 0u);
-      for (; j + 2 <= jhi; j += 4)
-        acc = addm(acc, redc_full((uint64_t)X[j] * Y[s - j] + (uint64_t)X[j + 2] * Y[s - j - 2], p, ninv));
-      if (j <= jhi) acc = addm(acc, redc_full((uint64_t)X[j] * Y[s - j], p, ninv));
-    }
-    return addm(acc, (uint32_t)__shfl_xor((int)acc, 1, 64));
+    return mid_product(X, nx, Y, ny, off, o, par, h, p, ninv);
   };
   auto stage = [&](uint32_t* dst, const uint32_t* src, uint32_t count, uint32_t cap) {
-    for (uint32_t i = t; i < cap; i += 2 * M) dst[i] = i < count ? src[i] : 0u;
+    for (uint32_t i = t; i < cap; i += kMidTPO * M) dst[i] = i < count ? src[i] : 0u;
   };
   if constexpr (MODE == R2M_INVSTEP) {
     const uint32_t k = J.in_b;
@@ -1007,39 +1024,7 @@ __global__ void __launch_bounds__(2 << LM) k_step_direct(RowJob2 j0, const RowJo
 // ---------------------------------------------------------------------------
 constexpr uint32_t kSmallSeries = 512;
 constexpr int kSmallSeriesLog = 9;  // the size key its launches are recorded under
-constexpr uint32_t kSmallTPO = 4;   // threads an output
-constexpr uint32_t kSmallThreads = kSmallSeries / 2 * kSmallTPO;
-
-// sum_j X[j] Y[o + off - j] over 0 <= j < nx, 0 <= o + off - j < ny, for an
-// output o < h: the terms with j = par mod kSmallTPO, added to those of the
-// output's other kSmallTPO - 1 lanes (adjacent lanes of one wave).
-__device__ __forceinline__ uint32_t mid_product(const uint32_t* X, uint32_t nx, const uint32_t* Y, uint32_t ny,
-                                                uint32_t off, uint32_t o, uint32_t par, uint32_t h, uint32_t p,
-                                                uint32_t ninv) {
-  constexpr uint32_t D = kSmallTPO;
-  uint32_t acc = 0;
-  const uint32_t s = o + off;
-  if (o < h && nx && ny) {
-    const uint32_t jlo = s >= ny ? s - (ny - 1) : 0u;
-    const uint32_t jhi = min(nx - 1, s);  // inclusive
-    uint32_t j = jlo + ((par - jlo) & (D - 1));  // the first j >= jlo with j = par mod D
-    for (; j + D <= jhi; j += 2 * D) {
-      const uint32_t r = redc_full((uint64_t)X[j] * Y[s - j] + (uint64_t)X[j + D] * Y[s - j - D], p, ninv);
-      const uint32_t z = acc + r;
-      acc = z >= p ? z - p : z;
-    }
-    if (j <= jhi) {
-      const uint32_t z = acc + redc_full((uint64_t)X[j] * Y[s - j], p, ninv);
-      acc = z >= p ? z - p : z;
-    }
-  }
-#pragma unroll
-  for (uint32_t d = 1; d < D; d <<= 1) {
-    const uint32_t z = acc + (uint32_t)__shfl_xor((int)acc, (int)d, 64);
-    acc = z >= p ? z - p : z;
-  }
-  return acc;
-}
+constexpr uint32_t kSmallThreads = kSmallSeries / 2 * kMidTPO;
 
 template <int MODE>
 __global__ void __launch_bounds__(kSmallThreads) k_newton_small(RowJob2 j0, const RowJob2* __restrict__ jobs) {
@@ -1048,7 +1033,7 @@ __global__ void __launch_bounds__(kSmallThreads) k_newton_small(RowJob2 j0, cons
   __shared__ uint32_t A[NM], B[NM], C[NM], E[NM / 2];
   const uint32_t t = threadIdx.x, p = J.f.p, ninv = J.f.ninv;
   const uint32_t N = min(J.out_hi, NM), na = min(J.in_a, N);
-  const uint32_t o = t / kSmallTPO, par = t % kSmallTPO;
+  const uint32_t o = t / kMidTPO, par = t % kMidTPO;
   for (uint32_t i = t; i < NM; i += NM) {
     A[i] = i < na ? J.src_a[i] : 0u;
     B[i] = 0u;
