@@ -63,7 +63,7 @@ def parse():
                     help="only the 2^24 transform passes and a 64 MiB device copy (for counter runs)")
     ap.add_argument("--serve-requests", type=int, default=1000)
     ap.add_argument("--serve-rate", type=float, default=1000.0, help="offered load, requests/s (Poisson)")
-    ap.add_argument("--serve-levels", type=int, default=8)
+    ap.add_argument("--serve-levels", type=int, default=16)
     return ap.parse_args()
 
 

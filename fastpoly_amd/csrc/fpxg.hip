@@ -1536,7 +1536,7 @@ fpxg_server* fpxg_server_create(size_t hbm_budget_bytes, int levels_per_iteratio
   fpxg_server* out = nullptr;
   const int rc = guarded([&](Context& c) {
     const size_t budget = hbm_budget_bytes ? hbm_budget_bytes : c.arena->capacity() / 2;
-    const int lv = levels_per_iteration > 0 ? levels_per_iteration : 8;
+    const int lv = levels_per_iteration > 0 ? levels_per_iteration : 16;
     auto h = std::make_unique<fpxg_server>();
     h->s = std::make_unique<Server>(c, budget, lv);
     out = h.release();

@@ -195,7 +195,7 @@ typedef struct {
   size_t peak_in_use_bytes;
 } fpxg_server_stats;
 
-/* hbm_budget_bytes 0: half the arena; levels_per_iteration 0: 8. */
+/* hbm_budget_bytes 0: half the arena; levels_per_iteration 0: 16. */
 fpxg_server* fpxg_server_create(size_t hbm_budget_bytes, int levels_per_iteration);
 /* FPXG_E_ARG for a null input with a nonzero length (nothing is queued). */
 int fpxg_server_submit(fpxg_server* s, fpxg_req* req, uint64_t* id);
