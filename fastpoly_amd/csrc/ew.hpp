@@ -381,6 +381,16 @@ struct ScanPart {
   uint32_t m1, m2, last, pad;
 };
 
+// Up to two records by value (one series input, or a product's operands):
+// such a scan needs no upload of its records.
+struct ScanPack {
+  ScanRec r[2];
+};
+__device__ __forceinline__ ScanRec scan_rec(const ScanRec* recs, const ScanPack& pk, uint32_t i) {
+  if (recs) return recs[i];
+  return i == 0 ? pk.r[0] : pk.r[1];
+}
+
 // (first, second) nonzero indices of two disjoint ranges, merged.
 __device__ __forceinline__ void scan_merge(uint32_t& a1, uint32_t& a2, uint32_t b1, uint32_t b2) {
   const uint32_t m2 = a1 < b1 ? min(a2, b1) : (b1 < a1 ? min(a1, b2) : min(a2, b2));
@@ -390,10 +400,10 @@ __device__ __forceinline__ void scan_merge(uint32_t& a1, uint32_t& a2, uint32_t 
 
 // Stage 1: block x of record y scans slice x of B (indices are record-relative;
 // the sentinel "none" is the record's n).
-__global__ void __launch_bounds__(256) k_scan_part(const ScanRec* __restrict__ recs, ScanPart* __restrict__ parts,
-                                                  uint32_t B) {
+__global__ void __launch_bounds__(256) k_scan_part(const ScanRec* __restrict__ recs, ScanPack pk,
+                                                  ScanPart* __restrict__ parts, uint32_t B) {
   __shared__ uint32_t m1s[256], m2s[256], lss[256];
-  const ScanRec R = recs[blockIdx.y];
+  const ScanRec R = scan_rec(recs, pk, blockIdx.y);
   const uint32_t per = (uint32_t)(((uint64_t)R.n + B - 1) / B);
   const uint32_t lo = (uint32_t)min((uint64_t)blockIdx.x * per, (uint64_t)R.n);
   const uint32_t hi = (uint32_t)min((uint64_t)lo + per, (uint64_t)R.n);
@@ -433,11 +443,11 @@ __global__ void __launch_bounds__(256) k_scan_part(const ScanRec* __restrict__ r
 }
 
 // Stage 2: one workgroup per record merges its B <= 256 partials.
-__global__ void __launch_bounds__(256) k_scan_final(const ScanRec* __restrict__ recs,
+__global__ void __launch_bounds__(256) k_scan_final(const ScanRec* __restrict__ recs, ScanPack pk,
                                                    const ScanPart* __restrict__ parts, uint32_t B,
                                                    ScanOut* __restrict__ out) {
   __shared__ uint32_t m1s[256], m2s[256], lss[256];
-  const ScanRec R = recs[blockIdx.x];
+  const ScanRec R = scan_rec(recs, pk, blockIdx.x);
   ScanPart q{R.n, R.n, 0u, 0u};
   if (threadIdx.x < B) q = parts[(size_t)blockIdx.x * B + threadIdx.x];
   m1s[threadIdx.x] = q.m1; m2s[threadIdx.x] = q.m2; lss[threadIdx.x] = q.last;
@@ -519,17 +529,35 @@ static std::vector<ScanOut> scan_inputs(Context& c, const std::vector<ScanRec>& 
   for (const ScanRec& r : recs) maxn = std::max(maxn, r.n);
   // Slices of >= 4096 words, at most 256 per record.
   const uint32_t B = std::max<uint32_t>(1, std::min<uint32_t>(256, (maxn + 4095) / 4096));
+  DevBuf dpart = c.buf(recs.size() * B * sizeof(ScanPart) / 4 + 1);
+  if (recs.size() <= 2) {
+    // The single calls' scan: records by value, results written straight
+    // into pinned host memory (no upload, no download copy).
+    if (!c.scan_host)
+      FPXG_HIP(hipHostMalloc(&c.scan_host, 2 * sizeof(ScanOut), hipHostMallocCoherent));
+    ScanPack pk{};
+    for (size_t i = 0; i < recs.size(); ++i) pk.r[i] = recs[i];
+    ScanOut* hout = static_cast<ScanOut*>(c.scan_host);
+    hipLaunchKernelGGL(k_scan_part, dim3(B, (unsigned)recs.size()), dim3(256), 0, c.stream, (const ScanRec*)nullptr,
+                       pk, (ScanPart*)dpart.get(), B);
+    FPXG_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_scan_final, dim3((unsigned)recs.size()), dim3(256), 0, c.stream, (const ScanRec*)nullptr, pk,
+                       (const ScanPart*)dpart.get(), B, hout);
+    FPXG_HIP(hipGetLastError());
+    FPXG_HIP(hipStreamSynchronize(c.stream));
+    for (size_t i = 0; i < recs.size(); ++i) out[i] = hout[i];
+    return out;
+  }
   DevBuf dr = c.buf(recs.size() * sizeof(ScanRec) / 4 + 1);
   DevBuf dout = c.buf(recs.size() * sizeof(ScanOut) / 4 + 1);
-  DevBuf dpart = c.buf(recs.size() * B * sizeof(ScanPart) / 4 + 1);
   FPXG_HIP(hipMemcpyAsync(dr.get(), recs.data(), recs.size() * sizeof(ScanRec), hipMemcpyHostToDevice, c.stream));
   for (size_t s = 0; s < recs.size(); s += 65535) {
     const size_t cnt = std::min<size_t>(65535, recs.size() - s);
     const ScanRec* rp = (const ScanRec*)dr.get() + s;
-    hipLaunchKernelGGL(k_scan_part, dim3(B, (unsigned)cnt), dim3(256), 0, c.stream, rp,
+    hipLaunchKernelGGL(k_scan_part, dim3(B, (unsigned)cnt), dim3(256), 0, c.stream, rp, ScanPack{},
                        (ScanPart*)dpart.get() + s * B, B);
     FPXG_HIP(hipGetLastError());
-    hipLaunchKernelGGL(k_scan_final, dim3((unsigned)cnt), dim3(256), 0, c.stream, rp,
+    hipLaunchKernelGGL(k_scan_final, dim3((unsigned)cnt), dim3(256), 0, c.stream, rp, ScanPack{},
                        (const ScanPart*)dpart.get() + s * B, B, (ScanOut*)dout.get() + s);
     FPXG_HIP(hipGetLastError());
   }

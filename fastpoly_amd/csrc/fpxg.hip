@@ -1059,6 +1059,7 @@ void fpxg_shutdown(void) {
     (void)hipEventDestroy(s.ev);
   }
   for (auto& kv : g_ctx->tickets) (void)hipEventDestroy(kv.second);
+  if (g_ctx->scan_host) (void)hipHostFree(g_ctx->scan_host);
   (void)hipStreamDestroy(g_ctx->stream);
   g_ctx.reset();
 }

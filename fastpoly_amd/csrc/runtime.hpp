@@ -226,6 +226,9 @@ struct Context {
   static constexpr int kStagingSlots = 4;
   Staging staging[kStagingSlots];
   int staging_next = 0;
+  // Pinned host memory the input scan of one or two records writes its
+  // results to (allocated on first use).
+  void* scan_host = nullptr;
   std::unordered_map<uint64_t, GarnerConst> garner_cache;  // by (target p, primes, accumulate)
   std::unordered_map<uint32_t, bool> prime_cache;          // modulus -> accepted
   // Outstanding batch tickets (completion events).
