@@ -19,18 +19,19 @@ struct GarnerConst {
   uint32_t primes, accumulate, neg;
 };
 
-__device__ __forceinline__ void garner_one(const uint32_t* __restrict__ c1, const uint32_t* __restrict__ c2,
-                                           const uint32_t* __restrict__ c3, uint32_t* __restrict__ out, uint32_t i,
-                                           const GarnerConst& g) {
+// The target limb from the residues c1, c2, c3 (those past g.primes are
+// ignored) and, when accumulating, the output's old limb.
+__device__ __forceinline__ uint32_t garner_value(uint32_t c1, uint32_t c2, uint32_t c3, uint32_t old,
+                                                 const GarnerConst& g) {
   {
-    const uint32_t x1 = redc_full(c1[i], g.P[0], g.Pninv[0]);
+    const uint32_t x1 = redc_full(c1, g.P[0], g.Pninv[0]);
     uint32_t x2 = 0, x3 = 0;
     if (g.primes >= 2) {
-      const uint32_t r2 = redc_full(c2[i], g.P[1], g.Pninv[1]);
+      const uint32_t r2 = redc_full(c2, g.P[1], g.Pninv[1]);
       const uint32_t d2 = r2 >= x1 ? r2 - x1 : r2 + g.P[1] - x1;
       x2 = mulmod_full(d2, g.inv12, g.P[1], g.Pninv[1]);
       if (g.primes == 3) {
-        const uint32_t r3 = redc_full(c3[i], g.P[2], g.Pninv[2]);
+        const uint32_t r3 = redc_full(c3, g.P[2], g.Pninv[2]);
         uint32_t x12 = x1 + mulmod_full(x2, g.p1_in3, g.P[2], g.Pninv[2]);
         x12 = x12 >= g.P[2] ? x12 - g.P[2] : x12;
         const uint32_t d3 = r3 >= x12 ? r3 - x12 : r3 + g.P[2] - x12;
@@ -40,7 +41,7 @@ __device__ __forceinline__ void garner_one(const uint32_t* __restrict__ c1, cons
     uint32_t v;
     if (g.p == 2) {
       v = (x1 + x2 + x3) & 1u;
-      if (g.accumulate) v ^= out[i];
+      if (g.accumulate) v ^= old;
     } else {
       v = redc_full(x1, g.p, g.ninv);
       uint32_t t = redc_full((uint64_t)x2 * g.p1_t, g.p, g.ninv);
@@ -48,14 +49,24 @@ __device__ __forceinline__ void garner_one(const uint32_t* __restrict__ c1, cons
       t = redc_full((uint64_t)x3 * g.p12_t, g.p, g.ninv);
       v = v + t >= g.p ? v + t - g.p : v + t;
       if (g.neg) v = v == 0 ? 0u : g.p - v;
-      if (g.accumulate) {
-        const uint32_t o = out[i];
-        v = v + o >= g.p ? v + o - g.p : v + o;
-      }
+      if (g.accumulate) v = v + old >= g.p ? v + old - g.p : v + old;
     }
-    out[i] = v;
+    return v;
   }
 }
+
+__device__ __forceinline__ void garner_one(const uint32_t* __restrict__ c1, const uint32_t* __restrict__ c2,
+                                           const uint32_t* __restrict__ c3, uint32_t* __restrict__ out, uint32_t i,
+                                           const GarnerConst& g) {
+  out[i] = garner_value(c1[i], g.primes >= 2 ? c2[i] : 0u, g.primes == 3 ? c3[i] : 0u, g.accumulate ? out[i] : 0u, g);
+}
+
+// Garner fused into the last auxiliary prime's inverse column pass: the
+// residues of the other primes, read at the output position.
+struct GarnerArgs {
+  const uint32_t* r[2];
+  GarnerConst g;
+};
 
 __global__ void k_garner(const uint32_t* __restrict__ c1, const uint32_t* __restrict__ c2,
                          const uint32_t* __restrict__ c3, uint32_t* __restrict__ out, uint32_t n,

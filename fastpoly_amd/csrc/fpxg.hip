@@ -444,30 +444,33 @@ static int tile_log_cols(int S, int L) {
 // njobs jobs from the device array dj, or (dj null) the njobs by-value jobs
 // j[0 .. njobs).
 template <int S, int LOGC, bool LAZY, bool INV, bool SGN>
-static void cols2_launch_c(Context& c, const ColJob2* j, const PassPlan& P, uint32_t njobs, const ColJob2* dj) {
+static void cols2_launch_c(Context& c, const ColJob2* j, const PassPlan& P, uint32_t njobs, const ColJob2* dj,
+                           const GarnerArgs& ga) {
   using CS = ColShape<S, LOGC>;
   const size_t lds = (size_t)CS::words * 4;
   JobPack<ColJob2, kColPack> pack;
   for (int i = 0; i < kColPack; ++i) pack.j[i] = j[(!dj && (uint32_t)i < njobs) ? i : 0];
   dim3 grid(1u << (P.b - LOGC), njobs), block(CS::T);
-  hipLaunchKernelGGL((k_cols2<S, LOGC, LAZY, INV, SGN>), grid, block, lds, c.stream, pack, dj, P.L);
+  hipLaunchKernelGGL((k_cols2<S, LOGC, LAZY, INV, SGN>), grid, block, lds, c.stream, pack, dj, P.L, ga);
 }
 template <int S, bool LAZY, bool INV, bool SGN>
-static void cols2_launch(Context& c, const ColJob2* j, const PassPlan& P, uint32_t njobs, const ColJob2* dj) {
+static void cols2_launch(Context& c, const ColJob2* j, const PassPlan& P, uint32_t njobs, const ColJob2* dj,
+                         const GarnerArgs& ga) {
   if constexpr (S == 10) {
     if (P.logC == 3) {
-      cols2_launch_c<S, 3, LAZY, INV, SGN>(c, j, P, njobs, dj);
+      cols2_launch_c<S, 3, LAZY, INV, SGN>(c, j, P, njobs, dj, ga);
       return;
     }
   }
-  cols2_launch_c<S, col_tile_log(S), LAZY, INV, SGN>(c, j, P, njobs, dj);
+  cols2_launch_c<S, col_tile_log(S), LAZY, INV, SGN>(c, j, P, njobs, dj, ga);
 }
 
 template <bool LAZY, bool INV, bool SGN = false>
-static void cols2_size(Context& c, const ColJob2* j, const PassPlan& P, uint32_t njobs, const ColJob2* dj) {
+static void cols2_size(Context& c, const ColJob2* j, const PassPlan& P, uint32_t njobs, const ColJob2* dj,
+                       const GarnerArgs& ga) {
   switch (P.S) {
 #define FPXG_C(S) \
-  case S: cols2_launch<S, LAZY, INV, SGN>(c, j, P, njobs, dj); break;
+  case S: cols2_launch<S, LAZY, INV, SGN>(c, j, P, njobs, dj, ga); break;
     FPXG_C(1) FPXG_C(2) FPXG_C(3) FPXG_C(4) FPXG_C(5) FPXG_C(6)
     FPXG_C(7) FPXG_C(8) FPXG_C(9) FPXG_C(10) FPXG_C(11) FPXG_C(12)
 #undef FPXG_C
@@ -479,8 +482,10 @@ static void cols2_size(Context& c, const ColJob2* j, const PassPlan& P, uint32_t
 // j[0 .. njobs); recorded as one record per job, all but the first marked as
 // sharing the first's launch level.
 static void launch_cols_n(Context& c, bool lazy, bool inv, const ColJob2* j, const PassPlan& P, uint32_t njobs,
-                          const ColJob2* dj) {
+                          const ColJob2* dj, const GarnerArgs* gap = nullptr) {
   if (!dj && (njobs < 1 || njobs > (uint32_t)kColPack)) throw Error(E_DEVICE, "column launch group size");
+  const GarnerArgs ga = gap ? *gap : GarnerArgs{};
+  if (gap && (c.rec || dj || !inv)) throw Error(E_DEVICE, "fused Garner: one inverse launch, not recorded");
   if (c.rec && !dj) {
     for (uint32_t i = 0; i < njobs; ++i) c.rec->push_cols(P.S, lazy, inv, P.L, j[i], i > 0);
     return;
@@ -490,12 +495,12 @@ static void launch_cols_n(Context& c, bool lazy, bool inv, const ColJob2* j, con
   bool sgn = lazy && !inv && !dj;
   for (uint32_t i = 0; sgn && i < njobs; ++i) sgn = signed_ok(j[i].f.p);
   if (lazy) {
-    if (inv) cols2_size<true, true>(c, j, P, njobs, dj);
-    else if (sgn) cols2_size<true, false, true>(c, j, P, njobs, dj);
-    else cols2_size<true, false>(c, j, P, njobs, dj);
+    if (inv) cols2_size<true, true>(c, j, P, njobs, dj, ga);
+    else if (sgn) cols2_size<true, false, true>(c, j, P, njobs, dj, ga);
+    else cols2_size<true, false>(c, j, P, njobs, dj, ga);
   } else {
-    if (inv) cols2_size<false, true>(c, j, P, njobs, dj);
-    else cols2_size<false, false>(c, j, P, njobs, dj);
+    if (inv) cols2_size<false, true>(c, j, P, njobs, dj, ga);
+    else cols2_size<false, false>(c, j, P, njobs, dj, ga);
   }
   FPXG_HIP(hipGetLastError());
 }
@@ -695,11 +700,21 @@ void ntt_inverse(Context& c, const HostField& F, uint32_t* dst, const uint32_t* 
 // a CRT product: one launch per pass carries every field's jobs).  out[i] gets
 // the window [0, outc) in field F[i]; lift reads a and b as limbs of another
 // field (times r2).  All fields must share the lazy mode.
-static void conv_core_fields(Context& c, const HostField* const* F, int k, const uint32_t* a, size_t la,
+// Garner fused into the product's last inverse pass (CRT products): the
+// target limbs go to out, reconstructed with g.
+struct GarnerFuse {
+  uint32_t* out;
+  GarnerConst g;
+};
+
+static bool conv_core_fields(Context& c, const HostField* const* F, int k, const uint32_t* a, size_t la,
                              const uint32_t* b, size_t lb, uint32_t* const* out, size_t outc, bool lift,
-                             int L_cyclic = 0, size_t lo = 0) {
+                             int L_cyclic = 0, size_t lo = 0, const GarnerFuse* gf = nullptr) {
   // L_cyclic: the 2^L_cyclic-point cyclic product, whose window [lo, lo +
   // outc) is written (lo + outc <= 2^L_cyclic); else the whole product from 0.
+  // With gf (and a column pass, outside a recorded batch) the last prime's
+  // inverse pass ends in Garner and writes gf->out: returns true; otherwise
+  // every prime's residues are in out[i] and the caller runs Garner.
   const int L = L_cyclic ? L_cyclic : std::max(1, ceil_log2(la + lb - 1));
   const uint32_t N = 1u << L;
   const PassPlan P = c.conv_plan(L);
@@ -719,7 +734,7 @@ static void conv_core_fields(Context& c, const HostField* const* F, int k, const
       rj[i].flags = R2_SCALE | R2_CANON | (lift ? R2_LIFT : 0);
     }
     launch_rows_group(c, lazy, R2M_CONV, rj, k, L, 1);
-    return;
+    return false;
   }
   DevBuf A[kRowPack], B[kRowPack];
   ColJob2 cj[2 * kRowPack];
@@ -745,7 +760,22 @@ static void conv_core_fields(Context& c, const HostField* const* F, int k, const
     ij[i].src = A[i].get(); ij[i].dst = out[i]; ij[i].out_lo = (uint32_t)lo; ij[i].out_hi = (uint32_t)(lo + outc);
     ij[i].out_shift = (uint32_t)lo; ij[i].inv_n = inv_pow2(*F[i], L); ij[i].f = F[i]->dev;
   }
+  if (gf && !c.rec) {
+    // The other primes' residues first; the last prime's pass reads them at
+    // each output position and writes the target limb (restore_crt fused).
+    if (k > 1) launch_cols_n(c, lazy, true, ij, P, (uint32_t)(k - 1), nullptr);
+    ColJob2 last = ij[k - 1];
+    last.dst = gf->out;
+    last.flags |= R2_GARNER;
+    GarnerArgs ga{};
+    ga.r[0] = k > 1 ? out[0] : nullptr;
+    ga.r[1] = k > 2 ? out[1] : nullptr;
+    ga.g = gf->g;
+    launch_cols_n(c, lazy, true, &last, P, 1, nullptr, &ga);
+    return true;
+  }
   launch_cols_n(c, lazy, true, ij, P, (uint32_t)k, nullptr);
+  return false;
 }
 
 static void conv_core(Context& c, const HostField& F, const uint32_t* a, size_t la, const uint32_t* b, size_t lb,
@@ -804,6 +834,9 @@ static void crt_window(Context& c, const HostField& T, const uint32_t* a, size_t
     Fs[k] = &c.field(kCrtP[k]);
     outs[k] = res[k].get();
   }
+  // Garner stays a launch of its own here: fused into the last prime's
+  // inverse pass it made lone products slower (1e9+7, 2^14 x 2^14: 0.035
+  // against 0.022 ms), where the Newton steps' window products gain from it.
   conv_core_fields(c, Fs, primes, a, la, b, lb, outs, outc, true);
   GarnerRec g{};
   g.c1 = res[0].get();

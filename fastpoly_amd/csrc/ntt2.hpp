@@ -17,6 +17,7 @@
 // bit-reversed order, and its exact DIT inverse.
 #pragma once
 
+#include "crt.hpp"
 #include "device.hpp"
 
 namespace fpxg {
@@ -417,7 +418,13 @@ struct RowJob2 {
 };
 // By-value jobs per row launch: one for each of the three auxiliary primes.
 constexpr int kRowPack = 3;
-enum : uint32_t { R2_LIFT = 1, R2_CANON = 2, R2_SCALE = 4, R2_NEG = 8, R2_ACC = 16, R2_SUBFROM = 32, R2_MULTAB = 64 };
+enum : uint32_t { R2_LIFT = 1, R2_CANON = 2, R2_SCALE = 4, R2_NEG = 8, R2_ACC = 16, R2_SUBFROM = 32, R2_MULTAB = 64,
+                  R2_GARNER = 128 };
+// R2_GARNER (column inverse only): the job is the last auxiliary prime of a
+// CRT product; its canonical residue v and the other primes' residues
+// (GarnerArgs, at the output position) give the target limb by Garner
+// (restore_crt, crt_convolution.hpp:88-127), negated / accumulated as the
+// GarnerConst says, written to dst.
 // R2_ACC (column inverse only): dst[i] = dst[i] + result, both canonical.
 // Inverse-pass epilogues on the canonical result v at output position t (the
 // row pass needs R2_CANON with them): R2_SUBFROM v = (t < epi_avail ? epi[t]
@@ -1151,7 +1158,7 @@ struct ColShape {
 // brings them back, + p gives the usual [0, 2p).
 template <int S, int LOGC, bool LAZY, bool INV, bool SGN = false>
 __global__ void __launch_bounds__((ColShape<S, LOGC>::T)) __attribute__((amdgpu_waves_per_eu(FPXG_COL_WPE)))
-k_cols2(JobPack<ColJob2, kColPack> jp, const ColJob2* __restrict__ jobs, int L) {
+k_cols2(JobPack<ColJob2, kColPack> jp, const ColJob2* __restrict__ jobs, int L, GarnerArgs ga) {
   using SC = Sched<S>;
   static_assert(!SGN || (LAZY && !INV), "signed residues: lazy forward passes only");
   constexpr int NPH = SC::nph;
@@ -1245,23 +1252,40 @@ k_cols2(JobPack<ColJob2, kColPack> jp, const ColJob2* __restrict__ jobs, int L) 
     inv_phases<S, PL - 1, LOGC, LAZY>(x, X, g, c, J.f.planes_inv, A);
     constexpr int r0 = SC::r_of(0);
     const bool acc = (flags & R2_ACC) != 0;
+    if (!(flags & R2_GARNER)) {
+#pragma unroll
+      for (int u = 0; u < PhaseGroups<S, 0, LOGC>::NG; ++u)
+#pragma unroll
+        for (int k = 0; k < (1 << r0); ++k) {
+          const uint32_t e = elem<S, 0>(PhaseGroups<S, 0, LOGC>::gid(g, u), k);
+          const uint32_t gi = e * M + col;
+          const uint32_t v = A.canon(A.half(x[u * (1 << r0) + k]));
+          if (gi >= J.out_lo && gi < J.out_hi) {
+            uint32_t w = neg ? (v == 0 ? 0u : J.f.p - v) : v;
+            if (flags & (R2_SUBFROM | R2_MULTAB))
+              w = out_epilogue(flags, J.epi, J.epi_avail, gi - J.out_shift, w, J.f.p, J.f.ninv);
+            if (acc) {
+              const uint32_t s = gload(J.dst + (gi - J.out_shift)) + w;
+              w = min(s, s - J.f.p);
+            }
+            gstore(J.dst + (gi - J.out_shift), w);
+          }
+        }
+      return;
+    }
+    // R2_GARNER: the other primes' residues at each output position.
+    const uint32_t np = ga.g.primes;
 #pragma unroll
     for (int u = 0; u < PhaseGroups<S, 0, LOGC>::NG; ++u)
 #pragma unroll
       for (int k = 0; k < (1 << r0); ++k) {
-        const uint32_t e = elem<S, 0>(PhaseGroups<S, 0, LOGC>::gid(g, u), k);
-        const uint32_t gi = e * M + col;
+        const uint32_t gi = elem<S, 0>(PhaseGroups<S, 0, LOGC>::gid(g, u), k) * M + col;
+        if (gi < J.out_lo || gi >= J.out_hi) continue;
         const uint32_t v = A.canon(A.half(x[u * (1 << r0) + k]));
-        if (gi >= J.out_lo && gi < J.out_hi) {
-          uint32_t w = neg ? (v == 0 ? 0u : J.f.p - v) : v;
-          if (flags & (R2_SUBFROM | R2_MULTAB))
-            w = out_epilogue(flags, J.epi, J.epi_avail, gi - J.out_shift, w, J.f.p, J.f.ninv);
-          if (acc) {
-            const uint32_t s = gload(J.dst + (gi - J.out_shift)) + w;
-            w = min(s, s - J.f.p);
-          }
-          gstore(J.dst + (gi - J.out_shift), w);
-        }
+        const uint32_t o = gi - J.out_shift;
+        const uint32_t r1 = np == 1 ? v : gload(ga.r[0] + o);
+        const uint32_t r2 = np == 2 ? v : (np == 3 ? gload(ga.r[1] + o) : 0u);
+        gstore(J.dst + o, garner_value(r1, r2, np == 3 ? v : 0u, ga.g.accumulate ? gload(J.dst + o) : 0u, ga.g));
       }
   }
 }

@@ -117,11 +117,12 @@ static void crt_conv(Context& c, const HostField& T, int L, const uint32_t* x, s
     Fs[i] = &c.field(kCrtP[i]);
     outs[i] = res[i].get();
   }
-  conv_core_fields(c, Fs, 3, x, cx, y, cy, outs, hi - lo, true, L, lo);
+  GarnerFuse gf{out, garner_const(c, T, 3, false)};
+  gf.g.neg = neg;
+  if (conv_core_fields(c, Fs, 3, x, cx, y, cy, outs, hi - lo, true, L, lo, &gf)) return;
   GarnerRec g{};
   g.c1 = res[0].get(); g.c2 = res[1].get(); g.c3 = res[2].get(); g.out = out; g.n = (uint32_t)(hi - lo);
-  g.g = garner_const(c, T, 3, false);
-  g.g.neg = neg;
+  g.g = gf.g;
   emit_garner(c, g);
 }
 
