@@ -264,6 +264,23 @@ def test_series_random_vs_oracle(fa, oracle, p):
         assert np.array_equal(fa.sqrt(p, sq, n), oracle.series("sqrt", p, sq, n)), ("sqrt", p, n)
 
 
+@pytest.mark.parametrize("p", [3, 5, 7, 13, 97, 65537])
+def test_series_at_domain_limit(fa, oracle, p):
+    """log and exp up to n = p (the largest the domain allows) over small
+    primes: the reciprocal tables 1/(c + i) and 1/i run to the end of the
+    field, four elements sharing each Fermat inverse."""
+    rng = random.Random(p)
+    one = oracle.from_int(p, 1)
+    for n in sorted({1, 2, min(p, 4096) - 1, min(p, 4096)} - {0}):
+        a = np.array([oracle.from_int(p, rng.randrange(p)) for _ in range(n)], dtype=np.uint32)
+        la = a.copy()
+        la[0] = one
+        assert np.array_equal(fa.series("log", p, la, n), oracle.series("log", p, la, n)), (p, n)
+        ea = a.copy()
+        ea[0] = 0
+        assert np.array_equal(fa.series("exp", p, ea, n), oracle.series("exp", p, ea, n)), (p, n)
+
+
 def test_series_edge_cases(fa, oracle):
     po = oracle
     one = po.from_int(P, 1)
