@@ -364,6 +364,7 @@ static void rows2_mode(Context& c, int mode, const RowJob2* j, uint32_t nrows, u
     case R2M_CONV: rows2_launch<LOGM, LAZY, R2M_CONV>(c, j, nrows, njobs, dj); break;
     case R2M_SPEC: rows2_launch<LOGM, LAZY, R2M_SPEC>(c, j, nrows, njobs, dj); break;
     case R2M_SPEC_ONLY: rows2_launch<LOGM, LAZY, R2M_SPEC_ONLY>(c, j, nrows, njobs, dj); break;
+    case R2M_SPEC_SUM2: rows2_launch<LOGM, LAZY, R2M_SPEC_SUM2>(c, j, nrows, njobs, dj); break;
     case R2M_INVSTEP:
     case R2M_EXPSTEP:
     case R2M_SQRTSTEP:
@@ -540,8 +541,8 @@ static void context_init(Context& c) {
   // LDS per workgroup (gfx950: 160 KB per CU).  Without it, stop at 2^13.
   bool big = true;
 #define X(L, M) big = rows2_attr<L, true, M>() && rows2_attr<L, false, M>() && big;
-  X(13, R2M_FWD) X(13, R2M_INV) X(13, R2M_CONV) X(13, R2M_SPEC) X(13, R2M_SPEC_ONLY)
-  X(14, R2M_FWD) X(14, R2M_INV) X(14, R2M_CONV) X(14, R2M_SPEC) X(14, R2M_SPEC_ONLY)
+  X(13, R2M_FWD) X(13, R2M_INV) X(13, R2M_CONV) X(13, R2M_SPEC) X(13, R2M_SPEC_ONLY) X(13, R2M_SPEC_SUM2)
+  X(14, R2M_FWD) X(14, R2M_INV) X(14, R2M_CONV) X(14, R2M_SPEC) X(14, R2M_SPEC_ONLY) X(14, R2M_SPEC_SUM2)
 #undef X
   big = rows2_attr<13, true, R2M_FWDS>() && rows2_attr<14, true, R2M_FWDS>() && big;
 #define X(S) big = cols2_attr<S, true, false>() && cols2_attr<S, true, true>() && cols2_attr<S, false, false>() && \
@@ -895,6 +896,10 @@ static void split_conv(Context& c, const HostField& T, const uint32_t* a, size_t
     if (i0 == i1) {  // one pair: product and row inverse in one pass
       rj.src_a = SA[i0].get(); rj.src_b = spec_b(k - i0);
       launch_rows(c, T.lazy, R2M_SPEC_ONLY, rj, P.b, rj.nrows);
+    } else if (i1 == i0 + 1) {  // two pairs: both products summed in the row pass's loads
+      rj.src_a = SA[i0].get(); rj.src_b = spec_b(k - i0);
+      rj.src_c = SA[i1].get(); rj.epi = spec_b(k - i1);
+      launch_rows(c, T.lazy, R2M_SPEC_SUM2, rj, P.b, rj.nrows);
     } else {
       for (size_t i = i0; i <= i1; ++i)
         emit_ew(c, T, EW_MULADD, W.get(), SA[i].get(), spec_b(k - i), N, 0, i == i0 ? 0u : 1u);

@@ -439,7 +439,8 @@ __device__ __forceinline__ uint32_t out_epilogue(uint32_t flags, const uint32_t*
   return v;
 }
 enum : int { R2M_FWD = 1, R2M_INV = 2, R2M_CONV = 3, R2M_SPEC = 4, R2M_SPEC_ONLY = 5, R2M_INVSTEP = 6,
-             R2M_EXPSTEP = 7, R2M_SQRTSTEP = 8, R2M_NINV = 9, R2M_NEXP = 10, R2M_NSQRT = 11, R2M_FWDS = 12 };
+             R2M_EXPSTEP = 7, R2M_SQRTSTEP = 8, R2M_NINV = 9, R2M_NEXP = 10, R2M_NSQRT = 11, R2M_FWDS = 12,
+             R2M_SPEC_SUM2 = 13 };
 // R2M_FWDS: R2M_FWD on signed residues (ArithS, p < 2^28), chosen at launch
 // time for a recorded R2M_FWD; never recorded itself.
 // R2M_NINV / R2M_NEXP / R2M_NSQRT: the first coefficients of a series in one
@@ -459,6 +460,7 @@ enum : int { R2M_FWD = 1, R2M_INV = 2, R2M_CONV = 3, R2M_SPEC = 4, R2M_SPEC_ONLY
 // R2M_CONV: forward a, forward b, a*b, inverse
 // R2M_SPEC: forward a, a*spectrum(b), inverse
 // R2M_SPEC_ONLY: a (already a spectrum) * spectrum(b), inverse
+// R2M_SPEC_SUM2: a * b + c * d (four spectra: src_a, src_b, src_c, epi), inverse
 
 // Target waves per SIMD (register budget 512 / waves).  Eight waves (64
 // registers) lets two 1024-thread workgroups share a CU; the fused modes hold
@@ -730,7 +732,7 @@ k_rows2(JobPack<RowJob2, kRowPack> jp, const RowJob2* __restrict__ jobs) {
       forward_regs(x);
 #pragma unroll
       for (int i = 0; i < NGL * (1 << rl); ++i) x[i] = A.mul(x[i], w[i]);
-    } else {  // R2M_INV / R2M_SPEC_ONLY: load the spectrum in phase PL's grouping
+    } else {  // R2M_INV / R2M_SPEC_ONLY / R2M_SPEC_SUM2: load the spectrum in phase PL's grouping
 #pragma unroll
       for (int u = 0; u < NGL; ++u)
 #pragma unroll
@@ -738,6 +740,8 @@ k_rows2(JobPack<RowJob2, kRowPack> jp, const RowJob2* __restrict__ jobs) {
           const uint32_t gi = active ? base + elem<LOGM, PL>(PhaseGroups<LOGM, PL>::gid(g, u), k) : 0u;
           uint32_t v = gload(J.src_a + gi);
           if constexpr (MODE == R2M_SPEC_ONLY) v = A.mul(v, gload(J.src_b + gi));
+          if constexpr (MODE == R2M_SPEC_SUM2)
+            v = A.add(A.mul(v, gload(J.src_b + gi)), A.mul(gload(J.src_c + gi), gload(J.epi + gi)));
           x[u * (1 << rl) + k] = v;
         }
     }
