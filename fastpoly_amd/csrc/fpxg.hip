@@ -885,7 +885,10 @@ static void split_conv(Context& c, const HostField& T, const uint32_t* a, size_t
     ntt_forward(c, T, SB[j].get(), b + j * h, (uint32_t)std::min(h, lb - j * h), M, false, true);
   }
   auto spec_b = [&](size_t j) -> const uint32_t* { return same ? SA[j].get() : SB[j].get(); };
-  fill(c, out, outc, 0);
+  // Block k covers [k h, k h + 2h - 1); it overlaps block k - 1 on its first
+  // h - 1 positions only: block 0 stores, every later block adds below h - 1
+  // and stores above, so every output position is written and none is
+  // filled beforehand.
   DevBuf W = c.buf(N);
   const PassPlan P = c.plan(M);
   for (size_t k = 0; k + 1 < pa + pb && k * h < outc; ++k) {
@@ -908,7 +911,7 @@ static void split_conv(Context& c, const HostField& T, const uint32_t* a, size_t
     }
     ColJob2 cj{};
     cj.src = W.get(); cj.dst = out + lo; cj.out_lo = 0; cj.out_hi = (uint32_t)(hi - lo); cj.out_shift = 0;
-    cj.inv_n = inv_pow2(T, M); cj.flags = R2_ACC; cj.f = T.dev;
+    cj.inv_n = inv_pow2(T, M); cj.flags = k == 0 ? 0u : R2_ACCLO; cj.acc_end = (uint32_t)(h - 1); cj.f = T.dev;
     launch_cols(c, T.lazy, true, cj, P);
   }
 }

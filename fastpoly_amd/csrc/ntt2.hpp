@@ -419,7 +419,9 @@ struct RowJob2 {
 // By-value jobs per row launch: one for each of the three auxiliary primes.
 constexpr int kRowPack = 3;
 enum : uint32_t { R2_LIFT = 1, R2_CANON = 2, R2_SCALE = 4, R2_NEG = 8, R2_ACC = 16, R2_SUBFROM = 32, R2_MULTAB = 64,
-                  R2_GARNER = 128 };
+                  R2_GARNER = 128, R2_ACCLO = 256 };
+// R2_ACCLO (column inverse only): like R2_ACC for output positions below
+// acc_end (relative to out_shift), a plain store from there on.
 // R2_GARNER (column inverse only): the job is the last auxiliary prime of a
 // CRT product; its canonical residue v and the other primes' residues
 // (GarnerArgs, at the output position) give the target limb by Garner
@@ -1141,7 +1143,7 @@ struct ColJob2 {
   uint32_t inv_n;
   uint32_t flags;  // R2_LIFT, R2_CANON, R2_NEG, R2_ACC, R2_SUBFROM, R2_MULTAB
   const uint32_t* epi;
-  uint32_t epi_avail, pad_;
+  uint32_t epi_avail, acc_end;  // acc_end: R2_ACCLO
   FieldDev f;
 };
 
@@ -1268,7 +1270,7 @@ k_cols2(JobPack<ColJob2, kColPack> jp, const ColJob2* __restrict__ jobs, int L, 
             uint32_t w = neg ? (v == 0 ? 0u : J.f.p - v) : v;
             if (flags & (R2_SUBFROM | R2_MULTAB))
               w = out_epilogue(flags, J.epi, J.epi_avail, gi - J.out_shift, w, J.f.p, J.f.ninv);
-            if (acc) {
+            if (acc || ((flags & R2_ACCLO) && gi - J.out_shift < J.acc_end)) {
               const uint32_t s = gload(J.dst + (gi - J.out_shift)) + w;
               w = min(s, s - J.f.p);
             }
