@@ -449,7 +449,11 @@ __global__ void __launch_bounds__(256) k_scan_final(const ScanRec* __restrict__ 
   __shared__ uint32_t m1s[256], m2s[256], lss[256];
   const ScanRec R = scan_rec(recs, pk, blockIdx.x);
   ScanPart q{R.n, R.n, 0u, 0u};
-  if (threadIdx.x < B) q = parts[(size_t)blockIdx.x * B + threadIdx.x];
+  for (uint32_t t = threadIdx.x; t < B; t += 256) {
+    const ScanPart r = parts[(size_t)blockIdx.x * B + t];
+    scan_merge(q.m1, q.m2, r.m1, r.m2);
+    q.last = max(q.last, r.last);
+  }
   m1s[threadIdx.x] = q.m1; m2s[threadIdx.x] = q.m2; lss[threadIdx.x] = q.last;
   __syncthreads();
   for (int st = 128; st > 0; st >>= 1) {
@@ -527,8 +531,9 @@ static std::vector<ScanOut> scan_inputs(Context& c, const std::vector<ScanRec>& 
   if (recs.empty()) return out;
   uint32_t maxn = 0;
   for (const ScanRec& r : recs) maxn = std::max(maxn, r.n);
-  // Slices of >= 4096 words, at most 256 per record.
-  const uint32_t B = std::max<uint32_t>(1, std::min<uint32_t>(256, (maxn + 4095) / 4096));
+  // Slices of >= 1024 words (one round of loads a thread), at most 1024 per
+  // record; the final kernel folds up to four slices a thread.
+  const uint32_t B = std::max<uint32_t>(1, std::min<uint32_t>(1024, (maxn + 1023) / 1024));
   DevBuf dpart = c.buf(recs.size() * B * sizeof(ScanPart) / 4 + 1);
   if (recs.size() <= 2) {
     // The single calls' scan: records by value, results written straight
