@@ -333,7 +333,7 @@ static Prepared prepare(Context& c, fpxg_req& q) {
   if (q.op < FPXG_INV || q.op > FPXG_SQRT) throw Error(E_ARG, "unknown series op");
   if (q.n == 0) return r;
   if (!q.out || (q.la && !q.a) || q.out_cap < q.n) throw Error(E_ARG, "null pointer or output capacity too small");
-  r.xa = std::min(q.la, q.n);
+  r.xa = series_input_words_dev(q.op, q.la, q.n);
   r.outc = q.n;
   q.out_len = q.n;
   if (q.op == FPXG_SQRT && c.field(q.p).p == 2) r.char2 = true;

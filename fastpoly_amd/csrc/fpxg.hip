@@ -231,8 +231,22 @@ void* Arena::alloc(size_t bytes) {
     }
   }
   const size_t chunk = std::max(sz, (size_t)256 << 20);
-  if (reserved_ + chunk > cap_) {
-    if (reserved_ + sz > cap_) throw Error(E_NOMEM, "device arena exhausted");
+  if (reserved_ + sz > cap_) reclaim_idle_locked();
+  if (reserved_ + sz > cap_) {
+    // At the cap: an idle block of a larger class serves the request before
+    // the arena gives up (it returns to its own class when freed).
+    for (size_t j = (size_t)k + 1; j < free_.size(); ++j) {
+      auto& big = free_[j];
+      if (big.empty()) continue;
+      void* p = big.back();
+      big.pop_back();
+      ++hits_;
+      idle_bytes_ -= (size_t)1 << j;
+      --idle_blocks_;
+      in_use_ += (size_t)1 << j;
+      return p;
+    }
+    throw Error(E_NOMEM, "device arena exhausted");
   }
   const size_t take = reserved_ + chunk > cap_ ? sz : chunk;
   char* base = nullptr;
@@ -276,11 +290,46 @@ void Arena::free(void* p) {
 void Arena::release_idle() {
   std::lock_guard<std::mutex> g(mu_);
   if (in_use_ != 0) return;  // chunks are carved; only an idle arena can return them
+  release_all_locked();
+}
+
+void Arena::release_all_locked() {
   for (auto& c : chunks_) (void)hipFree(c.base);
   chunks_.clear();
   for (auto& fl : free_) fl.clear();
   owner_.clear();
   reserved_ = idle_bytes_ = idle_blocks_ = 0;
+}
+
+// Under cap pressure: an arena with nothing in use starts over; otherwise
+// every idle block that is a whole allocation of its own (taken at the cap,
+// not carved from a chunk) goes back to the device.  hipFree waits for the
+// device, so no queued kernel still reads the memory.
+void Arena::reclaim_idle_locked() {
+  if (in_use_ == 0) {
+    release_all_locked();
+    return;
+  }
+  for (size_t i = 0; i < chunks_.size();) {
+    Chunk& ch = chunks_[i];
+    auto ow = owner_.find(ch.base);
+    bool freed = false;
+    if (ow != owner_.end() && ((size_t)1 << ow->second) == ch.size && ch.used == ch.size) {
+      auto& fl = free_[ow->second];
+      auto f = std::find(fl.begin(), fl.end(), (void*)ch.base);
+      if (f != fl.end()) {
+        fl.erase(f);
+        (void)hipFree(ch.base);
+        reserved_ -= ch.size;
+        idle_bytes_ -= ch.size;
+        --idle_blocks_;
+        owner_.erase(ow);
+        chunks_.erase(chunks_.begin() + (ptrdiff_t)i);
+        freed = true;
+      }
+    }
+    if (!freed) ++i;
+  }
 }
 
 fpxg_stats Arena::stats() const {
@@ -1399,7 +1448,7 @@ int fpxg_series_u32(int op, uint32_t p, const uint32_t* a, size_t la, size_t n, 
     if (n == 0) return;
     if (!out || (la && !a)) throw Error(E_ARG, "null pointer");
     const HostField& T = c.field(p);
-    const size_t use = std::min(la, n);
+    const size_t use = series_input_words(op, a, la, n);
     Upload u(c, a, use);
     DevBuf o = c.buf(n);
     series_call(c, T, op, u.buf.get(), use, n, o.get());
@@ -1413,7 +1462,7 @@ int fpxg_series_dev(int op, uint32_t p, const uint32_t* a, size_t la, size_t n, 
     check_modulus(p);
     if (n == 0) return;
     if (!out || (la && !a)) throw Error(E_ARG, "null pointer");
-    series_call(c, c.field(p), op, a, std::min(la, n), n, out);
+    series_call(c, c.field(p), op, a, series_input_words_dev(op, la, n), n, out);
   });
 }
 
@@ -1531,7 +1580,7 @@ int fpxg_batch_u32(fpxg_req* reqs, size_t count) {
     for (size_t i = 0; i < count; ++i) {
       const fpxg_req& q = reqs[i];
       const bool conv = q.op == FPXG_CONV;
-      in_a[i] = q.a ? std::min(q.la, conv ? q.n : q.n) : 0;
+      in_a[i] = !q.a ? 0 : conv ? std::min(q.la, q.n) : (q.n ? series_input_words(q.op, q.a, q.la, q.n) : 0);
       in_b[i] = conv && q.b ? std::min(q.lb, q.n) : 0;
       out_n[i] = q.out ? std::min(q.out_cap, conv ? std::min(q.n, q.la + q.lb) : q.n) : 0;
       off_a[i] = total; total += in_a[i];
@@ -1589,26 +1638,51 @@ fpxg_server* fpxg_server_create(size_t hbm_budget_bytes, int levels_per_iteratio
 int fpxg_server_submit(fpxg_server* h, fpxg_req* req, uint64_t* id) {
   if (!h || !req || !id) return E_ARG;
   Server& S = *h->s;
-  auto it = std::make_unique<SItem>();
-  it->user = req;
-  it->t_submit = S.now();
-  const int rc = S.stage(*it);  // this thread copies and scans the input
-  if (rc != OK) {
-    S.pinned.free(it->pin_in, it->pin_in_cls);
-    t_last_error = "invalid request";
-    return rc;
-  }
-  {
-    std::lock_guard<std::mutex> g(S.mu);
-    if (!S.fatal.empty() || S.stopping) {
-      t_last_error = S.fatal.empty() ? "server is stopping" : S.fatal;
-      return E_DEVICE;
+  std::unique_ptr<SItem> it;
+  // Staging allocates pinned host memory and may throw; nothing may escape
+  // into a C caller, and the staged block goes back to the pool on every
+  // early return.
+  auto give_back = [&] {
+    if (it) {
+      S.pinned.free(it->pin_in, it->pin_in_cls);
+      it->pin_in = nullptr;
     }
-    it->id = S.next_id++;
-    *id = it->id;
-    S.outstanding.insert(it->id);
-    ++S.st.submitted;
-    S.incoming.push_back(std::move(it));
+  };
+  try {
+    it = std::make_unique<SItem>();
+    it->user = req;
+    it->t_submit = S.now();
+    const int rc = S.stage(*it);  // this thread copies and scans the input
+    if (rc != OK) {
+      give_back();
+      t_last_error = "invalid request";
+      return rc;
+    }
+    {
+      std::lock_guard<std::mutex> g(S.mu);
+      if (!S.fatal.empty() || S.stopping) {
+        t_last_error = S.fatal.empty() ? "server is stopping" : S.fatal;
+        give_back();
+        return E_DEVICE;
+      }
+      it->id = S.next_id++;
+      *id = it->id;
+      S.outstanding.insert(it->id);
+      ++S.st.submitted;
+      S.incoming.push_back(std::move(it));
+    }
+  } catch (const Error& e) {
+    give_back();
+    t_last_error = e.what();
+    return e.code;
+  } catch (const std::bad_alloc&) {
+    give_back();
+    t_last_error = "host allocation failed";
+    return E_NOMEM;
+  } catch (const std::exception& e) {
+    give_back();
+    t_last_error = e.what();
+    return E_DEVICE;
   }
   S.cv_work.notify_one();
   return OK;

@@ -107,6 +107,8 @@ class Arena {
 
  private:
   struct Chunk { char* base; size_t size, used; };
+  void release_all_locked();
+  void reclaim_idle_locked();
   size_t cap_;
   size_t reserved_ = 0, in_use_ = 0, hits_ = 0, sys_allocs_ = 0, idle_bytes_ = 0, idle_blocks_ = 0;
   std::vector<Chunk> chunks_;                  // blocks are carved from the last chunk only

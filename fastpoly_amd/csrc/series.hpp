@@ -22,6 +22,26 @@
 
 namespace fpxg {
 
+// Input words a series call reads from a host array a[0, la): min(la, n),
+// except sqrt, which finds the valuation v over the whole input and then reads
+// a[v + i] for i < n - v/2, i.e. up to n + v/2 > n words (square_root.hpp:26-44).
+static size_t series_input_words(int op, const uint32_t* a, size_t la, size_t n) {
+  if (op != FPXG_SQRT) return std::min(la, n);
+  size_t v = 0;
+  while (v < la && a[v] == 0) ++v;
+  if (v == la) return std::min(la, n);
+  return std::min(la, std::max(v + 1, n + v / 2));
+}
+
+// The same for device inputs, whose valuation is found by the device scan: the
+// whole input for sqrt (the scan's positions are 32-bit words).
+static size_t series_input_words_dev(int op, size_t la, size_t n) {
+  if (op != FPXG_SQRT) return std::min(la, n);
+  if (la >= (size_t)UINT32_MAX)
+    throw Error(E_SIZE, "poly::sqrt: inputs of 2^32 - 1 or more coefficients are not supported");
+  return la;
+}
+
 static bool series_native(const HostField& T, int L) {
   return T.p != 2 && T.maxlog >= 6 && L <= std::min(T.maxlog, 26);
 }

@@ -40,7 +40,10 @@ class PinnedPool {
       return p;
     }
     void* p = nullptr;
-    FPXG_HIP(hipHostMalloc(&p, (size_t)1 << k, hipHostMallocDefault));
+    if (hipHostMalloc(&p, (size_t)1 << k, hipHostMallocDefault) != hipSuccess || !p) {
+      (void)hipGetLastError();
+      throw Error(E_NOMEM, "hipHostMalloc failed for a pinned staging block");
+    }
     all_.push_back(p);
     return p;
   }
@@ -73,6 +76,7 @@ struct SItem {
   int status = OK;
   fpxg_result result{};
   bool has_scan = false;  // input scanned at submit (series ops)
+  size_t staged_a = 0;    // series ops: input words staged at submit
 };
 
 // Host scan of a series input (same fields as the device scan).
@@ -180,7 +184,9 @@ struct Server {
         xb = std::min(q.lb, q.n);
       }
     } else {
-      xa = std::min(q.la, q.n);
+      if (q.la && q.n && !q.a) return E_ARG;
+      xa = q.n ? series_input_words(q.op, q.a, q.la, q.n) : 0;
+      it.staged_a = xa;
     }
     if ((xa && !q.a) || (xb && !q.b)) return E_ARG;
     if (xa + xb > ((size_t)1 << 33)) return E_SIZE;
@@ -218,6 +224,7 @@ struct Server {
   bool admit(std::unique_ptr<SItem>& it) {
     fpxg_req& q = it->q;
     q = *it->user;
+    if (q.op != FPXG_CONV) q.la = it->staged_a;  // what stage() copied into pin_in
     try {
       it->pr = prepare(c, q);
     } catch (const Error& e) {

@@ -18,10 +18,12 @@
 
 #include <algorithm>
 #include <array>
+#include <bit>
 #include <cstddef>
 #include <cstdint>
 #include <initializer_list>
 #include <memory>
+#include <mutex>
 #include <new>
 #include <ostream>
 #include <stdexcept>
@@ -352,15 +354,12 @@ class Poly : public std::vector<M> {
 template <class M>
 class NttPlan {
  public:
-  static std::shared_ptr<const NttPlan> get(uint32_t n) { return std::shared_ptr<const NttPlan>(new NttPlan(n)); }
-  static const NttPlan& get_ref(uint32_t n) {
-    static thread_local std::array<std::unique_ptr<NttPlan>, 32> cache;
-    validate(n);
-    int k = 0;
-    while ((uint32_t(1) << k) < n) ++k;
-    if (!cache[k]) cache[k].reset(new NttPlan(n));
-    return *cache[k];
-  }
+  // One plan per size for the whole process, built once even when many
+  // threads ask for a cold size together: get(n) == get(n) and
+  // &get_ref(n) == get(n).get(), as the reference's cache guarantees
+  // (ntt.hpp:130-135, 250-276).
+  static std::shared_ptr<const NttPlan> get(uint32_t n) { return cached(n); }
+  static const NttPlan& get_ref(uint32_t n) { return *cached(n); }
   uint32_t size() const { return n_; }
   int log_size() const {
     int k = 0;
@@ -382,6 +381,17 @@ class NttPlan {
     int k = 0;
     while (t && t % 2 == 0) { t /= 2; ++k; }
     if (M::mod == 2 || (uint64_t(1) << k) < n) throw ntt_size_error("NTT size exceeds 2^v2(mod-1) for this modulus");
+  }
+  static const std::shared_ptr<const NttPlan>& cached(uint32_t n) {
+    validate(n);
+    struct Slot {
+      std::once_flag built;
+      std::shared_ptr<const NttPlan> plan;
+    };
+    static std::array<Slot, 32> slots;
+    Slot& s = slots[std::countr_zero(n)];
+    std::call_once(s.built, [&s, n] { s.plan = std::shared_ptr<const NttPlan>(new NttPlan(n)); });
+    return s.plan;
   }
   void run(uint32_t* a, int variant, size_t count) const {
     detail::raise_status(fpxg_ntt_u32(M::mod, M::primitive_root, n_, variant, count, a), "NttPlan");

@@ -89,8 +89,13 @@ int fpxg_divmod_u32(uint32_t p, const uint32_t* a, size_t la, const uint32_t* b,
  * root == 0 selects the smallest primitive root (modint.hpp:28-52). */
 int fpxg_ntt_u32(uint32_t p, uint32_t root, uint32_t n, int variant, size_t count, uint32_t* data);
 
-/* Device-resident forms of conv, series, ntt, pow and divmod (asynchronous,
- * except that divmod and pow read back a few scan words). */
+/* Device-resident forms of conv, series, ntt, pow and divmod.  conv and ntt
+ * are asynchronous (stream-ordered on the library stream).  series, pow and
+ * divmod first read back a few words of a device scan of the input (the
+ * domain checks and special cases are decided on the host before anything is
+ * written, as the reference raises its errors), so they wait for the work
+ * queued before them on the library stream; the Newton schedule they then
+ * enqueue runs asynchronously. */
 int fpxg_conv_dev(uint32_t p, const uint32_t* a, size_t la, const uint32_t* b, size_t lb,
                   size_t n, uint32_t* out, size_t out_cap, size_t* out_len);
 int fpxg_series_dev(int op, uint32_t p, const uint32_t* a, size_t la, size_t n, uint32_t* out);

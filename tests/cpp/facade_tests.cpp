@@ -3,7 +3,11 @@
 // inputs (std::mt19937_64, bench/bench.cpp:216-220) against its published
 // FNV-1a checksums (bench/bench.cpp:152-159).  Built into a shared object by
 // tests/cpp/build.sh; tests/test_facade.py calls fpx_facade_tests on the GPU.
+#include <array>
+#include <barrier>
 #include <bit>
+#include <memory>
+#include <thread>
 #include <cstdio>
 #include <cstring>
 #include <random>
@@ -49,6 +53,48 @@ template <class F>
 bool throws_size(F&& f) {
   try { f(); } catch (const fpx::ntt_size_error&) { return true; } catch (...) { return false; }
   return false;
+}
+
+// Eight threads ask for cold plans of three sizes at once; each size is built
+// once and every thread gets that one plan (tests/test_ntt.cpp:338-378 of the
+// reference).  Mont<998244353, 243> is a field no other test touches, so its
+// cache starts cold.
+void cold_plans_concurrently() {
+  using Field = fpx::Mont<998244353u, 243u>;
+  using Plan = fpx::NttPlan<Field>;
+  constexpr size_t threads = 8;
+  constexpr std::array<uint32_t, threads> sizes{256, 512, 2048, 256, 512, 2048, 256, 512};
+  std::array<std::shared_ptr<const Plan>, threads> plans;
+  std::array<bool, threads> correct{};
+  std::barrier start(static_cast<std::ptrdiff_t>(threads));
+  std::array<std::thread, threads> workers;
+  for (size_t t = 0; t < threads; ++t) {
+    workers[t] = std::thread([&, t] {
+      start.arrive_and_wait();
+      try {
+        const uint32_t n = sizes[t];
+        if (t & 1) plans[t] = Plan::get(n);
+        const auto& borrowed = Plan::get_ref(n);
+        if (!(t & 1)) plans[t] = Plan::get(n);
+        std::vector<uint32_t> a(n);
+        for (uint32_t i = 0; i < n; ++i) a[i] = Field::from_int(uint64_t(i + 1) * (t + 1)).raw_val();
+        const auto original = a;
+        borrowed.forward_lazy(a.data());
+        borrowed.inverse(a.data());
+        correct[t] = a == original && plans[t] == Plan::get(n) && &borrowed == plans[t].get();
+      } catch (...) {
+        correct[t] = false;
+      }
+    });
+  }
+  for (auto& w : workers) w.join();
+  bool all = true, shared = true;
+  for (size_t t = 0; t < threads; ++t) {
+    all = all && correct[t] && plans[t] == Plan::get(sizes[t]) && plans[t].get() == &Plan::get_ref(sizes[t]);
+    for (size_t u = 0; u < t; ++u) shared = shared && ((plans[t] == plans[u]) == (sizes[t] == sizes[u]));
+  }
+  check(all, "8 threads, cold plans: one plan per size, transforms round-trip");
+  check(shared, "8 threads, cold plans: equal sizes share a plan, different sizes do not");
 }
 
 void run_all() {
@@ -125,6 +171,14 @@ void run_all() {
   check(fpx::poly::conv(std::vector<M>{}, std::vector<M>{M(1)}).empty(), "conv with an empty operand");
   const auto st = fpx::scratch_memory_stats();
   check(st.system_allocations >= 1, "scratch_memory_stats reports the device arena");
+  {
+    // Plan identity (tests/test_ntt.cpp:197-203 of the reference).
+    const auto owning = fpx::NttPlan<M>::get(1024);
+    check(owning == fpx::NttPlan<M>::get(1024), "NttPlan::get(n) == NttPlan::get(n)");
+    check(&fpx::NttPlan<M>::get_ref(1024) == owning.get(), "&NttPlan::get_ref(n) == NttPlan::get(n).get()");
+    check(fpx::NttPlan<M>::get_ref(1024).size() == 1024u, "NttPlan::get_ref(n).size()");
+  }
+  cold_plans_concurrently();
 }
 
 }  // namespace
@@ -134,6 +188,54 @@ extern "C" int fpx_facade_tests(char* report, size_t cap) {
   log_text.clear();
   try {
     run_all();
+  } catch (const std::exception& e) {
+    log_text += std::string("FAIL exception: ") + e.what() + "\n";
+    ++failures;
+  }
+  if (report && cap) {
+    std::strncpy(report, log_text.c_str(), cap - 1);
+    report[cap - 1] = 0;
+  }
+  return failures;
+}
+
+// Run in a process whose arena was capped at a few MiB (fpxg_init): a call
+// whose scratch does not fit raises std::bad_alloc (the reference's
+// allocation failure) and leaves the caller's data as it was; a call that
+// fits still works afterwards.
+extern "C" int fpx_facade_nomem_tests(char* report, size_t cap) {
+  failures = 0;
+  log_text.clear();
+  try {
+    using M = fpx::mod998244353;
+    std::mt19937_64 rng(99);
+    const fpx::Poly<M> big(rnd<M>(size_t(1) << 21, rng));
+    bool threw = false;
+    try {
+      (void)(big * big);
+    } catch (const std::bad_alloc&) {
+      threw = true;
+    }
+    check(threw, "conv past the arena cap -> std::bad_alloc");
+    threw = false;
+    fpx::Poly<M> inv_in = big;
+    inv_in[0] = M::from_int(1);
+    try {
+      (void)inv_in.inv(size_t(1) << 21);
+    } catch (const std::bad_alloc&) {
+      threw = true;
+    }
+    check(threw, "inv past the arena cap -> std::bad_alloc");
+    const fpx::Poly<M> x(rnd<M>(300, rng)), y(rnd<M>(200, rng));
+    const auto z = x * y;
+    bool ok = z.size() == 499;
+    for (size_t k = 0; ok && k < z.size(); k += 37) {
+      M s{};
+      for (size_t i = 0; i <= k; ++i)
+        if (i < x.size() && k - i < y.size()) s += x[i] * y[k - i];
+      ok = s == z[k];
+    }
+    check(ok, "a product that fits still works after the failures");
   } catch (const std::exception& e) {
     log_text += std::string("FAIL exception: ") + e.what() + "\n";
     ++failures;

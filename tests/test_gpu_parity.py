@@ -314,3 +314,79 @@ def test_workload_checksums(fa, oracle, key):
         from test_oracle import run_series_op
         got = run_series_op(po, p, op, n, series=fa.series, conv=fa.conv)
     assert gi.hexsum(po, got) == want
+
+
+# ------------------------------------------------------------------ sqrt past n
+def _sqrt_long_cases(po):
+    """sqrt inputs longer than n: the valuation v is found over the whole input
+    and a[v + i] is read for i < n - v/2 (square_root.hpp:26-44), so
+    coefficients at and past x^n matter."""
+    f = lambda x: po.from_int(P, x)  # noqa: E731
+    cases = [([0, 0, 0, 0, f(4)], 3), ([0, 0, f(1), f(1), f(1), f(1)], 4), ([0, 0, 0, 0, 0, f(4)], 3),
+             ([0] * 8 + [f(9), f(5)], 3), ([0, 0, 0, 0, f(3)], 2), ([0] * 12 + [f(16)] + [f(7)] * 9, 5)]
+    rng = random.Random(77)
+    for n in (5, 70, 600, 3000):
+        for half in (1, 3, 40):
+            v = 2 * half
+            sq = po.MT(1000 * n + v).rnd(P, n)
+            sq[0] = f(rng.randrange(1, 1000))
+            body = po.conv(P, sq, sq, n)
+            a = np.concatenate([np.zeros(v, np.uint32), body, po.MT(n + v).rnd(P, rng.randrange(0, 2 * v + 3))])
+            cases.append((a, n))
+        cases.append(([0] * (2 * n + 1) + [f(4)], n))  # odd valuation past n
+        cases.append(([0] * (2 * n + 2) + [f(4)], n))  # even, half >= n: zeros
+    return cases
+
+
+def _want_sqrt(po, a, n):
+    try:
+        return po.series("sqrt", P, a, n)
+    except po.OracleError:
+        return None
+
+
+def test_sqrt_input_longer_than_n(fa, oracle):
+    for a, n in _sqrt_long_cases(oracle):
+        want = _want_sqrt(oracle, a, n)
+        if want is None:
+            with pytest.raises(fa.DomainError):
+                fa.sqrt(P, a, n)
+        else:
+            assert np.array_equal(fa.sqrt(P, a, n), want), (len(a), n)
+
+
+def test_sqrt_input_longer_than_n_reference(fa, oracle):
+    if oracle.ref() is None:
+        pytest.skip("oracle/_ref (the reference library built from its headers) is not built")
+    f = lambda x: oracle.from_int(P, x)  # noqa: E731
+    for a, n in [([0, 0, 0, 0, f(4)], 3), ([0, 0, f(1), f(1), f(1), f(1)], 4)]:
+        assert np.array_equal(fa.sqrt(P, a, n), oracle.ref_series("sqrt", P, np.asarray(a, np.uint32), n))
+    got = fa.sqrt(P, [0, 0, 0, 0, f(4)], 3)  # x^2 * (+-2): the root's sign is Mont::sqrt's choice
+    assert got[0] == 0 and got[1] == 0 and got[2] in (f(2), f(P - 2))
+
+
+def test_sqrt_input_longer_than_n_dev_batch_server(fa, oracle):
+    cases = _sqrt_long_cases(oracle)
+    wants = [_want_sqrt(oracle, a, n) for a, n in cases]
+    # device pointers
+    for (a, n), want in zip(cases, wants):
+        d = fa.DeviceArray.from_host(np.asarray(a, np.uint32))
+        o = fa.DeviceArray(n)
+        if want is None:
+            with pytest.raises(fa.DomainError):
+                fa.series_dev("sqrt", P, d, len(a), n, o)
+        else:
+            fa.series_dev("sqrt", P, d, len(a), n, o)
+            assert np.array_equal(o.to_host(), want), (len(a), n)
+    # merged batch (host arrays) and the server
+    reqs = [fa.Request("sqrt", P, a, n=n) for a, n in cases]
+    got = fa.batch(reqs)
+    with fa.Server() as srv:
+        ids = [srv.submit(fa.Request("sqrt", P, a, n=n)) for a, n in cases]
+        served = [srv.wait(i)[0] for i in ids]
+    for g, s, want, (a, n) in zip(got, served, wants, cases):
+        for res in (g, s):
+            if want is None:
+                assert isinstance(res, fa.DomainError), (len(a), n)
+            else:
+                assert np.array_equal(res, want), (len(a), n)

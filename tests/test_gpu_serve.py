@@ -110,3 +110,25 @@ def test_serve_tight_budget_and_poll(fa):
         assert len(got) == len(ids)
     for r, i in zip(reqs, ids):
         assert _same(got[i][0], _single(fa, r)), (r.op, r.p, r.a.size)
+
+
+def test_serve_rejected_submit_returns_a_code(fa):
+    """A request the server cannot stage comes back as an error code from
+    submit (never an abort), and the server keeps serving afterwards."""
+    import ctypes
+    z = np.zeros(8, np.uint32)
+    good = fa.Request("conv", P, np.arange(1, 100, dtype=np.uint32), b=np.arange(5, 50, dtype=np.uint32))
+    with fa.Server() as srv:
+        q = fa.Req()
+        q.op, q.p = fa.OPS["conv"], P
+        # 2^33 + 2^33 input words: refused by size before any copy is made
+        q.a, q.la, q.b, q.lb = z.ctypes.data, 1 << 33, z.ctypes.data, 1 << 33
+        q.n, q.out, q.out_cap = 1 << 34, z.ctypes.data, 8
+        rid = ctypes.c_uint64(0)
+        rc = fa.lib().fpxg_server_submit(srv._h, ctypes.byref(q), ctypes.byref(rid))
+        assert rc == fa.E_SIZE
+        q2 = fa.Req()
+        q2.op, q2.p, q2.a, q2.la, q2.n = fa.OPS["inv"], P, None, 5, 4  # null input with la > 0
+        assert fa.lib().fpxg_server_submit(srv._h, ctypes.byref(q2), ctypes.byref(rid)) == fa.E_ARG
+        got = srv.wait(srv.submit(good))[0]
+    assert np.array_equal(got, _single(fa, good))
