@@ -44,6 +44,7 @@ EXPORTED_SYMBOLS = [
     "fpxg_server_create", "fpxg_server_submit", "fpxg_server_wait", "fpxg_server_poll", "fpxg_server_now",
     "fpxg_server_stats_get", "fpxg_server_destroy", "fpxg_autotune", "fpxg_set_option",
     "fpxg_pow_u32", "fpxg_pow_dev", "fpxg_divmod_u32", "fpxg_divmod_dev", "fpxg_product_plan_info",
+    "fpxg_conv_dev_noalloc",
 ]
 
 
@@ -130,6 +131,7 @@ def lib():
                                     ctypes.c_size_t, _u32p, ctypes.c_size_t, _size_p]
         L.fpxg_conv_dev.argtypes = [ctypes.c_uint32, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_void_p,
                                     ctypes.c_size_t, ctypes.c_size_t, ctypes.c_void_p, ctypes.c_size_t, _size_p]
+        L.fpxg_conv_dev_noalloc.argtypes = L.fpxg_conv_dev.argtypes
         L.fpxg_series_u32.argtypes = [ctypes.c_int, ctypes.c_uint32, _u32p, ctypes.c_size_t, ctypes.c_size_t, _u32p]
         L.fpxg_series_dev.argtypes = [ctypes.c_int, ctypes.c_uint32, ctypes.c_void_p, ctypes.c_size_t,
                                       ctypes.c_size_t, ctypes.c_void_p]
@@ -380,6 +382,17 @@ def conv_dev(p: int, a: DeviceArray, la: int, b: DeviceArray, lb: int, out: Devi
     got = ctypes.c_size_t(0)
     lim = SIZE_MAX if n is None else int(n)
     _check(lib().fpxg_conv_dev(p, a.ptr, la, b.ptr, lb, lim, out.ptr, out.n, ctypes.byref(got)), "conv_dev")
+    return got.value
+
+
+def conv_dev_noalloc(p: int, a: DeviceArray, la: int, b: DeviceArray, lb: int, out: DeviceArray,
+                     n: int | None = None) -> int:
+    """conv_dev through fpxg_conv_dev_noalloc: scratch from the library's static
+    device array, no device allocation."""
+    got = ctypes.c_size_t(0)
+    lim = SIZE_MAX if n is None else int(n)
+    _check(lib().fpxg_conv_dev_noalloc(p, a.ptr, la, b.ptr, lb, lim, out.ptr, out.n, ctypes.byref(got)),
+           "conv_dev_noalloc")
     return got.value
 
 

@@ -348,6 +348,13 @@ fpxg_stats Arena::stats() const {
 // Context and launch helpers
 // ===========================================================================
 
+// Scratch of fpxg_conv_dev_noalloc: 2^23 words (32 MiB) of static device
+// memory, one copy per device, held until the process ends.  Host code never
+// uses this name as an address: context_init asks the runtime for the device's
+// copy (Context::scratch_dev) and only that pointer reaches kernels.
+constexpr size_t kStaticScratchWords = (size_t)1 << 23;
+static __device__ __align__(256) uint32_t scratch[kStaticScratchWords];
+
 static std::unique_ptr<Context> g_ctx;
 static std::mutex g_init_mu;
 static size_t g_arena_bytes = (size_t)64 << 30;
@@ -603,6 +610,13 @@ static void context_init(Context& c) {
   c.row_log_max = big ? 14 : 13;
   c.arena = std::make_unique<Arena>(g_arena_bytes);
   c.graphs_enabled = getenv("FPXG_NO_GRAPHS") == nullptr;
+  // This device's copy of the static scratch array (the device is current).
+  void* sp = nullptr;
+  FPXG_HIP(hipGetSymbolAddress(&sp, HIP_SYMBOL(scratch)));
+  c.scratch_dev = static_cast<uint32_t*>(sp);
+  // Plans of the reference's named NTT modulus, 1e9+7 and the three auxiliary
+  // primes, built at set-up so that the no-allocation entry point finds them.
+  for (uint32_t p : {998244353u, 1000000007u, 167772161u, 469762049u, 754974721u}) (void)c.field(p);
 }
 
 Context& ctx() {
@@ -965,52 +979,96 @@ static void split_conv(Context& c, const HostField& T, const uint32_t* a, size_t
   }
 }
 
-void conv_dev(Context& c, const HostField& T, const uint32_t* a, size_t la, const uint32_t* b, size_t lb,
-              uint32_t* out, size_t outc) {
-  if (la == 0 || lb == 0 || outc == 0) return;
-  la = std::min(la, outc);
-  lb = std::min(lb, outc);
+// The path conv_dev takes for a product of la x lb coefficients (both already
+// clamped to outc), decided from the sizes and the field alone.
+enum ConvRoute { CR_DIRECT, CR_TAIL, CR_NATIVE, CR_SPLIT, CR_CRT, CR_BLOCKS };
+
+static ConvRoute conv_route(const Context& c, const HostField& T, size_t la, size_t lb, size_t outc) {
   const size_t full = la + lb - 1;
   // A short side: direct product (lone calls; recorded schedules keep the
   // transform path, whose launches merge across requests).  Measured
   // crossover against the transforms (long side 2^20): a short side of ~150
   // over an NTT prime, ~350 over the auxiliary primes.
   const size_t direct_max = native_ok(T, full) || split_ok(T, la, lb) ? 128 : kDirectShort;
-  if (!c.rec && T.p != 2 && std::min(la, lb) <= direct_max && outc <= ((size_t)1 << 31)) {
-    const bool a_short = la <= lb;
-    const uint32_t* sh = a_short ? a : b;
-    const uint32_t* lg = a_short ? b : a;
-    const uint32_t ns = (uint32_t)std::min(la, lb), nl = (uint32_t)std::max(la, lb);
-    const unsigned blocks = (unsigned)((outc + kDirectTile - 1) / kDirectTile);
-    hipLaunchKernelGGL(k_conv_direct, dim3(blocks), dim3(256), 0, c.stream, sh, ns, lg, nl, out, (uint32_t)outc,
-                       T.p, T.ninv);
-    FPXG_HIP(hipGetLastError());
-    return;
-  }
+  if (!c.rec && T.p != 2 && std::min(la, lb) <= direct_max && outc <= ((size_t)1 << 31)) return CR_DIRECT;
   // A product a few coefficients past a power of two N: the N-point cyclic
   // product plus the high tail computed directly (convolution.hpp:102-137).
   if (!c.rec && T.p != 2 && full > 64) {
     const int Lf = ceil_log2(full) - 1;
     const size_t N = (size_t)1 << Lf, tail = full - N;
-    if (tail >= 1 && tail <= 40 && la <= N && lb <= N && native_ok(T, N)) {
+    if (tail >= 1 && tail <= 40 && la <= N && lb <= N && native_ok(T, N)) return CR_TAIL;
+  }
+  if (native_ok(T, full)) return CR_NATIVE;
+  if (split_ok(T, la, lb)) return CR_SPLIT;
+  if (la <= kCrtBlock && lb <= kCrtBlock) return CR_CRT;
+  return CR_BLOCKS;
+}
+
+// Device words of scratch the product takes (every buffer rounded to 64 words,
+// the arena's and the static region's 256-byte alignment), or SIZE_MAX for
+// the paths that are only served from the arena (the split and block-pair
+// products above 2^23 coefficients).  Mirrors the buffers of conv_core_fields
+// (two 2^L-point spectra per field when the product has a column pass) and
+// crt_window (one residue window per auxiliary prime).
+static size_t words64(size_t w) { return (std::max<size_t>(w, 1) + 63) & ~(size_t)63; }
+static size_t conv_scratch_words(const Context& c, const HostField& T, size_t la, size_t lb, size_t outc) {
+  if (la == 0 || lb == 0 || outc == 0) return 0;
+  la = std::min(la, outc);
+  lb = std::min(lb, outc);
+  const size_t full = la + lb - 1;
+  auto spectra = [&](int L, int fields) -> size_t {
+    return c.conv_plan(L).S == 0 ? 0 : 2 * (size_t)fields * words64((size_t)1 << L);
+  };
+  switch (conv_route(c, T, la, lb, outc)) {
+    case CR_DIRECT: return 0;
+    case CR_TAIL: return spectra(ceil_log2(full) - 1, 1);
+    case CR_NATIVE: return spectra(std::max(1, ceil_log2(full)), 1);
+    case CR_CRT: {
+      const int primes = crt_prime_count(T.p, std::min(la, lb));
+      return (size_t)primes * words64(outc) + spectra(std::max(1, ceil_log2(full)), primes);
+    }
+    default: return SIZE_MAX;
+  }
+}
+
+void conv_dev(Context& c, const HostField& T, const uint32_t* a, size_t la, const uint32_t* b, size_t lb,
+              uint32_t* out, size_t outc) {
+  if (la == 0 || lb == 0 || outc == 0) return;
+  la = std::min(la, outc);
+  lb = std::min(lb, outc);
+  const size_t full = la + lb - 1;
+  switch (conv_route(c, T, la, lb, outc)) {
+    case CR_DIRECT: {
+      const bool a_short = la <= lb;
+      const uint32_t* sh = a_short ? a : b;
+      const uint32_t* lg = a_short ? b : a;
+      const uint32_t ns = (uint32_t)std::min(la, lb), nl = (uint32_t)std::max(la, lb);
+      const unsigned blocks = (unsigned)((outc + kDirectTile - 1) / kDirectTile);
+      hipLaunchKernelGGL(k_conv_direct, dim3(blocks), dim3(256), 0, c.stream, sh, ns, lg, nl, out, (uint32_t)outc,
+                         T.p, T.ninv);
+      FPXG_HIP(hipGetLastError());
+      return;
+    }
+    case CR_TAIL: {
+      const int Lf = ceil_log2(full) - 1;
+      const size_t N = (size_t)1 << Lf, tail = full - N;
       conv_core(c, T, a, la, b, lb, out, std::min(outc, N), false, Lf);
       hipLaunchKernelGGL(k_tail_fix, dim3(1), dim3(64), 0, c.stream, a, (uint32_t)la, b, (uint32_t)lb, out,
                          (uint32_t)outc, (uint32_t)N, (uint32_t)tail, T.p, T.ninv);
       FPXG_HIP(hipGetLastError());
       return;
     }
-  }
-  if (native_ok(T, full)) {
-    conv_core(c, T, a, la, b, lb, out, outc, false);
-    return;
-  }
-  if (split_ok(T, la, lb)) {
-    split_conv(c, T, a, la, b, lb, out, outc);
-    return;
-  }
-  if (la <= kCrtBlock && lb <= kCrtBlock) {
-    crt_window(c, T, a, la, b, lb, out, outc, false);
-    return;
+    case CR_NATIVE:
+      conv_core(c, T, a, la, b, lb, out, outc, false);
+      return;
+    case CR_SPLIT:
+      split_conv(c, T, a, la, b, lb, out, outc);
+      return;
+    case CR_CRT:
+      crt_window(c, T, a, la, b, lb, out, outc, false);
+      return;
+    case CR_BLOCKS:
+      break;
   }
   // crt_convolution block pairs (crt_convolution.hpp:187-205).
   fill(c, out, outc, 0);
@@ -1216,6 +1274,38 @@ int fpxg_conv_dev(uint32_t p, const uint32_t* a, size_t la, const uint32_t* b, s
     if (outc == 0) return;
     if (!a || !b || !out || out_cap < outc) throw Error(E_ARG, "null pointer or output capacity too small");
     conv_dev(c, c.field(p), a, xa, b, xb, out, outc);
+    *out_len = outc;
+  });
+}
+
+int fpxg_conv_dev_noalloc(uint32_t p, const uint32_t* a, size_t la, const uint32_t* b, size_t lb, size_t n,
+                          uint32_t* out, size_t out_cap, size_t* out_len) {
+  return guarded([&](Context& c) {
+    check_modulus(p);
+    if (!out_len) throw Error(E_ARG, "out_len is null");
+    *out_len = 0;
+    size_t xa = 0, xb = 0;
+    const size_t outc = conv_len(la, lb, n, &xa, &xb);
+    if (outc == 0) return;
+    if (!a || !b || !out || out_cap < outc) throw Error(E_ARG, "null pointer or output capacity too small");
+    // Only plans built earlier (at set-up, or by an earlier call with this
+    // modulus): building one allocates device tables.
+    auto f = c.fields.find((uint64_t)p << 32);
+    if (f == c.fields.end()) throw Error(E_ARG, "modulus not prepared: no earlier call on this device used it");
+    const HostField& T = *f->second;
+    const size_t need = conv_scratch_words(c, T, xa, xb, outc);
+    if (need > kStaticScratchWords)
+      throw Error(E_SIZE, "product needs more scratch than the 32 MiB static array (fpxg_conv_dev serves it)");
+    if (!c.scratch_dev) throw Error(E_DEVICE, "static scratch address unavailable");
+    Context::Bump bump{c.scratch_dev, kStaticScratchWords, 0};
+    c.bump = &bump;
+    try {
+      conv_dev(c, T, a, xa, b, xb, out, outc);
+    } catch (...) {
+      c.bump = nullptr;
+      throw;
+    }
+    c.bump = nullptr;
     *out_len = outc;
   });
 }

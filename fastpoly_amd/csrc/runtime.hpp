@@ -125,6 +125,8 @@ class DevBuf {
   DevBuf(Arena* a, size_t words) : a_(a), n_(words) {
     p_ = static_cast<uint32_t*>(a->alloc(std::max<size_t>(words, 1) * 4));
   }
+  // A view of memory the buffer does not own (the static scratch region).
+  DevBuf(uint32_t* p, size_t words) : a_(nullptr), p_(p), n_(words) {}
   DevBuf(DevBuf&& o) noexcept : a_(o.a_), p_(o.p_), n_(o.n_) { o.p_ = nullptr; }
   DevBuf& operator=(DevBuf&& o) noexcept {
     reset();
@@ -135,7 +137,7 @@ class DevBuf {
   DevBuf& operator=(const DevBuf&) = delete;
   ~DevBuf() { reset(); }
   void reset() {
-    if (p_) a_->free(p_);
+    if (p_ && a_) a_->free(p_);
     p_ = nullptr;
   }
   uint32_t* get() const { return p_; }
@@ -245,7 +247,23 @@ struct Context {
   PassPlan conv_plan_base(int L) const;
   PassPlan plan_base(int L) const;
   HostField& field(uint32_t p, uint32_t root = 0);
-  DevBuf buf(size_t words) { return DevBuf(arena.get(), words); }
+  // Static scratch (fpxg_conv_dev_noalloc): this device's address of the
+  // file-scope scratch array, and, while a no-allocation call runs, the bump
+  // region every buffer is carved from instead of the arena.
+  uint32_t* scratch_dev = nullptr;
+  struct Bump {
+    uint32_t* base;
+    size_t cap, used;
+  };
+  Bump* bump = nullptr;
+  DevBuf buf(size_t words) {
+    if (!bump) return DevBuf(arena.get(), words);
+    const size_t w = (std::max<size_t>(words, 1) + 63) & ~(size_t)63;  // 256-byte aligned, as the arena
+    if (bump->used + w > bump->cap) throw Error(E_SIZE, "static scratch exhausted");
+    uint32_t* p = bump->base + bump->used;
+    bump->used += w;
+    return DevBuf(p, words);
+  }
 };
 
 Context& ctx();  // initialises on first use

@@ -99,6 +99,21 @@ int fpxg_ntt_u32(uint32_t p, uint32_t root, uint32_t n, int variant, size_t coun
 int fpxg_conv_dev(uint32_t p, const uint32_t* a, size_t la, const uint32_t* b, size_t lb,
                   size_t n, uint32_t* out, size_t out_cap, size_t* out_len);
 int fpxg_series_dev(int op, uint32_t p, const uint32_t* a, size_t la, size_t n, uint32_t* out);
+/* fpxg_conv_dev that allocates nothing.  No counterpart in the reference: it
+ * is for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  Same
+ * arguments, results and errors as fpxg_conv_dev; its scratch (spectra and
+ * CRT residues) is a static 32 MiB device array (2^23 words) that each device
+ * holds until the process ends.  Limits: products whose scratch exceeds the
+ * array are refused with FPXG_E_SIZE (an NTT-prime product up to 2^22
+ * coefficients fits; a three-prime product up to 2^19), and a modulus whose
+ * plan no earlier call built is refused with FPXG_E_ARG (998244353,
+ * 1000000007 and the auxiliary primes are built at set-up).  Every call
+ * shares the array and leaves its contents behind; calls through this
+ * library run in order on its one stream, but calls that overlap on one GPU
+ * (e.g. from two copies of the library in one process) race. */
+int fpxg_conv_dev_noalloc(uint32_t p, const uint32_t* a, size_t la, const uint32_t* b, size_t lb,
+                          size_t n, uint32_t* out, size_t out_cap, size_t* out_len);
 int fpxg_ntt_dev(uint32_t p, uint32_t root, uint32_t n, int variant, size_t count, uint32_t* data);
 int fpxg_pow_dev(uint32_t p, const uint32_t* a, size_t la, uint64_t k, size_t n, uint32_t* out);
 int fpxg_divmod_dev(uint32_t p, const uint32_t* a, size_t la, const uint32_t* b, size_t lb,
