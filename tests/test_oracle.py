@@ -412,3 +412,35 @@ def test_pow_divmod_match_reference_library(oracle, p):
         q1, r1 = oracle.divmod(p, a, b)
         q2, r2 = oracle.ref_divmod(p, a, b)
         assert np.array_equal(q1, q2) and np.array_equal(r1, r2), (p, la, lb)
+
+
+def test_oracle_matches_workload_checksums_sample(oracle):
+    """The per-request checksums of the full workloads (written by the
+    reference library) agree with the oracle on the requests small enough for
+    the scalar oracle."""
+    import json
+    import os
+    from fastpoly_amd.workloads import serving_stream, uneven_batch
+    with open(os.path.join(os.path.dirname(__file__), "golden", "workload_checksums.json")) as fh:
+        g = json.load(fh)
+
+    def run(r):
+        if r[0] == "conv":
+            return oracle.conv(r[1], r[2], r[3])
+        return oracle.series(r[0], r[1], r[2], r[3])
+
+    def size(r):
+        return r[2].size + (r[3].size if r[0] == "conv" else r[3])
+
+    batch = uneven_batch(g["uneven_batch"]["n_conv"], g["uneven_batch"]["n_series"], seed=g["uneven_batch"]["seed"])
+    picked = [i for i, r in enumerate(batch) if size(r) <= 1 << 13][:30]
+    picked += [i for i, r in enumerate(batch) if r[0] != "conv" and size(r) <= 1 << 12][:10]
+    assert len(picked) >= 30
+    for i in picked:
+        assert "%016x" % oracle.checksum(run(batch[i])) == g["uneven_batch"]["checksums"][i], i
+    s = g["serving_stream"]
+    stream = [r for _, r in serving_stream(s["count"], 1000.0, seed=s["seed"], length_log=tuple(s["length_log"]))]
+    picked = [i for i, r in enumerate(stream) if size(r) <= 1 << 12][:12]
+    assert len(picked) >= 5
+    for i in picked:
+        assert "%016x" % oracle.checksum(run(stream[i])) == s["checksums"][i], i
