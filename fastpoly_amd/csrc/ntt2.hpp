@@ -50,6 +50,10 @@ __host__ __device__ constexpr uint32_t plane_off(int logB, int r) {
 }
 constexpr uint32_t kPlaneWords = 4u * 32768u;
 
+// Loads a group's 2^r - 1 twiddles and its padding word tw[2^r - 1] (16-byte
+// loads).  keep_pad() marks the padding word as read where the twiddles are
+// used: a register the compiler believes dead is reused at once, and the
+// reuse must then wait for the load in flight to land.
 template <int r>
 __device__ __forceinline__ void load_tw(uint32_t* tw, const uint32_t* plane, uint32_t lo) {
   const uint32_t* p = plane + (lo << r);
@@ -62,6 +66,10 @@ __device__ __forceinline__ void load_tw(uint32_t* tw, const uint32_t* plane, uin
   } else {
     tw[0] = gload(p);
   }
+}
+template <int r>
+__device__ __forceinline__ void keep_pad(const uint32_t* tw) {
+  if constexpr (r >= 2) asm volatile("" ::"v"(tw[(1 << r) - 1]));
 }
 
 // LO0: the group's offset lo is 0 (last phase of a block), so every level's
@@ -314,35 +322,94 @@ __device__ __forceinline__ void phase_from_lds(uint32_t* x, const uint32_t* X, u
   }
 }
 
-// Phases [P, END) of a forward transform whose registers already hold phase
-// P-1's output (P >= 1): each step writes LDS, syncs, reads phase P's groups
-// and computes.  All threads of the workgroup must call it.  With signed
-// residues (AR = ArithS) phases hand over values below 2p in magnitude and
-// the last one's outputs are below BLAST * p.
-template <int LOGB, int P, int END, int LOGC, bool LAZY, int BLAST = 2, class AR = Arith<LAZY>>
-__device__ __forceinline__ void fwd_phases(uint32_t* x, uint32_t* X, uint32_t g, uint32_t c, const uint32_t* planes,
-                                           const AR& A) {
-  if constexpr (P < END) {
-    if constexpr (WaveLocal<LOGB, P, LOGC>::value) {
-      wave_sync();
-      phase_to_lds<LOGB, P - 1, LOGC>(X, x, g, c);
-      wave_sync();
-      phase_from_lds<LOGB, P, LOGC>(x, X, g, c);
-    } else {
-      if constexpr (P > 1) __syncthreads();  // earlier exchanges may still be reading
-      phase_to_lds<LOGB, P - 1, LOGC>(X, x, g, c);
-      __syncthreads();
-      phase_from_lds<LOGB, P, LOGC>(x, X, g, c);
-    }
-    phase_compute<LOGB, P, false, LAZY, LOGC, P + 1 == END ? BLAST : 2>(x, g, planes, A);
-    fwd_phases<LOGB, P + 1, END, LOGC, LAZY, BLAST>(x, X, g, c, planes, A);
+// Twiddles of phase P for thread g: the 2^r-word plane groups of its NG
+// groups, 16 words in all (some unused for r = 1).  Loading them is separate
+// from using them so that every load can be issued a phase (or a whole
+// memory round trip) before the butterflies that need it.
+template <int LOGB, int P, int LOGC = 0>
+__device__ __forceinline__ void tw_load(uint32_t* T, const uint32_t* planes, uint32_t g) {
+  constexpr int r = Sched<LOGB>::r_of(P);
+  const uint32_t* plane = planes + plane_off(LOGB - Sched<LOGB>::s_of(P), r);
+#pragma unroll
+  for (int u = 0; u < PhaseGroups<LOGB, P, LOGC>::NG; ++u)
+    load_tw<r>(T + u * (1 << r), plane, group_lo<LOGB, P>(PhaseGroups<LOGB, P, LOGC>::gid(g, u)));
+}
+template <int LOGB, int P, bool INV, bool LAZY, int LOGC = 0, int BOUT = 2, class AR = Arith<LAZY>>
+__device__ __forceinline__ void tw_apply(uint32_t* x, const uint32_t* T, const AR& A) {
+  constexpr int r = Sched<LOGB>::r_of(P);
+  constexpr bool lo0 = LOGB - Sched<LOGB>::s_of(P) - r == 0;
+#pragma unroll
+  for (int u = 0; u < PhaseGroups<LOGB, P, LOGC>::NG; ++u) {
+    if constexpr (INV) dit_block<r, LAZY, lo0>(x + u * (1 << r), T + u * (1 << r), A);
+    else dif_any<r, lo0, BOUT, LAZY>(x + u * (1 << r), T + u * (1 << r), A);
+    keep_pad<r>(T + u * (1 << r));
   }
 }
-// Inverse phases P, P-1, ..., 0 where registers hold phase P+1's inverse output.
-template <int LOGB, int P, int LOGC, bool LAZY>
+
+// Register exchange between phases P-1 (registers) and P (registers) of a
+// forward transform, through LDS.
+template <int LOGB, int P, int LOGC>
+__device__ __forceinline__ void fwd_exchange(uint32_t* x, uint32_t* X, uint32_t g, uint32_t c) {
+  if constexpr (WaveLocal<LOGB, P, LOGC>::value) {
+    wave_sync();
+    phase_to_lds<LOGB, P - 1, LOGC>(X, x, g, c);
+    wave_sync();
+    phase_from_lds<LOGB, P, LOGC>(x, X, g, c);
+  } else {
+    if constexpr (P > 1) __syncthreads();  // earlier exchanges may still be reading
+    phase_to_lds<LOGB, P - 1, LOGC>(X, x, g, c);
+    __syncthreads();
+    phase_from_lds<LOGB, P, LOGC>(x, X, g, c);
+  }
+}
+
+// Phases [P, END) of a forward transform whose registers already hold phase
+// P-1's output (P >= 1); T holds phase P's twiddles (loads issued earlier).
+// Each step issues the next phase's twiddle loads, exchanges through LDS and
+// computes, so a twiddle load overlaps a whole phase.  All threads of the
+// workgroup must call it.  With signed residues (AR = ArithS) phases hand
+// over values below 2p in magnitude and the last one's outputs are below
+// BLAST * p.
+// PF = false (kernels held to 64 registers): no prefetch; each phase loads
+// its own twiddles after its exchange and T is unused.
+template <int LOGB, int P, int END, int LOGC, bool LAZY, int BLAST = 2, bool PF = true, class AR = Arith<LAZY>>
+__device__ __forceinline__ void fwd_phases(uint32_t* x, uint32_t* X, uint32_t g, uint32_t c, const uint32_t* planes,
+                                           const AR& A, const uint32_t* T) {
+  if constexpr (P < END) {
+    uint32_t Tn[16];
+    if constexpr (PF && P + 1 < END) tw_load<LOGB, P + 1, LOGC>(Tn, planes, g);
+    fwd_exchange<LOGB, P, LOGC>(x, X, g, c);
+    if constexpr (PF) {
+      tw_apply<LOGB, P, false, LAZY, LOGC, P + 1 == END ? BLAST : 2>(x, T, A);
+    } else {
+      uint32_t Tp[16];
+      tw_load<LOGB, P, LOGC>(Tp, planes, g);
+      tw_apply<LOGB, P, false, LAZY, LOGC, P + 1 == END ? BLAST : 2>(x, Tp, A);
+    }
+    fwd_phases<LOGB, P + 1, END, LOGC, LAZY, BLAST, PF>(x, X, g, c, planes, A, Tn);
+  }
+}
+// A whole forward transform of registers holding phase 0's input; T0 holds
+// phase 0's twiddles (tw_load<LOGB, 0, LOGC>, best issued with the data
+// loads).  BOUT0: output bound of a one-phase transform (signed residues).
+template <int LOGB, int LOGC, bool LAZY, int BLAST = 2, int BOUT0 = 2, bool PF = true, class AR = Arith<LAZY>>
+__device__ __forceinline__ void fwd_transform(uint32_t* x, uint32_t* X, uint32_t g, uint32_t c, const uint32_t* planes,
+                                              const AR& A, const uint32_t* T0) {
+  constexpr int NPH = Sched<LOGB>::nph;
+  uint32_t T1[16];
+  if constexpr (PF && NPH > 1) tw_load<LOGB, 1, LOGC>(T1, planes, g);
+  tw_apply<LOGB, 0, false, LAZY, LOGC, NPH == 1 ? BOUT0 : 2>(x, T0, A);
+  fwd_phases<LOGB, 1, NPH, LOGC, LAZY, BLAST, PF>(x, X, g, c, planes, A, T1);
+}
+
+// Inverse phases P, P-1, ..., 0 where registers hold phase P+1's inverse
+// output and T holds phase P's twiddles.
+template <int LOGB, int P, int LOGC, bool LAZY, bool PF = true>
 __device__ __forceinline__ void inv_phases(uint32_t* x, uint32_t* X, uint32_t g, uint32_t c, const uint32_t* planes,
-                                           const Arith<LAZY>& A) {
+                                           const Arith<LAZY>& A, const uint32_t* T) {
   if constexpr (P >= 0) {
+    uint32_t Tn[16];
+    if constexpr (PF && P >= 1) tw_load<LOGB, P - 1, LOGC>(Tn, planes, g);
     if constexpr (WaveLocal<LOGB, P + 1, LOGC>::value) {
       wave_sync();
       phase_to_lds<LOGB, P + 1, LOGC>(X, x, g, c);
@@ -354,9 +421,26 @@ __device__ __forceinline__ void inv_phases(uint32_t* x, uint32_t* X, uint32_t g,
       __syncthreads();
       phase_from_lds<LOGB, P, LOGC>(x, X, g, c);
     }
-    phase_compute<LOGB, P, true, LAZY, LOGC>(x, g, planes, A);
-    inv_phases<LOGB, P - 1, LOGC, LAZY>(x, X, g, c, planes, A);
+    if constexpr (PF) {
+      tw_apply<LOGB, P, true, LAZY, LOGC>(x, T, A);
+    } else {
+      uint32_t Tp[16];
+      tw_load<LOGB, P, LOGC>(Tp, planes, g);
+      tw_apply<LOGB, P, true, LAZY, LOGC>(x, Tp, A);
+    }
+    inv_phases<LOGB, P - 1, LOGC, LAZY, PF>(x, X, g, c, planes, A, Tn);
   }
+}
+// A whole inverse transform of registers holding the spectrum in the last
+// phase's grouping; TL holds that phase's inverse twiddles (issued earlier).
+template <int LOGB, int LOGC, bool LAZY, bool PF = true>
+__device__ __forceinline__ void inv_transform(uint32_t* x, uint32_t* X, uint32_t g, uint32_t c, const uint32_t* planes,
+                                              const Arith<LAZY>& A, const uint32_t* TL) {
+  constexpr int PL = Sched<LOGB>::nph - 1;
+  uint32_t Tn[16];
+  if constexpr (PF && PL >= 1) tw_load<LOGB, PL - 1, LOGC>(Tn, planes, g);
+  tw_apply<LOGB, PL, true, LAZY, LOGC>(x, TL, A);
+  inv_phases<LOGB, PL - 1, LOGC, LAZY, PF>(x, X, g, c, planes, A, Tn);
 }
 
 // Up to K jobs passed by value at the start of the kernel arguments: a launch
@@ -471,7 +555,7 @@ enum : int { R2M_FWD = 1, R2M_INV = 2, R2M_CONV = 3, R2M_SPEC = 4, R2M_SPEC_ONLY
 #define FPXG_ROW_WPE 8
 #endif
 #ifndef FPXG_ROW_WPE_FUSED
-#define FPXG_ROW_WPE_FUSED 5
+#define FPXG_ROW_WPE_FUSED 4
 #endif
 #ifndef FPXG_COL_WPE
 #define FPXG_COL_WPE 8
@@ -502,6 +586,18 @@ k_rows2(JobPack<RowJob2, kRowPack> jp, const RowJob2* __restrict__ jobs) {
   const uint32_t base = row << LOGM;
   uint32_t* X = lds + ur * RS::lds_words_per_row;
   uint32_t x[16];
+  // The first forward phase's and the first inverse phase's twiddles, issued
+  // before any data load so that their latency hides behind it.
+  constexpr int PL = NPH - 1;  // last forward phase = first inverse phase
+  constexpr bool kHasFwd = MODE != R2M_INV && MODE != R2M_SPEC_ONLY && MODE != R2M_SPEC_SUM2;
+  constexpr bool kHasInv = MODE != R2M_FWD && MODE != R2M_FWDS;
+  // Modes held to 64 registers (the plain forward and inverse passes, which
+  // stream the largest transforms) load each phase's twiddles where they are
+  // used instead.
+  constexpr bool kPF = RowWaves<MODE>::value < 8;
+  uint32_t TF0[16], TI[16];
+  if constexpr (kPF && kHasFwd) tw_load<LOGM, 0>(TF0, J.f.planes_fwd, g);
+  if constexpr (kPF && kHasInv) tw_load<LOGM, PL>(TI, J.f.planes_inv, g);
 
   // Branch-free first-phase load: indices past in_count (or of an inactive
   // row) read a clamped valid word and are then replaced by zero.  load_issue
@@ -563,13 +659,12 @@ k_rows2(JobPack<RowJob2, kRowPack> jp, const RowJob2* __restrict__ jobs) {
   static_assert(MODE != R2M_FWDS || LAZY, "signed residues: lazy primes only");
   const ArithS Sg(J.f);
   auto forward_regs = [&](uint32_t* regs) {
+    if constexpr (!kPF) tw_load<LOGM, 0>(TF0, J.f.planes_fwd, g);
     if constexpr (MODE == R2M_FWDS) {
       // outputs below 4p in magnitude; the store brings them to [0, 2p)
-      phase_compute<LOGM, 0, false, LAZY, 0, NPH == 1 ? 4 : 2>(regs, g, J.f.planes_fwd, Sg);
-      fwd_phases<LOGM, 1, NPH, 0, LAZY, 4>(regs, X, g, 0u, J.f.planes_fwd, Sg);
+      fwd_transform<LOGM, 0, LAZY, 4, 4, kPF>(regs, X, g, 0u, J.f.planes_fwd, Sg, TF0);
     } else {
-      phase_compute<LOGM, 0, false, LAZY>(regs, g, J.f.planes_fwd, A);
-      fwd_phases<LOGM, 1, NPH, 0, LAZY>(regs, X, g, 0u, J.f.planes_fwd, A);
+      fwd_transform<LOGM, 0, LAZY, 2, 2, kPF>(regs, X, g, 0u, J.f.planes_fwd, A, TF0);
     }
   };
   auto forward = [&](const uint32_t* src, uint32_t in_count, uint32_t* regs) {
@@ -577,7 +672,6 @@ k_rows2(JobPack<RowJob2, kRowPack> jp, const RowJob2* __restrict__ jobs) {
     forward_regs(regs);
   };
 
-  constexpr int PL = NPH - 1;  // last forward phase = first inverse phase
   constexpr int rl = SC::r_of(PL);
   constexpr int NGL = PhaseGroups<LOGM, PL>::NG;
 
@@ -618,8 +712,7 @@ k_rows2(JobPack<RowJob2, kRowPack> jp, const RowJob2* __restrict__ jobs) {
       forward_regs(x);
 #pragma unroll
       for (int i = 0; i < NGL * (1 << rl); ++i) x[i] = A.mul(x[i], y[i]);
-      phase_compute<LOGM, PL, true, LAZY>(x, g, J.f.planes_inv, A);
-      inv_phases<LOGM, PL - 1, 0, LAZY>(x, X, g, 0u, J.f.planes_inv, A);
+      inv_transform<LOGM, 0, LAZY>(x, X, g, 0u, J.f.planes_inv, A, TI);
       // Register k of thread g holds position k * GT + g: the upper half
       // (k >= 8) is e's [0, k); keep e's first out_hi positions, zero the rest.
 #pragma unroll
@@ -662,8 +755,7 @@ k_rows2(JobPack<RowJob2, kRowPack> jp, const RowJob2* __restrict__ jobs) {
       // Inverse of regs into phase 0's layout (register k of thread g holds
       // position k * GT + g), times 1/N: values in [0, 2p).
       auto inverse_scaled = [&](uint32_t* r) {
-        phase_compute<LOGM, PL, true, LAZY>(r, g, J.f.planes_inv, A);
-        inv_phases<LOGM, PL - 1, 0, LAZY>(r, X, g, 0u, J.f.planes_inv, A);
+        inv_transform<LOGM, 0, LAZY>(r, X, g, 0u, J.f.planes_inv, A, TI);
 #pragma unroll
         for (int i = 0; i < 16; ++i) r[i] = A.mul(r[i], J.inv_n);
       };
@@ -748,8 +840,8 @@ k_rows2(JobPack<RowJob2, kRowPack> jp, const RowJob2* __restrict__ jobs) {
         }
     }
     // Inverse: phase PL on registers, then PL-1 .. 0 through LDS.
-    phase_compute<LOGM, PL, true, LAZY>(x, g, J.f.planes_inv, A);
-    inv_phases<LOGM, PL - 1, 0, LAZY>(x, X, g, 0u, J.f.planes_inv, A);
+    if constexpr (!kPF) tw_load<LOGM, PL>(TI, J.f.planes_inv, g);
+    inv_transform<LOGM, 0, LAZY, kPF>(x, X, g, 0u, J.f.planes_inv, A, TI);
     // Inverse outputs are in [0, 4p) (lazy); the scale brings them to [0, 2p).
     if (flags & R2_SCALE) {
 #pragma unroll
@@ -1162,8 +1254,18 @@ struct ColShape {
 // SGN (forward, p < 2^28): the phases on signed residues (ArithS); the
 // four-step twiddle product (factor < 2p, values below 4p in magnitude)
 // brings them back, + p gives the usual [0, 2p).
+// Column passes of up to 2^9 levels serve mid-size transforms, whose few
+// workgroups are latency-bound: 128 registers, room to prefetch every
+// phase's twiddles a phase ahead.  Longer columns (the streaming passes of
+// the largest transforms) keep 64 registers for two workgroups a CU.
+template <int S>
+struct ColTune {
+  static constexpr bool PF = S <= 9;
+  static constexpr int waves = S <= 9 ? 4 : FPXG_COL_WPE;
+};
+
 template <int S, int LOGC, bool LAZY, bool INV, bool SGN = false>
-__global__ void __launch_bounds__((ColShape<S, LOGC>::T)) __attribute__((amdgpu_waves_per_eu(FPXG_COL_WPE)))
+__global__ void __launch_bounds__((ColShape<S, LOGC>::T)) __attribute__((amdgpu_waves_per_eu(ColTune<S>::waves)))
 k_cols2(JobPack<ColJob2, kColPack> jp, const ColJob2* __restrict__ jobs, int L, GarnerArgs ga) {
   using SC = Sched<S>;
   static_assert(!SGN || (LAZY && !INV), "signed residues: lazy forward passes only");
@@ -1185,9 +1287,38 @@ k_cols2(JobPack<ColJob2, kColPack> jp, const ColJob2* __restrict__ jobs, int L, 
   const uint32_t shift = J.f.logmax - L;
   uint32_t x[16];
   constexpr int PL = NPH - 1;
+  constexpr int rl = SC::r_of(PL);
+  constexpr int NGL = PhaseGroups<S, PL, LOGC>::NG;
+  typedef __attribute__((address_space(1))) const uint32_t g32;
+  const uint32_t split = J.f.split, smask = (1u << split) - 1;
+  // Four-step twiddle w^(col * bitrev_S(e)) (inverse: w^-(...)).  In the last
+  // forward phase e = gid * 2^rl + k, so the exponent is col *
+  // bitrev_(S-rl)(gid) + j * col * 2^(S-rl) with j = bitrev_rl(k): one table
+  // lookup (two words, w^x = hi[x >> split] * lo[x & mask]) for each group's
+  // first factor and one for the ratio D, then a running product.  The
+  // lookups are issued here, before the data, so they cost no round trip of
+  // their own.
+  const uint32_t* twl = INV ? J.f.tw_lo_inv : J.f.tw_lo;
+  const uint32_t* twh = INV ? J.f.tw_hi_inv : J.f.tw_hi;
+  // (Passes held to 64 registers issue them where they are used.)
+  uint32_t d_hi, d_lo, q_hi[NGL], q_lo[NGL];
+  auto fourstep_load = [&] {
+    const uint32_t xd = (col << (S - rl)) << shift;
+    d_hi = *(g32*)(twh + (xd >> split));
+    d_lo = *(g32*)(twl + (xd & smask));
+#pragma unroll
+    for (int u = 0; u < NGL; ++u) {
+      const uint32_t xq = (col * bitrev(PhaseGroups<S, PL, LOGC>::gid(g, u) << rl, S)) << shift;
+      q_hi[u] = *(g32*)(twh + (xq >> split));
+      q_lo[u] = *(g32*)(twl + (xq & smask));
+    }
+  };
+  if constexpr (ColTune<S>::PF || INV) fourstep_load();
   if constexpr (!INV) {
     constexpr int r0 = SC::r_of(0);
     constexpr int NE0 = PhaseGroups<S, 0, LOGC>::NG * (1 << r0);
+    uint32_t T0[16];
+    if constexpr (ColTune<S>::PF) tw_load<S, 0, LOGC>(T0, J.f.planes_fwd, g);  // with the data
     if (J.in_count == 0) {
 #pragma unroll
       for (int i = 0; i < NE0; ++i) x[i] = 0u;
@@ -1210,23 +1341,15 @@ k_cols2(JobPack<ColJob2, kColPack> jp, const ColJob2* __restrict__ jobs, int L, 
           if (elem<S, 0>(PhaseGroups<S, 0, LOGC>::gid(g, u), k) * M + col >= J.in_count) x[u * (1 << r0) + k] = 0u;
     }
     const ArithS Sg(J.f);
-    if constexpr (SGN) {
-      phase_compute<S, 0, false, LAZY, LOGC, NPH == 1 ? 4 : 2>(x, g, J.f.planes_fwd, Sg);
-      fwd_phases<S, 1, NPH, LOGC, LAZY, 4>(x, X, g, c, J.f.planes_fwd, Sg);
-    } else {
-      phase_compute<S, 0, false, LAZY, LOGC>(x, g, J.f.planes_fwd, A);
-      fwd_phases<S, 1, NPH, LOGC, LAZY>(x, X, g, c, J.f.planes_fwd, A);
-    }
-    // Four-step twiddle w^(col * bitrev_S(e)).  In the last phase e = gid *
-    // 2^rl + k, so the exponent is col * bitrev_(S-rl)(gid) + j * col *
-    // 2^(S-rl) with j = bitrev_rl(k): one table lookup for the group's first
-    // factor, one for the ratio D, then a running product per element.
-    constexpr int rl = SC::r_of(PL);
-    const uint32_t D = twiddle<LAZY>(A, J.f.tw_lo, J.f.tw_hi, J.f.split, (col << (S - rl)) << shift);
+    if constexpr (!ColTune<S>::PF) tw_load<S, 0, LOGC>(T0, J.f.planes_fwd, g);  // after the data: registers
+    if constexpr (SGN) fwd_transform<S, LOGC, LAZY, 4, 4, ColTune<S>::PF>(x, X, g, c, J.f.planes_fwd, Sg, T0);
+    else fwd_transform<S, LOGC, LAZY, 2, 2, ColTune<S>::PF>(x, X, g, c, J.f.planes_fwd, A, T0);
+    if constexpr (!ColTune<S>::PF) fourstep_load();
+    const uint32_t D = A.mul(d_hi, d_lo);
 #pragma unroll
-    for (int u = 0; u < PhaseGroups<S, PL, LOGC>::NG; ++u) {
+    for (int u = 0; u < NGL; ++u) {
       const uint32_t gid = PhaseGroups<S, PL, LOGC>::gid(g, u);
-      uint32_t q = twiddle<LAZY>(A, J.f.tw_lo, J.f.tw_hi, J.f.split, (col * bitrev(gid << rl, S)) << shift);
+      uint32_t q = A.mul(q_hi[u], q_lo[u]);
 #pragma unroll
       for (int j = 0; j < (1 << rl); ++j) {
         const int k = (int)bitrev_c(j, rl);
@@ -1237,28 +1360,52 @@ k_cols2(JobPack<ColJob2, kColPack> jp, const ColJob2* __restrict__ jobs, int L, 
       }
     }
   } else {
-    constexpr int rl = SC::r_of(PL);
+    constexpr int r0 = SC::r_of(0);
+    constexpr int NE0 = PhaseGroups<S, 0, LOGC>::NG * (1 << r0);
     const bool neg = (flags & R2_NEG) != 0;
-    // Inverse four-step twiddle times 1/N, by the same running product.
-    const uint32_t D = twiddle<LAZY>(A, J.f.tw_lo_inv, J.f.tw_hi_inv, J.f.split, (col << (S - rl)) << shift);
+    uint32_t TI[16];
+    tw_load<S, PL, LOGC>(TI, J.f.planes_inv, g);
+    // The spectrum, in the last phase's grouping.
 #pragma unroll
-    for (int u = 0; u < PhaseGroups<S, PL, LOGC>::NG; ++u) {
-      const uint32_t gid = PhaseGroups<S, PL, LOGC>::gid(g, u);
-      uint32_t q = A.mul(twiddle<LAZY>(A, J.f.tw_lo_inv, J.f.tw_hi_inv, J.f.split, (col * bitrev(gid << rl, S)) << shift),
-                         J.inv_n);
+    for (int u = 0; u < NGL; ++u)
+#pragma unroll
+      for (int k = 0; k < (1 << rl); ++k)
+        x[u * (1 << rl) + k] = gload(J.src + elem<S, PL>(PhaseGroups<S, PL, LOGC>::gid(g, u), k) * M + col);
+    // The epilogue operand of every output (epi for R2_SUBFROM / R2_MULTAB,
+    // else the destination itself for R2_ACC / R2_ACCLO), issued with the
+    // data at clamped positions; unused values are never read.
+    const bool epi_op = (flags & (R2_SUBFROM | R2_MULTAB)) != 0;
+    const bool acc_op = (flags & (R2_ACC | R2_ACCLO)) != 0;
+    const uint32_t win = J.out_hi > J.out_shift ? J.out_hi - J.out_shift : 0u;  // output words past out_shift
+    const uint32_t elen = epi_op ? ((flags & R2_SUBFROM) ? J.epi_avail : win) : (acc_op ? win : 0u);
+    uint32_t ev[NE0];
+    if (!(flags & R2_GARNER) && elen != 0) {
+      const uint32_t* esrc = epi_op ? J.epi : J.dst;
+      const uint32_t elim = elen - 1;
+#pragma unroll
+      for (int u = 0; u < PhaseGroups<S, 0, LOGC>::NG; ++u)
+#pragma unroll
+        for (int k = 0; k < (1 << r0); ++k) {
+          const uint32_t gi = elem<S, 0>(PhaseGroups<S, 0, LOGC>::gid(g, u), k) * M + col;
+          const uint32_t t = gi >= J.out_shift ? gi - J.out_shift : 0u;
+          ev[u * (1 << r0) + k] = gload(esrc + min(t, elim));
+        }
+    }
+    // Inverse four-step twiddle times 1/N, by the same running product.
+    const uint32_t D = A.mul(d_hi, d_lo);
+#pragma unroll
+    for (int u = 0; u < NGL; ++u) {
+      uint32_t q = A.mul(A.mul(q_hi[u], q_lo[u]), J.inv_n);
 #pragma unroll
       for (int j = 0; j < (1 << rl); ++j) {
         const int k = (int)bitrev_c(j, rl);
-        const uint32_t e = elem<S, PL>(gid, k);
-        x[u * (1 << rl) + k] = A.mul(gload(J.src + e * M + col), q);
+        x[u * (1 << rl) + k] = A.mul(x[u * (1 << rl) + k], q);
         if (j + 1 < (1 << rl)) q = A.mul(q, D);
       }
     }
-    phase_compute<S, PL, true, LAZY, LOGC>(x, g, J.f.planes_inv, A);
-    inv_phases<S, PL - 1, LOGC, LAZY>(x, X, g, c, J.f.planes_inv, A);
-    constexpr int r0 = SC::r_of(0);
-    const bool acc = (flags & R2_ACC) != 0;
+    inv_transform<S, LOGC, LAZY, ColTune<S>::PF>(x, X, g, c, J.f.planes_inv, A, TI);
     if (!(flags & R2_GARNER)) {
+      const uint32_t p = J.f.p;
 #pragma unroll
       for (int u = 0; u < PhaseGroups<S, 0, LOGC>::NG; ++u)
 #pragma unroll
@@ -1267,14 +1414,20 @@ k_cols2(JobPack<ColJob2, kColPack> jp, const ColJob2* __restrict__ jobs, int L, 
           const uint32_t gi = e * M + col;
           const uint32_t v = A.canon(A.half(x[u * (1 << r0) + k]));
           if (gi >= J.out_lo && gi < J.out_hi) {
-            uint32_t w = neg ? (v == 0 ? 0u : J.f.p - v) : v;
-            if (flags & (R2_SUBFROM | R2_MULTAB))
-              w = out_epilogue(flags, J.epi, J.epi_avail, gi - J.out_shift, w, J.f.p, J.f.ninv);
-            if (acc || ((flags & R2_ACCLO) && gi - J.out_shift < J.acc_end)) {
-              const uint32_t s = gload(J.dst + (gi - J.out_shift)) + w;
-              w = min(s, s - J.f.p);
+            const uint32_t t = gi - J.out_shift;
+            uint32_t w = neg ? (v == 0 ? 0u : p - v) : v;
+            if (flags & R2_SUBFROM) {
+              const uint32_t sv = t < J.epi_avail ? ev[u * (1 << r0) + k] : 0u;
+              w = sv >= w ? sv - w : sv - w + p;
             }
-            gstore(J.dst + (gi - J.out_shift), w);
+            if (flags & R2_MULTAB) w = mulmod_full(w, ev[u * (1 << r0) + k], p, J.f.ninv);
+            if ((flags & R2_ACC) || ((flags & R2_ACCLO) && t < J.acc_end)) {
+              // the destination's old value: prefetched unless an epilogue operand took the registers
+              const uint32_t old = epi_op ? gload(J.dst + t) : ev[u * (1 << r0) + k];
+              const uint32_t s2 = old + w;
+              w = min(s2, s2 - p);
+            }
+            gstore(J.dst + t, w);
           }
         }
       return;
