@@ -16,9 +16,11 @@ Also reported on the same line:
                   (read + write N uint32), 16N per transform; traffic from the
                   FETCH_SIZE / WRITE_SIZE counter runs summarised in profiles/;
   pcie_inclusive  the 2^23 conv through the host-buffer entry point;
+  config1_conv_2p16  conv 2^16 x 2^16 mod 998244353 (device and host buffers);
+  config3_mod_1e9p7  conv 2^22 x 2^22 and inv 2^20 mod 1000000007 (auxiliary primes);
   batched         the uneven batch (8192 convs + 1024 inv/exp), merged vs per call;
   serving         a Poisson request stream through the continuous-batching
-                  server: throughput, p50 / p99 latency;
+                  server: throughput, p50 / p99 latency from arrival;
   cpu_baseline    the reference library (compiled from the reference sources into
                   oracle/_ref) timed on this host, one thread, on a bounded
                   sample of the step; its outputs are compared limb for limb.
@@ -195,6 +197,8 @@ def main():
     }
 
     result["pow_divmod"] = pow_divmod_leg(fa, args)
+    result["config1_conv_2p16"] = config1_leg(fa, args)
+    result["config3_mod_1e9p7"] = config3_leg(fa, args)
     if not args.no_batch:
         result["batched"] = batched_leg(fa, args)
     if not args.no_serve:
@@ -277,52 +281,99 @@ def ntt_only(fa, args):
     print(json.dumps({"ntt_only": True, "n": n, "reps": args.ntt_reps, "copy_bytes": 4 * n}), flush=True)
 
 
+def _dev_time(fa, fn, reps):
+    """Mean seconds per call of fn (device-resident inputs), HIP events on the
+    library stream, after one untimed call."""
+    fn()
+    fa.synchronize()
+    fa.event_record(2)
+    for _ in range(reps):
+        fn()
+    fa.event_record(3)
+    fa.synchronize()
+    return fa.event_elapsed_ms(2, 3) / 1e3 / reps
+
+
+def _uniform(rng, p, n):
+    return rng.integers(0, p, n, dtype=np.uint32)
+
+
 def pow_divmod_leg(fa, args):
-    """poly::pow(a, 1000003, 2^20) and poly::divmod of 2^20 by 2^19 limbs mod
-    998244353 (the reference bench's inputs), inputs in HBM, against the
-    reference library on one host core with its outputs compared limb for limb."""
-    sys.path.insert(0, os.path.join(ROOT, "oracle"))
-    import pyoracle as po
+    """poly::pow(a, 1000003, 2^20) (a0 = 1) and poly::divmod of 2^20 by 2^19
+    limbs mod 998244353 on uniform inputs in HBM."""
+    rng = np.random.default_rng(31)
     n = N_SERIES
-    g = po.MT(3 + n)
-    a = g.rnd(P, n)
-    a[0] = po.from_int(P, 1)
-    g = po.MT(4 + n)
-    x = g.rnd(P, n)
-    y = g.rnd(P, n // 2)
+    a = _uniform(rng, P, n)
+    a[0] = montgomery_one(P)
+    x, y = _uniform(rng, P, n), _uniform(rng, P, n // 2)
     D = fa.DeviceArray
     da, dx, dy = D.from_host(a), D.from_host(x), D.from_host(y)
     out, q, r = D(n), D(n), D(n)
-    reps = 5
-
-    def timed(fn):
-        fn()
-        fa.synchronize()
-        t0 = time.perf_counter()
-        for _ in range(reps):
-            fn()
-        fa.synchronize()
-        return (time.perf_counter() - t0) / reps
-
-    t_pow = timed(lambda: fa.pow_dev(P, da, n, 1000003, n, out))
+    t_pow = _dev_time(fa, lambda: fa.pow_dev(P, da, n, 1000003, n, out), 5)
     lens = []
-    t_div = timed(lambda: lens.append(fa.divmod_dev(P, dx, n, dy, n // 2, q, r)))
+    t_div = _dev_time(fa, lambda: lens.append(fa.divmod_dev(P, dx, n, dy, n // 2, q, r)), 5)
     lq, lr = lens[-1]
     res = {"workload": "pow(a, 1000003) at n = 2^20 and divmod of 2^20 by 2^19 limbs, mod 998244353",
            "pow_ms": round(1e3 * t_pow, 3), "divmod_ms": round(1e3 * t_div, 3),
            "pow_coeffs_per_s": n / t_pow, "divmod_coeffs_per_s": (lq + lr) / t_div}
-    if not args.no_cpu and po.ref() is not None and hasattr(po.ref(), "fpref_pow"):
-        t0 = time.perf_counter()
-        cp = po.ref_pow(P, a, 1000003, n)
-        t1 = time.perf_counter()
-        cq, cr = po.ref_divmod(P, x, y)
-        t2 = time.perf_counter()
-        res["cpu_baseline"] = {"pow_ms": round(1e3 * (t1 - t0), 1), "divmod_ms": round(1e3 * (t2 - t1), 1),
-                               "cores": 1, "kind": "reference",
-                               "bit_exact_vs_gpu": bool(np.array_equal(cp, out.to_host())
-                                                        and np.array_equal(cq, q.to_host(lq))
-                                                        and np.array_equal(cr, r.to_host(lr)))}
+    if not args.no_cpu:
+        res["cpu_baseline"] = cpu_baseline_pow_divmod(a, x, y, out.to_host(), q.to_host(lq), r.to_host(lr))
     for d in (da, dx, dy, out, q, r):
+        d.free()
+    return res
+
+
+def config1_leg(fa, args):
+    """Plain truncated convolution mod 998244353 of two 2^16-coefficient
+    polynomials (full product 2^17 - 1 limbs): inputs in HBM (fpxg_conv_dev),
+    and through the host-buffer entry point (PCIe-inclusive)."""
+    rng = np.random.default_rng(41)
+    n = 1 << 16
+    a, b = _uniform(rng, P, n), _uniform(rng, P, n)
+    D = fa.DeviceArray
+    da, db, do = D.from_host(a), D.from_host(b), D(2 * n - 1)
+    t_dev = _dev_time(fa, lambda: fa.conv_dev(P, da, n, db, n, do), 200)
+    fa.conv(P, a, b)
+    t0 = time.perf_counter()
+    for _ in range(20):
+        host = fa.conv(P, a, b)
+    t_host = (time.perf_counter() - t0) / 20
+    gpu = do.to_host()
+    res = {"workload": "conv 2^16 x 2^16 mod 998244353 (full product, 2^17 - 1 limbs)",
+           "ms": round(1e3 * t_dev, 4), "coeffs_per_s": (2 * n - 1) / t_dev,
+           "host_buffers_ms": round(1e3 * t_host, 4), "host_same_as_device": bool(np.array_equal(host, gpu)),
+           "published_ms": {"neon": 0.7005, "avx2_rosetta": 5.2948}}
+    if not args.no_cpu:
+        res["cpu_baseline"] = cpu_baseline_ops([("conv", P, a, b)], [gpu], 20)
+    for d in (da, db, do):
+        d.free()
+    return res
+
+
+def config3_leg(fa, args):
+    """Non-NTT-friendly prime 1000000007: conv of two 2^22-coefficient
+    polynomials through the auxiliary primes 167772161, 469762049, 754974721
+    with Garner back to 1e9+7, and inv at n = 2^20 with a0 uniform in [1, p);
+    inputs in HBM."""
+    Q = 1000000007
+    rng = np.random.default_rng(43)
+    n, m = 1 << 22, 1 << 20
+    a, b = _uniform(rng, Q, n), _uniform(rng, Q, n)
+    s = _uniform(rng, Q, m)
+    s[0] = int(fa.from_int(Q, int(rng.integers(1, Q))))
+    D = fa.DeviceArray
+    da, db, dc, ds, di = D.from_host(a), D.from_host(b), D(2 * n - 1), D.from_host(s), D(m)
+    t_conv = _dev_time(fa, lambda: fa.conv_dev(Q, da, n, db, n, dc), 10)
+    t_inv = _dev_time(fa, lambda: fa.series_dev("inv", Q, ds, m, m, di), 10)
+    gc, gi = dc.to_host(), di.to_host()
+    res = {"workload": "mod 1000000007: conv 2^22 x 2^22 (three auxiliary primes, Garner) and inv at n = 2^20 "
+                       "(a0 uniform in [1, p))",
+           "conv_ms": round(1e3 * t_conv, 3), "inv_ms": round(1e3 * t_inv, 3),
+           "conv_coeffs_per_s": (2 * n - 1) / t_conv, "inv_coeffs_per_s": m / t_inv,
+           "published_ms_2p16": {"conv_neon": 2.4671, "inv_neon": 4.7863}}
+    if not args.no_cpu:
+        res["cpu_baseline"] = cpu_baseline_ops([("conv", Q, a, b), ("inv", Q, s, m)], [gc, gi], 1)
+    for d in (da, db, dc, ds, di):
         d.free()
     return res
 
@@ -331,8 +382,12 @@ def serving_leg(fa, args):
     """Poisson stream of conv 50% / inv 20% / log 10% / exp 10% / sqrt 10%
     requests, lengths log-uniform in [2^10, 2^20], moduli 998244353 (70%) /
     1000000007 (30%), host inputs, submitted at their arrival times to the
-    continuous-batching server.  Latency = completion - submission on the
-    server's clock (includes upload, queueing, compute and download)."""
+    continuous-batching server by four client threads (request i by thread
+    i mod 4, so a large input being copied in does not hold back the next
+    arrival).  Latency = completion - arrival on the server's clock (client
+    queueing, the input copy, upload, compute and download); the same from
+    the submit call is reported beside it."""
+    import threading
     from fastpoly_amd.workloads import serving_stream
 
     stream = serving_stream(args.serve_requests, args.serve_rate)
@@ -340,25 +395,34 @@ def serving_leg(fa, args):
     for _, t in stream:
         reqs.append(fa.Request("conv", t[1], t[2], b=t[3]) if t[0] == "conv" else fa.Request(t[0], t[1], t[2], n=t[3]))
     out_coeffs = sum(r.out_capacity() for r in reqs)
+    feeders = 4
     with fa.Server(levels_per_iteration=args.serve_levels) as srv:
         # Warm-up: one request of each kind (fields, plans, pinned blocks).
         for r in reqs[:16]:
             srv.wait(srv.submit(r))
         t0 = srv.now() + 0.01
-        ids, lag = [], []
-        for (arr, _), r in zip(stream, reqs):
-            due = t0 + arr
-            while True:
-                now = srv.now()
-                if now >= due:
-                    break
-                if due - now > 0.0005:
-                    time.sleep(due - now - 0.0003)
-            ids.append(srv.submit(r))
-            lag.append(srv.now() - due)
+        ids, lag, due = [None] * len(reqs), [0.0] * len(reqs), [t0 + arr for arr, _ in stream]
+
+        def feed(k):
+            for i in range(k, len(reqs), feeders):
+                while True:
+                    now = srv.now()
+                    if now >= due[i]:
+                        break
+                    if due[i] - now > 0.0005:
+                        time.sleep(due[i] - now - 0.0003)
+                ids[i] = srv.submit(reqs[i])
+                lag[i] = srv.now() - due[i]
+
+        threads = [threading.Thread(target=feed, args=(k,)) for k in range(feeders)]
+        for th in threads:
+            th.start()
+        for th in threads:
+            th.join()
         res = [srv.wait(i) for i in ids]
         st = srv.stats()
-    lat = np.array([t["latency_s"] for _, t in res]) * 1e3
+    lat_submit = np.array([t["latency_s"] for _, t in res]) * 1e3
+    lat = np.array([t["done_s"] - d for (_, t), d in zip(res, due)]) * 1e3
     first = min(t["submit_s"] for _, t in res)
     last = max(t["done_s"] for _, t in res)
     errors = sum(isinstance(g, Exception) for g, _ in res)
@@ -382,7 +446,10 @@ def serving_leg(fa, args):
         "achieved_rate_per_s": round(len(reqs) / span, 1), "output_coeffs": int(out_coeffs),
         "coeffs_per_s": out_coeffs / span, "p50_ms": round(float(np.percentile(lat, 50)), 3),
         "p99_ms": round(float(np.percentile(lat, 99)), 3), "mean_ms": round(float(lat.mean()), 3),
-        "max_ms": round(float(lat.max()), 3), "submit_lag_p99_ms": round(float(np.percentile(lag, 99)) * 1e3, 3),
+        "max_ms": round(float(lat.max()), 3), "latency_from": "arrival (Poisson due time)",
+        "from_submit_p50_ms": round(float(np.percentile(lat_submit, 50)), 3),
+        "from_submit_p99_ms": round(float(np.percentile(lat_submit, 99)), 3), "client_threads": feeders,
+        "submit_lag_p99_ms": round(float(np.percentile(lag, 99)) * 1e3, 3),
         "errors": errors, "identical_to_single_calls": exact, "levels_per_iteration": args.serve_levels,
         "server": st,
     }
@@ -541,6 +608,46 @@ def cpu_parallel_baseline(reqs, gpu_outs, seconds, max_threads=16):
             "bit_exact_vs_gpu": bool(done) and all(d[2] for d in done),
             "request_ms_p50": round(float(np.percentile(lat, 50)), 3),
             "request_ms_p99": round(float(np.percentile(lat, 99)), 3)}
+
+
+def _ref_lib():
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle as po
+    return po, po.ref() is not None
+
+
+def cpu_baseline_ops(ops, gpu_outs, reps):
+    """The reference library (oracle/_ref, one thread) on the same requests:
+    best of `reps` runs of each, outputs compared limb for limb with the GPU's."""
+    po, have_ref = _ref_lib()
+    if not have_ref:
+        return None
+    times, exact = [], True
+    for op, g in zip(ops, gpu_outs):
+        best = 1e30
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            out = po.ref_conv(op[1], op[2], op[3]) if op[0] == "conv" else po.ref_series(op[0], op[1], op[2], op[3])
+            best = min(best, time.perf_counter() - t0)
+        times.append(round(1e3 * best, 3))
+        exact = exact and bool(np.array_equal(out, g))
+    return {"ms": times, "cores": 1, "kind": "reference", "backend": po.ref().fpref_backend().decode(),
+            "sample": f"best of {reps} run(s) of each operation", "bit_exact_vs_gpu": exact}
+
+
+def cpu_baseline_pow_divmod(a, x, y, g_pow, g_q, g_r):
+    po, have_ref = _ref_lib()
+    if not have_ref or not hasattr(po.ref(), "fpref_pow"):
+        return None
+    n = a.size
+    t0 = time.perf_counter()
+    cp = po.ref_pow(P, a, 1000003, n)
+    t1 = time.perf_counter()
+    cq, cr = po.ref_divmod(P, x, y)
+    t2 = time.perf_counter()
+    return {"pow_ms": round(1e3 * (t1 - t0), 1), "divmod_ms": round(1e3 * (t2 - t1), 1), "cores": 1,
+            "kind": "reference", "bit_exact_vs_gpu": bool(np.array_equal(cp, g_pow) and np.array_equal(cq, g_q)
+                                                            and np.array_equal(cr, g_r))}
 
 
 def cpu_baseline(args, a, b, inv_in, log_in, exp_in, sq_host, gpu_out):

@@ -610,6 +610,7 @@ static void context_init(Context& c) {
   c.row_log_max = big ? 14 : 13;
   c.arena = std::make_unique<Arena>(g_arena_bytes);
   c.graphs_enabled = getenv("FPXG_NO_GRAPHS") == nullptr;
+  if (const char* gf = getenv("FPXG_GARNER_FUSE_LOG")) c.garner_fuse_log = atoi(gf);
   // This device's copy of the static scratch array (the device is current).
   void* sp = nullptr;
   FPXG_HIP(hipGetSymbolAddress(&sp, HIP_SYMBOL(scratch)));
@@ -898,10 +899,13 @@ static void crt_window(Context& c, const HostField& T, const uint32_t* a, size_t
     Fs[k] = &c.field(kCrtP[k]);
     outs[k] = res[k].get();
   }
-  // Garner stays a launch of its own here: fused into the last prime's
-  // inverse pass it made lone products slower (1e9+7, 2^14 x 2^14: 0.035
-  // against 0.022 ms), where the Newton steps' window products gain from it.
-  conv_core_fields(c, Fs, primes, a, la, b, lb, outs, outc, true);
+  const int L = std::max(1, ceil_log2(la + lb - 1));
+  if (!c.rec && L >= c.garner_fuse_log) {
+    GarnerFuse gf{out, garner_const(c, T, primes, accumulate)};
+    if (conv_core_fields(c, Fs, primes, a, la, b, lb, outs, outc, true, 0, 0, &gf)) return;
+  } else {
+    conv_core_fields(c, Fs, primes, a, la, b, lb, outs, outc, true);
+  }
   GarnerRec g{};
   g.c1 = res[0].get();
   g.c2 = primes > 1 ? res[1].get() : res[0].get();

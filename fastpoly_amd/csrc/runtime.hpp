@@ -203,6 +203,7 @@ struct Context {
   int row_log_max = 14;      // largest row pass
   bool row_dma = true;       // persistent prefetching forward pass for many 2^14-word rows
   int single_log_max = 12;   // largest transform done as one row pass
+  int garner_fuse_log = 31;  // lone CRT products from 2^garner_fuse_log points fuse Garner into the last inverse pass
   int cus = 256;
   std::map<int, PassPlan> plan_override;  // autotuned splits by log2 n
   std::map<int, PassPlan> conv_override;  // autotuned splits of fused products by log2 n
