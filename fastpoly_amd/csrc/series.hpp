@@ -113,6 +113,75 @@ static void native_conv(Context& c, const HostField& T, int L, const uint32_t* x
   launch_cols(c, T.lazy, true, ij, P);
 }
 
+// One window product of native_conv_multi: out[i - lo] = (x * y)[i] for i in
+// [lo, hi) of the 2^L-point cyclic product, negated on request, with the
+// epilogue ep.
+struct WinProd {
+  const uint32_t* x;
+  size_t cx;
+  const uint32_t* y;
+  size_t cy;
+  size_t lo, hi;
+  uint32_t* out;
+  bool neg;
+  Epi ep;
+};
+
+// k <= 3 independent window products of one 2^L-point size (with a column
+// pass) in three launches shared by all of them: the forward column pass of
+// every distinct operand, the fused row pass of every product, the inverse
+// column pass into every window.  A Newton step's products are chains of
+// latency-bound launches, so two products that do not depend on each other
+// cost the launches of one.  An operand used by several products (same
+// pointer and length) is transformed once.
+static void native_conv_multi(Context& c, const HostField& T, int L, const WinProd* w, int k) {
+  const PassPlan P = c.conv_plan(L);
+  if (P.S == 0 || k < 1 || k > kRowPack) throw Error(E_DEVICE, "native_conv_multi: shape");
+  const uint32_t N = 1u << L;
+  const uint32_t* srcs[2 * kRowPack];
+  size_t cnts[2 * kRowPack];
+  DevBuf spec[2 * kRowPack], dst[kRowPack];
+  int nsrc = 0, ia[kRowPack], ib[kRowPack];
+  auto operand = [&](const uint32_t* p, size_t n) {
+    for (int i = 0; i < nsrc; ++i)
+      if (srcs[i] == p && cnts[i] == n) return i;
+    srcs[nsrc] = p;
+    cnts[nsrc] = n;
+    return nsrc++;
+  };
+  for (int i = 0; i < k; ++i) {
+    ia[i] = operand(w[i].x, w[i].cx);
+    ib[i] = operand(w[i].y, w[i].cy);
+  }
+  ColJob2 cj[2 * kRowPack];
+  for (int i = 0; i < nsrc; ++i) {
+    spec[i] = c.buf(N);
+    cj[i] = ColJob2{};
+    cj[i].src = srcs[i]; cj[i].dst = spec[i].get(); cj[i].in_count = (uint32_t)cnts[i]; cj[i].f = T.dev;
+  }
+  launch_cols_n(c, T.lazy, false, cj, P, (uint32_t)nsrc, nullptr);
+  RowJob2 rj[kRowPack];
+  for (int i = 0; i < k; ++i) {
+    dst[i] = c.buf(N);
+    rj[i] = RowJob2{};
+    rj[i].f = T.dev;
+    rj[i].src_a = spec[ia[i]].get(); rj[i].src_b = spec[ib[i]].get(); rj[i].dst = dst[i].get();
+    rj[i].in_a = N; rj[i].in_b = N; rj[i].out_lo = 0; rj[i].out_hi = N;
+  }
+  launch_rows_group(c, T.lazy, R2M_CONV, rj, k, P.b, 1u << P.S);
+  ColJob2 ij[kRowPack];
+  for (int i = 0; i < k; ++i) {
+    const Epi& ep = w[i].ep;
+    ij[i] = ColJob2{};
+    ij[i].src = dst[i].get(); ij[i].dst = w[i].out;
+    ij[i].out_lo = (uint32_t)w[i].lo; ij[i].out_hi = (uint32_t)w[i].hi; ij[i].out_shift = (uint32_t)w[i].lo;
+    ij[i].inv_n = ep.scale ? T.mul(inv_pow2(T, L), ep.scale) : inv_pow2(T, L);
+    ij[i].flags = (w[i].neg ? R2_NEG : 0u) | ep.flags; ij[i].epi = ep.src; ij[i].epi_avail = ep.avail;
+    ij[i].f = T.dev;
+  }
+  launch_cols_n(c, T.lazy, true, ij, P, (uint32_t)k, nullptr);
+}
+
 // A full doubling step of 2^L = 2m points that one launch serves: the direct
 // window products up to 2^9 points (any odd prime), the one-workgroup NTT
 // step modes up to 2^13 (NTT primes).
@@ -175,6 +244,24 @@ static void extend_inverse(Context& c, const HostField& T, const uint32_t* a, si
   }
   crt_conv(c, T, E.L, a, std::min(la, wanted), cc, k, k, wanted, e.get(), true);
   crt_conv(c, T, E.L, e.get(), hlen, cc, k, 0, hlen, cc + k);
+}
+
+// Whether extending c from k to `wanted` coefficients (extend_inverse), a
+// step of at least 2^10 points, can run its first product at 2^L points in the
+// launches of an independent product of that size (native_conv_multi): at
+// 2^L >= wanted + k points that product has no wraparound at all.  Its second
+// product stays at 2k points (one launch up to 2^13, where the one-workgroup
+// inverse step did both products in one slower launch).
+static bool cext_mergeable(Context& c, const HostField& T, size_t k, size_t wanted, int L) {
+  const int Li = ceil_log2(2 * k);
+  if (wanted <= k + 8 || Li < 10 || !series_native(T, Li) || !series_native(T, L)) return false;
+  return c.conv_plan(L).S > 0 && ((size_t)1 << L) >= wanted + k;
+}
+
+// The second half of that extension: c[k, wanted) = -(e * c)[0, wanted - k)
+// (inverse.hpp:18-65), e = (b * c)[k, wanted) computed beforehand.
+static void cext_finish(Context& c, const HostField& T, const uint32_t* e, uint32_t* cc, size_t k, size_t wanted) {
+  native_conv(c, T, ceil_log2(2 * k), e, wanted - k, cc, k, 0, wanted - k, cc + k);
 }
 
 // The first min(n, 512) coefficients of a small-series mode in one launch
@@ -301,11 +388,24 @@ static void series_emit(Context& c, const HostField& T, const SeriesReq& q, cons
           recur(c, T, REC_EXP, ap.get(), lap, nullptr, 0, b, m, m2, one, 0);
           continue;
         }
-        if (csize < hlen) {
+        Engine E(c, T, ceil_log2(2 * m));
+        // The c extension's first product and r = (a' b)[m - 1, ...) depend
+        // only on b and the old c: one set of launches for both.
+        const bool merged = csize < hlen && E.native && rtab.get() && !one_launch_step(c, T, E, m) &&
+                            cext_mergeable(c, T, csize, hlen, E.L);
+        DevBuf r;
+        if (merged) {
+          r = c.buf(hlen);
+          DevBuf e = c.buf(hlen - csize);
+          const WinProd w[2] = {{b, std::min(m, hlen), cbuf.get(), csize, csize, hlen, e.get(), true, Epi()},
+                                {ap.get(), std::min(lap, m2 - 1), b, m, m - 1, m - 1 + hlen, r.get(), false, Epi()}};
+          native_conv_multi(c, T, E.L, w, 2);
+          cext_finish(c, T, e.get(), cbuf.get(), csize, hlen);
+          csize = hlen;
+        } else if (csize < hlen) {
           extend_inverse(c, T, b, m, cbuf.get(), csize, hlen);
           csize = hlen;
         }
-        Engine E(c, T, ceil_log2(2 * m));
         if (rtab.get() && one_launch_step(c, T, E, m)) {
           // The step's three products in one workgroup (R2M_EXPSTEP).
           RowJob2 rj{};
@@ -319,9 +419,10 @@ static void series_emit(Context& c, const HostField& T, const SeriesReq& q, cons
           launch_rows(c, T.lazy, R2M_EXPSTEP, rj, E.L, 1);
           continue;
         }
-        DevBuf r = c.buf(hlen), t = c.buf(hlen);
+        if (!merged) r = c.buf(hlen);
+        DevBuf t = c.buf(hlen);
         if (E.native && rtab.get()) {
-          native_conv(c, T, E.L, ap.get(), std::min(lap, m2 - 1), b, m, m - 1, m - 1 + hlen, r.get());
+          if (!merged) native_conv(c, T, E.L, ap.get(), std::min(lap, m2 - 1), b, m, m - 1, m - 1 + hlen, r.get());
           Epi ep;
           ep.flags = R2_MULTAB;
           ep.src = rtab.get() + m;
@@ -372,11 +473,29 @@ static void series_emit(Context& c, const HostField& T, const SeriesReq& q, cons
           recur(c, T, REC_SQRT, nullptr, 0, av, au, b, m, m2, root0, tri);
           continue;
         }
-        if (csize < hlen) {
+        Engine E(c, T, ceil_log2(2 * m));
+        // res = a[v + m + i] - (b * b)[m + i]  (square_root.hpp:93-94)
+        Epi sub;
+        sub.flags = R2_SUBFROM;
+        sub.src = av + std::min(m, au);
+        sub.avail = (uint32_t)(au > m ? au - m : 0);
+        // The c extension's first product and b * b depend only on b and the
+        // old c: one set of launches for both.
+        const bool merged = csize < hlen && E.native && !one_launch_step(c, T, E, m) &&
+                            cext_mergeable(c, T, csize, hlen, E.L);
+        DevBuf res_m;
+        if (merged) {
+          res_m = c.buf(hlen);
+          DevBuf e = c.buf(hlen - csize);
+          const WinProd w[2] = {{b, std::min(m, hlen), cbuf.get(), csize, csize, hlen, e.get(), true, Epi()},
+                                {b, m, b, m, m, m2, res_m.get(), false, sub}};
+          native_conv_multi(c, T, E.L, w, 2);
+          cext_finish(c, T, e.get(), cbuf.get(), csize, hlen);
+          csize = hlen;
+        } else if (csize < hlen) {
           extend_inverse(c, T, b, m, cbuf.get(), csize, hlen);
           csize = hlen;
         }
-        Engine E(c, T, ceil_log2(2 * m));
         if (one_launch_step(c, T, E, m)) {
           // The step's two products in one workgroup (R2M_SQRTSTEP).
           RowJob2 rj{};
@@ -390,13 +509,9 @@ static void series_emit(Context& c, const HostField& T, const SeriesReq& q, cons
           launch_rows(c, T.lazy, R2M_SQRTSTEP, rj, E.L, 1);
           continue;
         }
-        DevBuf sq = c.buf(hlen), res = c.buf(hlen);
+        DevBuf sq = c.buf(hlen), res = merged ? std::move(res_m) : c.buf(hlen);
         if (E.native) {
-          Epi sub;  // res = a[v + m + i] - (b*b)[m + i]  (square_root.hpp:93-94)
-          sub.flags = R2_SUBFROM;
-          sub.src = av + std::min(m, au);
-          sub.avail = (uint32_t)(au > m ? au - m : 0);
-          native_conv(c, T, E.L, b, m, b, m, m, m2, res.get(), false, sub);
+          if (!merged) native_conv(c, T, E.L, b, m, b, m, m, m2, res.get(), false, sub);
           Epi half;  // times 1/2 (square_root.hpp:96-97)
           half.scale = half_inv;
           native_conv(c, T, E.L, res.get(), hlen, cbuf.get(), hlen, 0, hlen, b + m, false, half);
