@@ -365,6 +365,15 @@ static int g_device = 0;
 // Forward passes on signed residues (ArithS) need 8p < 2^31.
 static bool signed_ok(uint32_t p) { return p < (1u << 28); }
 
+// Elements a thread owns in a one-row launch (log2): fewer than 16 for the
+// fused products and Newton steps of 2^10 .. 2^13 points, whose single
+// workgroup would otherwise run on 1-8 waves.
+static constexpr int row_small_e(int logm, int mode) {
+  const bool fused = mode == R2M_CONV || mode == R2M_INVSTEP || mode == R2M_EXPSTEP || mode == R2M_SQRTSTEP;
+  if (!fused || logm < 10 || logm > 13) return 4;
+  return logm <= 12 ? 2 : 3;
+}
+
 // A row launch: njobs jobs from the device array dj, or (dj null) the njobs
 // by-value jobs j[0 .. njobs).
 template <int LOGM, bool LAZY, int MODE>
@@ -396,6 +405,17 @@ static void rows2_launch(Context& c, const RowJob2* j, uint32_t nrows, uint32_t 
   } else {
     JobPack<RowJob2, kRowPack> pack;
     for (int i = 0; i < kRowPack; ++i) pack.j[i] = j[(!dj && (uint32_t)i < njobs) ? i : 0];
+    // A lone row of a mid-size product or Newton step: one workgroup, so
+    // fewer elements a thread put it on more waves.
+    constexpr int ES = row_small_e(LOGM, MODE);
+    if constexpr (ES != 4) {
+      if (!dj && njobs == 1 && nrows == 1) {
+        using RSS = RowShape<LOGM, ES>;
+        hipLaunchKernelGGL((k_rows2<LOGM, LAZY, MODE, ES>), dim3(1, 1), dim3(RSS::T),
+                           (size_t)RSS::U * RSS::lds_words_per_row * 4, c.stream, pack, dj);
+        return;
+      }
+    }
     dim3 grid((nrows + RS::U - 1) / RS::U, njobs), block(RS::T);
     hipLaunchKernelGGL((k_rows2<LOGM, LAZY, MODE>), grid, block, lds, c.stream, pack, dj);
   }

@@ -175,35 +175,38 @@ __device__ __forceinline__ void dit_block(uint32_t* x, const uint32_t* tw, const
   }
 }
 
-// Phase schedule of a 2^LOGB block: radix-16 phases with the one shorter
-// phase (LOGB mod 4 levels) second, so that the first and last phases stay
-// radix 16 whenever LOGB >= 8.
-template <int LOGB>
+// Phase schedule of a 2^LOGB block whose threads own 2^E elements each
+// (E = 4 unless stated): radix-2^E phases with the one shorter phase
+// (LOGB mod E levels) second, so that the first and last phases stay radix
+// 2^E whenever LOGB >= 2E.  Fewer elements a thread (E = 2, 3) put more
+// threads on one block: the one-workgroup steps of mid-size Newton chains run
+// on 4-16 waves instead of 1-8.
+template <int LOGB, int E = 4>
 struct Sched {
-  static constexpr int a = LOGB / 4, b = LOGB % 4;
+  static constexpr int a = LOGB / E, b = LOGB % E;
   static constexpr int nph = a + (b ? 1 : 0);
   static constexpr int short_at = a >= 1 ? 1 : 0;
-  static constexpr int r_of(int p) { return (b && p == short_at) ? b : 4; }
+  static constexpr int r_of(int p) { return (b && p == short_at) ? b : E; }
   static constexpr int s_of(int p) {
     int s = 0;
     for (int i = 0; i < p; ++i) s += r_of(i);
     return s;
   }
-  // threads per unit (one 16-point group each in a radix-16 phase)
-  static constexpr int GT = LOGB >= 4 ? (1 << (LOGB - 4)) : 1;
+  // threads per unit (one 2^E-point group each in a radix-2^E phase)
+  static constexpr int GT = LOGB >= E ? (1 << (LOGB - E)) : 1;
 };
 
 // Element index within a unit for group id gid, register k of phase p.
-template <int LOGB, int P>
+template <int LOGB, int P, int E = 4>
 __device__ __forceinline__ uint32_t elem(uint32_t gid, int k) {
-  constexpr int s = Sched<LOGB>::s_of(P), r = Sched<LOGB>::r_of(P);
+  constexpr int s = Sched<LOGB, E>::s_of(P), r = Sched<LOGB, E>::r_of(P);
   constexpr int lo_bits = LOGB - s - r;
   const uint32_t lo = gid & ((1u << lo_bits) - 1), hi = gid >> lo_bits;
   return (hi << (LOGB - s)) | ((uint32_t)k << lo_bits) | lo;
 }
-template <int LOGB, int P>
+template <int LOGB, int P, int E = 4>
 __device__ __forceinline__ uint32_t group_lo(uint32_t gid) {
-  constexpr int s = Sched<LOGB>::s_of(P), r = Sched<LOGB>::r_of(P);
+  constexpr int s = Sched<LOGB, E>::s_of(P), r = Sched<LOGB, E>::r_of(P);
   constexpr int lo_bits = LOGB - s - r;
   return gid & ((1u << lo_bits) - 1);
 }
@@ -213,11 +216,11 @@ __device__ __forceinline__ uint32_t group_lo(uint32_t gid) {
 // phase with NG > 1 groups per thread, the WG threads of a wave take WG * NG
 // consecutive group ids, so a wave's groups cover whole sub-blocks whenever
 // possible and the phase transitions around them stay inside the wave.
-template <int LOGB, int P, int LOGC = 0>
+template <int LOGB, int P, int LOGC = 0, int E = 4>
 struct PhaseGroups {
-  static constexpr int r = Sched<LOGB>::r_of(P);
-  static constexpr int NG = LOGB >= 4 ? (1 << (4 - r)) : 1;
-  static constexpr int GT = Sched<LOGB>::GT;
+  static constexpr int r = Sched<LOGB, E>::r_of(P);
+  static constexpr int NG = LOGB >= E ? (1 << (E - r)) : 1;
+  static constexpr int GT = Sched<LOGB, E>::GT;
   static constexpr int WG = (64 >> LOGC) < GT ? (64 >> LOGC) : GT;
   __device__ static __forceinline__ uint32_t gid(uint32_t g, int u) {
     if constexpr (NG == 1) return g;
@@ -228,11 +231,11 @@ struct PhaseGroups {
 // Whether the exchange between phases P-1 and P stays inside each wave: true
 // when a wave's groups of phase P-1 cover whole 2^(LOGB - s(P-1)) sub-blocks
 // (then its phase-P groups are the same elements), or the unit fits a wave.
-template <int LOGB, int P, int LOGC>
+template <int LOGB, int P, int LOGC, int E = 4>
 struct WaveLocal {
-  using G = PhaseGroups<LOGB, P - 1, LOGC>;
-  static constexpr int lb = LOGB - Sched<LOGB>::s_of(P - 1) - Sched<LOGB>::r_of(P - 1);
-  static constexpr bool value = (Sched<LOGB>::GT << LOGC) <= 64 || (G::WG * G::NG >= (1 << lb));
+  using G = PhaseGroups<LOGB, P - 1, LOGC, E>;
+  static constexpr int lb = LOGB - Sched<LOGB, E>::s_of(P - 1) - Sched<LOGB, E>::r_of(P - 1);
+  static constexpr bool value = (Sched<LOGB, E>::GT << LOGC) <= 64 || (G::WG * G::NG >= (1 << lb));
 };
 
 // Orders one wave's LDS accesses (a wave's LDS operations execute in order;
@@ -243,37 +246,20 @@ __device__ __forceinline__ void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// Compute phase P on registers (forward or inverse).  AR is Arith<LAZY>, or
-// ArithS for a forward phase on signed residues with outputs below BOUT * p.
-template <int LOGB, int P, bool INV, bool LAZY, int LOGC = 0, int BOUT = 2, class AR = Arith<LAZY>>
-__device__ __forceinline__ void phase_compute(uint32_t* x, uint32_t g, const uint32_t* planes, const AR& A) {
-  constexpr int r = Sched<LOGB>::r_of(P);
-  constexpr int NG = PhaseGroups<LOGB, P, LOGC>::NG;
-  const uint32_t* plane = planes + plane_off(LOGB - Sched<LOGB>::s_of(P), r);
-#pragma unroll
-  for (int u = 0; u < NG; ++u) {
-    uint32_t tw[16];
-    load_tw<r>(tw, plane, group_lo<LOGB, P>(PhaseGroups<LOGB, P, LOGC>::gid(g, u)));
-    constexpr bool lo0 = LOGB - Sched<LOGB>::s_of(P) - r == 0;
-    if constexpr (INV) dit_block<r, LAZY, lo0>(x + u * (1 << r), tw, A);
-    else dif_any<r, lo0, BOUT, LAZY>(x + u * (1 << r), tw, A);
-  }
-}
-
 // Register <-> LDS for phase P.  The unit's element e lives at padded word
 // pidx(e << LOGC | c) (LOGC = 0 for rows, with a per-row base).  For a group
 // (hi, lo) of phase P the padded address is base(gid) + off(k) with off a
 // compile-time function of k whenever the block spans >= 32 words, so every
 // LDS access uses an immediate offset.
-template <int LOGB, int P, int LOGC>
+template <int LOGB, int P, int LOGC, int E = 4>
 struct LdsMap {
-  static constexpr int s = Sched<LOGB>::s_of(P), r = Sched<LOGB>::r_of(P);
+  static constexpr int s = Sched<LOGB, E>::s_of(P), r = Sched<LOGB, E>::r_of(P);
   static constexpr int lo_bits = LOGB - s - r;
   static constexpr int span = LOGB - s + LOGC;  // log2 words of one hi-block
   static constexpr bool affine = span >= 5;
   // unpadded word of (hi, k, lo, c)
   __device__ static __forceinline__ uint32_t word(uint32_t gid, int k, uint32_t c) {
-    return (elem<LOGB, P>(gid, k) << LOGC) | c;
+    return (elem<LOGB, P, E>(gid, k) << LOGC) | c;
   }
   __device__ static __forceinline__ uint32_t base(uint32_t gid, uint32_t c) { return pidx(word(gid, 0, c)); }
   // pidx(word(gid, k, c)) - base(gid, c), independent of gid and c when affine
@@ -285,14 +271,14 @@ struct LdsMap {
   }
 };
 
-template <int LOGB, int P, int LOGC>
+template <int LOGB, int P, int LOGC, int E = 4>
 __device__ __forceinline__ void phase_to_lds(uint32_t* X, const uint32_t* x, uint32_t g, uint32_t c) {
-  using LM = LdsMap<LOGB, P, LOGC>;
+  using LM = LdsMap<LOGB, P, LOGC, E>;
   constexpr int r = LM::r;
-  constexpr int NG = PhaseGroups<LOGB, P, LOGC>::NG;
+  constexpr int NG = PhaseGroups<LOGB, P, LOGC, E>::NG;
 #pragma unroll
   for (int u = 0; u < NG; ++u) {
-    const uint32_t gid = PhaseGroups<LOGB, P, LOGC>::gid(g, u);
+    const uint32_t gid = PhaseGroups<LOGB, P, LOGC, E>::gid(g, u);
     if constexpr (LM::affine) {
       uint32_t* b = X + LM::base(gid, c);
 #pragma unroll
@@ -303,14 +289,14 @@ __device__ __forceinline__ void phase_to_lds(uint32_t* X, const uint32_t* x, uin
     }
   }
 }
-template <int LOGB, int P, int LOGC>
+template <int LOGB, int P, int LOGC, int E = 4>
 __device__ __forceinline__ void phase_from_lds(uint32_t* x, const uint32_t* X, uint32_t g, uint32_t c) {
-  using LM = LdsMap<LOGB, P, LOGC>;
+  using LM = LdsMap<LOGB, P, LOGC, E>;
   constexpr int r = LM::r;
-  constexpr int NG = PhaseGroups<LOGB, P, LOGC>::NG;
+  constexpr int NG = PhaseGroups<LOGB, P, LOGC, E>::NG;
 #pragma unroll
   for (int u = 0; u < NG; ++u) {
-    const uint32_t gid = PhaseGroups<LOGB, P, LOGC>::gid(g, u);
+    const uint32_t gid = PhaseGroups<LOGB, P, LOGC, E>::gid(g, u);
     if constexpr (LM::affine) {
       const uint32_t* b = X + LM::base(gid, c);
 #pragma unroll
@@ -326,20 +312,20 @@ __device__ __forceinline__ void phase_from_lds(uint32_t* x, const uint32_t* X, u
 // groups, 16 words in all (some unused for r = 1).  Loading them is separate
 // from using them so that every load can be issued a phase (or a whole
 // memory round trip) before the butterflies that need it.
-template <int LOGB, int P, int LOGC = 0>
+template <int LOGB, int P, int LOGC = 0, int E = 4>
 __device__ __forceinline__ void tw_load(uint32_t* T, const uint32_t* planes, uint32_t g) {
-  constexpr int r = Sched<LOGB>::r_of(P);
-  const uint32_t* plane = planes + plane_off(LOGB - Sched<LOGB>::s_of(P), r);
+  constexpr int r = Sched<LOGB, E>::r_of(P);
+  const uint32_t* plane = planes + plane_off(LOGB - Sched<LOGB, E>::s_of(P), r);
 #pragma unroll
-  for (int u = 0; u < PhaseGroups<LOGB, P, LOGC>::NG; ++u)
-    load_tw<r>(T + u * (1 << r), plane, group_lo<LOGB, P>(PhaseGroups<LOGB, P, LOGC>::gid(g, u)));
+  for (int u = 0; u < PhaseGroups<LOGB, P, LOGC, E>::NG; ++u)
+    load_tw<r>(T + u * (1 << r), plane, group_lo<LOGB, P, E>(PhaseGroups<LOGB, P, LOGC, E>::gid(g, u)));
 }
-template <int LOGB, int P, bool INV, bool LAZY, int LOGC = 0, int BOUT = 2, class AR = Arith<LAZY>>
+template <int LOGB, int P, bool INV, bool LAZY, int LOGC = 0, int BOUT = 2, int E = 4, class AR = Arith<LAZY>>
 __device__ __forceinline__ void tw_apply(uint32_t* x, const uint32_t* T, const AR& A) {
-  constexpr int r = Sched<LOGB>::r_of(P);
-  constexpr bool lo0 = LOGB - Sched<LOGB>::s_of(P) - r == 0;
+  constexpr int r = Sched<LOGB, E>::r_of(P);
+  constexpr bool lo0 = LOGB - Sched<LOGB, E>::s_of(P) - r == 0;
 #pragma unroll
-  for (int u = 0; u < PhaseGroups<LOGB, P, LOGC>::NG; ++u) {
+  for (int u = 0; u < PhaseGroups<LOGB, P, LOGC, E>::NG; ++u) {
     if constexpr (INV) dit_block<r, LAZY, lo0>(x + u * (1 << r), T + u * (1 << r), A);
     else dif_any<r, lo0, BOUT, LAZY>(x + u * (1 << r), T + u * (1 << r), A);
     keep_pad<r>(T + u * (1 << r));
@@ -348,18 +334,18 @@ __device__ __forceinline__ void tw_apply(uint32_t* x, const uint32_t* T, const A
 
 // Register exchange between phases P-1 (registers) and P (registers) of a
 // forward transform, through LDS.
-template <int LOGB, int P, int LOGC>
+template <int LOGB, int P, int LOGC, int E = 4>
 __device__ __forceinline__ void fwd_exchange(uint32_t* x, uint32_t* X, uint32_t g, uint32_t c) {
-  if constexpr (WaveLocal<LOGB, P, LOGC>::value) {
+  if constexpr (WaveLocal<LOGB, P, LOGC, E>::value) {
     wave_sync();
-    phase_to_lds<LOGB, P - 1, LOGC>(X, x, g, c);
+    phase_to_lds<LOGB, P - 1, LOGC, E>(X, x, g, c);
     wave_sync();
-    phase_from_lds<LOGB, P, LOGC>(x, X, g, c);
+    phase_from_lds<LOGB, P, LOGC, E>(x, X, g, c);
   } else {
     if constexpr (P > 1) __syncthreads();  // earlier exchanges may still be reading
-    phase_to_lds<LOGB, P - 1, LOGC>(X, x, g, c);
+    phase_to_lds<LOGB, P - 1, LOGC, E>(X, x, g, c);
     __syncthreads();
-    phase_from_lds<LOGB, P, LOGC>(x, X, g, c);
+    phase_from_lds<LOGB, P, LOGC, E>(x, X, g, c);
   }
 }
 
@@ -372,75 +358,77 @@ __device__ __forceinline__ void fwd_exchange(uint32_t* x, uint32_t* X, uint32_t 
 // BLAST * p.
 // PF = false (kernels held to 64 registers): no prefetch; each phase loads
 // its own twiddles after its exchange and T is unused.
-template <int LOGB, int P, int END, int LOGC, bool LAZY, int BLAST = 2, bool PF = true, class AR = Arith<LAZY>>
+template <int LOGB, int P, int END, int LOGC, bool LAZY, int BLAST = 2, bool PF = true, int E = 4,
+          class AR = Arith<LAZY>>
 __device__ __forceinline__ void fwd_phases(uint32_t* x, uint32_t* X, uint32_t g, uint32_t c, const uint32_t* planes,
                                            const AR& A, const uint32_t* T) {
   if constexpr (P < END) {
     uint32_t Tn[16];
-    if constexpr (PF && P + 1 < END) tw_load<LOGB, P + 1, LOGC>(Tn, planes, g);
-    fwd_exchange<LOGB, P, LOGC>(x, X, g, c);
+    if constexpr (PF && P + 1 < END) tw_load<LOGB, P + 1, LOGC, E>(Tn, planes, g);
+    fwd_exchange<LOGB, P, LOGC, E>(x, X, g, c);
     if constexpr (PF) {
-      tw_apply<LOGB, P, false, LAZY, LOGC, P + 1 == END ? BLAST : 2>(x, T, A);
+      tw_apply<LOGB, P, false, LAZY, LOGC, P + 1 == END ? BLAST : 2, E>(x, T, A);
     } else {
       uint32_t Tp[16];
-      tw_load<LOGB, P, LOGC>(Tp, planes, g);
-      tw_apply<LOGB, P, false, LAZY, LOGC, P + 1 == END ? BLAST : 2>(x, Tp, A);
+      tw_load<LOGB, P, LOGC, E>(Tp, planes, g);
+      tw_apply<LOGB, P, false, LAZY, LOGC, P + 1 == END ? BLAST : 2, E>(x, Tp, A);
     }
-    fwd_phases<LOGB, P + 1, END, LOGC, LAZY, BLAST, PF>(x, X, g, c, planes, A, Tn);
+    fwd_phases<LOGB, P + 1, END, LOGC, LAZY, BLAST, PF, E>(x, X, g, c, planes, A, Tn);
   }
 }
 // A whole forward transform of registers holding phase 0's input; T0 holds
 // phase 0's twiddles (tw_load<LOGB, 0, LOGC>, best issued with the data
 // loads).  BOUT0: output bound of a one-phase transform (signed residues).
-template <int LOGB, int LOGC, bool LAZY, int BLAST = 2, int BOUT0 = 2, bool PF = true, class AR = Arith<LAZY>>
+template <int LOGB, int LOGC, bool LAZY, int BLAST = 2, int BOUT0 = 2, bool PF = true, int E = 4,
+          class AR = Arith<LAZY>>
 __device__ __forceinline__ void fwd_transform(uint32_t* x, uint32_t* X, uint32_t g, uint32_t c, const uint32_t* planes,
                                               const AR& A, const uint32_t* T0) {
-  constexpr int NPH = Sched<LOGB>::nph;
+  constexpr int NPH = Sched<LOGB, E>::nph;
   uint32_t T1[16];
-  if constexpr (PF && NPH > 1) tw_load<LOGB, 1, LOGC>(T1, planes, g);
-  tw_apply<LOGB, 0, false, LAZY, LOGC, NPH == 1 ? BOUT0 : 2>(x, T0, A);
-  fwd_phases<LOGB, 1, NPH, LOGC, LAZY, BLAST, PF>(x, X, g, c, planes, A, T1);
+  if constexpr (PF && NPH > 1) tw_load<LOGB, 1, LOGC, E>(T1, planes, g);
+  tw_apply<LOGB, 0, false, LAZY, LOGC, NPH == 1 ? BOUT0 : 2, E>(x, T0, A);
+  fwd_phases<LOGB, 1, NPH, LOGC, LAZY, BLAST, PF, E>(x, X, g, c, planes, A, T1);
 }
 
 // Inverse phases P, P-1, ..., 0 where registers hold phase P+1's inverse
 // output and T holds phase P's twiddles.
-template <int LOGB, int P, int LOGC, bool LAZY, bool PF = true>
+template <int LOGB, int P, int LOGC, bool LAZY, bool PF = true, int E = 4>
 __device__ __forceinline__ void inv_phases(uint32_t* x, uint32_t* X, uint32_t g, uint32_t c, const uint32_t* planes,
                                            const Arith<LAZY>& A, const uint32_t* T) {
   if constexpr (P >= 0) {
     uint32_t Tn[16];
-    if constexpr (PF && P >= 1) tw_load<LOGB, P - 1, LOGC>(Tn, planes, g);
-    if constexpr (WaveLocal<LOGB, P + 1, LOGC>::value) {
+    if constexpr (PF && P >= 1) tw_load<LOGB, P - 1, LOGC, E>(Tn, planes, g);
+    if constexpr (WaveLocal<LOGB, P + 1, LOGC, E>::value) {
       wave_sync();
-      phase_to_lds<LOGB, P + 1, LOGC>(X, x, g, c);
+      phase_to_lds<LOGB, P + 1, LOGC, E>(X, x, g, c);
       wave_sync();
-      phase_from_lds<LOGB, P, LOGC>(x, X, g, c);
+      phase_from_lds<LOGB, P, LOGC, E>(x, X, g, c);
     } else {
       __syncthreads();  // a forward pass or an earlier exchange may still be reading
-      phase_to_lds<LOGB, P + 1, LOGC>(X, x, g, c);
+      phase_to_lds<LOGB, P + 1, LOGC, E>(X, x, g, c);
       __syncthreads();
-      phase_from_lds<LOGB, P, LOGC>(x, X, g, c);
+      phase_from_lds<LOGB, P, LOGC, E>(x, X, g, c);
     }
     if constexpr (PF) {
-      tw_apply<LOGB, P, true, LAZY, LOGC>(x, T, A);
+      tw_apply<LOGB, P, true, LAZY, LOGC, 2, E>(x, T, A);
     } else {
       uint32_t Tp[16];
-      tw_load<LOGB, P, LOGC>(Tp, planes, g);
-      tw_apply<LOGB, P, true, LAZY, LOGC>(x, Tp, A);
+      tw_load<LOGB, P, LOGC, E>(Tp, planes, g);
+      tw_apply<LOGB, P, true, LAZY, LOGC, 2, E>(x, Tp, A);
     }
-    inv_phases<LOGB, P - 1, LOGC, LAZY, PF>(x, X, g, c, planes, A, Tn);
+    inv_phases<LOGB, P - 1, LOGC, LAZY, PF, E>(x, X, g, c, planes, A, Tn);
   }
 }
 // A whole inverse transform of registers holding the spectrum in the last
 // phase's grouping; TL holds that phase's inverse twiddles (issued earlier).
-template <int LOGB, int LOGC, bool LAZY, bool PF = true>
+template <int LOGB, int LOGC, bool LAZY, bool PF = true, int E = 4>
 __device__ __forceinline__ void inv_transform(uint32_t* x, uint32_t* X, uint32_t g, uint32_t c, const uint32_t* planes,
                                               const Arith<LAZY>& A, const uint32_t* TL) {
-  constexpr int PL = Sched<LOGB>::nph - 1;
+  constexpr int PL = Sched<LOGB, E>::nph - 1;
   uint32_t Tn[16];
-  if constexpr (PF && PL >= 1) tw_load<LOGB, PL - 1, LOGC>(Tn, planes, g);
-  tw_apply<LOGB, PL, true, LAZY, LOGC>(x, TL, A);
-  inv_phases<LOGB, PL - 1, LOGC, LAZY, PF>(x, X, g, c, planes, A, Tn);
+  if constexpr (PF && PL >= 1) tw_load<LOGB, PL - 1, LOGC, E>(Tn, planes, g);
+  tw_apply<LOGB, PL, true, LAZY, LOGC, 2, E>(x, TL, A);
+  inv_phases<LOGB, PL - 1, LOGC, LAZY, PF, E>(x, X, g, c, planes, A, Tn);
 }
 
 // Up to K jobs passed by value at the start of the kernel arguments: a launch
@@ -477,9 +465,9 @@ __device__ __forceinline__ Job load_job(const Job& j0, const Job* jobs) {
 // ---------------------------------------------------------------------------
 // Row pass: U rows of 2^LOGM contiguous words per workgroup.
 // ---------------------------------------------------------------------------
-template <int LOGM>
+template <int LOGM, int E = 4>
 struct RowShape {
-  static constexpr int GT = Sched<LOGM>::GT;
+  static constexpr int GT = Sched<LOGM, E>::GT;
   static constexpr int U = GT >= 64 ? 1 : 64 / GT;
   static constexpr int T = GT * U;
   static constexpr uint32_t M = 1u << LOGM;
@@ -567,11 +555,15 @@ struct RowWaves {
                                                                                              : FPXG_ROW_WPE_FUSED;
 };
 
-template <int LOGM, bool LAZY, int MODE>
-__global__ void __launch_bounds__(RowShape<LOGM>::T) __attribute__((amdgpu_waves_per_eu(RowWaves<MODE>::value)))
+// E: log2 of the elements a thread owns (4; 2 or 3 for the one-row launches
+// of mid-size Newton steps and products, which then run on 4x or 2x the
+// waves).
+template <int LOGM, bool LAZY, int MODE, int E = 4>
+__global__ void __launch_bounds__((RowShape<LOGM, E>::T)) __attribute__((amdgpu_waves_per_eu(RowWaves<MODE>::value)))
 k_rows2(JobPack<RowJob2, kRowPack> jp, const RowJob2* __restrict__ jobs) {
-  using RS = RowShape<LOGM>;
-  using SC = Sched<LOGM>;
+  using RS = RowShape<LOGM, E>;
+  using SC = Sched<LOGM, E>;
+  constexpr int NE = 1 << E;  // elements a thread owns
   constexpr int NPH = SC::nph;
   extern __shared__ uint32_t lds[];
   const RowJob2 J = load_job(jp.j[0], jobs);
@@ -596,8 +588,8 @@ k_rows2(JobPack<RowJob2, kRowPack> jp, const RowJob2* __restrict__ jobs) {
   // used instead.
   constexpr bool kPF = RowWaves<MODE>::value < 8;
   uint32_t TF0[16], TI[16];
-  if constexpr (kPF && kHasFwd) tw_load<LOGM, 0>(TF0, J.f.planes_fwd, g);
-  if constexpr (kPF && kHasInv) tw_load<LOGM, PL>(TI, J.f.planes_inv, g);
+  if constexpr (kPF && kHasFwd) tw_load<LOGM, 0, 0, E>(TF0, J.f.planes_fwd, g);
+  if constexpr (kPF && kHasInv) tw_load<LOGM, PL, 0, E>(TI, J.f.planes_inv, g);
 
   // Branch-free first-phase load: indices past in_count (or of an inactive
   // row) read a clamped valid word and are then replaced by zero.  load_issue
@@ -611,9 +603,9 @@ k_rows2(JobPack<RowJob2, kRowPack> jp, const RowJob2* __restrict__ jobs) {
   // is a uniform base plus the thread's 32-bit offset (no 64-bit address
   // arithmetic per load).
   constexpr int r0 = SC::r_of(0);
-  constexpr int NE0 = PhaseGroups<LOGM, 0>::NG * (1 << r0);
+  constexpr int NE0 = PhaseGroups<LOGM, 0, 0, E>::NG * (1 << r0);
   auto whole_row = [&](uint32_t in_count) {
-    return RS::U == 1 && PhaseGroups<LOGM, 0>::NG == 1 && base + RS::M <= in_count;
+    return RS::U == 1 && PhaseGroups<LOGM, 0, 0, E>::NG == 1 && base + RS::M <= in_count;
   };
   auto load_issue = [&](const uint32_t* src, uint32_t in_count, uint32_t* dst_regs) {
     if (in_count == 0) return;
@@ -625,11 +617,11 @@ k_rows2(JobPack<RowJob2, kRowPack> jp, const RowJob2* __restrict__ jobs) {
     }
     const uint32_t last = in_count - 1;
 #pragma unroll
-    for (int u = 0; u < PhaseGroups<LOGM, 0>::NG; ++u)
+    for (int u = 0; u < PhaseGroups<LOGM, 0, 0, E>::NG; ++u)
 #pragma unroll
       for (int k = 0; k < (1 << r0); ++k)
         dst_regs[u * (1 << r0) + k] =
-            gload(src + min(base + elem<LOGM, 0>(PhaseGroups<LOGM, 0>::gid(g, u), k), last));
+            gload(src + min(base + elem<LOGM, 0, E>(PhaseGroups<LOGM, 0, 0, E>::gid(g, u), k), last));
   };
   auto load_fix = [&](uint32_t in_count, uint32_t* dst_regs) {
     if (in_count == 0) {
@@ -643,10 +635,10 @@ k_rows2(JobPack<RowJob2, kRowPack> jp, const RowJob2* __restrict__ jobs) {
     }
     if (whole_row(in_count)) return;
 #pragma unroll
-    for (int u = 0; u < PhaseGroups<LOGM, 0>::NG; ++u)
+    for (int u = 0; u < PhaseGroups<LOGM, 0, 0, E>::NG; ++u)
 #pragma unroll
       for (int k = 0; k < (1 << r0); ++k) {
-        const uint32_t gi = base + elem<LOGM, 0>(PhaseGroups<LOGM, 0>::gid(g, u), k);
+        const uint32_t gi = base + elem<LOGM, 0, E>(PhaseGroups<LOGM, 0, 0, E>::gid(g, u), k);
         if (!(active && gi < in_count)) dst_regs[u * (1 << r0) + k] = 0u;
       }
   };
@@ -659,12 +651,12 @@ k_rows2(JobPack<RowJob2, kRowPack> jp, const RowJob2* __restrict__ jobs) {
   static_assert(MODE != R2M_FWDS || LAZY, "signed residues: lazy primes only");
   const ArithS Sg(J.f);
   auto forward_regs = [&](uint32_t* regs) {
-    if constexpr (!kPF) tw_load<LOGM, 0>(TF0, J.f.planes_fwd, g);
+    if constexpr (!kPF) tw_load<LOGM, 0, 0, E>(TF0, J.f.planes_fwd, g);
     if constexpr (MODE == R2M_FWDS) {
       // outputs below 4p in magnitude; the store brings them to [0, 2p)
-      fwd_transform<LOGM, 0, LAZY, 4, 4, kPF>(regs, X, g, 0u, J.f.planes_fwd, Sg, TF0);
+      fwd_transform<LOGM, 0, LAZY, 4, 4, kPF, E>(regs, X, g, 0u, J.f.planes_fwd, Sg, TF0);
     } else {
-      fwd_transform<LOGM, 0, LAZY, 2, 2, kPF>(regs, X, g, 0u, J.f.planes_fwd, A, TF0);
+      fwd_transform<LOGM, 0, LAZY, 2, 2, kPF, E>(regs, X, g, 0u, J.f.planes_fwd, A, TF0);
     }
   };
   auto forward = [&](const uint32_t* src, uint32_t in_count, uint32_t* regs) {
@@ -673,7 +665,7 @@ k_rows2(JobPack<RowJob2, kRowPack> jp, const RowJob2* __restrict__ jobs) {
   };
 
   constexpr int rl = SC::r_of(PL);
-  constexpr int NGL = PhaseGroups<LOGM, PL>::NG;
+  constexpr int NGL = PhaseGroups<LOGM, PL, 0, E>::NG;
 
   // Canonical outputs: min(v, v - p) with p, or a no-op min(v, v) with 0.
   const uint32_t canon_p = (flags & R2_CANON) ? J.f.p : 0u;
@@ -683,7 +675,7 @@ k_rows2(JobPack<RowJob2, kRowPack> jp, const RowJob2* __restrict__ jobs) {
     for (int u = 0; u < NGL; ++u)
 #pragma unroll
       for (int k = 0; k < (1 << rl); ++k) {
-        const uint32_t gi = base + elem<LOGM, PL>(PhaseGroups<LOGM, PL>::gid(g, u), k);
+        const uint32_t gi = base + elem<LOGM, PL, E>(PhaseGroups<LOGM, PL, 0, E>::gid(g, u), k);
         const uint32_t v = MODE == R2M_FWDS ? Sg.mul(x[u * (1 << rl) + k], Sg.one) + Sg.p : x[u * (1 << rl) + k];
         if (active) gstore(J.dst + gi, min(v, v - canon_p));
       }
@@ -701,7 +693,7 @@ k_rows2(JobPack<RowJob2, kRowPack> jp, const RowJob2* __restrict__ jobs) {
 #pragma unroll
       for (int i = 0; i < NGL * (1 << rl); ++i) x[i] = A.mul(x[i], y[i]);
     } else if constexpr (MODE == R2M_INVSTEP) {
-      static_assert(LOGM >= 4 && PhaseGroups<LOGM, 0>::NG == 1, "the half shift below needs a radix-16 first phase");
+      static_assert(LOGM >= E && PhaseGroups<LOGM, 0, 0, E>::NG == 1, "the half shift below needs a radix-2^E first phase");
       uint32_t y[16];
       load_issue(J.src_b, J.in_b, y);  // c
       load_issue(J.src_a, J.in_a, x);  // a
@@ -712,23 +704,23 @@ k_rows2(JobPack<RowJob2, kRowPack> jp, const RowJob2* __restrict__ jobs) {
       forward_regs(x);
 #pragma unroll
       for (int i = 0; i < NGL * (1 << rl); ++i) x[i] = A.mul(x[i], y[i]);
-      inv_transform<LOGM, 0, LAZY>(x, X, g, 0u, J.f.planes_inv, A, TI);
+      inv_transform<LOGM, 0, LAZY, true, E>(x, X, g, 0u, J.f.planes_inv, A, TI);
       // Register k of thread g holds position k * GT + g: the upper half
-      // (k >= 8) is e's [0, k); keep e's first out_hi positions, zero the rest.
+      // (k >= NE/2) is e's [0, k); keep e's first out_hi positions, zero the rest.
 #pragma unroll
-      for (int i = 0; i < 8; ++i) {
-        const uint32_t pos = (uint32_t)i * RowShape<LOGM>::GT + g;
-        const uint32_t v = A.mul(x[i + 8], J.inv_n);
+      for (int i = 0; i < NE / 2; ++i) {
+        const uint32_t pos = (uint32_t)i * RowShape<LOGM, E>::GT + g;
+        const uint32_t v = A.mul(x[i + NE / 2], J.inv_n);
         x[i] = pos < J.out_hi ? v : 0u;
-        x[i + 8] = 0u;
+        x[i + NE / 2] = 0u;
       }
       __syncthreads();  // the exchanges above are done with LDS
       forward_regs(x);
 #pragma unroll
       for (int i = 0; i < NGL * (1 << rl); ++i) x[i] = A.mul(x[i], y[i]);
     } else if constexpr (MODE == R2M_EXPSTEP || MODE == R2M_SQRTSTEP) {
-      static_assert(LOGM >= 4 && PhaseGroups<LOGM, 0>::NG == 1, "positions k * GT + g need a radix-16 first phase");
-      constexpr uint32_t GT = RowShape<LOGM>::GT;
+      static_assert(LOGM >= E && PhaseGroups<LOGM, 0, 0, E>::NG == 1, "positions k * GT + g need a radix-2^E first phase");
+      constexpr uint32_t GT = RowShape<LOGM, E>::GT;
       const uint32_t h = J.out_hi;
       uint32_t yb[16], yc[16], ep[16];
       load_issue(J.src_b, J.in_b, yb);  // b[0, m)
@@ -740,10 +732,10 @@ k_rows2(JobPack<RowJob2, kRowPack> jp, const RowJob2* __restrict__ jobs) {
         const uint32_t ecount = MODE == R2M_EXPSTEP ? h : J.epi_avail;
         if (ecount) {
 #pragma unroll
-          for (int i = 0; i < 16; ++i) ep[i] = gload(J.epi + min((uint32_t)i * GT + g, ecount - 1));
+          for (int i = 0; i < NE; ++i) ep[i] = gload(J.epi + min((uint32_t)i * GT + g, ecount - 1));
         } else {
 #pragma unroll
-          for (int i = 0; i < 16; ++i) ep[i] = 0u;
+          for (int i = 0; i < NE; ++i) ep[i] = 0u;
         }
       }
       load_fix(J.in_b, yb);
@@ -755,61 +747,61 @@ k_rows2(JobPack<RowJob2, kRowPack> jp, const RowJob2* __restrict__ jobs) {
       // Inverse of regs into phase 0's layout (register k of thread g holds
       // position k * GT + g), times 1/N: values in [0, 2p).
       auto inverse_scaled = [&](uint32_t* r) {
-        inv_transform<LOGM, 0, LAZY>(r, X, g, 0u, J.f.planes_inv, A, TI);
+        inv_transform<LOGM, 0, LAZY, true, E>(r, X, g, 0u, J.f.planes_inv, A, TI);
 #pragma unroll
-        for (int i = 0; i < 16; ++i) r[i] = A.mul(r[i], J.inv_n);
+        for (int i = 0; i < NE; ++i) r[i] = A.mul(r[i], J.inv_n);
       };
       if constexpr (MODE == R2M_EXPSTEP) {
         load_fix(J.in_a, x);
         forward_regs(x);
 #pragma unroll
-        for (int i = 0; i < 16; ++i) x[i] = A.mul(x[i], yb[i]);
+        for (int i = 0; i < NE; ++i) x[i] = A.mul(x[i], yb[i]);
         inverse_scaled(x);
         // r = window [aux, aux + h): a shift by aux positions through LDS
         // (aux + h <= 2^LOGM - 1, inside the row's words).
         __syncthreads();
 #pragma unroll
-        for (int i = 0; i < 16; ++i) X[pidx((uint32_t)i * GT + g)] = x[i];
+        for (int i = 0; i < NE; ++i) X[pidx((uint32_t)i * GT + g)] = x[i];
         __syncthreads();
 #pragma unroll
-        for (int i = 0; i < 16; ++i) {
+        for (int i = 0; i < NE; ++i) {
           const uint32_t pos = (uint32_t)i * GT + g;
           x[i] = pos < h ? X[pidx(pos + J.aux)] : 0u;
         }
         __syncthreads();
         forward_regs(x);
 #pragma unroll
-        for (int i = 0; i < 16; ++i) x[i] = A.mul(x[i], yc[i]);
+        for (int i = 0; i < NE; ++i) x[i] = A.mul(x[i], yc[i]);
         inverse_scaled(x);
         // t = (r * c)[0, h) / (m + i)
 #pragma unroll
-        for (int i = 0; i < 16; ++i) {
+        for (int i = 0; i < NE; ++i) {
           const uint32_t pos = (uint32_t)i * GT + g;
           x[i] = pos < h ? A.mul(x[i], ep[i]) : 0u;
         }
         __syncthreads();
         forward_regs(x);
 #pragma unroll
-        for (int i = 0; i < 16; ++i) x[i] = A.mul(x[i], yb[i]);
+        for (int i = 0; i < NE; ++i) x[i] = A.mul(x[i], yb[i]);
       } else {
 #pragma unroll
-        for (int i = 0; i < 16; ++i) x[i] = A.mul(yb[i], yb[i]);
+        for (int i = 0; i < NE; ++i) x[i] = A.mul(yb[i], yb[i]);
         inverse_scaled(x);
-        // (b * b)[m + i] is the upper half, registers 8..15; res = a - it.
+        // (b * b)[m + i] is the upper half, registers NE/2 .. NE-1; res = a - it.
         const uint32_t p = J.f.p;
 #pragma unroll
-        for (int i = 0; i < 8; ++i) {
+        for (int i = 0; i < NE / 2; ++i) {
           const uint32_t pos = (uint32_t)i * GT + g;
-          const uint32_t v = A.canon(x[i + 8]);
+          const uint32_t v = A.canon(x[i + NE / 2]);
           const uint32_t s = pos < J.epi_avail ? ep[i] : 0u;
           const uint32_t r = s >= v ? s - v : s - v + p;
           x[i] = pos < h ? r : 0u;
-          x[i + 8] = 0u;
+          x[i + NE / 2] = 0u;
         }
         __syncthreads();
         forward_regs(x);
 #pragma unroll
-        for (int i = 0; i < 16; ++i) x[i] = A.mul(A.mul(x[i], yc[i]), J.aux);  // times aux = 1/2
+        for (int i = 0; i < NE; ++i) x[i] = A.mul(A.mul(x[i], yc[i]), J.aux);  // times aux = 1/2
       }
     } else if constexpr (MODE == R2M_SPEC) {
       // The spectrum (phase PL's grouping) is loaded with the operand.
@@ -819,7 +811,7 @@ k_rows2(JobPack<RowJob2, kRowPack> jp, const RowJob2* __restrict__ jobs) {
       for (int u = 0; u < NGL; ++u)
 #pragma unroll
         for (int k = 0; k < (1 << rl); ++k) {
-          const uint32_t gi = base + elem<LOGM, PL>(PhaseGroups<LOGM, PL>::gid(g, u), k);
+          const uint32_t gi = base + elem<LOGM, PL, E>(PhaseGroups<LOGM, PL, 0, E>::gid(g, u), k);
           w[u * (1 << rl) + k] = gload(J.src_b + (active ? gi : 0u));
         }
       load_fix(J.in_a, x);
@@ -831,7 +823,7 @@ k_rows2(JobPack<RowJob2, kRowPack> jp, const RowJob2* __restrict__ jobs) {
       for (int u = 0; u < NGL; ++u)
 #pragma unroll
         for (int k = 0; k < (1 << rl); ++k) {
-          const uint32_t gi = active ? base + elem<LOGM, PL>(PhaseGroups<LOGM, PL>::gid(g, u), k) : 0u;
+          const uint32_t gi = active ? base + elem<LOGM, PL, E>(PhaseGroups<LOGM, PL, 0, E>::gid(g, u), k) : 0u;
           uint32_t v = gload(J.src_a + gi);
           if constexpr (MODE == R2M_SPEC_ONLY) v = A.mul(v, gload(J.src_b + gi));
           if constexpr (MODE == R2M_SPEC_SUM2)
@@ -840,8 +832,8 @@ k_rows2(JobPack<RowJob2, kRowPack> jp, const RowJob2* __restrict__ jobs) {
         }
     }
     // Inverse: phase PL on registers, then PL-1 .. 0 through LDS.
-    if constexpr (!kPF) tw_load<LOGM, PL>(TI, J.f.planes_inv, g);
-    inv_transform<LOGM, 0, LAZY, kPF>(x, X, g, 0u, J.f.planes_inv, A, TI);
+    if constexpr (!kPF) tw_load<LOGM, PL, 0, E>(TI, J.f.planes_inv, g);
+    inv_transform<LOGM, 0, LAZY, kPF, E>(x, X, g, 0u, J.f.planes_inv, A, TI);
     // Inverse outputs are in [0, 4p) (lazy); the scale brings them to [0, 2p).
     if (flags & R2_SCALE) {
 #pragma unroll
@@ -855,10 +847,10 @@ k_rows2(JobPack<RowJob2, kRowPack> jp, const RowJob2* __restrict__ jobs) {
       for (int i = 0; i < NE0; ++i) x[i] = x[i] == 0 ? 0u : A.R - x[i];
     }
 #pragma unroll
-    for (int u = 0; u < PhaseGroups<LOGM, 0>::NG; ++u)
+    for (int u = 0; u < PhaseGroups<LOGM, 0, 0, E>::NG; ++u)
 #pragma unroll
       for (int k = 0; k < (1 << r0); ++k) {
-        const uint32_t gi = base + elem<LOGM, 0>(PhaseGroups<LOGM, 0>::gid(g, u), k);
+        const uint32_t gi = base + elem<LOGM, 0, E>(PhaseGroups<LOGM, 0, 0, E>::gid(g, u), k);
         const uint32_t v = x[u * (1 << r0) + k];
         if (active && gi >= J.out_lo && gi < J.out_hi) {
           uint32_t w = min(v, v - canon_p);
