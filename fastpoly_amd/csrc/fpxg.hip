@@ -374,6 +374,10 @@ static constexpr int row_small_e(int logm, int mode) {
   return logm <= 12 ? 2 : 3;
 }
 
+static constexpr int row_multi_e(int logm, int mode) {
+  return (mode == R2M_CONV && logm >= 6 && logm <= 10) ? 2 : 4;
+}
+
 // A row launch: njobs jobs from the device array dj, or (dj null) the njobs
 // by-value jobs j[0 .. njobs).
 template <int LOGM, bool LAZY, int MODE>
@@ -413,6 +417,18 @@ static void rows2_launch(Context& c, const RowJob2* j, uint32_t nrows, uint32_t 
         using RSS = RowShape<LOGM, ES>;
         hipLaunchKernelGGL((k_rows2<LOGM, LAZY, MODE, ES>), dim3(1, 1), dim3(RSS::T),
                            (size_t)RSS::U * RSS::lds_words_per_row * 4, c.stream, pack, dj);
+        return;
+      }
+    }
+    // The row pass of a mid-size four-step product (rows of 2^6 .. 2^10
+    // words): four elements a thread, so each thread issues a quarter of the
+    // loads (the pass is latency-bound at these sizes).
+    constexpr int EM = row_multi_e(LOGM, MODE);
+    if constexpr (EM != 4) {
+      if (!dj && nrows > 1) {
+        using RSM = RowShape<LOGM, EM>;
+        hipLaunchKernelGGL((k_rows2<LOGM, LAZY, MODE, EM>), dim3((nrows + RSM::U - 1) / RSM::U, njobs),
+                           dim3(RSM::T), (size_t)RSM::U * RSM::lds_words_per_row * 4, c.stream, pack, dj);
         return;
       }
     }
