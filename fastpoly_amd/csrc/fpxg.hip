@@ -375,7 +375,7 @@ static constexpr int row_small_e(int logm, int mode) {
 }
 
 static constexpr int row_multi_e(int logm, int mode) {
-  return (mode == R2M_CONV && logm >= 6 && logm <= 10) ? 2 : 4;
+  return (mode == R2M_CONV && logm >= 6 && logm <= 12) ? 2 : 4;
 }
 
 // A row launch: njobs jobs from the device array dj, or (dj null) the njobs
@@ -420,12 +420,12 @@ static void rows2_launch(Context& c, const RowJob2* j, uint32_t nrows, uint32_t 
         return;
       }
     }
-    // The row pass of a mid-size four-step product (rows of 2^6 .. 2^10
-    // words): four elements a thread, so each thread issues a quarter of the
-    // loads (the pass is latency-bound at these sizes).
+    // The row pass of a four-step product of up to 2^21 points (rows of
+    // 2^6 .. 2^12 words): four elements a thread, so each thread issues a
+    // quarter of the loads (the pass is latency-bound at these sizes).
     constexpr int EM = row_multi_e(LOGM, MODE);
     if constexpr (EM != 4) {
-      if (!dj && nrows > 1) {
+      if (!dj && nrows > 1 && ((size_t)nrows << LOGM) <= ((size_t)1 << 21)) {
         using RSM = RowShape<LOGM, EM>;
         hipLaunchKernelGGL((k_rows2<LOGM, LAZY, MODE, EM>), dim3((nrows + RSM::U - 1) / RSM::U, njobs),
                            dim3(RSM::T), (size_t)RSM::U * RSM::lds_words_per_row * 4, c.stream, pack, dj);
@@ -543,6 +543,19 @@ static void cols2_launch_c(Context& c, const ColJob2* j, const PassPlan& P, uint
   const size_t lds = (size_t)CS::words * 4;
   JobPack<ColJob2, kColPack> pack;
   for (int i = 0; i < kColPack; ++i) pack.j[i] = j[(!dj && (uint32_t)i < njobs) ? i : 0];
+  // Columns of lone calls and Newton steps up to 2^20 points (2^5 .. 2^10
+  // levels): four elements a thread (at most 1024 threads a tile, so 8
+  // columns a tile at 2^9, 4 at 2^10).  Merged batches and larger transforms
+  // keep 16: their launches are throughput-bound.
+  if constexpr (S >= 5 && S <= 10) {
+    if (!dj && P.L <= 20) {
+      constexpr int LC = LOGC < 10 - (S - 2) ? LOGC : 10 - (S - 2);
+      using CS2 = ColShape<S, LC, 2>;
+      hipLaunchKernelGGL((k_cols2<S, LC, LAZY, INV, SGN, 2>), dim3(1u << (P.b - LC), njobs), dim3(CS2::T),
+                         (size_t)CS2::words * 4, c.stream, pack, dj, P.L, ga);
+      return;
+    }
+  }
   dim3 grid(1u << (P.b - LOGC), njobs), block(CS::T);
   hipLaunchKernelGGL((k_cols2<S, LOGC, LAZY, INV, SGN>), grid, block, lds, c.stream, pack, dj, P.L, ga);
 }

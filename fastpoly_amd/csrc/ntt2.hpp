@@ -1235,9 +1235,9 @@ struct ColJob2 {
 // the three auxiliary primes.
 constexpr int kColPack = 6;
 
-template <int S, int LOGC>
+template <int S, int LOGC, int E = 4>
 struct ColShape {
-  static constexpr int GT = Sched<S>::GT;
+  static constexpr int GT = Sched<S, E>::GT;
   static constexpr int C = 1 << LOGC;
   static constexpr int T = GT * C;
   static constexpr uint32_t words = lds_words((1u << S) << LOGC);
@@ -1256,10 +1256,12 @@ struct ColTune {
   static constexpr int waves = S <= 9 ? 4 : FPXG_COL_WPE;
 };
 
-template <int S, int LOGC, bool LAZY, bool INV, bool SGN = false>
-__global__ void __launch_bounds__((ColShape<S, LOGC>::T)) __attribute__((amdgpu_waves_per_eu(ColTune<S>::waves)))
+// E: log2 of the elements a thread owns (4; 2 for the by-value launches of
+// mid-size columns, whose threads then issue a quarter of the loads).
+template <int S, int LOGC, bool LAZY, bool INV, bool SGN = false, int E = 4>
+__global__ void __launch_bounds__((ColShape<S, LOGC, E>::T)) __attribute__((amdgpu_waves_per_eu(ColTune<S>::waves)))
 k_cols2(JobPack<ColJob2, kColPack> jp, const ColJob2* __restrict__ jobs, int L, GarnerArgs ga) {
-  using SC = Sched<S>;
+  using SC = Sched<S, E>;
   static_assert(!SGN || (LAZY && !INV), "signed residues: lazy forward passes only");
   constexpr int NPH = SC::nph;
   constexpr int C = 1 << LOGC;
@@ -1280,7 +1282,7 @@ k_cols2(JobPack<ColJob2, kColPack> jp, const ColJob2* __restrict__ jobs, int L, 
   uint32_t x[16];
   constexpr int PL = NPH - 1;
   constexpr int rl = SC::r_of(PL);
-  constexpr int NGL = PhaseGroups<S, PL, LOGC>::NG;
+  constexpr int NGL = PhaseGroups<S, PL, LOGC, E>::NG;
   typedef __attribute__((address_space(1))) const uint32_t g32;
   const uint32_t split = J.f.split, smask = (1u << split) - 1;
   // Four-step twiddle w^(col * bitrev_S(e)) (inverse: w^-(...)).  In the last
@@ -1300,7 +1302,7 @@ k_cols2(JobPack<ColJob2, kColPack> jp, const ColJob2* __restrict__ jobs, int L, 
     d_lo = *(g32*)(twl + (xd & smask));
 #pragma unroll
     for (int u = 0; u < NGL; ++u) {
-      const uint32_t xq = (col * bitrev(PhaseGroups<S, PL, LOGC>::gid(g, u) << rl, S)) << shift;
+      const uint32_t xq = (col * bitrev(PhaseGroups<S, PL, LOGC, E>::gid(g, u) << rl, S)) << shift;
       q_hi[u] = *(g32*)(twh + (xq >> split));
       q_lo[u] = *(g32*)(twl + (xq & smask));
     }
@@ -1308,9 +1310,9 @@ k_cols2(JobPack<ColJob2, kColPack> jp, const ColJob2* __restrict__ jobs, int L, 
   if constexpr (ColTune<S>::PF || INV) fourstep_load();
   if constexpr (!INV) {
     constexpr int r0 = SC::r_of(0);
-    constexpr int NE0 = PhaseGroups<S, 0, LOGC>::NG * (1 << r0);
+    constexpr int NE0 = PhaseGroups<S, 0, LOGC, E>::NG * (1 << r0);
     uint32_t T0[16];
-    if constexpr (ColTune<S>::PF) tw_load<S, 0, LOGC>(T0, J.f.planes_fwd, g);  // with the data
+    if constexpr (ColTune<S>::PF) tw_load<S, 0, LOGC, E>(T0, J.f.planes_fwd, g);  // with the data
     if (J.in_count == 0) {
 #pragma unroll
       for (int i = 0; i < NE0; ++i) x[i] = 0u;
@@ -1318,34 +1320,34 @@ k_cols2(JobPack<ColJob2, kColPack> jp, const ColJob2* __restrict__ jobs, int L, 
       // Unconditional (clamped) loads first so they overlap; lift and zero fill after.
       const uint32_t last = J.in_count - 1;
 #pragma unroll
-      for (int u = 0; u < PhaseGroups<S, 0, LOGC>::NG; ++u)
+      for (int u = 0; u < PhaseGroups<S, 0, LOGC, E>::NG; ++u)
 #pragma unroll
         for (int k = 0; k < (1 << r0); ++k)
-          x[u * (1 << r0) + k] = gload(J.src + min(elem<S, 0>(PhaseGroups<S, 0, LOGC>::gid(g, u), k) * M + col, last));
+          x[u * (1 << r0) + k] = gload(J.src + min(elem<S, 0, E>(PhaseGroups<S, 0, LOGC, E>::gid(g, u), k) * M + col, last));
       if (lift) {
 #pragma unroll
         for (int i = 0; i < NE0; ++i) x[i] = A.mul(x[i], J.f.r2);
       }
 #pragma unroll
-      for (int u = 0; u < PhaseGroups<S, 0, LOGC>::NG; ++u)
+      for (int u = 0; u < PhaseGroups<S, 0, LOGC, E>::NG; ++u)
 #pragma unroll
         for (int k = 0; k < (1 << r0); ++k)
-          if (elem<S, 0>(PhaseGroups<S, 0, LOGC>::gid(g, u), k) * M + col >= J.in_count) x[u * (1 << r0) + k] = 0u;
+          if (elem<S, 0, E>(PhaseGroups<S, 0, LOGC, E>::gid(g, u), k) * M + col >= J.in_count) x[u * (1 << r0) + k] = 0u;
     }
     const ArithS Sg(J.f);
-    if constexpr (!ColTune<S>::PF) tw_load<S, 0, LOGC>(T0, J.f.planes_fwd, g);  // after the data: registers
-    if constexpr (SGN) fwd_transform<S, LOGC, LAZY, 4, 4, ColTune<S>::PF>(x, X, g, c, J.f.planes_fwd, Sg, T0);
-    else fwd_transform<S, LOGC, LAZY, 2, 2, ColTune<S>::PF>(x, X, g, c, J.f.planes_fwd, A, T0);
+    if constexpr (!ColTune<S>::PF) tw_load<S, 0, LOGC, E>(T0, J.f.planes_fwd, g);  // after the data: registers
+    if constexpr (SGN) fwd_transform<S, LOGC, LAZY, 4, 4, ColTune<S>::PF, E>(x, X, g, c, J.f.planes_fwd, Sg, T0);
+    else fwd_transform<S, LOGC, LAZY, 2, 2, ColTune<S>::PF, E>(x, X, g, c, J.f.planes_fwd, A, T0);
     if constexpr (!ColTune<S>::PF) fourstep_load();
     const uint32_t D = A.mul(d_hi, d_lo);
 #pragma unroll
     for (int u = 0; u < NGL; ++u) {
-      const uint32_t gid = PhaseGroups<S, PL, LOGC>::gid(g, u);
+      const uint32_t gid = PhaseGroups<S, PL, LOGC, E>::gid(g, u);
       uint32_t q = A.mul(q_hi[u], q_lo[u]);
 #pragma unroll
       for (int j = 0; j < (1 << rl); ++j) {
         const int k = (int)bitrev_c(j, rl);
-        const uint32_t e = elem<S, PL>(gid, k);
+        const uint32_t e = elem<S, PL, E>(gid, k);
         const uint32_t v = SGN ? Sg.mul(x[u * (1 << rl) + k], q) + Sg.p : A.mul(x[u * (1 << rl) + k], q);
         gstore(J.dst + e * M + col, min(v, v - canon_p));
         if (j + 1 < (1 << rl)) q = A.mul(q, D);
@@ -1353,16 +1355,16 @@ k_cols2(JobPack<ColJob2, kColPack> jp, const ColJob2* __restrict__ jobs, int L, 
     }
   } else {
     constexpr int r0 = SC::r_of(0);
-    constexpr int NE0 = PhaseGroups<S, 0, LOGC>::NG * (1 << r0);
+    constexpr int NE0 = PhaseGroups<S, 0, LOGC, E>::NG * (1 << r0);
     const bool neg = (flags & R2_NEG) != 0;
     uint32_t TI[16];
-    tw_load<S, PL, LOGC>(TI, J.f.planes_inv, g);
+    tw_load<S, PL, LOGC, E>(TI, J.f.planes_inv, g);
     // The spectrum, in the last phase's grouping.
 #pragma unroll
     for (int u = 0; u < NGL; ++u)
 #pragma unroll
       for (int k = 0; k < (1 << rl); ++k)
-        x[u * (1 << rl) + k] = gload(J.src + elem<S, PL>(PhaseGroups<S, PL, LOGC>::gid(g, u), k) * M + col);
+        x[u * (1 << rl) + k] = gload(J.src + elem<S, PL, E>(PhaseGroups<S, PL, LOGC, E>::gid(g, u), k) * M + col);
     // The epilogue operand of every output (epi for R2_SUBFROM / R2_MULTAB,
     // else the destination itself for R2_ACC / R2_ACCLO), issued with the
     // data at clamped positions; unused values are never read.
@@ -1375,10 +1377,10 @@ k_cols2(JobPack<ColJob2, kColPack> jp, const ColJob2* __restrict__ jobs, int L, 
       const uint32_t* esrc = epi_op ? J.epi : J.dst;
       const uint32_t elim = elen - 1;
 #pragma unroll
-      for (int u = 0; u < PhaseGroups<S, 0, LOGC>::NG; ++u)
+      for (int u = 0; u < PhaseGroups<S, 0, LOGC, E>::NG; ++u)
 #pragma unroll
         for (int k = 0; k < (1 << r0); ++k) {
-          const uint32_t gi = elem<S, 0>(PhaseGroups<S, 0, LOGC>::gid(g, u), k) * M + col;
+          const uint32_t gi = elem<S, 0, E>(PhaseGroups<S, 0, LOGC, E>::gid(g, u), k) * M + col;
           const uint32_t t = gi >= J.out_shift ? gi - J.out_shift : 0u;
           ev[u * (1 << r0) + k] = gload(esrc + min(t, elim));
         }
@@ -1395,14 +1397,14 @@ k_cols2(JobPack<ColJob2, kColPack> jp, const ColJob2* __restrict__ jobs, int L, 
         if (j + 1 < (1 << rl)) q = A.mul(q, D);
       }
     }
-    inv_transform<S, LOGC, LAZY, ColTune<S>::PF>(x, X, g, c, J.f.planes_inv, A, TI);
+    inv_transform<S, LOGC, LAZY, ColTune<S>::PF, E>(x, X, g, c, J.f.planes_inv, A, TI);
     if (!(flags & R2_GARNER)) {
       const uint32_t p = J.f.p;
 #pragma unroll
-      for (int u = 0; u < PhaseGroups<S, 0, LOGC>::NG; ++u)
+      for (int u = 0; u < PhaseGroups<S, 0, LOGC, E>::NG; ++u)
 #pragma unroll
         for (int k = 0; k < (1 << r0); ++k) {
-          const uint32_t e = elem<S, 0>(PhaseGroups<S, 0, LOGC>::gid(g, u), k);
+          const uint32_t e = elem<S, 0, E>(PhaseGroups<S, 0, LOGC, E>::gid(g, u), k);
           const uint32_t gi = e * M + col;
           const uint32_t v = A.canon(A.half(x[u * (1 << r0) + k]));
           if (gi >= J.out_lo && gi < J.out_hi) {
@@ -1427,10 +1429,10 @@ k_cols2(JobPack<ColJob2, kColPack> jp, const ColJob2* __restrict__ jobs, int L, 
     // R2_GARNER: the other primes' residues at each output position.
     const uint32_t np = ga.g.primes;
 #pragma unroll
-    for (int u = 0; u < PhaseGroups<S, 0, LOGC>::NG; ++u)
+    for (int u = 0; u < PhaseGroups<S, 0, LOGC, E>::NG; ++u)
 #pragma unroll
       for (int k = 0; k < (1 << r0); ++k) {
-        const uint32_t gi = elem<S, 0>(PhaseGroups<S, 0, LOGC>::gid(g, u), k) * M + col;
+        const uint32_t gi = elem<S, 0, E>(PhaseGroups<S, 0, LOGC, E>::gid(g, u), k) * M + col;
         if (gi < J.out_lo || gi >= J.out_hi) continue;
         const uint32_t v = A.canon(A.half(x[u * (1 << r0) + k]));
         const uint32_t o = gi - J.out_shift;
