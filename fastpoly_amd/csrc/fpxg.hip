@@ -1201,21 +1201,46 @@ struct Upload {
 
 namespace fpxg {
 
-// Microseconds per run of fn: the least of three timed trials of `reps` runs
-// each (after one untimed run), so a trial disturbed by other work is ignored.
+// Microseconds per run of fn on the device alone: `reps` runs captured into
+// one hipGraph (after an untimed eager run that warms the arena, so nothing
+// is allocated during capture), the least of three timed replays.  Eager
+// timing would charge every launch its host overhead, which is as long as
+// the short passes being compared and is not paid by graph replay.
 template <class Fn>
-static float time_us(Context& c, int reps, Fn&& fn) {
+static float time_us_graph(Context& c, int reps, Fn&& fn) {
   fn();
-  float best = 1e30f;
-  for (int trial = 0; trial < 3; ++trial) {
-    FPXG_HIP(hipEventRecord(c.events[14], c.stream));
+  FPXG_HIP(hipStreamSynchronize(c.stream));
+  hipGraph_t g = nullptr;
+  FPXG_HIP(hipStreamBeginCapture(c.stream, hipStreamCaptureModeThreadLocal));
+  try {
     for (int r = 0; r < reps; ++r) fn();
-    FPXG_HIP(hipEventRecord(c.events[15], c.stream));
-    FPXG_HIP(hipEventSynchronize(c.events[15]));
-    float ms = 0;
-    FPXG_HIP(hipEventElapsedTime(&ms, c.events[14], c.events[15]));
-    best = std::min(best, 1e3f * ms / (float)reps);
+  } catch (...) {
+    (void)hipStreamEndCapture(c.stream, &g);
+    if (g) (void)hipGraphDestroy(g);
+    throw;
   }
+  FPXG_HIP(hipStreamEndCapture(c.stream, &g));
+  hipGraphExec_t ge = nullptr;
+  const hipError_t ie = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(g);
+  if (ie != hipSuccess) throw Error(E_DEVICE, std::string("hipGraphInstantiate: ") + hipGetErrorString(ie));
+  float best = 1e30f;
+  try {
+    FPXG_HIP(hipGraphLaunch(ge, c.stream));
+    for (int trial = 0; trial < 3; ++trial) {
+      FPXG_HIP(hipEventRecord(c.events[14], c.stream));
+      FPXG_HIP(hipGraphLaunch(ge, c.stream));
+      FPXG_HIP(hipEventRecord(c.events[15], c.stream));
+      FPXG_HIP(hipEventSynchronize(c.events[15]));
+      float ms = 0;
+      FPXG_HIP(hipEventElapsedTime(&ms, c.events[14], c.events[15]));
+      best = std::min(best, 1e3f * ms / (float)reps);
+    }
+  } catch (...) {
+    (void)hipGraphExecDestroy(ge);
+    throw;
+  }
+  (void)hipGraphExecDestroy(ge);
   return best;
 }
 
@@ -1459,7 +1484,8 @@ int fpxg_autotune(int min_log, int max_log, int reps, int* chosen_col_levels, fl
       std::vector<float> times;
       for (const PassPlan& P : cand) {
         c.plan_override[L] = P;
-        times.push_back(time_us(c, reps, [&] { ntt_forward(c, F, buf.get(), buf.get(), 1u << L, L, false, false); }));
+        times.push_back(time_us_graph(c, std::max(reps, 8),
+                                      [&] { ntt_forward(c, F, buf.get(), buf.get(), 1u << L, L, false, false); }));
       }
       float best = 0;
       const PassPlan pick = tuned_pick(cand, times, def, &best);
@@ -1471,17 +1497,15 @@ int fpxg_autotune(int min_log, int max_log, int reps, int* chosen_col_levels, fl
     // one workgroup's launch up to 2^13 points, else every admissible split.
     DevBuf out = c.buf((size_t)1 << max_log);
     for (int L = std::max(min_log, 13); L <= max_log; ++L) {
-      const HostField& F = c.field(L <= 25 ? kCrtP[0] : kCrtP[1]);
+      // Timed over 998244353 (the Newton chains' prime, unsigned forward
+      // passes) where it has the transform, else an auxiliary prime.
+      const HostField& F = c.field(L <= 23 ? 998244353u : (L <= 25 ? kCrtP[0] : kCrtP[1]));
       const size_t half = (size_t)1 << (L - 1);
       c.conv_override.erase(L);
       const PassPlan def = c.conv_plan_base(L);
       // Splits within two column levels of the default (and one workgroup
-      // while the size fits it): the product's passes are short, latency-bound
-      // launches, where far-off splits only win by timing noise.
-      // One workgroup only where it is also the default (<= 2^13): the eager
-      // timing here charges each launch its host gap, which graph replay (the
-      // series calls) does not pay, and a one-workgroup 2^14 product occupies
-      // a single CU for ~25 us against ~17 us split.
+      // while the size fits it, up to 2^13): the product's passes are short,
+      // latency-bound launches, where far-off splits only win by noise.
       const int s_ref = def.S ? def.S : std::max(1, L / 2);
       std::vector<PassPlan> cand;
       if (L <= std::min(c.row_log_max, 13)) cand.push_back({L, L, 0, 0});
@@ -1491,7 +1515,7 @@ int fpxg_autotune(int min_log, int max_log, int reps, int* chosen_col_levels, fl
       std::vector<float> times;
       for (const PassPlan& P : cand) {
         c.conv_override[L] = P;
-        times.push_back(time_us(c, reps, [&] {
+        times.push_back(time_us_graph(c, std::max(reps, 8), [&] {
           native_conv(c, F, L, buf.get(), half, buf.get() + half, half, 0, 2 * half, out.get());
         }));
       }
