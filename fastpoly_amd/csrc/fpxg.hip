@@ -387,13 +387,18 @@ static void rows2_launch(Context& c, const RowJob2* j, uint32_t nrows, uint32_t 
   // Many whole rows of 2^14 words, forward: the persistent prefetching pass.
   if constexpr (LOGM == kDmaRowLog && (MODE == R2M_FWD || MODE == R2M_FWDS)) {
     // 16-byte row loads (LDS DMA) and stores need 16-byte aligned rows.
-    if (!dj && njobs == 1 && c.row_dma && !(j[0].flags & R2_LIFT) && nrows >= 2u * (uint32_t)c.cus &&
-        (size_t)j[0].in_a >= ((size_t)nrows << LOGM) && ((uintptr_t)j[0].src_a & 15) == 0 &&
-        ((uintptr_t)j[0].dst & 15) == 0) {
-      RowJob2 jj = j[0];
-      jj.nrows = nrows;
+    bool dma = !dj && c.row_dma && (size_t)nrows * njobs >= 2u * (size_t)c.cus;
+    for (uint32_t i = 0; dma && i < njobs; ++i)
+      dma = !(j[i].flags & R2_LIFT) && (size_t)j[i].in_a >= ((size_t)nrows << LOGM) &&
+            ((uintptr_t)j[i].src_a & 15) == 0 && ((uintptr_t)j[i].dst & 15) == 0;
+    if (dma) {
+      JobPack<RowJob2, kRowPack> jp;
+      for (int i = 0; i < kRowPack; ++i) {
+        jp.j[i] = j[(uint32_t)i < njobs ? i : 0];
+        jp.j[i].nrows = nrows;
+      }
       hipLaunchKernelGGL((k_rows_fwd_dma<LAZY, MODE == R2M_FWDS>), dim3((uint32_t)c.cus), dim3(RS::T), 0, c.stream,
-                         jj);
+                         jp, njobs);
       return;
     }
   }
@@ -785,6 +790,43 @@ void ntt_forward(Context& c, const HostField& F, uint32_t* dst, const uint32_t* 
   launch_rows(c, F.lazy, R2M_FWD, rj, P.b, rj.nrows);
 }
 
+// k forward transforms of one size and field (canonical outputs), their
+// column passes kColPack jobs to a launch and their row passes kRowPack: the
+// split product's spectra, a few launches that each fill the GPU several
+// times over instead of one under-filled launch per pass and transform.
+static void ntt_forward_multi(Context& c, const HostField& F, uint32_t* const* dst, const uint32_t* const* src,
+                              const uint32_t* in_count, int k, int L) {
+  // Transforms of 2^22 points and more: rows of 2^14 words, so that the row
+  // passes of all k transforms run as one persistent prefetching pass.
+  const PassPlan P = (L >= 22 && L - kDmaRowLog <= 12 && c.row_log_max >= kDmaRowLog)
+                         ? PassPlan{L, kDmaRowLog, L - kDmaRowLog, tile_log_cols(L - kDmaRowLog, L)}
+                         : c.plan(L);
+  if (P.S == 0 || c.rec) {
+    for (int i = 0; i < k; ++i) ntt_forward(c, F, dst[i], src[i], in_count[i], L, false, true);
+    return;
+  }
+  const uint32_t N = 1u << L;
+  for (int i0 = 0; i0 < k; i0 += kColPack) {
+    ColJob2 cj[kColPack];
+    const int n = std::min(kColPack, k - i0);
+    for (int i = 0; i < n; ++i) {
+      cj[i] = ColJob2{};
+      cj[i].src = src[i0 + i]; cj[i].dst = dst[i0 + i]; cj[i].in_count = in_count[i0 + i]; cj[i].f = F.dev;
+    }
+    launch_cols_n(c, F.lazy, false, cj, P, (uint32_t)n, nullptr);
+  }
+  for (int i0 = 0; i0 < k; i0 += kRowPack) {
+    RowJob2 rj[kRowPack];
+    const int n = std::min(kRowPack, k - i0);
+    for (int i = 0; i < n; ++i) {
+      rj[i] = RowJob2{};
+      rj[i].f = F.dev; rj[i].out_lo = 0; rj[i].out_hi = N; rj[i].flags = R2_CANON;
+      rj[i].src_a = dst[i0 + i]; rj[i].dst = dst[i0 + i]; rj[i].in_a = N;
+    }
+    launch_rows_group(c, F.lazy, R2M_FWD, rj, n, P.b, 1u << P.S);
+  }
+}
+
 void ntt_inverse(Context& c, const HostField& F, uint32_t* dst, const uint32_t* src, int L, uint32_t out_lo,
                  uint32_t out_hi, uint32_t out_shift) {
   // src is clobbered (row pass in place); dst may alias src only if out_shift == 0.
@@ -992,14 +1034,18 @@ static void split_conv(Context& c, const HostField& T, const uint32_t* a, size_t
   const size_t pa = (la + h - 1) / h, pb = (lb + h - 1) / h;
   const bool same = a == b && la == lb;
   std::vector<DevBuf> SA(pa), SB(same ? 0 : pb);
+  std::vector<uint32_t*> fd;
+  std::vector<const uint32_t*> fs;
+  std::vector<uint32_t> fc;
   for (size_t i = 0; i < pa; ++i) {
     SA[i] = c.buf(N);
-    ntt_forward(c, T, SA[i].get(), a + i * h, (uint32_t)std::min(h, la - i * h), M, false, true);
+    fd.push_back(SA[i].get()); fs.push_back(a + i * h); fc.push_back((uint32_t)std::min(h, la - i * h));
   }
   for (size_t j = 0; j < SB.size(); ++j) {
     SB[j] = c.buf(N);
-    ntt_forward(c, T, SB[j].get(), b + j * h, (uint32_t)std::min(h, lb - j * h), M, false, true);
+    fd.push_back(SB[j].get()); fs.push_back(b + j * h); fc.push_back((uint32_t)std::min(h, lb - j * h));
   }
+  ntt_forward_multi(c, T, fd.data(), fs.data(), fc.data(), (int)fd.size(), M);
   auto spec_b = [&](size_t j) -> const uint32_t* { return same ? SA[j].get() : SB[j].get(); };
   // Block k covers [k h, k h + 2h - 1); it overlaps block k - 1 on its first
   // h - 1 positions only: block 0 stores, every later block adds below h - 1

@@ -905,7 +905,7 @@ constexpr int kDmaRowLog = 14;
 // back to [0, 2p) by one product with the Montgomery one and + p.
 template <bool LAZY, bool SGN>
 __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(4)))
-k_rows_fwd_dma(RowJob2 j0) {
+k_rows_fwd_dma(JobPack<RowJob2, kRowPack> jp, uint32_t njobs) {
   constexpr int LOGM = kDmaRowLog;
   static_assert(!SGN || LAZY, "signed residues: lazy primes only");
   using SC = Sched<LOGM>;
@@ -916,7 +916,14 @@ k_rows_fwd_dma(RowJob2 j0) {
   static_assert(RowShape<LOGM>::T == 1024 && RowShape<LOGM>::U == 1, "one row a workgroup");
   __shared__ uint32_t LA[1u << LOGM];          // prefetched row, unpadded
   __shared__ uint32_t LE[lds_words(1u << LOGM)];  // phase exchanges, padded
-  const RowJob2 J = load_job(j0, (const RowJob2*)nullptr);
+  // njobs jobs of J.nrows rows each (one field, flags and size; their own
+  // src / dst): rows are numbered across the jobs.
+  const RowJob2 J = load_job(jp.j[0], (const RowJob2*)nullptr);
+  const uint32_t total = J.nrows * njobs;
+  auto row_src = [&](uint32_t r) -> const uint32_t* {
+    return jp.j[r / J.nrows].src_a + ((size_t)(r % J.nrows) << LOGM);
+  };
+  auto row_dst = [&](uint32_t r) -> uint32_t* { return jp.j[r / J.nrows].dst + ((size_t)(r % J.nrows) << LOGM); };
   const Arith<LAZY> A(J.f);
   const ArithS Sg(J.f);
   auto comp = [&](auto ph, uint32_t* x, const uint32_t* T) {
@@ -929,7 +936,7 @@ k_rows_fwd_dma(RowJob2 j0) {
   const uint32_t canon_p = (J.flags & R2_CANON) ? J.f.p : 0u;
   // Row r into LA: 64 chunks of 1 KiB (64 lanes x 16 bytes), four per wave.
   auto prefetch = [&](uint32_t r) {
-    const uint32_t* src = J.src_a + ((size_t)r << LOGM);
+    const uint32_t* src = row_src(r);
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
       const uint32_t chunk = wv * 4 + i;
@@ -954,7 +961,7 @@ k_rows_fwd_dma(RowJob2 j0) {
     const uint32_t next = row + gridDim.x;
     lds_barrier();  // LA consumed by every wave; the previous row's LE reads are done
     // Always issued (the last row re-reads a valid row) so the code stays straight-line.
-    prefetch(min(next, J.nrows - 1));
+    prefetch(min(next, total - 1));
     phase_to_lds<LOGM, 0, 0>(LE, x, g, 0);
     lds_barrier();
     phase_from_lds<LOGM, 1, 0>(x, LE, g, 0);
@@ -982,13 +989,14 @@ k_rows_fwd_dma(RowJob2 j0) {
     }
     wave_sync();
     wait_vmem();  // the prefetch of the next row and this wave's previous stores
+    uint32_t* const out = row_dst(row);
 #pragma unroll
     for (int j = 0; j < 16; ++j) {
       const uint32_t e = wv * 1024 + j * 64 + ln;
-      gstore(J.dst + ((size_t)row << LOGM) + e, LE[pidx(e)]);
+      gstore(out + e, LE[pidx(e)]);
     }
     row = next;
-    if (row >= J.nrows) break;
+    if (row >= total) break;
   }
   wait_vmem();  // the redundant last prefetch lands before the workgroup's LDS is released
 }
