@@ -443,9 +443,12 @@ __global__ void __launch_bounds__(256) k_scan_part(const ScanRec* __restrict__ r
 }
 
 // Stage 2: one workgroup per record merges its B <= 256 partials.
+// token != 0: out is host memory the caller polls; the fields are written
+// first and out[i].pad[0] = token last, after a system-scope fence, so a
+// host that sees the token sees the fields.
 __global__ void __launch_bounds__(256) k_scan_final(const ScanRec* __restrict__ recs, ScanPack pk,
                                                    const ScanPart* __restrict__ parts, uint32_t B,
-                                                   ScanOut* __restrict__ out) {
+                                                   ScanOut* __restrict__ out, uint32_t token) {
   __shared__ uint32_t m1s[256], m2s[256], lss[256];
   const ScanRec R = scan_rec(recs, pk, blockIdx.x);
   ScanPart q{R.n, R.n, 0u, 0u};
@@ -473,7 +476,14 @@ __global__ void __launch_bounds__(256) k_scan_final(const ScanRec* __restrict__ 
     o.last1 = lss[0];
     o.a0 = R.n ? R.a[0] : 0u;
     o.afirst = o.first < R.n ? R.a[o.first] : 0u;
-    out[blockIdx.x] = o;
+    if (token) {
+      volatile uint32_t* w = reinterpret_cast<volatile uint32_t*>(out + blockIdx.x);
+      w[0] = o.first; w[1] = o.second; w[2] = o.last1; w[3] = o.a0; w[4] = o.afirst;
+      __threadfence_system();
+      w[5] = token;
+    } else {
+      out[blockIdx.x] = o;
+    }
   }
 }
 
@@ -546,11 +556,34 @@ static std::vector<ScanOut> scan_inputs(Context& c, const std::vector<ScanRec>& 
     hipLaunchKernelGGL(k_scan_part, dim3(B, (unsigned)recs.size()), dim3(256), 0, c.stream, (const ScanRec*)nullptr,
                        pk, (ScanPart*)dpart.get(), B);
     FPXG_HIP(hipGetLastError());
+    if (++c.scan_token == 0) c.scan_token = 1;
+    const uint32_t token = c.scan_token;
     hipLaunchKernelGGL(k_scan_final, dim3((unsigned)recs.size()), dim3(256), 0, c.stream, (const ScanRec*)nullptr, pk,
-                       (const ScanPart*)dpart.get(), B, hout);
+                       (const ScanPart*)dpart.get(), B, hout, token);
     FPXG_HIP(hipGetLastError());
-    FPXG_HIP(hipStreamSynchronize(c.stream));
-    for (size_t i = 0; i < recs.size(); ++i) out[i] = hout[i];
+    // The host needs the five words, not an idle stream: poll the tokens for
+    // up to 200 us (a stream synchronisation costs ~6 us more than the poll),
+    // then fall back to synchronising, which also reports any device error.
+    volatile const uint32_t* hw = reinterpret_cast<volatile const uint32_t*>(hout);
+    auto arrived = [&] {
+      for (size_t i = 0; i < recs.size(); ++i)
+        if (hw[i * (sizeof(ScanOut) / 4) + 5] != token) return false;
+      return true;
+    };
+    const auto t0 = std::chrono::steady_clock::now();
+    while (!arrived()) {
+      if (std::chrono::steady_clock::now() - t0 > std::chrono::microseconds(200)) {
+        FPXG_HIP(hipStreamSynchronize(c.stream));
+        if (!arrived()) throw Error(E_DEVICE, "input scan did not complete");
+        break;
+      }
+    }
+    std::atomic_thread_fence(std::memory_order_acquire);
+    for (size_t i = 0; i < recs.size(); ++i) {
+      const size_t b = i * (sizeof(ScanOut) / 4);
+      out[i].first = hw[b]; out[i].second = hw[b + 1]; out[i].last1 = hw[b + 2];
+      out[i].a0 = hw[b + 3]; out[i].afirst = hw[b + 4];
+    }
     return out;
   }
   DevBuf dr = c.buf(recs.size() * sizeof(ScanRec) / 4 + 1);
@@ -563,7 +596,7 @@ static std::vector<ScanOut> scan_inputs(Context& c, const std::vector<ScanRec>& 
                        (ScanPart*)dpart.get() + s * B, B);
     FPXG_HIP(hipGetLastError());
     hipLaunchKernelGGL(k_scan_final, dim3((unsigned)cnt), dim3(256), 0, c.stream, rp, ScanPack{},
-                       (const ScanPart*)dpart.get() + s * B, B, (ScanOut*)dout.get() + s);
+                       (const ScanPart*)dpart.get() + s * B, B, (ScanOut*)dout.get() + s, 0u);
     FPXG_HIP(hipGetLastError());
   }
   FPXG_HIP(hipMemcpyAsync(out.data(), dout.get(), recs.size() * sizeof(ScanOut), hipMemcpyDeviceToHost, c.stream));

@@ -12,6 +12,8 @@
 #include <unordered_map>
 #include <unordered_set>
 #include <memory>
+#include <atomic>
+#include <chrono>
 #include <mutex>
 #include <stdexcept>
 #include <string>
@@ -234,6 +236,7 @@ struct Context {
   // Pinned host memory the input scan of one or two records writes its
   // results to (allocated on first use).
   void* scan_host = nullptr;
+  uint32_t scan_token = 0;  // last token the scan wrote into scan_host
   std::unordered_map<uint64_t, GarnerConst> garner_cache;  // by (target p, primes, accumulate)
   std::unordered_map<uint32_t, bool> prime_cache;          // modulus -> accepted
   // Outstanding batch tickets (completion events).
