@@ -63,6 +63,11 @@ def parse():
     ap.add_argument("--no-serve", action="store_true", help="skip the serving leg")
     ap.add_argument("--ntt-only", action="store_true",
                     help="only the 2^24 transform passes and a 64 MiB device copy (for counter runs)")
+    ap.add_argument("--series-only", default=None, metavar="OPS",
+                    help="profiling: only the Newton series chains at n = 2^20 (comma-separated ops), "
+                         "each called --steps times after --warmup calls (graph replay)")
+    ap.add_argument("--batch-only", action="store_true",
+                    help="profiling: only the merged uneven batch, --batch-reps times after one warm-up")
     ap.add_argument("--serve-requests", type=int, default=1000)
     ap.add_argument("--serve-rate", type=float, default=1000.0, help="offered load, requests/s (Poisson)")
     ap.add_argument("--serve-levels", type=int, default=16)
@@ -112,6 +117,12 @@ def main():
     fa.init(0, 0)
     if args.ntt_only:
         ntt_only(fa, args)
+        return
+    if args.series_only:
+        series_only(fa, args)
+        return
+    if args.batch_only:
+        batch_only(fa, args)
         return
     rng = np.random.default_rng(20261203)
     roofline = roofline_leg(fa, args, np.random.default_rng(24))
@@ -279,6 +290,67 @@ def ntt_only(fa, args):
             raise RuntimeError("hipMemcpyDtoD failed")
     hip.hipDeviceSynchronize()
     print(json.dumps({"ntt_only": True, "n": n, "reps": args.ntt_reps, "copy_bytes": 4 * n}), flush=True)
+
+
+def series_only(fa, args):
+    """The step's Newton series chains alone (after the autotuner, inputs as
+    in the step), each op called args.warmup + args.steps times on the same
+    buffers: the second call captures its hipGraph, later calls replay it."""
+    fa.autotune(13, 24, 3)
+    _, _, inv_in, log_in, exp_in, d = make_inputs(np.random.default_rng(20261203))
+    D = fa.DeviceArray
+    ins = {"inv": D.from_host(inv_in), "log": D.from_host(log_in), "exp": D.from_host(exp_in)}
+    dd = D.from_host(d)
+    ins["sqrt"] = D(N_SERIES)
+    fa.conv_dev(P, dd, N_SERIES, dd, N_SERIES, ins["sqrt"], N_SERIES)
+    out = D(N_SERIES)
+    res = {}
+    for op in args.series_only.split(","):
+        for _ in range(max(args.warmup, 2)):
+            fa.series_dev(op, P, ins[op], N_SERIES, N_SERIES, out)
+        fa.synchronize()
+        fa.event_record(2)
+        for _ in range(args.steps):
+            fa.series_dev(op, P, ins[op], N_SERIES, N_SERIES, out)
+        fa.event_record(3)
+        fa.synchronize()
+        res[op + "_ms"] = round(fa.event_elapsed_ms(2, 3) / args.steps, 4)
+    print(json.dumps({"series_only": True, "n": N_SERIES, "steps": args.steps} | res), flush=True)
+
+
+def batch_only(fa, args):
+    """The merged uneven batch alone (one warm-up submit, then
+    args.batch_reps timed submits)."""
+    from fastpoly_amd.workloads import uneven_batch
+    reqs = uneven_batch(args.batch_convs, args.batch_series)
+    sin = [(r[2].size, r[3].size if r[0] == "conv" else 0) for r in reqs]
+    sout = [r[2].size + r[3].size - 1 if r[0] == "conv" else r[3] for r in reqs]
+    oi = np.concatenate([[0], np.cumsum([x + y for x, y in sin])])
+    oo = np.concatenate([[0], np.cumsum(sout)])
+    host = np.empty(int(oi[-1]), np.uint32)
+    for i, r in enumerate(reqs):
+        o = int(oi[i])
+        host[o:o + sin[i][0]] = r[2]
+        if r[0] == "conv":
+            host[o + sin[i][0]:o + sin[i][0] + sin[i][1]] = r[3]
+    din, dout = fa.DeviceArray.from_host(host), fa.DeviceArray(int(oo[-1]))
+    arr = (fa.Req * len(reqs))()
+    for i, r in enumerate(reqs):
+        q = arr[i]
+        q.op, q.p = fa.OPS[r[0]], r[1]
+        q.a, q.la = din.ptr + 4 * int(oi[i]), sin[i][0]
+        if r[0] == "conv":
+            q.b, q.lb, q.n = din.ptr + 4 * (int(oi[i]) + sin[i][0]), sin[i][1], fa.SIZE_MAX
+        else:
+            q.n = r[3]
+        q.out, q.out_cap = dout.ptr + 4 * int(oo[i]), sout[i]
+    fa.wait(fa.batch_submit(arr))
+    times = []
+    for _ in range(args.batch_reps):
+        t0 = time.perf_counter()
+        fa.wait(fa.batch_submit(arr))
+        times.append(round(time.perf_counter() - t0, 5))
+    print(json.dumps({"batch_only": True, "requests": len(reqs), "batched_s": times}), flush=True)
 
 
 def _dev_time(fa, fn, reps):
