@@ -51,6 +51,7 @@ static LaunchKey key_of(const LaunchRec& r) {
   switch (r.kind) {
     case LK_ROWS: return pack_key(LK_ROWS, r.k0, r.k1, r.k2, r.row.nrows);
     case LK_COLS: return pack_key(LK_COLS, r.k0, r.k1, r.k2, (uint32_t)r.k3);
+    case LK_COLG: return pack_key(LK_COLG, r.k0, 1, 0, (uint32_t)r.k3);
     case LK_RECUR: return pack_key(LK_RECUR, r.rec.hi <= 128 ? 0 : 1, 0, 0, 0);
     default: return pack_key(r.kind, 0, 0, 0, 0);
   }
@@ -88,6 +89,7 @@ static size_t group_bytes(LaunchKey k, const std::pair<LaunchKey, const LaunchRe
     switch (kind) {
       case LK_ROWS: take(cnt * sizeof(RowJob2)); break;
       case LK_COLS: take(cnt * sizeof(ColJob2)); break;
+      case LK_COLG: take(cnt * sizeof(CrtColJob)); break;
       case LK_RECUR: take(cnt * sizeof(RecurRec)); break;
       default: {
         const bool ew = kind == LK_EW;
@@ -124,6 +126,12 @@ static void stage_group(Plan& plan, LaunchKey k, const std::pair<LaunchKey, cons
         st.jobs = plan.alloc(cnt * sizeof(ColJob2));
         ColJob2* d = (ColJob2*)(plan.data + st.jobs);
         for (size_t i = 0; i < cnt; ++i) d[i] = g[s + i].second->col;
+        break;
+      }
+      case LK_COLG: {
+        st.jobs = plan.alloc(cnt * sizeof(CrtColJob));
+        CrtColJob* d = (CrtColJob*)(plan.data + st.jobs);
+        for (size_t i = 0; i < cnt; ++i) d[i] = g[s + i].second->crt;
         break;
       }
       case LK_RECUR: {
@@ -178,6 +186,12 @@ static void issue(Context& c, const Staged& s, const uint8_t* base) {
       const int S = key_a(k), L = (int)key_d(k);
       const PassPlan P{L, L - S, S, tile_log_cols(S, L)};
       launch_cols(c, key_b(k) != 0, key_c(k) != 0, s.first->col, P, s.count, (const ColJob2*)(base + s.jobs));
+      return;
+    }
+    case LK_COLG: {
+      const int S = key_a(k), L = (int)key_d(k);
+      const PassPlan P{L, L - S, S, tile_log_cols(S, L)};
+      launch_cols_crt(c, &s.first->crt, P, s.count, (const CrtColJob*)(base + s.jobs));
       return;
     }
     case LK_EW:

@@ -542,8 +542,7 @@ static int tile_log_cols(int S, int L) {
 // njobs jobs from the device array dj, or (dj null) the njobs by-value jobs
 // j[0 .. njobs).
 template <int S, int LOGC, bool LAZY, bool INV, bool SGN>
-static void cols2_launch_c(Context& c, const ColJob2* j, const PassPlan& P, uint32_t njobs, const ColJob2* dj,
-                           const GarnerArgs& ga) {
+static void cols2_launch_c(Context& c, const ColJob2* j, const PassPlan& P, uint32_t njobs, const ColJob2* dj) {
   using CS = ColShape<S, LOGC>;
   const size_t lds = (size_t)CS::words * 4;
   JobPack<ColJob2, kColPack> pack;
@@ -557,31 +556,29 @@ static void cols2_launch_c(Context& c, const ColJob2* j, const PassPlan& P, uint
       constexpr int LC = LOGC < 10 - (S - 2) ? LOGC : 10 - (S - 2);
       using CS2 = ColShape<S, LC, 2>;
       hipLaunchKernelGGL((k_cols2<S, LC, LAZY, INV, SGN, 2>), dim3(1u << (P.b - LC), njobs), dim3(CS2::T),
-                         (size_t)CS2::words * 4, c.stream, pack, dj, P.L, ga);
+                         (size_t)CS2::words * 4, c.stream, pack, dj, P.L);
       return;
     }
   }
   dim3 grid(1u << (P.b - LOGC), njobs), block(CS::T);
-  hipLaunchKernelGGL((k_cols2<S, LOGC, LAZY, INV, SGN>), grid, block, lds, c.stream, pack, dj, P.L, ga);
+  hipLaunchKernelGGL((k_cols2<S, LOGC, LAZY, INV, SGN>), grid, block, lds, c.stream, pack, dj, P.L);
 }
 template <int S, bool LAZY, bool INV, bool SGN>
-static void cols2_launch(Context& c, const ColJob2* j, const PassPlan& P, uint32_t njobs, const ColJob2* dj,
-                         const GarnerArgs& ga) {
+static void cols2_launch(Context& c, const ColJob2* j, const PassPlan& P, uint32_t njobs, const ColJob2* dj) {
   if constexpr (S == 10) {
     if (P.logC == 3) {
-      cols2_launch_c<S, 3, LAZY, INV, SGN>(c, j, P, njobs, dj, ga);
+      cols2_launch_c<S, 3, LAZY, INV, SGN>(c, j, P, njobs, dj);
       return;
     }
   }
-  cols2_launch_c<S, col_tile_log(S), LAZY, INV, SGN>(c, j, P, njobs, dj, ga);
+  cols2_launch_c<S, col_tile_log(S), LAZY, INV, SGN>(c, j, P, njobs, dj);
 }
 
 template <bool LAZY, bool INV, bool SGN = false>
-static void cols2_size(Context& c, const ColJob2* j, const PassPlan& P, uint32_t njobs, const ColJob2* dj,
-                       const GarnerArgs& ga) {
+static void cols2_size(Context& c, const ColJob2* j, const PassPlan& P, uint32_t njobs, const ColJob2* dj) {
   switch (P.S) {
 #define FPXG_C(S) \
-  case S: cols2_launch<S, LAZY, INV, SGN>(c, j, P, njobs, dj, ga); break;
+  case S: cols2_launch<S, LAZY, INV, SGN>(c, j, P, njobs, dj); break;
     FPXG_C(1) FPXG_C(2) FPXG_C(3) FPXG_C(4) FPXG_C(5) FPXG_C(6)
     FPXG_C(7) FPXG_C(8) FPXG_C(9) FPXG_C(10) FPXG_C(11) FPXG_C(12)
 #undef FPXG_C
@@ -593,10 +590,8 @@ static void cols2_size(Context& c, const ColJob2* j, const PassPlan& P, uint32_t
 // j[0 .. njobs); recorded as one record per job, all but the first marked as
 // sharing the first's launch level.
 static void launch_cols_n(Context& c, bool lazy, bool inv, const ColJob2* j, const PassPlan& P, uint32_t njobs,
-                          const ColJob2* dj, const GarnerArgs* gap = nullptr) {
+                          const ColJob2* dj) {
   if (!dj && (njobs < 1 || njobs > (uint32_t)kColPack)) throw Error(E_DEVICE, "column launch group size");
-  const GarnerArgs ga = gap ? *gap : GarnerArgs{};
-  if (gap && (c.rec || dj || !inv)) throw Error(E_DEVICE, "fused Garner: one inverse launch, not recorded");
   if (c.rec && !dj) {
     for (uint32_t i = 0; i < njobs; ++i) c.rec->push_cols(P.S, lazy, inv, P.L, j[i], i > 0);
     return;
@@ -606,12 +601,12 @@ static void launch_cols_n(Context& c, bool lazy, bool inv, const ColJob2* j, con
   bool sgn = lazy && !inv && !dj;
   for (uint32_t i = 0; sgn && i < njobs; ++i) sgn = signed_ok(j[i].f.p);
   if (lazy) {
-    if (inv) cols2_size<true, true>(c, j, P, njobs, dj, ga);
-    else if (sgn) cols2_size<true, false, true>(c, j, P, njobs, dj, ga);
-    else cols2_size<true, false>(c, j, P, njobs, dj, ga);
+    if (inv) cols2_size<true, true>(c, j, P, njobs, dj);
+    else if (sgn) cols2_size<true, false, true>(c, j, P, njobs, dj);
+    else cols2_size<true, false>(c, j, P, njobs, dj);
   } else {
-    if (inv) cols2_size<false, true>(c, j, P, njobs, dj, ga);
-    else cols2_size<false, false>(c, j, P, njobs, dj, ga);
+    if (inv) cols2_size<false, true>(c, j, P, njobs, dj);
+    else cols2_size<false, false>(c, j, P, njobs, dj);
   }
   FPXG_HIP(hipGetLastError());
 }
@@ -626,6 +621,75 @@ static void launch_cols_pair(Context& c, bool lazy, bool inv, const ColJob2& j0,
   launch_cols_n(c, lazy, inv, js, P, 2, nullptr);
 }
 
+// The inverse column pass of CRT products with Garner fused (k_cols2_crt):
+// tiles as in launch_cols; the auxiliary primes are lazy (all below 2^30).
+template <int S, int LOGC, int E>
+static void colcrt_go(Context& c, const CrtColJob* j, const PassPlan& P, uint32_t njobs, const CrtColJob* dj,
+                      const CrtFields& F) {
+  using CS = ColShape<S, LOGC, E>;
+  JobPack<CrtColJob, kCrtPack> pack;
+  for (int i = 0; i < kCrtPack; ++i) pack.j[i] = j[(!dj && (uint32_t)i < njobs) ? i : 0];
+  hipLaunchKernelGGL((k_cols2_crt<S, LOGC, true, E>), dim3(1u << (P.b - LOGC), njobs), dim3(CS::T),
+                     (size_t)CS::words * 4, c.stream, pack, dj, P.L, F);
+}
+template <int S, int LOGC>
+static void colcrt_launch_c(Context& c, const CrtColJob* j, const PassPlan& P, uint32_t njobs, const CrtColJob* dj,
+                            const CrtFields& F) {
+  if constexpr (S >= 5 && S <= 10) {
+    if (!dj && P.L <= 20) {  // four elements a thread, as in cols2_launch_c
+      constexpr int LC = LOGC < 10 - (S - 2) ? LOGC : 10 - (S - 2);
+      colcrt_go<S, LC, 2>(c, j, P, njobs, dj, F);
+      return;
+    }
+  }
+  colcrt_go<S, LOGC, 4>(c, j, P, njobs, dj, F);
+}
+template <int S>
+static void colcrt_launch(Context& c, const CrtColJob* j, const PassPlan& P, uint32_t njobs, const CrtColJob* dj,
+                          const CrtFields& F) {
+  if constexpr (S == 10) {
+    if (P.logC == 3) {
+      colcrt_launch_c<S, 3>(c, j, P, njobs, dj, F);
+      return;
+    }
+  }
+  colcrt_launch_c<S, col_tile_log(S)>(c, j, P, njobs, dj, F);
+}
+
+static const CrtFields& crt_fields(Context& c);
+
+// Whether a CRT product of 2^L points (with a column pass) runs k_cols2_crt:
+// no residue arrays, no Garner launch, but each workgroup takes every prime's
+// tile in turn.  Merged launches (batches, serving) are throughput-bound and
+// gain at every size; a lone call's or a Newton step's pass has few
+// workgroups below 2^20 points and is faster as one workgroup a prime tile
+// plus a Garner launch.
+static bool crt_cols_fused(Context& c, int L) {
+  const int from = c.rec && c.rec_merged ? c.crt_cols_log_merged : c.crt_cols_log;
+  return L >= from && c.conv_plan(L).S > 0;
+}
+
+// njobs CRT products' inverse column passes (by value, or the device array
+// dj); recorded as one record per product.
+static void launch_cols_crt(Context& c, const CrtColJob* j, const PassPlan& P, uint32_t njobs,
+                            const CrtColJob* dj = nullptr) {
+  if (!dj && (njobs < 1 || njobs > (uint32_t)kCrtPack)) throw Error(E_DEVICE, "CRT column launch group size");
+  if (c.rec && !dj) {
+    for (uint32_t i = 0; i < njobs; ++i) c.rec->push_colg(P.S, P.L, j[i], i > 0);
+    return;
+  }
+  const CrtFields& F = crt_fields(c);
+  switch (P.S) {
+#define FPXG_C(S) \
+  case S: colcrt_launch<S>(c, j, P, njobs, dj, F); break;
+    FPXG_C(1) FPXG_C(2) FPXG_C(3) FPXG_C(4) FPXG_C(5) FPXG_C(6)
+    FPXG_C(7) FPXG_C(8) FPXG_C(9) FPXG_C(10) FPXG_C(11) FPXG_C(12)
+#undef FPXG_C
+    default: throw Error(E_SIZE, "column pass size out of range");
+  }
+  FPXG_HIP(hipGetLastError());
+}
+
 template <int LOGM, bool LAZY, int MODE>
 static bool rows2_attr() {
   using RS = RowShape<LOGM>;
@@ -636,6 +700,13 @@ template <int S, bool LAZY, bool INV, bool SGN = false>
 static bool cols2_attr() {
   constexpr int LOGC = col_tile_log(S);
   return hipFuncSetAttribute((const void*)k_cols2<S, LOGC, LAZY, INV, SGN>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                             (int)(ColShape<S, LOGC>::words * 4)) == hipSuccess;
+}
+
+template <int S>
+static bool colcrt_attr() {
+  constexpr int LOGC = col_tile_log(S);
+  return hipFuncSetAttribute((const void*)k_cols2_crt<S, LOGC, true, 4>, hipFuncAttributeMaxDynamicSharedMemorySize,
                              (int)(ColShape<S, LOGC>::words * 4)) == hipSuccess;
 }
 
@@ -659,12 +730,13 @@ static void context_init(Context& c) {
                    cols2_attr<S, false, true>() && cols2_attr<S, true, false, true>() && big;
   X(9) X(10) X(11) X(12)
 #undef X
+  big = colcrt_attr<10>() && colcrt_attr<11>() && colcrt_attr<12>() && big;
   (void)hipGetLastError();
   c.lds_bytes = big ? (size_t)lds_words(1u << 14) * 4 : 65536;
   c.row_log_max = big ? 14 : 13;
   c.arena = std::make_unique<Arena>(g_arena_bytes);
   c.graphs_enabled = getenv("FPXG_NO_GRAPHS") == nullptr;
-  if (const char* gf = getenv("FPXG_GARNER_FUSE_LOG")) c.garner_fuse_log = atoi(gf);
+  if (const char* gc = getenv("FPXG_CRT_COLS_LOG")) c.crt_cols_log = c.crt_cols_log_merged = atoi(gc);
   // This device's copy of the static scratch array (the device is current).
   void* sp = nullptr;
   FPXG_HIP(hipGetSymbolAddress(&sp, HIP_SYMBOL(scratch)));
@@ -856,7 +928,7 @@ void ntt_inverse(Context& c, const HostField& F, uint32_t* dst, const uint32_t* 
 // a CRT product: one launch per pass carries every field's jobs).  out[i] gets
 // the window [0, outc) in field F[i]; lift reads a and b as limbs of another
 // field (times r2).  All fields must share the lazy mode.
-// Garner fused into the product's last inverse pass (CRT products): the
+// Garner fused into the product's inverse column pass (CRT products): the
 // target limbs go to out, reconstructed with g.
 struct GarnerFuse {
   uint32_t* out;
@@ -868,9 +940,10 @@ static bool conv_core_fields(Context& c, const HostField* const* F, int k, const
                              int L_cyclic = 0, size_t lo = 0, const GarnerFuse* gf = nullptr) {
   // L_cyclic: the 2^L_cyclic-point cyclic product, whose window [lo, lo +
   // outc) is written (lo + outc <= 2^L_cyclic); else the whole product from 0.
-  // With gf (and a column pass, outside a recorded batch) the last prime's
-  // inverse pass ends in Garner and writes gf->out: returns true; otherwise
-  // every prime's residues are in out[i] and the caller runs Garner.
+  // With gf, where crt_cols_fused says so, the inverse column pass of every
+  // prime runs in one launch with Garner in its epilogue and writes gf->out:
+  // returns true; otherwise every prime's residues are in out[i] and the
+  // caller runs Garner.
   const int L = L_cyclic ? L_cyclic : std::max(1, ceil_log2(la + lb - 1));
   const uint32_t N = 1u << L;
   const PassPlan P = c.conv_plan(L);
@@ -916,18 +989,21 @@ static bool conv_core_fields(Context& c, const HostField* const* F, int k, const
     ij[i].src = A[i].get(); ij[i].dst = out[i]; ij[i].out_lo = (uint32_t)lo; ij[i].out_hi = (uint32_t)(lo + outc);
     ij[i].out_shift = (uint32_t)lo; ij[i].inv_n = inv_pow2(*F[i], L); ij[i].f = F[i]->dev;
   }
-  if (gf && !c.rec) {
-    // The other primes' residues first; the last prime's pass reads them at
-    // each output position and writes the target limb (restore_crt fused).
-    if (k > 1) launch_cols_n(c, lazy, true, ij, P, (uint32_t)(k - 1), nullptr);
-    ColJob2 last = ij[k - 1];
-    last.dst = gf->out;
-    last.flags |= R2_GARNER;
-    GarnerArgs ga{};
-    ga.r[0] = k > 1 ? out[0] : nullptr;
-    ga.r[1] = k > 2 ? out[1] : nullptr;
-    ga.g = gf->g;
-    launch_cols_n(c, lazy, true, &last, P, 1, nullptr, &ga);
+  if (gf && crt_cols_fused(c, L)) {
+    // Every prime's inverse column pass in one workgroup a tile, Garner in
+    // its epilogue (also recorded): no residues are written.
+    const CrtFields& CF = crt_fields(c);
+    if (!lazy) throw Error(E_DEVICE, "CRT column pass: lazy fields only");
+    CrtColJob cj{};
+    for (int i = 0; i < 3; ++i) {
+      const int s = i < k ? i : 0;
+      if (F[s]->p != CF.f[s].p) throw Error(E_DEVICE, "CRT column pass: fields out of order");
+      cj.src[i] = A[s].get();
+      cj.inv_n[i] = inv_pow2(*F[s], L);
+    }
+    cj.dst = gf->out; cj.out_lo = (uint32_t)lo; cj.out_hi = (uint32_t)(lo + outc); cj.out_shift = (uint32_t)lo;
+    cj.g = gf->g;
+    launch_cols_crt(c, &cj, P, 1);
     return true;
   }
   launch_cols_n(c, lazy, true, ij, P, (uint32_t)k, nullptr);
@@ -945,6 +1021,15 @@ static const uint32_t kCrtP[3] = {167772161u, 469762049u, 754974721u};  // crt_c
 static const size_t kCrtBlock = (size_t)1 << 23;                          // crt_convolution.hpp:19
 
 // crt_prime_count (crt_convolution.hpp:78-83).
+static const CrtFields& crt_fields(Context& c) {
+  if (!c.crt_f_ok) {
+    for (int k = 0; k < 3; ++k) c.crt_f.f[k] = c.field(kCrtP[k]).dev;
+    c.crt_f_ok = true;
+  }
+  return c.crt_f;
+}
+
+
 static int crt_prime_count(uint32_t p, size_t terms) {
   const uint64_t sq = (uint64_t)(p - 1) * (p - 1);
   if (terms == 0) terms = 1;
@@ -982,21 +1067,23 @@ static GarnerConst garner_const(Context& c, const HostField& T, int primes, bool
 static void crt_window(Context& c, const HostField& T, const uint32_t* a, size_t la, const uint32_t* b, size_t lb,
                        uint32_t* out, size_t outc, bool accumulate) {
   const int primes = crt_prime_count(T.p, std::min(la, lb));
+  const int L = std::max(1, ceil_log2(la + lb - 1));
+  const bool fused = crt_cols_fused(c, L);
   DevBuf res[3];
   const HostField* Fs[3];
   uint32_t* outs[3];
   for (int k = 0; k < primes; ++k) {
-    res[k] = c.buf(outc);
+    if (!fused) res[k] = c.buf(outc);
     Fs[k] = &c.field(kCrtP[k]);
     outs[k] = res[k].get();
   }
-  const int L = std::max(1, ceil_log2(la + lb - 1));
-  if (!c.rec && L >= c.garner_fuse_log) {
+  if (fused) {
     GarnerFuse gf{out, garner_const(c, T, primes, accumulate)};
-    if (conv_core_fields(c, Fs, primes, a, la, b, lb, outs, outc, true, 0, 0, &gf)) return;
-  } else {
-    conv_core_fields(c, Fs, primes, a, la, b, lb, outs, outc, true);
+    if (!conv_core_fields(c, Fs, primes, a, la, b, lb, outs, outc, true, 0, 0, &gf))
+      throw Error(E_DEVICE, "CRT product: fused column pass not taken");
+    return;
   }
+  conv_core_fields(c, Fs, primes, a, la, b, lb, outs, outc, true);
   GarnerRec g{};
   g.c1 = res[0].get();
   g.c2 = primes > 1 ? res[1].get() : res[0].get();

@@ -35,6 +35,11 @@ static void issue_one(Context& c, const LaunchRec& r) {
       else
         hipLaunchKernelGGL(k_recur<1024>, dim3(1), dim3(1024), 0, c.stream, (const RecurRec*)nullptr, r.rec);
       break;
+    case LK_COLG: {
+      const PassPlan P{r.k3, r.k3 - r.k0, r.k0, tile_log_cols(r.k0, r.k3)};
+      launch_cols_crt(c, &r.crt, P, 1);
+      return;
+    }
     case LK_GARNER:
       hipLaunchKernelGGL(k_garner, dim3(grid_for(r.gar.n)), dim3(256), 0, c.stream, r.gar.c1, r.gar.c2, r.gar.c3,
                          r.gar.out, r.gar.n, r.gar.g);
@@ -49,14 +54,19 @@ static void issue_one(Context& c, const LaunchRec& r) {
 // (and for row passes the same row count), each after the first marked par.
 static bool same_launch(const LaunchRec& a, const LaunchRec& b) {
   if (!b.par || a.kind != b.kind || a.k0 != b.k0 || a.k1 != b.k1 || a.k2 != b.k2 || a.k3 != b.k3) return false;
-  if (a.kind == LK_COLS) return true;
+  if (a.kind == LK_COLS || a.kind == LK_COLG) return true;
   return a.kind == LK_ROWS && a.row.nrows == b.row.nrows;
 }
 
 // Issues recs[0 .. n) as one launch (n > 1 records of same_launch shape).
 static void issue_group(Context& c, const LaunchRec* recs, size_t n) {
   const LaunchRec& r = recs[0];
-  if (r.kind == LK_COLS) {
+  if (r.kind == LK_COLG) {
+    CrtColJob js[kCrtPack];
+    for (size_t i = 0; i < n; ++i) js[i] = recs[i].crt;
+    const PassPlan P{r.k3, r.k3 - r.k0, r.k0, tile_log_cols(r.k0, r.k3)};
+    launch_cols_crt(c, js, P, (uint32_t)n);
+  } else if (r.kind == LK_COLS) {
     ColJob2 js[kColPack];
     for (size_t i = 0; i < n; ++i) js[i] = recs[i].col;
     const PassPlan P{r.k3, r.k3 - r.k0, r.k0, tile_log_cols(r.k0, r.k3)};
@@ -160,7 +170,10 @@ static void series_call(Context& c, const HostField& T, int op, const uint32_t* 
     try {
       for (size_t i = 0; i < recs.size();) {
         const LaunchRec& r = recs[i];
-        const size_t cap = r.kind == LK_COLS ? (size_t)kColPack : r.kind == LK_ROWS ? (size_t)kRowPack : 1;
+        const size_t cap = r.kind == LK_COLS   ? (size_t)kColPack
+                           : r.kind == LK_COLG ? (size_t)kCrtPack
+                           : r.kind == LK_ROWS ? (size_t)kRowPack
+                                               : 1;
         size_t e = i + 1;
         while (e < recs.size() && e - i < cap && same_launch(r, recs[e])) ++e;
         if (e - i > 1) issue_group(c, recs.data() + i, e - i);

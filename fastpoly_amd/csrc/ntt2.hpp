@@ -491,14 +491,9 @@ struct RowJob2 {
 // By-value jobs per row launch: one for each of the three auxiliary primes.
 constexpr int kRowPack = 3;
 enum : uint32_t { R2_LIFT = 1, R2_CANON = 2, R2_SCALE = 4, R2_NEG = 8, R2_ACC = 16, R2_SUBFROM = 32, R2_MULTAB = 64,
-                  R2_GARNER = 128, R2_ACCLO = 256 };
+                  R2_ACCLO = 256 };
 // R2_ACCLO (column inverse only): like R2_ACC for output positions below
 // acc_end (relative to out_shift), a plain store from there on.
-// R2_GARNER (column inverse only): the job is the last auxiliary prime of a
-// CRT product; its canonical residue v and the other primes' residues
-// (GarnerArgs, at the output position) give the target limb by Garner
-// (restore_crt, crt_convolution.hpp:88-127), negated / accumulated as the
-// GarnerConst says, written to dst.
 // R2_ACC (column inverse only): dst[i] = dst[i] + result, both canonical.
 // Inverse-pass epilogues on the canonical result v at output position t (the
 // row pass needs R2_CANON with them): R2_SUBFROM v = (t < epi_avail ? epi[t]
@@ -1268,7 +1263,7 @@ struct ColTune {
 // mid-size columns, whose threads then issue a quarter of the loads).
 template <int S, int LOGC, bool LAZY, bool INV, bool SGN = false, int E = 4>
 __global__ void __launch_bounds__((ColShape<S, LOGC, E>::T)) __attribute__((amdgpu_waves_per_eu(ColTune<S>::waves)))
-k_cols2(JobPack<ColJob2, kColPack> jp, const ColJob2* __restrict__ jobs, int L, GarnerArgs ga) {
+k_cols2(JobPack<ColJob2, kColPack> jp, const ColJob2* __restrict__ jobs, int L) {
   using SC = Sched<S, E>;
   static_assert(!SGN || (LAZY && !INV), "signed residues: lazy forward passes only");
   constexpr int NPH = SC::nph;
@@ -1381,7 +1376,7 @@ k_cols2(JobPack<ColJob2, kColPack> jp, const ColJob2* __restrict__ jobs, int L, 
     const uint32_t win = J.out_hi > J.out_shift ? J.out_hi - J.out_shift : 0u;  // output words past out_shift
     const uint32_t elen = epi_op ? ((flags & R2_SUBFROM) ? J.epi_avail : win) : (acc_op ? win : 0u);
     uint32_t ev[NE0];
-    if (!(flags & R2_GARNER) && elen != 0) {
+    if (elen != 0) {
       const uint32_t* esrc = epi_op ? J.epi : J.dst;
       const uint32_t elim = elen - 1;
 #pragma unroll
@@ -1406,49 +1401,146 @@ k_cols2(JobPack<ColJob2, kColPack> jp, const ColJob2* __restrict__ jobs, int L, 
       }
     }
     inv_transform<S, LOGC, LAZY, ColTune<S>::PF, E>(x, X, g, c, J.f.planes_inv, A, TI);
-    if (!(flags & R2_GARNER)) {
-      const uint32_t p = J.f.p;
-#pragma unroll
-      for (int u = 0; u < PhaseGroups<S, 0, LOGC, E>::NG; ++u)
-#pragma unroll
-        for (int k = 0; k < (1 << r0); ++k) {
-          const uint32_t e = elem<S, 0, E>(PhaseGroups<S, 0, LOGC, E>::gid(g, u), k);
-          const uint32_t gi = e * M + col;
-          const uint32_t v = A.canon(A.half(x[u * (1 << r0) + k]));
-          if (gi >= J.out_lo && gi < J.out_hi) {
-            const uint32_t t = gi - J.out_shift;
-            uint32_t w = neg ? (v == 0 ? 0u : p - v) : v;
-            if (flags & R2_SUBFROM) {
-              const uint32_t sv = t < J.epi_avail ? ev[u * (1 << r0) + k] : 0u;
-              w = sv >= w ? sv - w : sv - w + p;
-            }
-            if (flags & R2_MULTAB) w = mulmod_full(w, ev[u * (1 << r0) + k], p, J.f.ninv);
-            if ((flags & R2_ACC) || ((flags & R2_ACCLO) && t < J.acc_end)) {
-              // the destination's old value: prefetched unless an epilogue operand took the registers
-              const uint32_t old = epi_op ? gload(J.dst + t) : ev[u * (1 << r0) + k];
-              const uint32_t s2 = old + w;
-              w = min(s2, s2 - p);
-            }
-            gstore(J.dst + t, w);
-          }
-        }
-      return;
-    }
-    // R2_GARNER: the other primes' residues at each output position.
-    const uint32_t np = ga.g.primes;
+    const uint32_t p = J.f.p;
 #pragma unroll
     for (int u = 0; u < PhaseGroups<S, 0, LOGC, E>::NG; ++u)
 #pragma unroll
       for (int k = 0; k < (1 << r0); ++k) {
-        const uint32_t gi = elem<S, 0, E>(PhaseGroups<S, 0, LOGC, E>::gid(g, u), k) * M + col;
-        if (gi < J.out_lo || gi >= J.out_hi) continue;
+        const uint32_t e = elem<S, 0, E>(PhaseGroups<S, 0, LOGC, E>::gid(g, u), k);
+        const uint32_t gi = e * M + col;
         const uint32_t v = A.canon(A.half(x[u * (1 << r0) + k]));
-        const uint32_t o = gi - J.out_shift;
-        const uint32_t r1 = np == 1 ? v : gload(ga.r[0] + o);
-        const uint32_t r2 = np == 2 ? v : (np == 3 ? gload(ga.r[1] + o) : 0u);
-        gstore(J.dst + o, garner_value(r1, r2, np == 3 ? v : 0u, ga.g.accumulate ? gload(J.dst + o) : 0u, ga.g));
+        if (gi >= J.out_lo && gi < J.out_hi) {
+          const uint32_t t = gi - J.out_shift;
+          uint32_t w = neg ? (v == 0 ? 0u : p - v) : v;
+          if (flags & R2_SUBFROM) {
+            const uint32_t sv = t < J.epi_avail ? ev[u * (1 << r0) + k] : 0u;
+            w = sv >= w ? sv - w : sv - w + p;
+          }
+          if (flags & R2_MULTAB) w = mulmod_full(w, ev[u * (1 << r0) + k], p, J.f.ninv);
+          if ((flags & R2_ACC) || ((flags & R2_ACCLO) && t < J.acc_end)) {
+            // the destination's old value: prefetched unless an epilogue operand took the registers
+            const uint32_t old = epi_op ? gload(J.dst + t) : ev[u * (1 << r0) + k];
+            const uint32_t s2 = old + w;
+            w = min(s2, s2 - p);
+          }
+          gstore(J.dst + t, w);
+        }
       }
   }
+}
+
+// ---------------------------------------------------------------------------
+// The inverse column pass of a CRT product over all its primes, Garner in the
+// epilogue (restore_crt, crt_convolution.hpp:88-127, fused): each workgroup
+// takes its tile of every prime's row-pass output in turn, runs the inverse
+// levels, and folds the canonical residues into Garner's running state (two
+// words an element); after the last prime it writes the target limbs of its
+// output window.  No residue array is written or read back, and there is no
+// Garner launch.
+// ---------------------------------------------------------------------------
+struct CrtFields {
+  FieldDev f[3];
+};
+// By-value jobs per launch (one product; two when products are merged).
+constexpr int kCrtPack = 2;
+
+// 128 registers (the Garner state is two words an element beside the data);
+// twiddles are not loaded a phase ahead, which would spill.
+template <int S>
+struct ColTuneCrt {
+  static constexpr bool PF = false;
+  static constexpr int waves = 4;
+};
+
+template <int S, int LOGC, bool LAZY, int E = 4>
+__global__ void __launch_bounds__((ColShape<S, LOGC, E>::T)) __attribute__((amdgpu_waves_per_eu(ColTuneCrt<S>::waves)))
+k_cols2_crt(JobPack<CrtColJob, kCrtPack> jp, const CrtColJob* __restrict__ jobs, int L, CrtFields F) {
+  using SC = Sched<S, E>;
+  constexpr int NPH = SC::nph;
+  constexpr int C = 1 << LOGC;
+  extern __shared__ uint32_t X[];
+  const CrtColJob J = load_job(jp.j[0], jobs);
+  const uint32_t np = __builtin_amdgcn_readfirstlane(J.g.primes);
+  const int b = L - S;
+  const uint32_t M = 1u << b;
+  const uint32_t tiles = M >> LOGC;
+  uint32_t tile = blockIdx.x;
+  if (tiles >= 64 && (tiles & 7) == 0) tile = (tile & 7) * (tiles >> 3) + (tile >> 3);
+  const uint32_t c = threadIdx.x % C, g = threadIdx.x / C;
+  const uint32_t col = (tile << LOGC) + c;
+  constexpr int PL = NPH - 1;
+  constexpr int rl = SC::r_of(PL);
+  constexpr int NGL = PhaseGroups<S, PL, LOGC, E>::NG;
+  constexpr int r0 = SC::r_of(0);
+  constexpr int NG0 = PhaseGroups<S, 0, LOGC, E>::NG;
+  constexpr int NE0 = NG0 * (1 << r0);
+  typedef __attribute__((address_space(1))) const uint32_t g32;
+  uint32_t s0[NE0], s1[NE0];  // Garner state: x1; then the target's share of x1 + p1 x2, and x1 + p1 x2 mod P3
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    if ((uint32_t)k >= np) break;  // uniform
+    if (k > 0) __syncthreads();    // the previous prime's last LDS reads are done
+    const FieldDev& f = F.f[k];
+    const Arith<LAZY> A(f);
+    uint32_t TI[16], x[16];
+    tw_load<S, PL, LOGC, E>(TI, f.planes_inv, g);
+    // Four-step twiddle w^-(col * bitrev_S(e)) times 1/N by a running product
+    // (as in k_cols2's inverse).
+    const uint32_t shift = f.logmax - L, split = f.split, smask = (1u << split) - 1;
+    const uint32_t xd = (col << (S - rl)) << shift;
+    const uint32_t d_hi = *(g32*)(f.tw_hi_inv + (xd >> split));
+    const uint32_t d_lo = *(g32*)(f.tw_lo_inv + (xd & smask));
+    uint32_t q_hi[NGL], q_lo[NGL];
+#pragma unroll
+    for (int u = 0; u < NGL; ++u) {
+      const uint32_t xq = (col * bitrev(PhaseGroups<S, PL, LOGC, E>::gid(g, u) << rl, S)) << shift;
+      q_hi[u] = *(g32*)(f.tw_hi_inv + (xq >> split));
+      q_lo[u] = *(g32*)(f.tw_lo_inv + (xq & smask));
+    }
+    const uint32_t* src = J.src[k];
+#pragma unroll
+    for (int u = 0; u < NGL; ++u)
+#pragma unroll
+      for (int kk = 0; kk < (1 << rl); ++kk)
+        x[u * (1 << rl) + kk] = gload(src + elem<S, PL, E>(PhaseGroups<S, PL, LOGC, E>::gid(g, u), kk) * M + col);
+    const uint32_t D = A.mul(d_hi, d_lo);
+#pragma unroll
+    for (int u = 0; u < NGL; ++u) {
+      uint32_t q = A.mul(A.mul(q_hi[u], q_lo[u]), J.inv_n[k]);
+#pragma unroll
+      for (int j = 0; j < (1 << rl); ++j) {
+        const int kk = (int)bitrev_c(j, rl);
+        x[u * (1 << rl) + kk] = A.mul(x[u * (1 << rl) + kk], q);
+        if (j + 1 < (1 << rl)) q = A.mul(q, D);
+      }
+    }
+    inv_transform<S, LOGC, LAZY, ColTuneCrt<S>::PF, E>(x, X, g, c, f.planes_inv, A, TI);
+#pragma unroll
+    for (int i = 0; i < NE0; ++i) {
+      const uint32_t v = A.canon(A.half(x[i]));  // Montgomery residue mod P_k
+      if (k == 0) {
+        s0[i] = redc_full(v, J.g.P[0], J.g.Pninv[0]);
+      } else if (k == 1) {
+        const uint32_t x1 = s0[i], x2 = garner_x2(x1, v, J.g);
+        s1[i] = garner_x12(x1, x2, J.g);
+        s0[i] = garner_tadd(garner_t1(x1, J.g), x2, J.g.p1_t, J.g);
+      } else {
+        s0[i] = garner_tadd(s0[i], garner_x3(s1[i], v, J.g), J.g.p12_t, J.g);
+      }
+    }
+  }
+  const bool acc = J.g.accumulate != 0;
+#pragma unroll
+  for (int u = 0; u < NG0; ++u)
+#pragma unroll
+    for (int kk = 0; kk < (1 << r0); ++kk) {
+      const uint32_t gi = elem<S, 0, E>(PhaseGroups<S, 0, LOGC, E>::gid(g, u), kk) * M + col;
+      if (gi < J.out_lo || gi >= J.out_hi) continue;
+      const uint32_t o = gi - J.out_shift;
+      uint32_t v = s0[u * (1 << r0) + kk];
+      if (np == 1) v = garner_t1(v, J.g);
+      gstore(J.dst + o, garner_finish(v, acc ? gload(J.dst + o) : 0u, J.g));
+    }
 }
 
 }  // namespace fpxg

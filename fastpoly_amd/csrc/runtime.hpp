@@ -159,10 +159,10 @@ struct PassPlan {
 };
 
 // One recorded kernel launch of a request's schedule.
-enum { LK_ROWS = 0, LK_COLS = 1, LK_EW = 2, LK_RECUR = 3, LK_GARNER = 4 };
+enum { LK_ROWS = 0, LK_COLS = 1, LK_EW = 2, LK_RECUR = 3, LK_GARNER = 4, LK_COLG = 5 };
 struct LaunchRec {
   int kind;
-  int k0, k1, k2, k3;  // shape: rows (logM, lazy, mode, -), cols (S, lazy, inv, L)
+  int k0, k1, k2, k3;  // shape: rows (logM, lazy, mode, -), cols (S, lazy, inv, L), CRT cols (S, -, -, L)
   int par;             // independent of the previous record: may run in the same launch level
   union {
     RowJob2 row;
@@ -170,6 +170,7 @@ struct LaunchRec {
     EwRec ew;
     RecurRec rec;
     GarnerRec gar;
+    CrtColJob crt;
   };
   LaunchRec() {}
 };
@@ -195,6 +196,10 @@ struct Recorder {
   void push_ew(const EwRec& e) { next(LK_EW).ew = e; }
   void push_recur(const RecurRec& e) { next(LK_RECUR).rec = e; }
   void push_garner(const GarnerRec& g) { next(LK_GARNER).gar = g; }
+  void push_colg(int S, int L, const CrtColJob& j, bool par = false) {
+    LaunchRec& r = next(LK_COLG);
+    r.k0 = S; r.k1 = 1; r.k3 = L; r.crt = j; r.par = par;
+  }
 };
 
 struct Context {
@@ -205,7 +210,8 @@ struct Context {
   int row_log_max = 14;      // largest row pass
   bool row_dma = true;       // persistent prefetching forward pass for many 2^14-word rows
   int single_log_max = 12;   // largest transform done as one row pass
-  int garner_fuse_log = 31;  // lone CRT products from 2^garner_fuse_log points fuse Garner into the last inverse pass
+  int crt_cols_log = 20;         // CRT products with a column pass from 2^crt_cols_log points: k_cols2_crt
+  int crt_cols_log_merged = 1;   // the same in merged (batch, serving) schedules
   int cus = 256;
   std::map<int, PassPlan> plan_override;  // autotuned splits by log2 n
   std::map<int, PassPlan> conv_override;  // autotuned splits of fused products by log2 n
@@ -238,6 +244,8 @@ struct Context {
   void* scan_host = nullptr;
   uint32_t scan_token = 0;  // last token the scan wrote into scan_host
   std::unordered_map<uint64_t, GarnerConst> garner_cache;  // by (target p, primes, accumulate)
+  CrtFields crt_f{};  // the auxiliary primes' tables, k_cols2_crt's kernel argument
+  bool crt_f_ok = false;
   std::unordered_map<uint32_t, bool> prime_cache;          // modulus -> accepted
   // Outstanding batch tickets (completion events).
   std::map<uint64_t, hipEvent_t> tickets;

@@ -192,23 +192,30 @@ static bool one_launch_step(const Context& c, const HostField& T, const Engine& 
 
 // Fields without their own transform: out[i - lo] = (x * y)[i] for i in
 // [lo, hi) of the 2^L-point cyclic product through the three auxiliary primes,
-// each pass one launch for all three, then Garner (negated on request).  No
+// each pass one launch for all three, then Garner (negated on request), in the
+// inverse column pass where crt_cols_fused says so.  No
 // spectrum is stored for reuse: fewer launches a step beat the saved
 // transforms at every size measured (exp mod 1e9+7 at 2^20: 1.42 ms, against
 // 1.50 ms with stored spectra).
 static void crt_conv(Context& c, const HostField& T, int L, const uint32_t* x, size_t cx, const uint32_t* y,
                      size_t cy, size_t lo, size_t hi, uint32_t* out, bool neg = false) {
+  const bool fused = crt_cols_fused(c, L);
   DevBuf res[3];
   const HostField* Fs[3];
   uint32_t* outs[3];
   for (int i = 0; i < 3; ++i) {
-    res[i] = c.buf(hi - lo);
+    if (!fused) res[i] = c.buf(hi - lo);
     Fs[i] = &c.field(kCrtP[i]);
     outs[i] = res[i].get();
   }
   GarnerFuse gf{out, garner_const(c, T, 3, false)};
   gf.g.neg = neg;
-  if (conv_core_fields(c, Fs, 3, x, cx, y, cy, outs, hi - lo, true, L, lo, &gf)) return;
+  if (fused) {
+    if (!conv_core_fields(c, Fs, 3, x, cx, y, cy, outs, hi - lo, true, L, lo, &gf))
+      throw Error(E_DEVICE, "CRT product: fused column pass not taken");
+    return;
+  }
+  conv_core_fields(c, Fs, 3, x, cx, y, cy, outs, hi - lo, true, L, lo);
   GarnerRec g{};
   g.c1 = res[0].get(); g.c2 = res[1].get(); g.c3 = res[2].get(); g.out = out; g.n = (uint32_t)(hi - lo);
   g.g = gf.g;
