@@ -596,6 +596,7 @@ def autotune(min_log: int = 13, max_log: int = 24, reps: int = 5) -> dict:
 
 OPT_GRAPHS = 0
 OPT_ROW_DMA = 1
+OPT_CRT_FUSED_LOG = 2
 
 
 def set_graphs(enabled: bool) -> None:
@@ -607,6 +608,13 @@ def set_row_dma(enabled: bool) -> None:
     """Forward row passes over many 2^14-word rows: one persistent prefetching
     workgroup per CU (default) or one workgroup per row (same results)."""
     _check(lib().fpxg_set_option(OPT_ROW_DMA, 1 if enabled else 0), "set_option")
+
+
+def set_crt_fused_log(log2_points: int) -> None:
+    """Three-prime products of at least 2^log2_points points reconstruct by
+    Garner inside their inverse column pass; 0 restores the default (same
+    results)."""
+    _check(lib().fpxg_set_option(OPT_CRT_FUSED_LOG, int(log2_points)), "set_option")
 
 
 def pow(p: int, a, k: int, n: int) -> np.ndarray:  # noqa: A001 (the reference's name)

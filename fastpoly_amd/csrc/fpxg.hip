@@ -736,7 +736,6 @@ static void context_init(Context& c) {
   c.row_log_max = big ? 14 : 13;
   c.arena = std::make_unique<Arena>(g_arena_bytes);
   c.graphs_enabled = getenv("FPXG_NO_GRAPHS") == nullptr;
-  if (const char* gc = getenv("FPXG_CRT_COLS_LOG")) c.crt_cols_log = c.crt_cols_log_merged = atoi(gc);
   // This device's copy of the static scratch array (the device is current).
   void* sp = nullptr;
   FPXG_HIP(hipGetSymbolAddress(&sp, HIP_SYMBOL(scratch)));
@@ -1681,6 +1680,14 @@ int fpxg_set_option(int option, int64_t value) {
         // Captured graphs hold the old choice: drop them.
         FPXG_HIP(hipStreamSynchronize(c.stream));
         c.row_dma = value != 0;
+        drop_graphs(c);
+        c.graph_seen.clear();
+        return;
+      case FPXG_OPT_CRT_FUSED_LOG:
+        if (value < 0 || value > 31) throw Error(E_ARG, "CRT fused pass size out of range");
+        FPXG_HIP(hipStreamSynchronize(c.stream));
+        c.crt_cols_log = value ? (int)value : Context::kCrtColsLog;
+        c.crt_cols_log_merged = value ? (int)value : Context::kCrtColsLogMerged;
         drop_graphs(c);
         c.graph_seen.clear();
         return;

@@ -210,8 +210,12 @@ struct Context {
   int row_log_max = 14;      // largest row pass
   bool row_dma = true;       // persistent prefetching forward pass for many 2^14-word rows
   int single_log_max = 12;   // largest transform done as one row pass
-  int crt_cols_log = 20;         // CRT products with a column pass from 2^crt_cols_log points: k_cols2_crt
-  int crt_cols_log_merged = 1;   // the same in merged (batch, serving) schedules
+  // CRT products with a column pass from 2^crt_cols_log points run
+  // k_cols2_crt; crt_cols_log_merged: the same in merged (batch, serving)
+  // schedules.
+  static constexpr int kCrtColsLog = 20, kCrtColsLogMerged = 1;
+  int crt_cols_log = kCrtColsLog;
+  int crt_cols_log_merged = kCrtColsLogMerged;
   int cus = 256;
   std::map<int, PassPlan> plan_override;  // autotuned splits by log2 n
   std::map<int, PassPlan> conv_override;  // autotuned splits of fused products by log2 n

@@ -145,8 +145,15 @@ int fpxg_product_plan_info(uint32_t n, int* row_log, int* col_levels);
  *   captured hipGraphs; 0 drops the cached graphs and launches every call.
  *   FPXG_OPT_ROW_DMA: 1 (default) runs the forward row pass over many
  *   2^14-word rows as one persistent prefetching workgroup per CU; 0 runs one
- *   workgroup per row.  Results are identical either way. */
-enum { FPXG_OPT_GRAPHS = 0, FPXG_OPT_ROW_DMA = 1 };
+ *   workgroup per row.
+ *   FPXG_OPT_CRT_FUSED_LOG: v in [1, 31] makes every three-prime (CRT)
+ *   product of at least 2^v points with a column pass reconstruct by Garner
+ *   in its inverse column pass (one workgroup a tile takes every prime in
+ *   turn); smaller ones write the primes' residues and run Garner after.
+ *   0 restores the default (2^20 points for single calls, every size in
+ *   batches and the server).
+ * Results are identical under every option. */
+enum { FPXG_OPT_GRAPHS = 0, FPXG_OPT_ROW_DMA = 1, FPXG_OPT_CRT_FUSED_LOG = 2 };
 int fpxg_set_option(int option, int64_t value);
 
 /* Device memory and stream plumbing for the *_dev entry points. */

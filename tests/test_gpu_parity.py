@@ -149,6 +149,80 @@ def test_conv_random_shapes(fa, oracle, p):
     assert fa.conv(p, [], x).size == 0 and fa.conv(p, x, x, 0).size == 0
 
 
+@pytest.mark.parametrize("fused_log", [1, 31])
+def test_crt_fused_column_pass(fa, oracle, fused_log):
+    """Three-prime (CRT) products with Garner in the inverse column pass at
+    every size with a column pass (1) and never (31, residues then a Garner
+    launch): products over 1e9+7 (three primes), 100003 (two), 3 and 2 (one),
+    truncated and squared; Newton series over 1e9+7 (window products,
+    negated); a merged batch.  Each against the oracle."""
+    fa.set_crt_fused_log(fused_log)
+    try:
+        for p in (Q, 100003, 3, 2):
+            rng = random.Random(p * 7 + fused_log)
+            for la, lb in ((4097, 4096), (16384, 16385), (33000, 40000)):
+                a = _rnd(oracle, p, la, rng.randrange(1 << 30))
+                b = _rnd(oracle, p, lb, rng.randrange(1 << 30))
+                want = oracle.conv(p, a, b)
+                assert np.array_equal(fa.conv(p, a, b), want), (p, la, lb)
+                n = (la + lb) // 3 + 1
+                assert np.array_equal(fa.conv(p, a, b, n), want[:n]), (p, la, lb, n)
+            x = _rnd(oracle, p, 20000, rng.randrange(1 << 30))
+            assert np.array_equal(fa.conv(p, x, x), oracle.conv(p, x, x)), p
+        rng = random.Random(fused_log)
+        n = 20000
+        a = _rnd(oracle, Q, n, rng.randrange(1 << 30))
+        a[0] = oracle.from_int(Q, 12345)
+        a1 = a.copy(); a1[0] = oracle.from_int(Q, 1)
+        a0 = a.copy(); a0[0] = 0
+        sq = oracle.conv(Q, a1, a1, n)
+        want = {"inv": oracle.series("inv", Q, a, n), "log": oracle.series("log", Q, a1, n),
+                "exp": oracle.series("exp", Q, a0, n), "sqrt": oracle.series("sqrt", Q, sq, n)}
+        ins = {"inv": a, "log": a1, "exp": a0, "sqrt": sq}
+        for op in ("inv", "log", "exp", "sqrt"):
+            for _ in range(3):  # eager, then captured, then replayed
+                assert np.array_equal(fa.series(op, Q, ins[op], n), want[op]), (op, fused_log)
+        b = _rnd(oracle, Q, 30000, 5)
+        reqs = [fa.Request("conv", Q, a, b=b), fa.Request("conv", 100003, b, b=b[:9000]),
+                fa.Request("exp", Q, a0, n=n), fa.Request("sqrt", Q, sq, n=n)]
+        got = fa.batch(reqs)
+        assert np.array_equal(got[0], oracle.conv(Q, a, b))
+        assert np.array_equal(got[1], oracle.conv(100003, b, b[:9000]))
+        assert np.array_equal(got[2], want["exp"]) and np.array_equal(got[3], want["sqrt"])
+    finally:
+        fa.set_crt_fused_log(0)
+    with pytest.raises(fa.ArgumentError):
+        fa.set_crt_fused_log(32)
+
+
+@pytest.mark.parametrize("fused_log", [0, 31])
+def test_crt_block_pairs_accumulate(fa, oracle, fused_log):
+    """A 1e9+7 product with an operand past 2^23 coefficients runs as block
+    pairs accumulated into the output (crt_convolution.hpp:187-205): here a
+    2^24-point three-prime block (Garner accumulating in the inverse column
+    pass by default, after it with 31) and a one-row block.  Checked at the
+    block boundary, the ends and random positions against exact integer sums
+    (limb = sum of a_i b_j times 2^-32 mod p)."""
+    fa.set_crt_fused_log(fused_log)
+    try:
+        la, lb = (1 << 23) + 100, 300
+        a = _rnd(oracle, Q, la, 77)
+        b = _rnd(oracle, Q, lb, 78)
+        got = fa.conv(Q, a, b)
+        assert got.size == la + lb - 1
+        rinv = pow(1 << 32, Q - 2, Q)
+        bl = [int(v) for v in b]
+        rng = random.Random(fused_log + 3)
+        pos = list(range(0, 64)) + list(range((1 << 23) - 40, (1 << 23) + 340)) + \
+            list(range(la + lb - 65, la + lb - 1)) + [rng.randrange(la + lb - 1) for _ in range(200)]
+        for i in pos:
+            lo, hi = max(0, i - la + 1), min(lb - 1, i)
+            acc = sum(int(a[i - j]) * bl[j] for j in range(lo, hi + 1))
+            assert int(got[i]) == acc % Q * rinv % Q, i
+    finally:
+        fa.set_crt_fused_log(0)
+
+
 @pytest.mark.parametrize("p", [P, Q, 1224736769, 3])
 def test_conv_short_side_direct(fa, oracle, p):
     """Products with a short side (at most 128 coefficients over an NTT prime,
