@@ -40,6 +40,10 @@ __device__ __forceinline__ uint32_t bload(__amdgpu_buffer_rsrc_t r, uint32_t lan
   return (uint32_t)__builtin_amdgcn_raw_buffer_load_b32(r, (int)lane_bytes, (int)uniform_bytes, 0);
 }
 __device__ __forceinline__ void gstore(uint32_t* p, uint32_t v) { *(gu32*)p = v; }
+__device__ __forceinline__ void bstore(__amdgpu_buffer_rsrc_t r, uint32_t v, uint32_t lane_bytes,
+                                       uint32_t uniform_bytes) {
+  __builtin_amdgcn_raw_buffer_store_b32(v, r, (int)lane_bytes, (int)uniform_bytes, 0);
+}
 
 // Twiddle planes.  plane(logB, r) starts at word plane_off(logB, r) and holds,
 // for lo < 2^(logB-r), the 2^r - 1 twiddles of group lo (level by level,
@@ -1320,22 +1324,22 @@ k_cols2(JobPack<ColJob2, kColPack> jp, const ColJob2* __restrict__ jobs, int L) 
 #pragma unroll
       for (int i = 0; i < NE0; ++i) x[i] = 0u;
     } else {
-      // Unconditional (clamped) loads first so they overlap; lift and zero fill after.
-      const uint32_t last = J.in_count - 1;
+      // Buffer loads whose range ends at in_count: the zero tail reads 0
+      // without clamps or compares.  Element (u, k) is at word
+      // elem(gid_u, 0) * M + col + (k << lo_bits) * M.
+      const uint32_t N = M << S;
+      const __amdgpu_buffer_rsrc_t rs = buf_rsrc(J.src, (J.in_count < N ? J.in_count : N) * 4);
+      constexpr int lo0 = S - r0;
 #pragma unroll
-      for (int u = 0; u < PhaseGroups<S, 0, LOGC, E>::NG; ++u)
+      for (int u = 0; u < PhaseGroups<S, 0, LOGC, E>::NG; ++u) {
+        const uint32_t lane = (elem<S, 0, E>(PhaseGroups<S, 0, LOGC, E>::gid(g, u), 0) * M + col) * 4;
 #pragma unroll
-        for (int k = 0; k < (1 << r0); ++k)
-          x[u * (1 << r0) + k] = gload(J.src + min(elem<S, 0, E>(PhaseGroups<S, 0, LOGC, E>::gid(g, u), k) * M + col, last));
+        for (int k = 0; k < (1 << r0); ++k) x[u * (1 << r0) + k] = bload(rs, lane + (((uint32_t)k * M) << (lo0 + 2)), 0);
+      }
       if (lift) {
 #pragma unroll
         for (int i = 0; i < NE0; ++i) x[i] = A.mul(x[i], J.f.r2);
       }
-#pragma unroll
-      for (int u = 0; u < PhaseGroups<S, 0, LOGC, E>::NG; ++u)
-#pragma unroll
-        for (int k = 0; k < (1 << r0); ++k)
-          if (elem<S, 0, E>(PhaseGroups<S, 0, LOGC, E>::gid(g, u), k) * M + col >= J.in_count) x[u * (1 << r0) + k] = 0u;
     }
     const ArithS Sg(J.f);
     if constexpr (!ColTune<S>::PF) tw_load<S, 0, LOGC, E>(T0, J.f.planes_fwd, g);  // after the data: registers
@@ -1343,16 +1347,20 @@ k_cols2(JobPack<ColJob2, kColPack> jp, const ColJob2* __restrict__ jobs, int L) 
     else fwd_transform<S, LOGC, LAZY, 2, 2, ColTune<S>::PF, E>(x, X, g, c, J.f.planes_fwd, A, T0);
     if constexpr (!ColTune<S>::PF) fourstep_load();
     const uint32_t D = A.mul(d_hi, d_lo);
+    // Stores at a per-lane offset plus a uniform one: element (u, k) of the
+    // last phase is at word elem(gid_u, 0) * M + col + (k << lo_bits) * M.
+    const __amdgpu_buffer_rsrc_t rd = buf_rsrc(J.dst, (M << S) * 4);
+    constexpr int lol = S - Sched<S, E>::s_of(PL) - rl;
 #pragma unroll
     for (int u = 0; u < NGL; ++u) {
       const uint32_t gid = PhaseGroups<S, PL, LOGC, E>::gid(g, u);
+      const uint32_t lane = (elem<S, PL, E>(gid, 0) * M + col) * 4;
       uint32_t q = A.mul(q_hi[u], q_lo[u]);
 #pragma unroll
       for (int j = 0; j < (1 << rl); ++j) {
         const int k = (int)bitrev_c(j, rl);
-        const uint32_t e = elem<S, PL, E>(gid, k);
         const uint32_t v = SGN ? Sg.mul(x[u * (1 << rl) + k], q) + Sg.p : A.mul(x[u * (1 << rl) + k], q);
-        gstore(J.dst + e * M + col, min(v, v - canon_p));
+        bstore(rd, min(v, v - canon_p), lane, ((uint32_t)k * M) << (lol + 2));
         if (j + 1 < (1 << rl)) q = A.mul(q, D);
       }
     }
@@ -1375,8 +1383,10 @@ k_cols2(JobPack<ColJob2, kColPack> jp, const ColJob2* __restrict__ jobs, int L) 
     const bool acc_op = (flags & (R2_ACC | R2_ACCLO)) != 0;
     const uint32_t win = J.out_hi > J.out_shift ? J.out_hi - J.out_shift : 0u;  // output words past out_shift
     const uint32_t elen = epi_op ? ((flags & R2_SUBFROM) ? J.epi_avail : win) : (acc_op ? win : 0u);
+    // (Passes held to 64 registers load them where they are used.)
+    constexpr bool kEvPF = ColTune<S>::PF;
     uint32_t ev[NE0];
-    if (elen != 0) {
+    if (kEvPF && elen != 0) {
       const uint32_t* esrc = epi_op ? J.epi : J.dst;
       const uint32_t elim = elen - 1;
 #pragma unroll
@@ -1413,13 +1423,14 @@ k_cols2(JobPack<ColJob2, kColPack> jp, const ColJob2* __restrict__ jobs, int L) 
           const uint32_t t = gi - J.out_shift;
           uint32_t w = neg ? (v == 0 ? 0u : p - v) : v;
           if (flags & R2_SUBFROM) {
-            const uint32_t sv = t < J.epi_avail ? ev[u * (1 << r0) + k] : 0u;
+            const uint32_t sv = t < J.epi_avail ? (kEvPF ? ev[u * (1 << r0) + k] : gload(J.epi + t)) : 0u;
             w = sv >= w ? sv - w : sv - w + p;
           }
-          if (flags & R2_MULTAB) w = mulmod_full(w, ev[u * (1 << r0) + k], p, J.f.ninv);
+          if (flags & R2_MULTAB)
+            w = mulmod_full(w, kEvPF ? ev[u * (1 << r0) + k] : gload(J.epi + t), p, J.f.ninv);
           if ((flags & R2_ACC) || ((flags & R2_ACCLO) && t < J.acc_end)) {
             // the destination's old value: prefetched unless an epilogue operand took the registers
-            const uint32_t old = epi_op ? gload(J.dst + t) : ev[u * (1 << r0) + k];
+            const uint32_t old = (epi_op || !kEvPF) ? gload(J.dst + t) : ev[u * (1 << r0) + k];
             const uint32_t s2 = old + w;
             w = min(s2, s2 - p);
           }
@@ -1497,12 +1508,17 @@ k_cols2_crt(JobPack<CrtColJob, kCrtPack> jp, const CrtColJob* __restrict__ jobs,
       q_hi[u] = *(g32*)(f.tw_hi_inv + (xq >> split));
       q_lo[u] = *(g32*)(f.tw_lo_inv + (xq & smask));
     }
-    const uint32_t* src = J.src[k];
+    {
+      const __amdgpu_buffer_rsrc_t rs = buf_rsrc(J.src[k], (M << S) * 4);
+      constexpr int lol = S - Sched<S, E>::s_of(PL) - rl;
 #pragma unroll
-    for (int u = 0; u < NGL; ++u)
+      for (int u = 0; u < NGL; ++u) {
+        const uint32_t lane = (elem<S, PL, E>(PhaseGroups<S, PL, LOGC, E>::gid(g, u), 0) * M + col) * 4;
 #pragma unroll
-      for (int kk = 0; kk < (1 << rl); ++kk)
-        x[u * (1 << rl) + kk] = gload(src + elem<S, PL, E>(PhaseGroups<S, PL, LOGC, E>::gid(g, u), kk) * M + col);
+        for (int kk = 0; kk < (1 << rl); ++kk)
+          x[u * (1 << rl) + kk] = bload(rs, lane, ((uint32_t)kk * M) << (lol + 2));
+      }
+    }
     const uint32_t D = A.mul(d_hi, d_lo);
 #pragma unroll
     for (int u = 0; u < NGL; ++u) {
