@@ -110,7 +110,8 @@ def main():
     args = parse()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not offered "
-              "(batches shard across GPUs with no exchange; see DESIGN.md)", file=sys.stderr)
+              "(batches and serving streams would shard across GPUs by request, with no exchange)",
+              file=sys.stderr)
         sys.exit(2)
     import fastpoly_amd as fa
 

@@ -1,20 +1,27 @@
 """Build libfpxg.so (hipcc, gfx950) in-tree.
 
+The library is several translation units (fpxg.hip and the launch_*.hip
+kernel tables), compiled in parallel into objects under fastpoly_amd/_build/
+and linked into fastpoly_amd/libfpxg.so.
+
     python -m fastpoly_amd.build        # or __graft_entry__.build()
 """
+import concurrent.futures as cf
+import glob
 import os
 import subprocess
 import sys
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
+CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libfpxg.so")
-SOURCES = [os.path.join(HERE, "csrc", "fpxg.hip")]
-DEPS = SOURCES + [os.path.join(HERE, "csrc", f) for f in ("device.hpp", "crt.hpp", "ntt2.hpp", "runtime.hpp", "series.hpp", "records.hpp", "ew.hpp", "batch.hpp", "serve.hpp", "graphs.hpp", "ops.hpp")] \
-    + [os.path.join(ROOT, "include", "fpxg.h")]
+OBJDIR = os.path.join(HERE, "_build")
+SOURCES = [os.path.join(CSRC, "fpxg.hip")] + sorted(glob.glob(os.path.join(CSRC, "launch_*.hip")))
+DEPS = SOURCES + sorted(glob.glob(os.path.join(CSRC, "*.hpp"))) + [os.path.join(ROOT, "include", "fpxg.h")]
 
 HIPCC_FLAGS = [
-    "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",
+    "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC",
     # packed FP32 forms are kept out of every kernel (none are needed: integer code)
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-I", os.path.join(ROOT, "include"),
@@ -28,19 +35,29 @@ def hipcc() -> str:
     raise RuntimeError("hipcc not found")
 
 
-def build(force: bool = True, verbose: bool = False) -> str:
-    if not force and os.path.exists(LIB):
-        lib_m = os.path.getmtime(LIB)
-        if all(os.path.getmtime(d) <= lib_m for d in DEPS):
-            return LIB
-    tmp = LIB + ".tmp"
-    cmd = [hipcc()] + HIPCC_FLAGS + ["-o", tmp] + SOURCES
+def _run(cmd, verbose):
     if verbose:
         print(" ".join(cmd), flush=True)
     res = subprocess.run(cmd, capture_output=True, text=True)
     if res.returncode != 0:
         sys.stderr.write(res.stdout + res.stderr)
-        raise RuntimeError("hipcc failed building libfpxg.so")
+        raise RuntimeError("hipcc failed: " + " ".join(cmd[-3:]))
+
+
+def build(force: bool = True, verbose: bool = False) -> str:
+    if not force and os.path.exists(LIB):
+        lib_m = os.path.getmtime(LIB)
+        if all(os.path.getmtime(d) <= lib_m for d in DEPS):
+            return LIB
+    os.makedirs(OBJDIR, exist_ok=True)
+    objs = [os.path.join(OBJDIR, os.path.basename(s)[:-4] + ".o") for s in SOURCES]
+    jobs = max(1, min(len(SOURCES), int(os.environ.get("MAX_JOBS", os.cpu_count() or 1))))
+    with cf.ThreadPoolExecutor(jobs) as ex:
+        futs = [ex.submit(_run, [hipcc()] + HIPCC_FLAGS + ["-c", "-o", o, s], verbose) for s, o in zip(SOURCES, objs)]
+        for f in futs:
+            f.result()
+    tmp = LIB + ".tmp"
+    _run([hipcc(), "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp] + objs, verbose)
     os.replace(tmp, LIB)
     return LIB
 

@@ -3,6 +3,8 @@
 // (immediate launch, or a record appended to the active batch recorder).
 #pragma once
 
+#include "crt_kernels.hpp"
+
 namespace fpxg {
 
 __device__ __forceinline__ uint32_t fs_mul(const FieldScalar& f, uint32_t a, uint32_t b) {

@@ -1,9 +1,10 @@
-// fastpoly_amd: the one translation unit of libfpxg.so (hipcc, gfx950).
+// fastpoly_amd: the main translation unit of libfpxg.so (hipcc, gfx950); the
+// kernel launch tables live in the launch_*.hip units (launch.hpp).
 //
 // Host runtime (context, arena, per-prime device plans), the transform and
 // convolution drivers, the Newton series drivers (series.hpp) and the C ABI
 // declared in include/fpxg.h.
-#include "runtime.hpp"
+#include "launch.hpp"
 
 #include <cstdio>
 #include <cstdlib>
@@ -360,143 +361,17 @@ static std::mutex g_init_mu;
 static size_t g_arena_bytes = (size_t)64 << 30;
 static int g_device = 0;
 
-// ---- launch dispatch over the compile-time shapes of ntt2.hpp ----
+// ---- launch routing: recording, then the part that instantiates the shape ----
 
-// Forward passes on signed residues (ArithS) need 8p < 2^31.
-static bool signed_ok(uint32_t p) { return p < (1u << 28); }
-
-// Elements a thread owns in a one-row launch (log2): fewer than 16 for the
-// fused products and Newton steps of 2^10 .. 2^13 points, whose single
-// workgroup would otherwise run on 1-8 waves.
-static constexpr int row_small_e(int logm, int mode) {
-  const bool fused = mode == R2M_CONV || mode == R2M_INVSTEP || mode == R2M_EXPSTEP || mode == R2M_SQRTSTEP;
-  if (!fused || logm < 10 || logm > 13) return 4;
-  return logm <= 12 ? 2 : 3;
-}
-
-static constexpr int row_multi_e(int logm, int mode) {
-  return (mode == R2M_CONV && logm >= 6 && logm <= 12) ? 2 : 4;
-}
-
-// A row launch: njobs jobs from the device array dj, or (dj null) the njobs
-// by-value jobs j[0 .. njobs).
-template <int LOGM, bool LAZY, int MODE>
-static void rows2_launch(Context& c, const RowJob2* j, uint32_t nrows, uint32_t njobs, const RowJob2* dj) {
-  using RS = RowShape<LOGM>;
-  const size_t lds = (size_t)RS::U * RS::lds_words_per_row * 4;
-  // Many whole rows of 2^14 words, forward: the persistent prefetching pass.
-  if constexpr (LOGM == kDmaRowLog && (MODE == R2M_FWD || MODE == R2M_FWDS)) {
-    // 16-byte row loads (LDS DMA) and stores need 16-byte aligned rows.
-    bool dma = !dj && c.row_dma && (size_t)nrows * njobs >= 2u * (size_t)c.cus;
-    for (uint32_t i = 0; dma && i < njobs; ++i)
-      dma = !(j[i].flags & R2_LIFT) && (size_t)j[i].in_a >= ((size_t)nrows << LOGM) &&
-            ((uintptr_t)j[i].src_a & 15) == 0 && ((uintptr_t)j[i].dst & 15) == 0;
-    if (dma) {
-      JobPack<RowJob2, kRowPack> jp;
-      for (int i = 0; i < kRowPack; ++i) {
-        jp.j[i] = j[(uint32_t)i < njobs ? i : 0];
-        jp.j[i].nrows = nrows;
-      }
-      hipLaunchKernelGGL((k_rows_fwd_dma<LAZY, MODE == R2M_FWDS>), dim3((uint32_t)c.cus), dim3(RS::T), 0, c.stream,
-                         jp, njobs);
-      return;
-    }
-  }
-  if constexpr (MODE == R2M_NINV || MODE == R2M_NEXP || MODE == R2M_NSQRT) {
-    if (!dj && njobs != 1) throw Error(E_DEVICE, "small-series launches take one job by value");
-    hipLaunchKernelGGL((k_newton_small<MODE>), dim3(1, njobs), dim3(kSmallThreads), 0, c.stream, j[0], dj);
-  } else if constexpr ((MODE == R2M_INVSTEP || MODE == R2M_EXPSTEP || MODE == R2M_SQRTSTEP) && LOGM <= 9) {
-    // Small Newton steps: their window products computed directly (one job
-    // by value, or a device job array).
-    if (!dj && njobs != 1) throw Error(E_DEVICE, "direct Newton steps take one job by value");
-    hipLaunchKernelGGL((k_step_direct<MODE, LOGM - 1>), dim3(1, njobs), dim3(kMidTPO << (LOGM - 1)), 0, c.stream,
-                       j[0], dj);
-  } else {
-    JobPack<RowJob2, kRowPack> pack;
-    for (int i = 0; i < kRowPack; ++i) pack.j[i] = j[(!dj && (uint32_t)i < njobs) ? i : 0];
-    // A lone row of a mid-size product or Newton step: one workgroup, so
-    // fewer elements a thread put it on more waves.
-    constexpr int ES = row_small_e(LOGM, MODE);
-    if constexpr (ES != 4) {
-      if (!dj && njobs == 1 && nrows == 1) {
-        using RSS = RowShape<LOGM, ES>;
-        hipLaunchKernelGGL((k_rows2<LOGM, LAZY, MODE, ES>), dim3(1, 1), dim3(RSS::T),
-                           (size_t)RSS::U * RSS::lds_words_per_row * 4, c.stream, pack, dj);
-        return;
-      }
-    }
-    // The row pass of a four-step product of up to 2^21 points (rows of
-    // 2^6 .. 2^12 words): four elements a thread, so each thread issues a
-    // quarter of the loads (the pass is latency-bound at these sizes).
-    constexpr int EM = row_multi_e(LOGM, MODE);
-    if constexpr (EM != 4) {
-      if (!dj && nrows > 1 && ((size_t)nrows << LOGM) <= ((size_t)1 << 21)) {
-        using RSM = RowShape<LOGM, EM>;
-        hipLaunchKernelGGL((k_rows2<LOGM, LAZY, MODE, EM>), dim3((nrows + RSM::U - 1) / RSM::U, njobs),
-                           dim3(RSM::T), (size_t)RSM::U * RSM::lds_words_per_row * 4, c.stream, pack, dj);
-        return;
-      }
-    }
-    dim3 grid((nrows + RS::U - 1) / RS::U, njobs), block(RS::T);
-    hipLaunchKernelGGL((k_rows2<LOGM, LAZY, MODE>), grid, block, lds, c.stream, pack, dj);
-  }
-}
-
-template <int LOGM, bool LAZY>
-static void rows2_mode(Context& c, int mode, const RowJob2* j, uint32_t nrows, uint32_t njobs, const RowJob2* dj) {
-  switch (mode) {
-    case R2M_FWD:
-      if constexpr (LAZY) {
-        // Signed residues when every by-value job's prime allows them.
-        bool sgn = !dj;
-        for (uint32_t i = 0; sgn && i < njobs; ++i) sgn = signed_ok(j[i].f.p);
-        if (sgn) {
-          rows2_launch<LOGM, LAZY, R2M_FWDS>(c, j, nrows, njobs, dj);
-          break;
-        }
-      }
-      rows2_launch<LOGM, LAZY, R2M_FWD>(c, j, nrows, njobs, dj);
-      break;
-    case R2M_INV: rows2_launch<LOGM, LAZY, R2M_INV>(c, j, nrows, njobs, dj); break;
-    case R2M_CONV: rows2_launch<LOGM, LAZY, R2M_CONV>(c, j, nrows, njobs, dj); break;
-    case R2M_SPEC: rows2_launch<LOGM, LAZY, R2M_SPEC>(c, j, nrows, njobs, dj); break;
-    case R2M_SPEC_ONLY: rows2_launch<LOGM, LAZY, R2M_SPEC_ONLY>(c, j, nrows, njobs, dj); break;
-    case R2M_SPEC_SUM2: rows2_launch<LOGM, LAZY, R2M_SPEC_SUM2>(c, j, nrows, njobs, dj); break;
-    case R2M_INVSTEP:
-    case R2M_EXPSTEP:
-    case R2M_SQRTSTEP:
-      if constexpr (LOGM >= 6 && LOGM <= 13) {
-        if (mode == R2M_INVSTEP) rows2_launch<LOGM, LAZY, R2M_INVSTEP>(c, j, nrows, njobs, dj);
-        else if (mode == R2M_EXPSTEP) rows2_launch<LOGM, LAZY, R2M_EXPSTEP>(c, j, nrows, njobs, dj);
-        else rows2_launch<LOGM, LAZY, R2M_SQRTSTEP>(c, j, nrows, njobs, dj);
-        break;
-      }
-      throw Error(E_DEVICE, "Newton-step row pass outside 2^6 .. 2^13 points");
-    case R2M_NINV:
-    case R2M_NEXP:
-    case R2M_NSQRT:
-      if constexpr (LOGM == kSmallSeriesLog) {
-        if (mode == R2M_NINV) rows2_launch<LOGM, LAZY, R2M_NINV>(c, j, nrows, njobs, dj);
-        else if (mode == R2M_NEXP) rows2_launch<LOGM, LAZY, R2M_NEXP>(c, j, nrows, njobs, dj);
-        else rows2_launch<LOGM, LAZY, R2M_NSQRT>(c, j, nrows, njobs, dj);
-        break;
-      }
-      throw Error(E_DEVICE, "small-series launch recorded with the wrong size key");
-    default: throw Error(E_DEVICE, "unsupported row-pass mode");
-  }
-}
-
-template <bool LAZY>
-static void rows2_size(Context& c, int logM, int mode, const RowJob2* j, uint32_t nrows, uint32_t njobs,
-                       const RowJob2* dj) {
-  switch (logM) {
-#define FPXG_R(L) \
-  case L: rows2_mode<L, LAZY>(c, mode, j, nrows, njobs, dj); break;
-    FPXG_R(1) FPXG_R(2) FPXG_R(3) FPXG_R(4) FPXG_R(5) FPXG_R(6) FPXG_R(7)
-    FPXG_R(8) FPXG_R(9) FPXG_R(10) FPXG_R(11) FPXG_R(12) FPXG_R(13) FPXG_R(14)
-#undef FPXG_R
-    default: throw Error(E_SIZE, "row pass size out of range");
-  }
+// A row launch through the part that instantiates its shape.
+static void rows2_size(Context& c, bool lazy, int logM, int mode, const RowJob2* j, uint32_t nrows,
+                       uint32_t njobs, const RowJob2* dj) {
+  if (logM < 1 || logM > 14) throw Error(E_SIZE, "row pass size out of range");
+  if (!lazy) rows2_part_canon(c, logM, mode, j, nrows, njobs, dj);
+  else if (logM <= 11) rows2_part_lazy_lo(c, logM, mode, j, nrows, njobs, dj);
+  else if (logM == 12) rows2_part_lazy_12(c, logM, mode, j, nrows, njobs, dj);
+  else if (logM == 13) rows2_part_lazy_13(c, logM, mode, j, nrows, njobs, dj);
+  else rows2_part_lazy_14(c, logM, mode, j, nrows, njobs, dj);
 }
 
 static void launch_rows(Context& c, bool lazy, int mode, const RowJob2& j, int logM, uint32_t nrows,
@@ -507,10 +382,10 @@ static void launch_rows(Context& c, bool lazy, int mode, const RowJob2& j, int l
     c.rec->push_rows(logM, lazy, mode, jj);
     return;
   }
-  if (lazy) rows2_size<true>(c, logM, mode, &j, nrows, njobs, dj);
-  else rows2_size<false>(c, logM, mode, &j, nrows, njobs, dj);
+  rows2_size(c, lazy, logM, mode, &j, nrows, njobs, dj);
   FPXG_HIP(hipGetLastError());
 }
+
 // k <= kRowPack independent row passes of one shape (nrows rows of 2^logM
 // each) in one launch; recorded as k records sharing one launch level.
 static void launch_rows_group(Context& c, bool lazy, int mode, const RowJob2* js, int k, int logM,
@@ -525,65 +400,8 @@ static void launch_rows_group(Context& c, bool lazy, int mode, const RowJob2* js
     for (int i = 0; i < k; ++i) c.rec->push_rows(logM, lazy, mode, jj[i], i > 0);
     return;
   }
-  if (lazy) rows2_size<true>(c, logM, mode, jj, nrows, (uint32_t)k, nullptr);
-  else rows2_size<false>(c, logM, mode, jj, nrows, (uint32_t)k, nullptr);
+  rows2_size(c, lazy, logM, mode, jj, nrows, (uint32_t)k, nullptr);
   FPXG_HIP(hipGetLastError());
-}
-
-// Column tiles: 16 columns up to 2^10 levels, 8 for 2^11, 4 for 2^12.  A
-// 2^10-level pass with fewer than 512 tiles of 16 uses tiles of 8, so a
-// mid-size transform still spreads over the CUs.
-static constexpr int col_tile_log(int S) { return S <= 10 ? 4 : (S <= 11 ? 3 : 2); }
-static int tile_log_cols(int S, int L) {
-  if (S == 10 && L - S - 4 < 9) return 3;
-  return col_tile_log(S);
-}
-
-// njobs jobs from the device array dj, or (dj null) the njobs by-value jobs
-// j[0 .. njobs).
-template <int S, int LOGC, bool LAZY, bool INV, bool SGN>
-static void cols2_launch_c(Context& c, const ColJob2* j, const PassPlan& P, uint32_t njobs, const ColJob2* dj) {
-  using CS = ColShape<S, LOGC>;
-  const size_t lds = (size_t)CS::words * 4;
-  JobPack<ColJob2, kColPack> pack;
-  for (int i = 0; i < kColPack; ++i) pack.j[i] = j[(!dj && (uint32_t)i < njobs) ? i : 0];
-  // Columns of lone calls and Newton steps up to 2^20 points (2^5 .. 2^10
-  // levels): four elements a thread (at most 1024 threads a tile, so 8
-  // columns a tile at 2^9, 4 at 2^10).  Merged batches and larger transforms
-  // keep 16: their launches are throughput-bound.
-  if constexpr (S >= 5 && S <= 10) {
-    if (!dj && P.L <= 20) {
-      constexpr int LC = LOGC < 10 - (S - 2) ? LOGC : 10 - (S - 2);
-      using CS2 = ColShape<S, LC, 2>;
-      hipLaunchKernelGGL((k_cols2<S, LC, LAZY, INV, SGN, 2>), dim3(1u << (P.b - LC), njobs), dim3(CS2::T),
-                         (size_t)CS2::words * 4, c.stream, pack, dj, P.L);
-      return;
-    }
-  }
-  dim3 grid(1u << (P.b - LOGC), njobs), block(CS::T);
-  hipLaunchKernelGGL((k_cols2<S, LOGC, LAZY, INV, SGN>), grid, block, lds, c.stream, pack, dj, P.L);
-}
-template <int S, bool LAZY, bool INV, bool SGN>
-static void cols2_launch(Context& c, const ColJob2* j, const PassPlan& P, uint32_t njobs, const ColJob2* dj) {
-  if constexpr (S == 10) {
-    if (P.logC == 3) {
-      cols2_launch_c<S, 3, LAZY, INV, SGN>(c, j, P, njobs, dj);
-      return;
-    }
-  }
-  cols2_launch_c<S, col_tile_log(S), LAZY, INV, SGN>(c, j, P, njobs, dj);
-}
-
-template <bool LAZY, bool INV, bool SGN = false>
-static void cols2_size(Context& c, const ColJob2* j, const PassPlan& P, uint32_t njobs, const ColJob2* dj) {
-  switch (P.S) {
-#define FPXG_C(S) \
-  case S: cols2_launch<S, LAZY, INV, SGN>(c, j, P, njobs, dj); break;
-    FPXG_C(1) FPXG_C(2) FPXG_C(3) FPXG_C(4) FPXG_C(5) FPXG_C(6)
-    FPXG_C(7) FPXG_C(8) FPXG_C(9) FPXG_C(10) FPXG_C(11) FPXG_C(12)
-#undef FPXG_C
-    default: throw Error(E_SIZE, "column pass size out of range");
-  }
 }
 
 // njobs jobs from the device array dj, or (dj null) the njobs by-value jobs
@@ -600,60 +418,22 @@ static void launch_cols_n(Context& c, bool lazy, bool inv, const ColJob2* j, con
   // job array may mix primes).
   bool sgn = lazy && !inv && !dj;
   for (uint32_t i = 0; sgn && i < njobs; ++i) sgn = signed_ok(j[i].f.p);
-  if (lazy) {
-    if (inv) cols2_size<true, true>(c, j, P, njobs, dj);
-    else if (sgn) cols2_size<true, false, true>(c, j, P, njobs, dj);
-    else cols2_size<true, false>(c, j, P, njobs, dj);
-  } else {
-    if (inv) cols2_size<false, true>(c, j, P, njobs, dj);
-    else cols2_size<false, false>(c, j, P, njobs, dj);
-  }
+  if (!lazy) cols2_part_canon(c, inv, j, P, njobs, dj);
+  else if (inv) cols2_part_lazy_inv(c, j, P, njobs, dj);
+  else cols2_part_lazy_fwd(c, sgn, j, P, njobs, dj);
   FPXG_HIP(hipGetLastError());
 }
+
 static void launch_cols(Context& c, bool lazy, bool inv, const ColJob2& j, const PassPlan& P, uint32_t njobs = 1,
                         const ColJob2* dj = nullptr) {
   launch_cols_n(c, lazy, inv, &j, P, njobs, dj);
 }
+
 // Two independent column passes of one shape in one launch.
 static void launch_cols_pair(Context& c, bool lazy, bool inv, const ColJob2& j0, const ColJob2& j1,
                              const PassPlan& P) {
   const ColJob2 js[2] = {j0, j1};
   launch_cols_n(c, lazy, inv, js, P, 2, nullptr);
-}
-
-// The inverse column pass of CRT products with Garner fused (k_cols2_crt):
-// tiles as in launch_cols; the auxiliary primes are lazy (all below 2^30).
-template <int S, int LOGC, int E>
-static void colcrt_go(Context& c, const CrtColJob* j, const PassPlan& P, uint32_t njobs, const CrtColJob* dj,
-                      const CrtFields& F) {
-  using CS = ColShape<S, LOGC, E>;
-  JobPack<CrtColJob, kCrtPack> pack;
-  for (int i = 0; i < kCrtPack; ++i) pack.j[i] = j[(!dj && (uint32_t)i < njobs) ? i : 0];
-  hipLaunchKernelGGL((k_cols2_crt<S, LOGC, true, E>), dim3(1u << (P.b - LOGC), njobs), dim3(CS::T),
-                     (size_t)CS::words * 4, c.stream, pack, dj, P.L, F);
-}
-template <int S, int LOGC>
-static void colcrt_launch_c(Context& c, const CrtColJob* j, const PassPlan& P, uint32_t njobs, const CrtColJob* dj,
-                            const CrtFields& F) {
-  if constexpr (S >= 5 && S <= 10) {
-    if (!dj && P.L <= 20) {  // four elements a thread, as in cols2_launch_c
-      constexpr int LC = LOGC < 10 - (S - 2) ? LOGC : 10 - (S - 2);
-      colcrt_go<S, LC, 2>(c, j, P, njobs, dj, F);
-      return;
-    }
-  }
-  colcrt_go<S, LOGC, 4>(c, j, P, njobs, dj, F);
-}
-template <int S>
-static void colcrt_launch(Context& c, const CrtColJob* j, const PassPlan& P, uint32_t njobs, const CrtColJob* dj,
-                          const CrtFields& F) {
-  if constexpr (S == 10) {
-    if (P.logC == 3) {
-      colcrt_launch_c<S, 3>(c, j, P, njobs, dj, F);
-      return;
-    }
-  }
-  colcrt_launch_c<S, col_tile_log(S)>(c, j, P, njobs, dj, F);
 }
 
 static const CrtFields& crt_fields(Context& c);
@@ -678,36 +458,8 @@ static void launch_cols_crt(Context& c, const CrtColJob* j, const PassPlan& P, u
     for (uint32_t i = 0; i < njobs; ++i) c.rec->push_colg(P.S, P.L, j[i], i > 0);
     return;
   }
-  const CrtFields& F = crt_fields(c);
-  switch (P.S) {
-#define FPXG_C(S) \
-  case S: colcrt_launch<S>(c, j, P, njobs, dj, F); break;
-    FPXG_C(1) FPXG_C(2) FPXG_C(3) FPXG_C(4) FPXG_C(5) FPXG_C(6)
-    FPXG_C(7) FPXG_C(8) FPXG_C(9) FPXG_C(10) FPXG_C(11) FPXG_C(12)
-#undef FPXG_C
-    default: throw Error(E_SIZE, "column pass size out of range");
-  }
+  colcrt_part(c, j, P, njobs, dj, crt_fields(c));
   FPXG_HIP(hipGetLastError());
-}
-
-template <int LOGM, bool LAZY, int MODE>
-static bool rows2_attr() {
-  using RS = RowShape<LOGM>;
-  return hipFuncSetAttribute((const void*)k_rows2<LOGM, LAZY, MODE>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                             (int)(RS::U * RS::lds_words_per_row * 4)) == hipSuccess;
-}
-template <int S, bool LAZY, bool INV, bool SGN = false>
-static bool cols2_attr() {
-  constexpr int LOGC = col_tile_log(S);
-  return hipFuncSetAttribute((const void*)k_cols2<S, LOGC, LAZY, INV, SGN>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                             (int)(ColShape<S, LOGC>::words * 4)) == hipSuccess;
-}
-
-template <int S>
-static bool colcrt_attr() {
-  constexpr int LOGC = col_tile_log(S);
-  return hipFuncSetAttribute((const void*)k_cols2_crt<S, LOGC, true, 4>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                             (int)(ColShape<S, LOGC>::words * 4)) == hipSuccess;
 }
 
 static void context_init(Context& c) {
@@ -720,17 +472,13 @@ static void context_init(Context& c) {
   c.cus = prop.multiProcessorCount;
   // Row passes of 2^14 words and column tiles of 2^14 words need 67.6 KB of
   // LDS per workgroup (gfx950: 160 KB per CU).  Without it, stop at 2^13.
-  bool big = true;
-#define X(L, M) big = rows2_attr<L, true, M>() && rows2_attr<L, false, M>() && big;
-  X(13, R2M_FWD) X(13, R2M_INV) X(13, R2M_CONV) X(13, R2M_SPEC) X(13, R2M_SPEC_ONLY) X(13, R2M_SPEC_SUM2)
-  X(14, R2M_FWD) X(14, R2M_INV) X(14, R2M_CONV) X(14, R2M_SPEC) X(14, R2M_SPEC_ONLY) X(14, R2M_SPEC_SUM2)
-#undef X
-  big = rows2_attr<13, true, R2M_FWDS>() && rows2_attr<14, true, R2M_FWDS>() && big;
-#define X(S) big = cols2_attr<S, true, false>() && cols2_attr<S, true, true>() && cols2_attr<S, false, false>() && \
-                   cols2_attr<S, false, true>() && cols2_attr<S, true, false, true>() && big;
-  X(9) X(10) X(11) X(12)
-#undef X
-  big = colcrt_attr<10>() && colcrt_attr<11>() && colcrt_attr<12>() && big;
+  bool big = rows2_attrs_lazy_13();
+  big = rows2_attrs_lazy_14() && big;
+  big = rows2_attrs_canon() && big;
+  big = cols2_attrs_lazy_fwd() && big;
+  big = cols2_attrs_lazy_inv() && big;
+  big = cols2_attrs_canon() && big;
+  big = colcrt_attrs() && big;
   (void)hipGetLastError();
   c.lds_bytes = big ? (size_t)lds_words(1u << 14) * 4 : 65536;
   c.row_log_max = big ? 14 : 13;

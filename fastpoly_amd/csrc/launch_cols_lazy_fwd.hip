@@ -1,0 +1,33 @@
+// Forward column-pass launches, lazy limbs (unsigned, or signed residues for
+// p < 2^28) (see launch.hpp).
+#include "launch.hpp"
+
+namespace fpxg {
+
+template <bool SGN>
+static void cols_fwd(Context& c, const ColJob2* j, const PassPlan& P, uint32_t njobs, const ColJob2* dj) {
+  switch (P.S) {
+#define FPXG_C(S) \
+  case S: cols2_launch<S, true, false, SGN>(c, j, P, njobs, dj); break;
+    FPXG_C(1) FPXG_C(2) FPXG_C(3) FPXG_C(4) FPXG_C(5) FPXG_C(6)
+    FPXG_C(7) FPXG_C(8) FPXG_C(9) FPXG_C(10) FPXG_C(11) FPXG_C(12)
+#undef FPXG_C
+    default: throw Error(E_SIZE, "column pass size out of range");
+  }
+}
+
+void cols2_part_lazy_fwd(Context& c, bool sgn, const ColJob2* j, const PassPlan& P, uint32_t njobs,
+                         const ColJob2* dj) {
+  if (sgn) cols_fwd<true>(c, j, P, njobs, dj);
+  else cols_fwd<false>(c, j, P, njobs, dj);
+}
+
+bool cols2_attrs_lazy_fwd() {
+  bool ok = true;
+#define X(S) ok = cols2_attr<S, true, false>() && cols2_attr<S, true, false, true>() && ok;
+  X(9) X(10) X(11) X(12)
+#undef X
+  return ok;
+}
+
+}  // namespace fpxg

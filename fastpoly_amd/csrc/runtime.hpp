@@ -1,6 +1,6 @@
 // fastpoly_amd host runtime: device context, arena, per-prime plans and the
-// transform / convolution drivers.  One translation unit (fpxg.hip) includes
-// this header, the kernels and the series code.
+// transform / convolution drivers.  fpxg.hip includes it with the drivers and
+// the series code; the launch_*.hip units include it for the kernel tables.
 #pragma once
 
 #include <hip/hip_runtime.h>
