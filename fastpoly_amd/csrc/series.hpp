@@ -445,12 +445,16 @@ static void series_emit(Context& c, const HostField& T, const SeriesReq& q, cons
       return;
     }
     case FPXG_SQRT: {  // poly::sqrt (square_root.hpp:12-112)
-      emit_ew(c, T, EW_FILL, out, nullptr, nullptr, n, 0, 0);
+      // The domain checks come before the output's zero fill: a failing call
+      // writes nothing.
       const size_t v = s.first;
+      uint32_t root0 = 0;
+      if (v < la) {
+        if (v & 1) throw Error(E_DOMAIN, "poly::sqrt: odd valuation, no square root");
+        if (!T.sqrt(s.afirst, &root0)) throw Error(E_DOMAIN, "poly::sqrt: non-residue constant term");
+      }
+      emit_ew(c, T, EW_FILL, out, nullptr, nullptr, n, 0, 0);
       if (v >= la) return;
-      if (v & 1) throw Error(E_DOMAIN, "poly::sqrt: odd valuation, no square root");
-      uint32_t root0;
-      if (!T.sqrt(s.afirst, &root0)) throw Error(E_DOMAIN, "poly::sqrt: non-residue constant term");
       const size_t half = v / 2;
       if (half >= n) return;
       const size_t inner = n - half, au = la - v;
@@ -538,17 +542,21 @@ static void series_emit(Context& c, const HostField& T, const SeriesReq& q, cons
 }
 
 // Characteristic-two square root (square_root.hpp:15-25), immediate only.
+// The result is built in a buffer of its own and copied out only after the
+// check, so a failing call writes nothing.
 static void sqrt_char2(Context& c, const HostField& T, const uint32_t* a, size_t la, size_t n, uint32_t* out) {
-  emit_ew(c, T, EW_FILL, out, nullptr, nullptr, n, 0, 0);
+  DevBuf res = c.buf(n);
+  emit_ew(c, T, EW_FILL, res.get(), nullptr, nullptr, n, 0, 0);
   const size_t cnt = std::min(la, n);
   DevBuf flag = c.buf(1);
   emit_ew(c, T, EW_FILL, flag.get(), nullptr, nullptr, 1, 0, 0);
   if (cnt) {
-    hipLaunchKernelGGL(k_frobenius_sqrt, dim3(grid_for(cnt)), dim3(256), 0, c.stream, a, cnt, out,
+    hipLaunchKernelGGL(k_frobenius_sqrt, dim3(grid_for(cnt)), dim3(256), 0, c.stream, a, cnt, res.get(),
                        (unsigned int*)flag.get());
     FPXG_HIP(hipGetLastError());
   }
   if (read_word(c, flag.get())) throw Error(E_DOMAIN, "poly::sqrt: nonzero odd coefficient in characteristic 2");
+  FPXG_HIP(hipMemcpyAsync(out, res.get(), n * 4, hipMemcpyDeviceToDevice, c.stream));
 }
 
 static void series_dev(Context& c, const HostField& T, int op, const uint32_t* a, size_t la, size_t n, uint32_t* out) {

@@ -142,3 +142,52 @@ def test_autotune_keeps_results_exact(fa, oracle):
     for op in ("exp", "inv", "sqrt", "log"):
         s = _input(oracle, op, P, 40000, 9)
         assert np.array_equal(fa.series(op, P, s, 40000), oracle.series(op, P, s, 40000)), op
+
+
+@pytest.mark.parametrize("op", ["inv", "log", "exp", "sqrt"])
+def test_replays_with_changing_scan(fa, oracle, op):
+    """One call site (same buffers) whose input alternates between scan
+    signatures: the graph cache must pick the schedule of the current scan
+    (fewer nonzero coefficients, a leading zero for sqrt, another leading
+    coefficient for sqrt) and raise the domain error with the output
+    untouched for inputs that violate the domain, on every call."""
+    n = 20000
+    D = fa.DeviceArray
+    da, out = D(n), D(n)
+    base = _input(oracle, op, P, n, 61)
+
+    def variant(kind):
+        a = base.copy()
+        if kind == "short":  # a shorter polynomial: other last nonzero index
+            a[n // 3:] = 0
+        elif kind == "bad":  # domain error for every op
+            if op == "inv":
+                a[0] = 0
+            elif op == "exp":
+                a[0] = oracle.from_int(P, 1)
+            elif op == "log":
+                a[0] = oracle.from_int(P, 2)
+            else:
+                a[0] = 0
+                a[1] = oracle.from_int(P, 1)  # odd valuation
+        elif kind == "shift" and op == "sqrt":  # valuation 2: another schedule
+            a[2:] = base[:-2]
+            a[:2] = 0
+        elif kind == "lead" and op == "sqrt":  # another square root taken on the host
+            a[0] = oracle.from_int(P, 4)
+        return a
+
+    kinds = ["base", "base", "base", "short", "base", "bad", "base", "short", "short", "short", "bad", "base",
+             "base", "shift", "lead", "base", "lead", "lead", "base"]
+    sentinel = np.full(n, 7, dtype=np.uint32)
+    for i, kind in enumerate(kinds):
+        a = variant(kind)
+        assert fa.lib().fpxg_memcpy_h2d(da.ptr, a.ctypes.data, 4 * n) == 0
+        if kind == "bad":
+            assert fa.lib().fpxg_memcpy_h2d(out.ptr, sentinel.ctypes.data, 4 * n) == 0
+            with pytest.raises(fa.DomainError):
+                fa.series_dev(op, P, da, n, n, out)
+            assert np.array_equal(out.to_host(), sentinel), (op, i, kind)
+            continue
+        fa.series_dev(op, P, da, n, n, out)
+        assert np.array_equal(out.to_host(), oracle.series(op, P, a, n)), (op, i, kind)

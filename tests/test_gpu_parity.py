@@ -464,3 +464,25 @@ def test_sqrt_input_longer_than_n_dev_batch_server(fa, oracle):
                 assert isinstance(res, fa.DomainError), (len(a), n)
             else:
                 assert np.array_equal(res, want), (len(a), n)
+
+
+def test_series_domain_error_writes_nothing(fa, oracle):
+    """A series call that raises the reference's domain error leaves the
+    caller's output buffer as it was (the reference returns nothing), on a
+    first (eager) call at a fresh site, for every op and for p = 2."""
+    f = lambda p, x: oracle.from_int(p, x)  # noqa: E731
+    cases = [
+        ("inv", P, [0, f(P, 1), f(P, 2)], 5),
+        ("log", P, [f(P, 2), f(P, 1)], 5),
+        ("exp", P, [f(P, 1), f(P, 1)], 5),
+        ("sqrt", P, [0, f(P, 1), f(P, 1)], 5),   # odd valuation
+        ("sqrt", P, [f(P, 5), f(P, 1)], 5),      # 5 is a non-residue mod 998244353
+        ("sqrt", 2, [1, 1, 0, 1], 3),            # characteristic 2: nonzero odd coefficient
+    ]
+    for op, p, a, n in cases:
+        d = fa.DeviceArray.from_host(np.asarray(a, np.uint32))
+        sentinel = np.arange(n, dtype=np.uint32) + 11
+        o = fa.DeviceArray.from_host(sentinel)
+        with pytest.raises(fa.DomainError):
+            fa.series_dev(op, p, d, len(a), n, o)
+        assert np.array_equal(o.to_host(), sentinel), (op, p, a)
