@@ -147,6 +147,47 @@ inline int tile_log_cols(int S, int L) {
   return col_tile_log(S);
 }
 
+// Tile width (log2 of the columns a tile) of the by-value column launches of
+// up to 2^20 points (2^5 .. 2^10 levels, four elements a thread): these
+// launches are latency-bound, and a workgroup's arithmetic is serialised over
+// its waves, so narrow tiles on many CUs finish first.  One column a tile
+// while rows have at most 2^7 words (a tile's reads are then at most 512
+// bytes apart), four at 2^8 .. 2^10 words, eight for the forward pass at
+// 2^10; at least 64 and at most 1024 threads a tile.  Measured on the MI355X
+// by graph replay of the pass alone, 2^13 .. 2^20 points: 2.1 .. 7 us a
+// launch against 2.2 .. 9 us with the widest tile that fits 1024 threads.
+static int small_col_tile_log(int S, int b, bool inv) {
+  int lc = b <= 7 ? 0 : (b == 10 && !inv ? 3 : 2);
+  lc = std::max(lc, 8 - S);
+  lc = std::min(lc, 12 - S);
+  return lc;
+}
+
+template <int S, int LC, bool LAZY, bool INV, bool SGN>
+static void cols2_small_go(Context& c, const JobPack<ColJob2, kColPack>& pack, const PassPlan& P, uint32_t njobs) {
+  using CS2 = ColShape<S, LC, 2>;
+  hipLaunchKernelGGL((k_cols2<S, LC, LAZY, INV, SGN, 2>), dim3(1u << (P.b - LC), njobs), dim3(CS2::T),
+                     (size_t)CS2::words * 4, c.stream, pack, (const ColJob2*)nullptr, P.L);
+}
+
+// Dispatches the tile width lc (8 - S <= lc <= 12 - S, lc <= P.b); false if
+// none applies.
+template <int S, int LC, bool LAZY, bool INV, bool SGN>
+static bool cols2_small(Context& c, const JobPack<ColJob2, kColPack>& pack, const PassPlan& P, uint32_t njobs,
+                        int lc) {
+  if constexpr (LC > 4) {
+    return false;
+  } else {
+    if constexpr (LC >= 8 - S && LC <= 12 - S) {
+      if (lc == LC && P.b >= LC) {
+        cols2_small_go<S, LC, LAZY, INV, SGN>(c, pack, P, njobs);
+        return true;
+      }
+    }
+    return cols2_small<S, LC + 1, LAZY, INV, SGN>(c, pack, P, njobs, lc);
+  }
+}
+
 // njobs jobs from the device array dj, or (dj null) the njobs by-value jobs
 // j[0 .. njobs).
 template <int S, int LOGC, bool LAZY, bool INV, bool SGN>
@@ -156,11 +197,13 @@ static void cols2_launch_c(Context& c, const ColJob2* j, const PassPlan& P, uint
   JobPack<ColJob2, kColPack> pack;
   for (int i = 0; i < kColPack; ++i) pack.j[i] = j[(!dj && (uint32_t)i < njobs) ? i : 0];
   // Columns of lone calls and Newton steps up to 2^20 points (2^5 .. 2^10
-  // levels): four elements a thread (at most 1024 threads a tile, so 8
-  // columns a tile at 2^9, 4 at 2^10).  Merged batches and larger transforms
-  // keep 16: their launches are throughput-bound.
+  // levels): four elements a thread in narrow tiles (small_col_tile_log),
+  // except the 2^20-point forward pass of 2^10 levels, which keeps sixteen
+  // elements a thread in tiles of eight columns.  Merged batches and larger
+  // transforms keep the wide tiles: their launches are throughput-bound.
   if constexpr (S >= 5 && S <= 10) {
-    if (!dj && P.L <= 20) {
+    if (!dj && P.L <= 20 && !(S == 10 && P.b == 10 && !INV)) {
+      if (cols2_small<S, 0, LAZY, INV, SGN>(c, pack, P, njobs, small_col_tile_log(S, P.b, INV))) return;
       constexpr int LC = LOGC < 10 - (S - 2) ? LOGC : 10 - (S - 2);
       using CS2 = ColShape<S, LC, 2>;
       hipLaunchKernelGGL((k_cols2<S, LC, LAZY, INV, SGN, 2>), dim3(1u << (P.b - LC), njobs), dim3(CS2::T),
