@@ -24,8 +24,15 @@ static constexpr int row_small_e(int logm, int mode) {
   return logm <= 12 ? 2 : 3;
 }
 
+// Elements a thread (log2) of the fused product's row pass over many rows of
+// 2^4 .. 2^12 words, up to 2^21 points a job: four (2), or eight (3) once the
+// launch holds 2^21 points or more, where the exchanges of the four-element
+// form start to cost more than its extra waves save.  Measured on the MI355X
+// by graph replay of the pass alone: rows of 2^4 / 2^5 words 5.0 / 5.6 ->
+// 2.6 / 3.0 us (sixteen elements before), two 2^20-point jobs of 2^12-word
+// rows 15.8 -> 13.9 us.
 static constexpr int row_multi_e(int logm, int mode) {
-  return (mode == R2M_CONV && logm >= 6 && logm <= 12) ? 2 : 4;
+  return (mode == R2M_CONV && logm >= 4 && logm <= 12) ? 2 : 4;
 }
 
 // A row launch: njobs jobs from the device array dj, or (dj null) the njobs
@@ -76,11 +83,20 @@ static void rows2_launch(Context& c, const RowJob2* j, uint32_t nrows, uint32_t 
       }
     }
     // The row pass of a four-step product of up to 2^21 points (rows of
-    // 2^6 .. 2^12 words): four elements a thread, so each thread issues a
-    // quarter of the loads (the pass is latency-bound at these sizes).
+    // 2^4 .. 2^12 words): four or eight elements a thread (row_multi_e), so
+    // each thread issues a quarter or half of the loads (the pass is
+    // latency-bound at these sizes).
     constexpr int EM = row_multi_e(LOGM, MODE);
     if constexpr (EM != 4) {
       if (!dj && nrows > 1 && ((size_t)nrows << LOGM) <= ((size_t)1 << 21)) {
+        if constexpr (LOGM >= 8) {
+          if (((size_t)nrows << LOGM) * njobs >= ((size_t)1 << 21)) {
+            using RS3 = RowShape<LOGM, 3>;
+            hipLaunchKernelGGL((k_rows2<LOGM, LAZY, MODE, 3>), dim3((nrows + RS3::U - 1) / RS3::U, njobs),
+                               dim3(RS3::T), (size_t)RS3::U * RS3::lds_words_per_row * 4, c.stream, pack, dj);
+            return;
+          }
+        }
         using RSM = RowShape<LOGM, EM>;
         hipLaunchKernelGGL((k_rows2<LOGM, LAZY, MODE, EM>), dim3((nrows + RSM::U - 1) / RSM::U, njobs),
                            dim3(RSM::T), (size_t)RSM::U * RSM::lds_words_per_row * 4, c.stream, pack, dj);
