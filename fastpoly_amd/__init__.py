@@ -597,6 +597,7 @@ def autotune(min_log: int = 13, max_log: int = 24, reps: int = 5) -> dict:
 OPT_GRAPHS = 0
 OPT_ROW_DMA = 1
 OPT_CRT_FUSED_LOG = 2
+OPT_STEP_LOG_MAX = 3
 
 
 def set_graphs(enabled: bool) -> None:
@@ -615,6 +616,13 @@ def set_crt_fused_log(log2_points: int) -> None:
     Garner inside their inverse column pass; 0 restores the default (same
     results)."""
     _check(lib().fpxg_set_option(OPT_CRT_FUSED_LOG, int(log2_points)), "set_option")
+
+
+def set_step_log_max(log2_points: int) -> None:
+    """Newton steps of 2^10 .. 2^log2_points points (9 <= log2_points <= 13)
+    over a prime with its own transform run as one workgroup's launch, larger
+    ones as four-step products; 0 restores the default, 12 (same results)."""
+    _check(lib().fpxg_set_option(OPT_STEP_LOG_MAX, int(log2_points)), "set_option")
 
 
 def pow(p: int, a, k: int, n: int) -> np.ndarray:  # noqa: A001 (the reference's name)

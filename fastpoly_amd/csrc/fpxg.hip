@@ -1439,6 +1439,13 @@ int fpxg_set_option(int option, int64_t value) {
         drop_graphs(c);
         c.graph_seen.clear();
         return;
+      case FPXG_OPT_STEP_LOG_MAX:
+        if (value != 0 && (value < 9 || value > 13)) throw Error(E_ARG, "one-workgroup step size out of range");
+        FPXG_HIP(hipStreamSynchronize(c.stream));
+        c.step_log_max = value ? (int)value : 12;
+        drop_graphs(c);
+        c.graph_seen.clear();
+        return;
       default:
         throw Error(E_ARG, "unknown option");
     }

@@ -208,6 +208,10 @@ struct Context {
   hipEvent_t events[16] = {};
   size_t lds_bytes = 65536;  // usable dynamic LDS per workgroup
   int row_log_max = 14;      // largest row pass
+  // Largest Newton step run as one workgroup's launch (2^12 points: at 2^13
+  // the four-step products' launches, each on many CUs, finish first; inv /
+  // exp / sqrt at 2^20: -2 / -9 / -7 us).
+  int step_log_max = 12;
   bool row_dma = true;       // persistent prefetching forward pass for many 2^14-word rows
   int single_log_max = 12;   // largest transform done as one row pass
   // CRT products with a column pass from 2^crt_cols_log points run

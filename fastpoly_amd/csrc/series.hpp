@@ -187,7 +187,7 @@ static void native_conv_multi(Context& c, const HostField& T, int L, const WinPr
 // step modes up to 2^13 (NTT primes).
 static bool one_launch_step(const Context& c, const HostField& T, const Engine& E, size_t m) {
   if (E.L < 6 || ((size_t)1 << E.L) != 2 * m || T.p == 2) return false;
-  return E.L <= 9 || (E.native && E.L <= std::min(c.row_log_max, 13));
+  return E.L <= 9 || (E.native && E.L <= std::min(c.row_log_max, c.step_log_max));
 }
 
 // Fields without their own transform: out[i - lo] = (x * y)[i] for i in

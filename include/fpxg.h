@@ -152,8 +152,11 @@ int fpxg_product_plan_info(uint32_t n, int* row_log, int* col_levels);
  *   turn); smaller ones write the primes' residues and run Garner after.
  *   0 restores the default (2^20 points for single calls, every size in
  *   batches and the server).
+ *   FPXG_OPT_STEP_LOG_MAX: v in [9, 13]: Newton steps of 2^10 .. 2^v points
+ *   over a prime with its own transform run as one workgroup's launch, larger
+ *   ones as four-step products; 0 restores the default (12).
  * Results are identical under every option. */
-enum { FPXG_OPT_GRAPHS = 0, FPXG_OPT_ROW_DMA = 1, FPXG_OPT_CRT_FUSED_LOG = 2 };
+enum { FPXG_OPT_GRAPHS = 0, FPXG_OPT_ROW_DMA = 1, FPXG_OPT_CRT_FUSED_LOG = 2, FPXG_OPT_STEP_LOG_MAX = 3 };
 int fpxg_set_option(int option, int64_t value);
 
 /* Device memory and stream plumbing for the *_dev entry points. */

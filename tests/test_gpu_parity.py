@@ -486,3 +486,31 @@ def test_series_domain_error_writes_nothing(fa, oracle):
         with pytest.raises(fa.DomainError):
             fa.series_dev(op, p, d, len(a), n, o)
         assert np.array_equal(o.to_host(), sentinel), (op, p, a)
+
+
+@pytest.mark.parametrize("step_max", [9, 13])
+def test_newton_step_placement(fa, oracle, step_max):
+    """Newton steps of 2^10 .. 2^13 points run as one workgroup's launch up to
+    2^step_max points and as four-step products beyond (default 2^12): with
+    every step split (9) and with one-launch steps up to 2^13 (13), inv / log /
+    exp / sqrt over 998244353 give the oracle's limbs, eagerly and replayed."""
+    fa.set_step_log_max(step_max)
+    try:
+        rng = random.Random(step_max)
+        for n in (3000, 9000, 20000):
+            a = _rnd(oracle, P, n, rng.randrange(1 << 30))
+            a[0] = oracle.from_int(P, 777)
+            a1 = a.copy(); a1[0] = oracle.from_int(P, 1)
+            a0 = a.copy(); a0[0] = 0
+            sq = oracle.conv(P, a1, a1, n)
+            ins = {"inv": a, "log": a1, "exp": a0, "sqrt": sq}
+            for op in ("inv", "log", "exp", "sqrt"):
+                want = oracle.series(op, P, ins[op], n)
+                d, o = fa.DeviceArray.from_host(ins[op]), fa.DeviceArray(n)
+                for rep in range(3):  # eager, captured, replayed
+                    fa.series_dev(op, P, d, n, n, o)
+                    assert np.array_equal(o.to_host(), want), (op, n, step_max, rep)
+    finally:
+        fa.set_step_log_max(0)
+    with pytest.raises(fa.ArgumentError):
+        fa.set_step_log_max(14)
