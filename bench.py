@@ -296,8 +296,11 @@ def ntt_only(fa, args):
 def series_only(fa, args):
     """The step's Newton series chains alone (after the autotuner, inputs as
     in the step), each op called args.warmup + args.steps times on the same
-    buffers: the second call captures its hipGraph, later calls replay it."""
+    buffers: the second call captures its hipGraph, later calls replay it.
+    The JSON line names the fused products' pass splits the autotuner chose
+    (a run under a profiler may time, and so choose, differently)."""
     fa.autotune(13, 24, 3)
+    plans = {lg: fa.product_plan_info(1 << lg)["col_levels"] for lg in range(13, 22)}
     _, _, inv_in, log_in, exp_in, d = make_inputs(np.random.default_rng(20261203))
     D = fa.DeviceArray
     ins = {"inv": D.from_host(inv_in), "log": D.from_host(log_in), "exp": D.from_host(exp_in)}
@@ -316,7 +319,8 @@ def series_only(fa, args):
         fa.event_record(3)
         fa.synchronize()
         res[op + "_ms"] = round(fa.event_elapsed_ms(2, 3) / args.steps, 4)
-    print(json.dumps({"series_only": True, "n": N_SERIES, "steps": args.steps} | res), flush=True)
+    print(json.dumps({"series_only": True, "n": N_SERIES, "steps": args.steps,
+                      "product_col_levels_by_log2n": plans} | res), flush=True)
 
 
 def batch_only(fa, args):
