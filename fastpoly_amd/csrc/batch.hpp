@@ -91,6 +91,15 @@ static size_t group_bytes(LaunchKey k, const std::pair<LaunchKey, const LaunchRe
       case LK_COLS: take(cnt * sizeof(ColJob2)); break;
       case LK_COLG: take(cnt * sizeof(CrtColJob)); break;
       case LK_RECUR: take(cnt * sizeof(RecurRec)); break;
+      case LK_TAIL: take(cnt * sizeof(TailRec)); break;
+      case LK_DIRECT: {
+        size_t nch = 0;
+        for (size_t i = 0; i < cnt; ++i) nch += (g[s + i].second->dir.outc + kDirectTile - 1) / kDirectTile;
+        if (nch == 0) break;
+        take(cnt * sizeof(DirectRec));
+        take(nch * sizeof(uint2));
+        break;
+      }
       default: {
         const bool ew = kind == LK_EW;
         size_t nch = 0;
@@ -138,6 +147,29 @@ static void stage_group(Plan& plan, LaunchKey k, const std::pair<LaunchKey, cons
         st.jobs = plan.alloc(cnt * sizeof(RecurRec));
         RecurRec* d = (RecurRec*)(plan.data + st.jobs);
         for (size_t i = 0; i < cnt; ++i) d[i] = g[s + i].second->rec;
+        break;
+      }
+      case LK_TAIL: {
+        st.jobs = plan.alloc(cnt * sizeof(TailRec));
+        TailRec* d = (TailRec*)(plan.data + st.jobs);
+        for (size_t i = 0; i < cnt; ++i) d[i] = g[s + i].second->tail;
+        break;
+      }
+      case LK_DIRECT: {
+        // One chunk a tile of kDirectTile outputs.
+        size_t nch = 0;
+        for (size_t i = 0; i < cnt; ++i) nch += (g[s + i].second->dir.outc + kDirectTile - 1) / kDirectTile;
+        if (nch == 0) continue;
+        st.jobs = plan.alloc(cnt * sizeof(DirectRec));
+        st.chunks = plan.alloc(nch * sizeof(uint2));
+        DirectRec* d = (DirectRec*)(plan.data + st.jobs);
+        uint2* ch = (uint2*)(plan.data + st.chunks);
+        size_t c = 0;
+        for (size_t i = 0; i < cnt; ++i) {
+          d[i] = g[s + i].second->dir;
+          for (uint32_t o = 0; o < d[i].outc; o += kDirectTile) ch[c++] = make_uint2((uint32_t)i, o);
+        }
+        st.nchunks = (uint32_t)nch;
         break;
       }
       case LK_EW:
@@ -209,6 +241,13 @@ static void issue(Context& c, const Staged& s, const uint8_t* base) {
     case LK_GARNER:
       hipLaunchKernelGGL(k_garner_batch, dim3(s.nchunks), dim3(256), 0, c.stream, (const GarnerRec*)(base + s.jobs),
                          (const uint2*)(base + s.chunks));
+      break;
+    case LK_DIRECT:
+      hipLaunchKernelGGL(k_conv_direct_batch, dim3(s.nchunks), dim3(256), 0, c.stream,
+                         (const DirectRec*)(base + s.jobs), (const uint2*)(base + s.chunks));
+      break;
+    case LK_TAIL:
+      hipLaunchKernelGGL(k_tail_fix_batch, dim3(s.count), dim3(64), 0, c.stream, (const TailRec*)(base + s.jobs));
       break;
   }
   FPXG_HIP(hipGetLastError());

@@ -268,14 +268,14 @@ __global__ void __launch_bounds__(256) k_ew(const EwRec* __restrict__ recs, cons
 constexpr uint32_t kDirectShort = 256;
 constexpr uint32_t kDirectTile = 1024;
 
-__global__ void __launch_bounds__(256) k_conv_direct(const uint32_t* __restrict__ sh, uint32_t ns,
-                                                     const uint32_t* __restrict__ lg, uint32_t nl,
-                                                     uint32_t* __restrict__ out, uint32_t outc, uint32_t p,
-                                                     uint32_t ninv) {
+// Outputs [base, base + kDirectTile) of one direct product (256 threads).
+__device__ __forceinline__ void conv_direct_tile(const uint32_t* __restrict__ sh, uint32_t ns,
+                                                 const uint32_t* __restrict__ lg, uint32_t nl,
+                                                 uint32_t* __restrict__ out, uint32_t outc, uint32_t p,
+                                                 uint32_t ninv, uint32_t base) {
   __shared__ uint32_t S[kDirectShort];
   __shared__ uint32_t W[kDirectTile + kDirectShort];
   const uint32_t t = threadIdx.x;
-  const uint32_t base = blockIdx.x * kDirectTile;
   const int64_t w0 = (int64_t)base - (int64_t)(ns - 1);  // long-operand index of W[0]
   for (uint32_t i = t; i < ns; i += 256) S[i] = sh[i];
   for (uint32_t i = t; i < kDirectTile + ns - 1; i += 256) {
@@ -330,15 +330,30 @@ __global__ void __launch_bounds__(256) k_conv_direct(const uint32_t* __restrict_
   }
 }
 
+__global__ void __launch_bounds__(256) k_conv_direct(const uint32_t* __restrict__ sh, uint32_t ns,
+                                                     const uint32_t* __restrict__ lg, uint32_t nl,
+                                                     uint32_t* __restrict__ out, uint32_t outc, uint32_t p,
+                                                     uint32_t ninv) {
+  conv_direct_tile(sh, ns, lg, nl, out, outc, p, ninv, blockIdx.x * kDirectTile);
+}
+
+// Merged schedules: chunk (record, first output) per workgroup.
+__global__ void __launch_bounds__(256) k_conv_direct_batch(const DirectRec* __restrict__ recs,
+                                                           const uint2* __restrict__ chunks) {
+  const uint2 ch = chunks[blockIdx.x];
+  DirectRec R = recs[ch.x];
+  make_uniform(R);
+  conv_direct_tile(R.sh, R.ns, R.lg, R.nl, R.out, R.outc, R.p, R.ninv, __builtin_amdgcn_readfirstlane(ch.y));
+}
+
 // Tail repair (convolution.hpp:102-137): out[0, min(outc, N)) holds the
 // N-point cyclic product of a and b, whose true length is N + tail (tail <=
 // 40, la, lb <= N).  Thread i < tail computes the high coefficient h = (a
 // b)[N + i] directly (tail - i terms), stores it at N + i and removes its
 // wraparound from position i.
-__global__ void __launch_bounds__(64) k_tail_fix(const uint32_t* __restrict__ a, uint32_t la,
-                                                 const uint32_t* __restrict__ b, uint32_t lb,
-                                                 uint32_t* __restrict__ out, uint32_t outc, uint32_t N,
-                                                 uint32_t tail, uint32_t p, uint32_t ninv) {
+__device__ __forceinline__ void tail_fix(const uint32_t* __restrict__ a, uint32_t la, const uint32_t* __restrict__ b,
+                                         uint32_t lb, uint32_t* __restrict__ out, uint32_t outc, uint32_t N,
+                                         uint32_t tail, uint32_t p, uint32_t ninv) {
   const uint32_t i = threadIdx.x;
   if (i >= tail) return;
   const uint32_t s = N + i;  // a_j b_(s - j) with s - j < lb and j < la
@@ -353,6 +368,20 @@ __global__ void __launch_bounds__(64) k_tail_fix(const uint32_t* __restrict__ a,
     const uint32_t v = out[i];
     out[i] = v >= h ? v - h : v - h + p;
   }
+}
+
+__global__ void __launch_bounds__(64) k_tail_fix(const uint32_t* __restrict__ a, uint32_t la,
+                                                 const uint32_t* __restrict__ b, uint32_t lb,
+                                                 uint32_t* __restrict__ out, uint32_t outc, uint32_t N,
+                                                 uint32_t tail, uint32_t p, uint32_t ninv) {
+  tail_fix(a, la, b, lb, out, outc, N, tail, p, ninv);
+}
+
+// Merged schedules: one workgroup a record.
+__global__ void __launch_bounds__(64) k_tail_fix_batch(const TailRec* __restrict__ recs) {
+  TailRec R = recs[blockIdx.x];
+  make_uniform(R);
+  tail_fix(R.a, R.la, R.b, R.lb, R.out, R.outc, R.N, R.tail, R.p, R.ninv);
 }
 
 // Characteristic two (square_root.hpp:15-25): (sum b_i x^i)^2 = sum b_i x^(2i).

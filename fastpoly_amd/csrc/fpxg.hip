@@ -918,15 +918,16 @@ enum ConvRoute { CR_DIRECT, CR_TAIL, CR_NATIVE, CR_SPLIT, CR_CRT, CR_BLOCKS };
 
 static ConvRoute conv_route(const Context& c, const HostField& T, size_t la, size_t lb, size_t outc) {
   const size_t full = la + lb - 1;
-  // A short side: direct product (lone calls; recorded schedules keep the
-  // transform path, whose launches merge across requests).  Measured
-  // crossover against the transforms (long side 2^20): a short side of ~150
-  // over an NTT prime, ~350 over the auxiliary primes.
+  // A short side: direct product (lone calls, and recorded schedules, whose
+  // direct products merge into one launch per level over a chunk table).
+  // Measured crossover against the transforms (long side 2^20): a short side
+  // of ~150 over an NTT prime, ~350 over the auxiliary primes.
+  (void)c;
   const size_t direct_max = native_ok(T, full) || split_ok(T, la, lb) ? 128 : kDirectShort;
-  if (!c.rec && T.p != 2 && std::min(la, lb) <= direct_max && outc <= ((size_t)1 << 31)) return CR_DIRECT;
+  if (T.p != 2 && std::min(la, lb) <= direct_max && outc <= ((size_t)1 << 31)) return CR_DIRECT;
   // A product a few coefficients past a power of two N: the N-point cyclic
   // product plus the high tail computed directly (convolution.hpp:102-137).
-  if (!c.rec && T.p != 2 && full > 64) {
+  if (T.p != 2 && full > 64) {
     const int Lf = ceil_log2(full) - 1;
     const size_t N = (size_t)1 << Lf, tail = full - N;
     if (tail >= 1 && tail <= 40 && la <= N && lb <= N && native_ok(T, N)) return CR_TAIL;
@@ -976,6 +977,10 @@ void conv_dev(Context& c, const HostField& T, const uint32_t* a, size_t la, cons
       const uint32_t* sh = a_short ? a : b;
       const uint32_t* lg = a_short ? b : a;
       const uint32_t ns = (uint32_t)std::min(la, lb), nl = (uint32_t)std::max(la, lb);
+      if (c.rec) {
+        c.rec->push_direct(DirectRec{sh, lg, out, ns, nl, (uint32_t)outc, T.p, T.ninv, 0u});
+        return;
+      }
       const unsigned blocks = (unsigned)((outc + kDirectTile - 1) / kDirectTile);
       hipLaunchKernelGGL(k_conv_direct, dim3(blocks), dim3(256), 0, c.stream, sh, ns, lg, nl, out, (uint32_t)outc,
                          T.p, T.ninv);
@@ -986,6 +991,11 @@ void conv_dev(Context& c, const HostField& T, const uint32_t* a, size_t la, cons
       const int Lf = ceil_log2(full) - 1;
       const size_t N = (size_t)1 << Lf, tail = full - N;
       conv_core(c, T, a, la, b, lb, out, std::min(outc, N), false, Lf);
+      if (c.rec) {
+        c.rec->push_tail(TailRec{a, b, out, (uint32_t)la, (uint32_t)lb, (uint32_t)outc, (uint32_t)N, (uint32_t)tail,
+                                 T.p, T.ninv, 0u});
+        return;
+      }
       hipLaunchKernelGGL(k_tail_fix, dim3(1), dim3(64), 0, c.stream, a, (uint32_t)la, b, (uint32_t)lb, out,
                          (uint32_t)outc, (uint32_t)N, (uint32_t)tail, T.p, T.ninv);
       FPXG_HIP(hipGetLastError());

@@ -44,6 +44,14 @@ static void issue_one(Context& c, const LaunchRec& r) {
       hipLaunchKernelGGL(k_garner, dim3(grid_for(r.gar.n)), dim3(256), 0, c.stream, r.gar.c1, r.gar.c2, r.gar.c3,
                          r.gar.out, r.gar.n, r.gar.g);
       break;
+    case LK_DIRECT:
+      hipLaunchKernelGGL(k_conv_direct, dim3((r.dir.outc + kDirectTile - 1) / kDirectTile), dim3(256), 0, c.stream,
+                         r.dir.sh, r.dir.ns, r.dir.lg, r.dir.nl, r.dir.out, r.dir.outc, r.dir.p, r.dir.ninv);
+      break;
+    case LK_TAIL:
+      hipLaunchKernelGGL(k_tail_fix, dim3(1), dim3(64), 0, c.stream, r.tail.a, r.tail.la, r.tail.b, r.tail.lb,
+                         r.tail.out, r.tail.outc, r.tail.N, r.tail.tail, r.tail.p, r.tail.ninv);
+      break;
     default:
       throw Error(E_DEVICE, "unknown launch record");
   }

@@ -47,4 +47,23 @@ struct EwRec {
   uint32_t n, avail, c;
 };
 
+// Products with a short side, computed directly (k_conv_direct): out[0, outc)
+// of sh[0, ns) * lg[0, nl), ns <= kDirectShort.
+struct DirectRec {
+  const uint32_t* sh;
+  const uint32_t* lg;
+  uint32_t* out;
+  uint32_t ns, nl, outc, p, ninv, pad;
+};
+
+// Tail repair of a product N + tail coefficients long (k_tail_fix): the high
+// tail of a * b computed directly into out[N, N + tail) and its wraparound
+// removed from out[0, tail).
+struct TailRec {
+  const uint32_t* a;
+  const uint32_t* b;
+  uint32_t* out;
+  uint32_t la, lb, outc, N, tail, p, ninv, pad;
+};
+
 }  // namespace fpxg

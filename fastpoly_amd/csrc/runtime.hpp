@@ -159,7 +159,7 @@ struct PassPlan {
 };
 
 // One recorded kernel launch of a request's schedule.
-enum { LK_ROWS = 0, LK_COLS = 1, LK_EW = 2, LK_RECUR = 3, LK_GARNER = 4, LK_COLG = 5 };
+enum { LK_ROWS = 0, LK_COLS = 1, LK_EW = 2, LK_RECUR = 3, LK_GARNER = 4, LK_COLG = 5, LK_DIRECT = 6, LK_TAIL = 7 };
 struct LaunchRec {
   int kind;
   int k0, k1, k2, k3;  // shape: rows (logM, lazy, mode, -), cols (S, lazy, inv, L), CRT cols (S, -, -, L)
@@ -171,6 +171,8 @@ struct LaunchRec {
     RecurRec rec;
     GarnerRec gar;
     CrtColJob crt;
+    DirectRec dir;
+    TailRec tail;
   };
   LaunchRec() {}
 };
@@ -196,6 +198,8 @@ struct Recorder {
   void push_ew(const EwRec& e) { next(LK_EW).ew = e; }
   void push_recur(const RecurRec& e) { next(LK_RECUR).rec = e; }
   void push_garner(const GarnerRec& g) { next(LK_GARNER).gar = g; }
+  void push_direct(const DirectRec& d) { next(LK_DIRECT).dir = d; }
+  void push_tail(const TailRec& t) { next(LK_TAIL).tail = t; }
   void push_colg(int S, int L, const CrtColJob& j, bool par = false) {
     LaunchRec& r = next(LK_COLG);
     r.k0 = S; r.k1 = 1; r.k3 = L; r.crt = j; r.par = par;

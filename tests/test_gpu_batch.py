@@ -166,3 +166,41 @@ def test_batch_device_submit_wait(fa, oracle):
 
 def test_batch_empty(fa):
     assert fa.batch([]) == []
+
+
+def test_batch_short_side_and_tail_products(fa, oracle):
+    """Products with a short side (direct, k_conv_direct_batch) and products a
+    few coefficients past a power of two (cyclic product plus tail repair,
+    k_tail_fix_batch) in one merged batch and through the server: every
+    request against the oracle, across the boundaries of both routes (short
+    side 128 / 129 over NTT primes, 256 / 257 over 1e9+7; tails of 1 .. 41)."""
+    rng = np.random.default_rng(77)
+    reqs, wants = [], []
+    for p in (P, Q, 3):
+        for ns in (1, 2, 40, 127, 128, 129, 255, 256, 257):
+            for nl in (ns, 1000, 5000):
+                a = rng.integers(0, p, nl, dtype=np.uint32)
+                b = rng.integers(0, p, ns, dtype=np.uint32)
+                reqs.append(fa.Request("conv", p, a, b))
+                wants.append(oracle.conv(p, a, b))
+                reqs.append(fa.Request("conv", p, b, a, n=max(1, nl // 2)))
+                wants.append(oracle.conv(p, b, a)[:max(1, nl // 2)])
+    for p in (P, Q):
+        for N in (1024, 4096):
+            for t in (1, 2, 17, 40, 41):
+                la = N // 2 + t // 2 + 1
+                lb = N + t + 1 - la
+                a = rng.integers(0, p, la, dtype=np.uint32)
+                b = rng.integers(0, p, lb, dtype=np.uint32)
+                reqs.append(fa.Request("conv", p, a, b))
+                wants.append(oracle.conv(p, a, b))
+                reqs.append(fa.Request("conv", p, a, b, n=N + t // 2))
+                wants.append(oracle.conv(p, a, b)[:N + t // 2])
+    got = fa.batch(reqs)
+    for i, (g, w) in enumerate(zip(got, wants)):
+        assert np.array_equal(g, w), (i, reqs[i].p, len(reqs[i].a), len(reqs[i].b))
+    with fa.Server() as srv:
+        ids = [srv.submit(r) for r in reqs]
+        served = [srv.wait(i)[0] for i in ids]
+    for i, (g, w) in enumerate(zip(served, wants)):
+        assert np.array_equal(g, w), (i, reqs[i].p, len(reqs[i].a), len(reqs[i].b))
