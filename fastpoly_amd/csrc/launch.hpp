@@ -18,10 +18,13 @@ inline bool signed_ok(uint32_t p) { return p < (1u << 28); }
 // Elements a thread owns in a one-row launch (log2): fewer than 16 for the
 // fused products and Newton steps of 2^10 .. 2^13 points, whose single
 // workgroup would otherwise run on 1-8 waves.
+// (Measured by graph replay of the launch alone: at 2^12 points eight elements
+// a thread beat four by 0.3-1.1 us, INVSTEP 11.6 -> 11.1 us, EXPSTEP 18.2 ->
+// 17.2; below 2^12 four win.)
 static constexpr int row_small_e(int logm, int mode) {
   const bool fused = mode == R2M_CONV || mode == R2M_INVSTEP || mode == R2M_EXPSTEP || mode == R2M_SQRTSTEP;
   if (!fused || logm < 10 || logm > 13) return 4;
-  return logm <= 12 ? 2 : 3;
+  return logm <= 11 ? 2 : 3;
 }
 
 // Elements a thread (log2) of the fused product's row pass over many rows of

@@ -1134,7 +1134,9 @@ template <int MODE>
 __global__ void __launch_bounds__(kSmallThreads) k_newton_small(RowJob2 j0, const RowJob2* __restrict__ jobs) {
   constexpr uint32_t NM = kSmallSeries;
   const RowJob2 J = load_job(j0, jobs);
-  __shared__ uint32_t A[NM], B[NM], C[NM], E[NM / 2];
+  // RT (exp): the reciprocal table 1/(m + i) for every step, loaded with the
+  // input so that no step waits on a global load.
+  __shared__ uint32_t A[NM], B[NM], C[NM], E[NM / 2], RT[MODE == R2M_NEXP ? NM : 1];
   const uint32_t t = threadIdx.x, p = J.f.p, ninv = J.f.ninv;
   const uint32_t N = min(J.out_hi, NM), na = min(J.in_a, N);
   const uint32_t o = t / kMidTPO, par = t % kMidTPO;
@@ -1142,6 +1144,7 @@ __global__ void __launch_bounds__(kSmallThreads) k_newton_small(RowJob2 j0, cons
     A[i] = i < na ? J.src_a[i] : 0u;
     B[i] = 0u;
     C[i] = 0u;
+    if constexpr (MODE == R2M_NEXP) RT[i] = i < N ? J.epi[i] : 0u;
   }
   if (t == 0) {
     if constexpr (MODE == R2M_NINV) {
@@ -1193,7 +1196,7 @@ __global__ void __launch_bounds__(kSmallThreads) k_newton_small(RowJob2 j0, cons
         __syncthreads();
         const uint32_t rc = mp(E, h, C, h, 0, h);
         __syncthreads();  // E is rewritten below
-        if (par == 0 && o < h) E[o] = mulmod_full(rc, J.epi[m + o], p, ninv);
+        if (par == 0 && o < h) E[o] = mulmod_full(rc, RT[m + o], p, ninv);
         __syncthreads();
         const uint32_t v = mp(E, h, B, m, 0, h);
         if (par == 0 && o < h) B[m + o] = v;
