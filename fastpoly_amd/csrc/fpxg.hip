@@ -634,6 +634,27 @@ static void ntt_forward_multi(Context& c, const HostField& F, uint32_t* const* d
     }
     launch_cols_n(c, F.lazy, false, cj, P, (uint32_t)n, nullptr);
   }
+  // Rows of 2^14 words over a lazy prime: up to kDmaPack transforms' rows in
+  // one persistent launch (rows numbered across them), so that each CU walks
+  // several rows and the prefetch pipeline fills (four 2^23-point pieces:
+  // 61 + 26 us in two launches before).
+  if (P.b == kDmaRowLog && F.lazy) {
+    std::vector<RowJob2> rj((size_t)k);
+    for (int i = 0; i < k; ++i) {
+      rj[i] = RowJob2{};
+      rj[i].f = F.dev; rj[i].out_lo = 0; rj[i].out_hi = N; rj[i].flags = R2_CANON;
+      rj[i].src_a = dst[i]; rj[i].dst = dst[i]; rj[i].in_a = N;
+    }
+    bool all = true;
+    for (int i0 = 0; i0 < k && all; i0 += kDmaPack)
+      all = rows_dma_ok(c, rj.data() + i0, 1u << P.S, (uint32_t)std::min(kDmaPack, k - i0));
+    if (all) {
+      for (int i0 = 0; i0 < k; i0 += kDmaPack)
+        if (!rows_dma_part_lazy(c, signed_ok(F.p), rj.data() + i0, 1u << P.S, (uint32_t)std::min(kDmaPack, k - i0)))
+          throw Error(E_DEVICE, "persistent row pass refused a checked launch");
+      return;
+    }
+  }
   for (int i0 = 0; i0 < k; i0 += kRowPack) {
     RowJob2 rj[kRowPack];
     const int n = std::min(kRowPack, k - i0);

@@ -38,6 +38,29 @@ static constexpr int row_multi_e(int logm, int mode) {
   return (mode == R2M_CONV && logm >= 4 && logm <= 12) ? 2 : 4;
 }
 
+// Whether njobs by-value forward jobs of nrows rows of 2^14 words each can run
+// as the persistent prefetching pass: enough rows for every CU twice, no
+// lift, whole rows of input, 16-byte aligned rows (LDS DMA loads and 16-byte
+// stores), and at most kDmaPack jobs.
+static bool rows_dma_ok(const Context& c, const RowJob2* j, uint32_t nrows, uint32_t njobs) {
+  bool dma = c.row_dma && njobs >= 1 && njobs <= (uint32_t)kDmaPack && (size_t)nrows * njobs >= 2u * (size_t)c.cus;
+  for (uint32_t i = 0; dma && i < njobs; ++i)
+    dma = !(j[i].flags & R2_LIFT) && (size_t)j[i].in_a >= ((size_t)nrows << kDmaRowLog) &&
+          ((uintptr_t)j[i].src_a & 15) == 0 && ((uintptr_t)j[i].dst & 15) == 0;
+  return dma;
+}
+
+template <bool LAZY, bool SGN>
+static void rows_dma_go(Context& c, const RowJob2* j, uint32_t nrows, uint32_t njobs) {
+  JobPack<RowJob2, kDmaPack> jp;
+  for (int i = 0; i < kDmaPack; ++i) {
+    jp.j[i] = j[(uint32_t)i < njobs ? i : 0];
+    jp.j[i].nrows = nrows;
+  }
+  hipLaunchKernelGGL((k_rows_fwd_dma<LAZY, SGN>), dim3((uint32_t)c.cus), dim3(RowShape<kDmaRowLog>::T), 0, c.stream,
+                     jp, njobs);
+}
+
 // A row launch: njobs jobs from the device array dj, or (dj null) the njobs
 // by-value jobs j[0 .. njobs).
 template <int LOGM, bool LAZY, int MODE>
@@ -46,19 +69,8 @@ static void rows2_launch(Context& c, const RowJob2* j, uint32_t nrows, uint32_t 
   const size_t lds = (size_t)RS::U * RS::lds_words_per_row * 4;
   // Many whole rows of 2^14 words, forward: the persistent prefetching pass.
   if constexpr (LOGM == kDmaRowLog && (MODE == R2M_FWD || MODE == R2M_FWDS)) {
-    // 16-byte row loads (LDS DMA) and stores need 16-byte aligned rows.
-    bool dma = !dj && c.row_dma && (size_t)nrows * njobs >= 2u * (size_t)c.cus;
-    for (uint32_t i = 0; dma && i < njobs; ++i)
-      dma = !(j[i].flags & R2_LIFT) && (size_t)j[i].in_a >= ((size_t)nrows << LOGM) &&
-            ((uintptr_t)j[i].src_a & 15) == 0 && ((uintptr_t)j[i].dst & 15) == 0;
-    if (dma) {
-      JobPack<RowJob2, kRowPack> jp;
-      for (int i = 0; i < kRowPack; ++i) {
-        jp.j[i] = j[(uint32_t)i < njobs ? i : 0];
-        jp.j[i].nrows = nrows;
-      }
-      hipLaunchKernelGGL((k_rows_fwd_dma<LAZY, MODE == R2M_FWDS>), dim3((uint32_t)c.cus), dim3(RS::T), 0, c.stream,
-                         jp, njobs);
+    if (!dj && rows_dma_ok(c, j, nrows, njobs)) {
+      rows_dma_go<LAZY, MODE == R2M_FWDS>(c, j, nrows, njobs);
       return;
     }
   }
@@ -314,6 +326,10 @@ void rows2_part_lazy_13(Context& c, int logM, int mode, const RowJob2* j, uint32
                         const RowJob2* dj);
 void rows2_part_lazy_14(Context& c, int logM, int mode, const RowJob2* j, uint32_t nrows, uint32_t njobs,
                         const RowJob2* dj);
+// The forward row pass of up to kDmaPack transforms of nrows rows of 2^14
+// words each (lazy limbs) as one persistent launch; false (nothing launched)
+// where rows_dma_ok says no.
+bool rows_dma_part_lazy(Context& c, bool sgn, const RowJob2* j, uint32_t nrows, uint32_t njobs);
 void rows2_part_canon(Context& c, int logM, int mode, const RowJob2* j, uint32_t nrows, uint32_t njobs,
                       const RowJob2* dj);  // logM 1..14
 // Column passes of 2^S levels.

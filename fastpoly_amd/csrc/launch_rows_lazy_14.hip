@@ -9,6 +9,14 @@ void rows2_part_lazy_14(Context& c, int logM, int mode, const RowJob2* j, uint32
   rows2_mode<14, true>(c, mode, j, nrows, njobs, dj);
 }
 
+bool rows_dma_part_lazy(Context& c, bool sgn, const RowJob2* j, uint32_t nrows, uint32_t njobs) {
+  if (!rows_dma_ok(c, j, nrows, njobs)) return false;
+  if (sgn) rows_dma_go<true, true>(c, j, nrows, njobs);
+  else rows_dma_go<true, false>(c, j, nrows, njobs);
+  FPXG_HIP(hipGetLastError());
+  return true;
+}
+
 bool rows2_attrs_lazy_14() {
   return rows2_attr<14, true, R2M_FWD>() && rows2_attr<14, true, R2M_INV>() && rows2_attr<14, true, R2M_CONV>() &&
          rows2_attr<14, true, R2M_SPEC>() && rows2_attr<14, true, R2M_SPEC_ONLY>() &&

@@ -899,12 +899,15 @@ __device__ __forceinline__ void compute_with(uint32_t* x, const uint32_t* T, con
 }
 
 constexpr int kDmaRowLog = 14;
+// By-value jobs of one persistent row launch (the pieces of a split product
+// share one launch).
+constexpr int kDmaPack = 8;
 
 // SGN: signed residues (ArithS) for p < 2^28; the row's values are converted
 // back to [0, 2p) by one product with the Montgomery one and + p.
 template <bool LAZY, bool SGN>
 __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(4)))
-k_rows_fwd_dma(JobPack<RowJob2, kRowPack> jp, uint32_t njobs) {
+k_rows_fwd_dma(JobPack<RowJob2, kDmaPack> jp, uint32_t njobs) {
   constexpr int LOGM = kDmaRowLog;
   static_assert(!SGN || LAZY, "signed residues: lazy primes only");
   using SC = Sched<LOGM>;
