@@ -217,7 +217,7 @@ static void issue(Context& c, const Staged& s, const uint8_t* base) {
     case LK_COLS: {
       const int S = key_a(k), L = (int)key_d(k);
       const PassPlan P{L, L - S, S, tile_log_cols(S, L)};
-      launch_cols(c, key_b(k) != 0, key_c(k) != 0, s.first->col, P, s.count, (const ColJob2*)(base + s.jobs));
+      launch_cols(c, key_b(k) != 0, key_c(k), s.first->col, P, s.count, (const ColJob2*)(base + s.jobs));
       return;
     }
     case LK_COLG: {

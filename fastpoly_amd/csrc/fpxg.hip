@@ -407,26 +407,31 @@ static void launch_rows_group(Context& c, bool lazy, int mode, const RowJob2* js
 // njobs jobs from the device array dj, or (dj null) the njobs by-value jobs
 // j[0 .. njobs); recorded as one record per job, all but the first marked as
 // sharing the first's launch level.
-static void launch_cols_n(Context& c, bool lazy, bool inv, const ColJob2* j, const PassPlan& P, uint32_t njobs,
+// dir: kColFwd, kColInv, or kColChain (the chained inverse-forward pass, lazy
+// fields only).
+enum : int { kColFwd = 0, kColInv = 1, kColChain = 2 };
+static void launch_cols_n(Context& c, bool lazy, int dir, const ColJob2* j, const PassPlan& P, uint32_t njobs,
                           const ColJob2* dj) {
   if (!dj && (njobs < 1 || njobs > (uint32_t)kColPack)) throw Error(E_DEVICE, "column launch group size");
+  if (dir == kColChain && !lazy) throw Error(E_DEVICE, "chained column pass: lazy fields only");
   if (c.rec && !dj) {
-    for (uint32_t i = 0; i < njobs; ++i) c.rec->push_cols(P.S, lazy, inv, P.L, j[i], i > 0);
+    for (uint32_t i = 0; i < njobs; ++i) c.rec->push_cols(P.S, lazy, dir, P.L, j[i], i > 0);
     return;
   }
   // Signed residues when every by-value job's prime allows them (a device
   // job array may mix primes).
-  bool sgn = lazy && !inv && !dj;
+  bool sgn = lazy && dir == kColFwd && !dj;
   for (uint32_t i = 0; sgn && i < njobs; ++i) sgn = signed_ok(j[i].f.p);
-  if (!lazy) cols2_part_canon(c, inv, j, P, njobs, dj);
-  else if (inv) cols2_part_lazy_inv(c, j, P, njobs, dj);
+  if (!lazy) cols2_part_canon(c, dir != kColFwd, j, P, njobs, dj);
+  else if (dir == kColChain) cols2_part_lazy_chain(c, j, P, njobs, dj);
+  else if (dir == kColInv) cols2_part_lazy_inv(c, j, P, njobs, dj);
   else cols2_part_lazy_fwd(c, sgn, j, P, njobs, dj);
   FPXG_HIP(hipGetLastError());
 }
 
-static void launch_cols(Context& c, bool lazy, bool inv, const ColJob2& j, const PassPlan& P, uint32_t njobs = 1,
+static void launch_cols(Context& c, bool lazy, int dir, const ColJob2& j, const PassPlan& P, uint32_t njobs = 1,
                         const ColJob2* dj = nullptr) {
-  launch_cols_n(c, lazy, inv, &j, P, njobs, dj);
+  launch_cols_n(c, lazy, dir, &j, P, njobs, dj);
 }
 
 // Two independent column passes of one shape in one launch.

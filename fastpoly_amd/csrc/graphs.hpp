@@ -22,7 +22,7 @@ static void issue_one(Context& c, const LaunchRec& r) {
       return;
     case LK_COLS: {
       const PassPlan P{r.k3, r.k3 - r.k0, r.k0, tile_log_cols(r.k0, r.k3)};
-      launch_cols(c, r.k1 != 0, r.k2 != 0, r.col, P);
+      launch_cols(c, r.k1 != 0, r.k2, r.col, P);
       return;
     }
     case LK_EW:
@@ -78,7 +78,7 @@ static void issue_group(Context& c, const LaunchRec* recs, size_t n) {
     ColJob2 js[kColPack];
     for (size_t i = 0; i < n; ++i) js[i] = recs[i].col;
     const PassPlan P{r.k3, r.k3 - r.k0, r.k0, tile_log_cols(r.k0, r.k3)};
-    launch_cols_n(c, r.k1 != 0, r.k2 != 0, js, P, (uint32_t)n, nullptr);
+    launch_cols_n(c, r.k1 != 0, r.k2, js, P, (uint32_t)n, nullptr);
   } else {
     RowJob2 js[kRowPack];
     for (size_t i = 0; i < n; ++i) js[i] = recs[i].row;

@@ -194,16 +194,16 @@ static int small_col_tile_log(int S, int b, bool inv) {
   return lc;
 }
 
-template <int S, int LC, bool LAZY, bool INV, bool SGN>
+template <int S, int LC, bool LAZY, bool INV, bool SGN, bool CH>
 static void cols2_small_go(Context& c, const JobPack<ColJob2, kColPack>& pack, const PassPlan& P, uint32_t njobs) {
   using CS2 = ColShape<S, LC, 2>;
-  hipLaunchKernelGGL((k_cols2<S, LC, LAZY, INV, SGN, 2>), dim3(1u << (P.b - LC), njobs), dim3(CS2::T),
+  hipLaunchKernelGGL((k_cols2<S, LC, LAZY, INV, SGN, 2, CH>), dim3(1u << (P.b - LC), njobs), dim3(CS2::T),
                      (size_t)CS2::words * 4, c.stream, pack, (const ColJob2*)nullptr, P.L);
 }
 
 // Dispatches the tile width lc (8 - S <= lc <= 12 - S, lc <= P.b); false if
 // none applies.
-template <int S, int LC, bool LAZY, bool INV, bool SGN>
+template <int S, int LC, bool LAZY, bool INV, bool SGN, bool CH>
 static bool cols2_small(Context& c, const JobPack<ColJob2, kColPack>& pack, const PassPlan& P, uint32_t njobs,
                         int lc) {
   if constexpr (LC > 4) {
@@ -211,17 +211,17 @@ static bool cols2_small(Context& c, const JobPack<ColJob2, kColPack>& pack, cons
   } else {
     if constexpr (LC >= 8 - S && LC <= 12 - S) {
       if (lc == LC && P.b >= LC) {
-        cols2_small_go<S, LC, LAZY, INV, SGN>(c, pack, P, njobs);
+        cols2_small_go<S, LC, LAZY, INV, SGN, CH>(c, pack, P, njobs);
         return true;
       }
     }
-    return cols2_small<S, LC + 1, LAZY, INV, SGN>(c, pack, P, njobs, lc);
+    return cols2_small<S, LC + 1, LAZY, INV, SGN, CH>(c, pack, P, njobs, lc);
   }
 }
 
 // njobs jobs from the device array dj, or (dj null) the njobs by-value jobs
 // j[0 .. njobs).
-template <int S, int LOGC, bool LAZY, bool INV, bool SGN>
+template <int S, int LOGC, bool LAZY, bool INV, bool SGN, bool CH = false>
 static void cols2_launch_c(Context& c, const ColJob2* j, const PassPlan& P, uint32_t njobs, const ColJob2* dj) {
   using CS = ColShape<S, LOGC>;
   const size_t lds = (size_t)CS::words * 4;
@@ -234,26 +234,26 @@ static void cols2_launch_c(Context& c, const ColJob2* j, const PassPlan& P, uint
   // transforms keep the wide tiles: their launches are throughput-bound.
   if constexpr (S >= 5 && S <= 10) {
     if (!dj && P.L <= 20 && !(S == 10 && P.b == 10 && !INV)) {
-      if (cols2_small<S, 0, LAZY, INV, SGN>(c, pack, P, njobs, small_col_tile_log(S, P.b, INV))) return;
+      if (cols2_small<S, 0, LAZY, INV, SGN, CH>(c, pack, P, njobs, small_col_tile_log(S, P.b, INV))) return;
       constexpr int LC = LOGC < 10 - (S - 2) ? LOGC : 10 - (S - 2);
       using CS2 = ColShape<S, LC, 2>;
-      hipLaunchKernelGGL((k_cols2<S, LC, LAZY, INV, SGN, 2>), dim3(1u << (P.b - LC), njobs), dim3(CS2::T),
+      hipLaunchKernelGGL((k_cols2<S, LC, LAZY, INV, SGN, 2, CH>), dim3(1u << (P.b - LC), njobs), dim3(CS2::T),
                          (size_t)CS2::words * 4, c.stream, pack, dj, P.L);
       return;
     }
   }
   dim3 grid(1u << (P.b - LOGC), njobs), block(CS::T);
-  hipLaunchKernelGGL((k_cols2<S, LOGC, LAZY, INV, SGN>), grid, block, lds, c.stream, pack, dj, P.L);
+  hipLaunchKernelGGL((k_cols2<S, LOGC, LAZY, INV, SGN, 4, CH>), grid, block, lds, c.stream, pack, dj, P.L);
 }
-template <int S, bool LAZY, bool INV, bool SGN>
+template <int S, bool LAZY, bool INV, bool SGN, bool CH = false>
 static void cols2_launch(Context& c, const ColJob2* j, const PassPlan& P, uint32_t njobs, const ColJob2* dj) {
   if constexpr (S == 10) {
     if (P.logC == 3) {
-      cols2_launch_c<S, 3, LAZY, INV, SGN>(c, j, P, njobs, dj);
+      cols2_launch_c<S, 3, LAZY, INV, SGN, CH>(c, j, P, njobs, dj);
       return;
     }
   }
-  cols2_launch_c<S, col_tile_log(S), LAZY, INV, SGN>(c, j, P, njobs, dj);
+  cols2_launch_c<S, col_tile_log(S), LAZY, INV, SGN, CH>(c, j, P, njobs, dj);
 }
 
 
@@ -302,11 +302,12 @@ static bool rows2_attr() {
   return hipFuncSetAttribute((const void*)k_rows2<LOGM, LAZY, MODE>, hipFuncAttributeMaxDynamicSharedMemorySize,
                              (int)(RS::U * RS::lds_words_per_row * 4)) == hipSuccess;
 }
-template <int S, bool LAZY, bool INV, bool SGN = false>
+template <int S, bool LAZY, bool INV, bool SGN = false, bool CH = false>
 static bool cols2_attr() {
   constexpr int LOGC = col_tile_log(S);
-  return hipFuncSetAttribute((const void*)k_cols2<S, LOGC, LAZY, INV, SGN>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                             (int)(ColShape<S, LOGC>::words * 4)) == hipSuccess;
+  return hipFuncSetAttribute((const void*)k_cols2<S, LOGC, LAZY, INV, SGN, 4, CH>,
+                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)(ColShape<S, LOGC>::words * 4)) ==
+         hipSuccess;
 }
 
 template <int S>
@@ -336,6 +337,9 @@ void rows2_part_canon(Context& c, int logM, int mode, const RowJob2* j, uint32_t
 void cols2_part_lazy_fwd(Context& c, bool sgn, const ColJob2* j, const PassPlan& P, uint32_t njobs,
                          const ColJob2* dj);
 void cols2_part_lazy_inv(Context& c, const ColJob2* j, const PassPlan& P, uint32_t njobs, const ColJob2* dj);
+// The chained pass (k_cols2 with CH: inverse levels, window, forward levels)
+// of 2^5 .. 2^12 column levels, lazy limbs.
+void cols2_part_lazy_chain(Context& c, const ColJob2* j, const PassPlan& P, uint32_t njobs, const ColJob2* dj);
 void cols2_part_canon(Context& c, bool inv, const ColJob2* j, const PassPlan& P, uint32_t njobs,
                       const ColJob2* dj);
 void colcrt_part(Context& c, const CrtColJob* j, const PassPlan& P, uint32_t njobs, const CrtColJob* dj,

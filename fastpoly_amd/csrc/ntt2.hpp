@@ -1271,11 +1271,21 @@ struct ColTune {
 
 // E: log2 of the elements a thread owns (4; 2 for the by-value launches of
 // mid-size columns, whose threads then issue a quarter of the loads).
-template <int S, int LOGC, bool LAZY, bool INV, bool SGN = false, int E = 4>
+// CH (inverse only; the chained pass): the inverse column levels of one
+// product, then, in the same tile, the forward column levels of the next
+// product's operand taken from it: e[t] = v[t + out_shift] for t < out_hi -
+// out_lo (out_lo == out_shift, which is 0 or half the transform: a register
+// move, since register k of the first phase holds row k * 2^(S-E) + g),
+// negated with R2_NEG, times epi[t] with R2_MULTAB, zero beyond; its forward
+// column spectrum (with
+// the forward four-step twiddle, lazy limbs) goes to dst.  A Newton step's
+// two dependent products then share a launch and a memory round trip.
+template <int S, int LOGC, bool LAZY, bool INV, bool SGN = false, int E = 4, bool CH = false>
 __global__ void __launch_bounds__((ColShape<S, LOGC, E>::T)) __attribute__((amdgpu_waves_per_eu(ColTune<S>::waves)))
 k_cols2(JobPack<ColJob2, kColPack> jp, const ColJob2* __restrict__ jobs, int L) {
   using SC = Sched<S, E>;
   static_assert(!SGN || (LAZY && !INV), "signed residues: lazy forward passes only");
+  static_assert(!CH || (INV && S >= E && PhaseGroups<S, 0, LOGC, E>::NG == 1), "chained pass: inverse, one group a phase-0 thread");
   constexpr int NPH = SC::nph;
   constexpr int C = 1 << LOGC;
   extern __shared__ uint32_t X[];
@@ -1321,6 +1331,20 @@ k_cols2(JobPack<ColJob2, kColPack> jp, const ColJob2* __restrict__ jobs, int L) 
     }
   };
   if constexpr (ColTune<S>::PF || INV) fourstep_load();
+  // The chained pass's forward four-step factors (forward tables), issued
+  // with the inverse's.
+  uint32_t fd_hi = 0, fd_lo = 0, fq_hi[CH ? NGL : 1], fq_lo[CH ? NGL : 1];
+  if constexpr (CH) {
+    const uint32_t xd = (col << (S - rl)) << shift;
+    fd_hi = *(g32*)(J.f.tw_hi + (xd >> split));
+    fd_lo = *(g32*)(J.f.tw_lo + (xd & smask));
+#pragma unroll
+    for (int u = 0; u < NGL; ++u) {
+      const uint32_t xq = (col * bitrev(PhaseGroups<S, PL, LOGC, E>::gid(g, u) << rl, S)) << shift;
+      fq_hi[u] = *(g32*)(J.f.tw_hi + (xq >> split));
+      fq_lo[u] = *(g32*)(J.f.tw_lo + (xq & smask));
+    }
+  }
   if constexpr (!INV) {
     constexpr int r0 = SC::r_of(0);
     constexpr int NE0 = PhaseGroups<S, 0, LOGC, E>::NG * (1 << r0);
@@ -1392,7 +1416,14 @@ k_cols2(JobPack<ColJob2, kColPack> jp, const ColJob2* __restrict__ jobs, int L) 
     // (Passes held to 64 registers load them where they are used.)
     constexpr bool kEvPF = ColTune<S>::PF;
     uint32_t ev[NE0];
-    if (kEvPF && elen != 0) {
+    if constexpr (CH) {
+      // The chained operand's epilogue factors epi[t], t its own positions.
+      if (flags & R2_MULTAB) {
+        const uint32_t elim = J.out_hi - J.out_lo - 1;
+#pragma unroll
+        for (int k = 0; k < NE0; ++k) ev[k] = gload(J.epi + min(elem<S, 0, E>(g, k) * M + col, elim));
+      }
+    } else if (kEvPF && elen != 0) {
       const uint32_t* esrc = epi_op ? J.epi : J.dst;
       const uint32_t elim = elen - 1;
 #pragma unroll
@@ -1417,6 +1448,45 @@ k_cols2(JobPack<ColJob2, kColPack> jp, const ColJob2* __restrict__ jobs, int L) 
       }
     }
     inv_transform<S, LOGC, LAZY, ColTune<S>::PF, E>(x, X, g, c, J.f.planes_inv, A, TI);
+    if constexpr (CH) {
+      constexpr int NE = 1 << E, HALF = NE / 2;
+      static_assert(NE0 == NE, "phase 0 holds one full group a thread");
+      uint32_t T0[16];
+      tw_load<S, 0, LOGC, E>(T0, J.f.planes_fwd, g);
+      const bool up = J.out_shift != 0;  // uniform: the upper half moves down
+      const uint32_t ecount = J.out_hi - J.out_lo;
+      const bool mul = (flags & R2_MULTAB) != 0;
+      uint32_t y[NE];
+#pragma unroll
+      for (int k = 0; k < NE; ++k) {
+        const uint32_t t = elem<S, 0, E>(g, k) * M + col;  // position in the new operand
+        uint32_t v = 0u;
+        if (k < HALF || !up) v = A.canon(A.half(up ? x[k + HALF] : x[k]));
+        if (flags & R2_NEG) v = v == 0 ? 0u : J.f.p - v;
+        if (mul) v = mulmod_full(v, ev[k], J.f.p, J.f.ninv);
+        y[k] = t < ecount ? v : 0u;
+      }
+#pragma unroll
+      for (int k = 0; k < NE; ++k) x[k] = y[k];
+      __syncthreads();  // the inverse's last exchange has been read by every thread
+      fwd_transform<S, LOGC, LAZY, 2, 2, ColTune<S>::PF, E>(x, X, g, c, J.f.planes_fwd, A, T0);
+      const uint32_t D = A.mul(fd_hi, fd_lo);
+      const __amdgpu_buffer_rsrc_t rd = buf_rsrc(J.dst, (M << S) * 4);
+      constexpr int lol = S - Sched<S, E>::s_of(PL) - rl;
+#pragma unroll
+      for (int u = 0; u < NGL; ++u) {
+        const uint32_t gid = PhaseGroups<S, PL, LOGC, E>::gid(g, u);
+        const uint32_t lane = (elem<S, PL, E>(gid, 0) * M + col) * 4;
+        uint32_t q = A.mul(fq_hi[u], fq_lo[u]);
+#pragma unroll
+        for (int j = 0; j < (1 << rl); ++j) {
+          const int k = (int)bitrev_c(j, rl);
+          bstore(rd, A.mul(x[u * (1 << rl) + k], q), lane, ((uint32_t)k * M) << (lol + 2));
+          if (j + 1 < (1 << rl)) q = A.mul(q, D);
+        }
+      }
+      return;
+    }
     const uint32_t p = J.f.p;
 #pragma unroll
     for (int u = 0; u < PhaseGroups<S, 0, LOGC, E>::NG; ++u)

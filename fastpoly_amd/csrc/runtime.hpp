@@ -191,9 +191,9 @@ struct Recorder {
     LaunchRec& r = next(LK_ROWS);
     r.k0 = logM; r.k1 = lazy; r.k2 = mode; r.row = j; r.par = par;
   }
-  void push_cols(int S, bool lazy, bool inv, int L, const ColJob2& j, bool par = false) {
+  void push_cols(int S, bool lazy, int dir, int L, const ColJob2& j, bool par = false) {
     LaunchRec& r = next(LK_COLS);
-    r.k0 = S; r.k1 = lazy; r.k2 = inv; r.k3 = L; r.col = j; r.par = par;
+    r.k0 = S; r.k1 = lazy; r.k2 = dir; r.k3 = L; r.col = j; r.par = par;
   }
   void push_ew(const EwRec& e) { next(LK_EW).ew = e; }
   void push_recur(const RecurRec& e) { next(LK_RECUR).rec = e; }

@@ -243,6 +243,37 @@ static void extend_inverse(Context& c, const HostField& T, const uint32_t* a, si
     launch_rows(c, T.lazy, R2M_INVSTEP, rj, E.L, 1);
     return;
   }
+  if (E.native && T.lazy && ((size_t)1 << E.L) == 2 * k && c.conv_plan(E.L).S >= 5) {
+    // Four-step products: the first product's inverse column pass hands e to
+    // the second product's forward column pass in the same launch (the
+    // chained pass), and c's column spectrum from the first product's
+    // forward pass serves the second: five launches instead of six.
+    const PassPlan P = c.conv_plan(E.L);
+    const uint32_t N = 1u << E.L;
+    const uint32_t inv_n = inv_pow2(T, E.L);
+    DevBuf X = c.buf(N), Y = c.buf(N), Z = c.buf(N);
+    ColJob2 jx{};
+    jx.src = a; jx.dst = X.get(); jx.in_count = (uint32_t)std::min(la, wanted); jx.f = T.dev;
+    ColJob2 jy = jx;
+    jy.src = cc; jy.dst = Y.get(); jy.in_count = (uint32_t)k;
+    launch_cols_pair(c, T.lazy, false, jx, jy, P);
+    RowJob2 rj{};
+    rj.f = T.dev; rj.nrows = 1u << P.S; rj.in_a = N; rj.in_b = N; rj.out_lo = 0; rj.out_hi = N;
+    rj.src_a = X.get(); rj.src_b = Y.get(); rj.dst = X.get();
+    launch_rows(c, T.lazy, R2M_CONV, rj, P.b, rj.nrows);
+    // e = -(a c)[k, wanted), moved down by k, then its forward column levels.
+    ColJob2 ch{};
+    ch.src = X.get(); ch.dst = Z.get(); ch.out_lo = (uint32_t)k; ch.out_hi = (uint32_t)wanted;
+    ch.out_shift = (uint32_t)k; ch.inv_n = inv_n; ch.flags = R2_NEG; ch.f = T.dev;
+    launch_cols(c, T.lazy, kColChain, ch, P);
+    rj.src_a = Z.get(); rj.src_b = Y.get(); rj.dst = Z.get();
+    launch_rows(c, T.lazy, R2M_CONV, rj, P.b, rj.nrows);
+    ColJob2 ij{};
+    ij.src = Z.get(); ij.dst = cc + k; ij.out_lo = 0; ij.out_hi = (uint32_t)hlen; ij.out_shift = 0;
+    ij.inv_n = inv_n; ij.f = T.dev;
+    launch_cols(c, T.lazy, kColInv, ij, P);
+    return;
+  }
   DevBuf e = c.buf(hlen);
   if (E.native) {
     native_conv(c, T, E.L, a, std::min(la, wanted), cc, k, k, wanted, e.get(), true);
