@@ -458,9 +458,43 @@ static void series_emit(Context& c, const HostField& T, const SeriesReq& q, cons
           continue;
         }
         if (!merged) r = c.buf(hlen);
+        if (E.native && rtab.get() && !merged)
+          native_conv(c, T, E.L, ap.get(), std::min(lap, m2 - 1), b, m, m - 1, m - 1 + hlen, r.get());
+        if (E.native && rtab.get() && T.lazy && ((size_t)1 << E.L) == 2 * m && c.conv_plan(E.L).S >= 5) {
+          // t = (r c)[0, h) / (m + i) hands itself to out = (t b)[0, h)
+          // through the chained pass; b's column spectrum rides in the first
+          // product's column launch: five launches instead of six.
+          const PassPlan P = c.conv_plan(E.L);
+          const uint32_t N = 1u << E.L;
+          const uint32_t inv_n = inv_pow2(T, E.L);
+          DevBuf R = c.buf(N), Cs = c.buf(N), Bs = c.buf(N), Tz = c.buf(N);
+          ColJob2 cj[3];
+          for (ColJob2& j : cj) {
+            j = ColJob2{};
+            j.f = T.dev;
+          }
+          cj[0].src = r.get(); cj[0].dst = R.get(); cj[0].in_count = (uint32_t)hlen;
+          cj[1].src = cbuf.get(); cj[1].dst = Cs.get(); cj[1].in_count = (uint32_t)hlen;
+          cj[2].src = b; cj[2].dst = Bs.get(); cj[2].in_count = (uint32_t)m;
+          launch_cols_n(c, T.lazy, kColFwd, cj, P, 3, nullptr);
+          RowJob2 rj{};
+          rj.f = T.dev; rj.nrows = 1u << P.S; rj.in_a = N; rj.in_b = N; rj.out_lo = 0; rj.out_hi = N;
+          rj.src_a = R.get(); rj.src_b = Cs.get(); rj.dst = R.get();
+          launch_rows(c, T.lazy, R2M_CONV, rj, P.b, rj.nrows);
+          ColJob2 ch{};
+          ch.src = R.get(); ch.dst = Tz.get(); ch.out_lo = 0; ch.out_hi = (uint32_t)hlen; ch.out_shift = 0;
+          ch.inv_n = inv_n; ch.flags = R2_MULTAB; ch.epi = rtab.get() + m; ch.f = T.dev;
+          launch_cols(c, T.lazy, kColChain, ch, P);
+          rj.src_a = Tz.get(); rj.src_b = Bs.get(); rj.dst = Tz.get();
+          launch_rows(c, T.lazy, R2M_CONV, rj, P.b, rj.nrows);
+          ColJob2 ij{};
+          ij.src = Tz.get(); ij.dst = b + m; ij.out_lo = 0; ij.out_hi = (uint32_t)hlen; ij.out_shift = 0;
+          ij.inv_n = inv_n; ij.f = T.dev;
+          launch_cols(c, T.lazy, kColInv, ij, P);
+          continue;
+        }
         DevBuf t = c.buf(hlen);
         if (E.native && rtab.get()) {
-          if (!merged) native_conv(c, T, E.L, ap.get(), std::min(lap, m2 - 1), b, m, m - 1, m - 1 + hlen, r.get());
           Epi ep;
           ep.flags = R2_MULTAB;
           ep.src = rtab.get() + m;
