@@ -403,10 +403,11 @@ static void series_emit(Context& c, const HostField& T, const SeriesReq& q, cons
         return;
       }
       // 1/i for the steps' t[i] /= m + i (exponential.hpp:194-195), read by the
-      // small-series kernel and, on native fields, by the product epilogues.
-      // (Other fields read it only below 512.)
+      // small-series kernel, the direct steps (up to 2^9 points: indices
+      // below 512) and, on native fields, the product epilogues.  (Other
+      // fields read it only below 512.)
       DevBuf rtab;
-      const size_t nr = series_native(T, 6) ? n : std::min<size_t>(n, kSmallSeries);
+      const size_t nr = series_native(T, 6) ? n : std::min<size_t>(n, std::max<size_t>(kSmallSeries, 512));
       rtab = c.buf(nr);
       emit_ew(c, T, EW_RECIP, rtab.get(), nullptr, nullptr, nr, 0, 0);
       // The first min(n, 512) coefficients of b, and c = 1/b to half of that.
