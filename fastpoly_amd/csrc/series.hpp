@@ -390,11 +390,13 @@ static void series_emit(Context& c, const HostField& T, const SeriesReq& q, cons
       // past the last nonzero index below cnt (i < p makes every factor i invertible).
       const size_t last = std::min<size_t>(s.last1, cnt);
       const size_t lap = last > 1 ? last - 1 : 0;
-      emit_ew(c, T, EW_FILL, out, nullptr, nullptr, n, 0, 0);
       if (lap == 0) {
+        emit_ew(c, T, EW_FILL, out, nullptr, nullptr, n, 0, 0);
         emit_ew(c, T, EW_FILL, out, nullptr, nullptr, 1, 0, one);
         return;
       }
+      // (No zero fill of out: the small-series launch and the Newton steps
+      // below write every coefficient of b = out[0, n).)
       DevBuf ap = c.buf(lap);
       emit_ew(c, T, EW_DERIV, ap.get(), a, nullptr, lap, 0, 0);
       uint32_t* b = out;
@@ -519,17 +521,22 @@ static void series_emit(Context& c, const HostField& T, const SeriesReq& q, cons
         if (v & 1) throw Error(E_DOMAIN, "poly::sqrt: odd valuation, no square root");
         if (!T.sqrt(s.afirst, &root0)) throw Error(E_DOMAIN, "poly::sqrt: non-residue constant term");
       }
-      emit_ew(c, T, EW_FILL, out, nullptr, nullptr, n, 0, 0);
-      if (v >= la) return;
       const size_t half = v / 2;
-      if (half >= n) return;
+      if (v >= la || half >= n) {  // the zero series, or a root vanishing below x^n
+        emit_ew(c, T, EW_FILL, out, nullptr, nullptr, n, 0, 0);
+        return;
+      }
       const size_t inner = n - half, au = la - v;
       const size_t ucnt = std::min(au, inner);
       uint32_t* b = out + half;
       if (s.second >= v + ucnt) {  // no nonzero coefficient in (v, v + ucnt)
+        emit_ew(c, T, EW_FILL, out, nullptr, nullptr, n, 0, 0);
         emit_ew(c, T, EW_FILL, b, nullptr, nullptr, 1, 0, root0);
         return;
       }
+      // The Newton steps below write every coefficient of b = out[half, n);
+      // only the valuation's zeros need a fill.
+      emit_ew(c, T, EW_FILL, out, nullptr, nullptr, half, 0, 0);
       const uint32_t half_inv = T.from_int(((uint64_t)T.p + 1) / 2);
       const uint32_t tri = T.mul(T.inv(root0), half_inv);
       const uint32_t* av = a + v;
