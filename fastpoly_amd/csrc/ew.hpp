@@ -192,6 +192,7 @@ __device__ __forceinline__ uint32_t ew_value(const EwRec& R, uint32_t i) {
     case EW_SCALE: return fs_mul(f, R.src[i], R.c);
     case EW_MUL_RECIP: return fs_mul(f, R.src[i], fs_inv(f, fs_from_int(f, R.c + i)));
     case EW_DERIV: return fs_mul(f, R.src[i + 1], fs_from_int(f, i + 1));
+    case EW_THETA: return fs_mul(f, R.src[i], fs_from_int(f, i));
     case EW_INTEGRAL: return (i >= 1 && i - 1 < R.avail) ? fs_mul(f, R.src[i - 1], fs_inv(f, fs_from_int(f, i))) : 0u;
     case EW_REVERSE: return R.src[R.n - 1 - i];
     case EW_RECIP: return R.c + i == 0 ? 0u : fs_inv(f, fs_from_int(f, R.c + i));

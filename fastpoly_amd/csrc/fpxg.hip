@@ -414,6 +414,20 @@ static void launch_cols_n(Context& c, bool lazy, int dir, const ColJob2* j, cons
                           const ColJob2* dj) {
   if (!dj && (njobs < 1 || njobs > (uint32_t)kColPack)) throw Error(E_DEVICE, "column launch group size");
   if (dir == kColChain && !lazy) throw Error(E_DEVICE, "chained column pass: lazy fields only");
+  if (dir == kColChain && !dj) {
+    // The window moves by a register move: 0, a quarter, half or minus a
+    // quarter of the transform; a prefix lies below the moved window.
+    const uint32_t N = 1u << P.L;
+    for (uint32_t i = 0; i < njobs; ++i) {
+      const uint32_t sh = j[i].out_shift;
+      if (sh != 0 && sh != N / 2 && sh != N / 4 && sh != 0u - N / 4)
+        throw Error(E_DEVICE, "chained column pass: shift is not a quarter multiple");
+      const uint32_t epif = j[i].flags & (R2_PREFIX | R2_SUBFROM | R2_MULTAB);
+      if (epif & (epif - 1)) throw Error(E_DEVICE, "chained column pass: one epilogue operand");
+      if ((j[i].flags & R2_PREFIX) && (j[i].out_lo - sh < j[i].epi_avail || j[i].out_hi - sh > N))
+        throw Error(E_DEVICE, "chained column pass: prefix overlaps the window");
+    }
+  }
   if (c.rec && !dj) {
     for (uint32_t i = 0; i < njobs; ++i) c.rec->push_cols(P.S, lazy, dir, P.L, j[i], i > 0);
     return;

@@ -20,6 +20,8 @@
 #include "crt.hpp"
 #include "device.hpp"
 
+#include <type_traits>
+
 namespace fpxg {
 
 typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
@@ -495,7 +497,9 @@ struct RowJob2 {
 // By-value jobs per row launch: one for each of the three auxiliary primes.
 constexpr int kRowPack = 3;
 enum : uint32_t { R2_LIFT = 1, R2_CANON = 2, R2_SCALE = 4, R2_NEG = 8, R2_ACC = 16, R2_SUBFROM = 32, R2_MULTAB = 64,
-                  R2_ACCLO = 256 };
+                  R2_PREFIX = 128, R2_ACCLO = 256 };
+// R2_PREFIX (chained column pass only): the operand's positions below
+// epi_avail come from epi, and its window is also stored to epi.
 // R2_ACCLO (column inverse only): like R2_ACC for output positions below
 // acc_end (relative to out_shift), a plain store from there on.
 // R2_ACC (column inverse only): dst[i] = dst[i] + result, both canonical.
@@ -1273,13 +1277,17 @@ struct ColTune {
 // mid-size columns, whose threads then issue a quarter of the loads).
 // CH (inverse only; the chained pass): the inverse column levels of one
 // product, then, in the same tile, the forward column levels of the next
-// product's operand taken from it: e[t] = v[t + out_shift] for t < out_hi -
-// out_lo (out_lo == out_shift, which is 0 or half the transform: a register
-// move, since register k of the first phase holds row k * 2^(S-E) + g),
-// negated with R2_NEG, times epi[t] with R2_MULTAB, zero beyond; its forward
-// column spectrum (with
-// the forward four-step twiddle, lazy limbs) goes to dst.  A Newton step's
-// two dependent products then share a launch and a memory round trip.
+// product's operand taken from it: e[t] = v[t + out_shift] where t +
+// out_shift lies in the window [out_lo, out_hi) (out_shift is 0, a quarter,
+// half or minus a quarter of the transform, modulo 2^32: a register move,
+// since register k of the first phase holds row k * 2^(S-E) + g), negated
+// with R2_NEG, subtracted from epi[t] (0 from epi_avail on) with R2_SUBFROM,
+// times epi[t] with R2_MULTAB; outside the window e[t] = epi[t]
+// for t < epi_avail with R2_PREFIX (which also stores the window's e[t] to
+// epi[t], positions at or above epi_avail), zero otherwise.  Its forward
+// column spectrum (with the forward four-step twiddle, lazy limbs) goes to
+// dst.  A Newton step's two dependent products then share a launch and a
+// memory round trip.
 template <int S, int LOGC, bool LAZY, bool INV, bool SGN = false, int E = 4, bool CH = false>
 __global__ void __launch_bounds__((ColShape<S, LOGC, E>::T)) __attribute__((amdgpu_waves_per_eu(ColTune<S>::waves)))
 k_cols2(JobPack<ColJob2, kColPack> jp, const ColJob2* __restrict__ jobs, int L) {
@@ -1417,11 +1425,13 @@ k_cols2(JobPack<ColJob2, kColPack> jp, const ColJob2* __restrict__ jobs, int L) 
     constexpr bool kEvPF = ColTune<S>::PF;
     uint32_t ev[NE0];
     if constexpr (CH) {
-      // The chained operand's epilogue factors epi[t], t its own positions.
-      if (flags & R2_MULTAB) {
-        const uint32_t elim = J.out_hi - J.out_lo - 1;
+      // The chained operand's words at its own positions t: the factors
+      // epi[t] of its window (R2_MULTAB), the minuends epi[t] (R2_SUBFROM) or
+      // its prefix epi[t] (R2_PREFIX).
+      const uint32_t ecnt = (flags & (R2_PREFIX | R2_SUBFROM)) ? J.epi_avail : J.out_hi - J.out_shift;
+      if ((flags & (R2_MULTAB | R2_PREFIX | R2_SUBFROM)) && ecnt != 0) {
 #pragma unroll
-        for (int k = 0; k < NE0; ++k) ev[k] = gload(J.epi + min(elem<S, 0, E>(g, k) * M + col, elim));
+        for (int k = 0; k < NE0; ++k) ev[k] = gload(J.epi + min(elem<S, 0, E>(g, k) * M + col, ecnt - 1));
       }
     } else if (kEvPF && elen != 0) {
       const uint32_t* esrc = epi_op ? J.epi : J.dst;
@@ -1453,19 +1463,38 @@ k_cols2(JobPack<ColJob2, kColPack> jp, const ColJob2* __restrict__ jobs, int L) 
       static_assert(NE0 == NE, "phase 0 holds one full group a thread");
       uint32_t T0[16];
       tw_load<S, 0, LOGC, E>(T0, J.f.planes_fwd, g);
-      const bool up = J.out_shift != 0;  // uniform: the upper half moves down
-      const uint32_t ecount = J.out_hi - J.out_lo;
-      const bool mul = (flags & R2_MULTAB) != 0;
+      // New position t takes old position t + out_shift when that lies in
+      // the window [out_lo, out_hi): the shift is a whole number of quarter
+      // transforms, so register k takes register k + DK.
+      const uint32_t NN = M << S, sh = J.out_shift, wlo = J.out_lo, wn = J.out_hi - J.out_lo;
+      const bool mul = (flags & R2_MULTAB) != 0, pre = (flags & R2_PREFIX) != 0;
+      uint32_t* const pdst = const_cast<uint32_t*>(J.epi);
       uint32_t y[NE];
+      auto take = [&](auto dkc) {
+        constexpr int DK = decltype(dkc)::value;
 #pragma unroll
-      for (int k = 0; k < NE; ++k) {
-        const uint32_t t = elem<S, 0, E>(g, k) * M + col;  // position in the new operand
-        uint32_t v = 0u;
-        if (k < HALF || !up) v = A.canon(A.half(up ? x[k + HALF] : x[k]));
-        if (flags & R2_NEG) v = v == 0 ? 0u : J.f.p - v;
-        if (mul) v = mulmod_full(v, ev[k], J.f.p, J.f.ninv);
-        y[k] = t < ecount ? v : 0u;
-      }
+        for (int k = 0; k < NE; ++k) {
+          const uint32_t t = elem<S, 0, E>(g, k) * M + col;  // position in the new operand
+          uint32_t v = 0u;
+          if (k + DK >= 0 && k + DK < NE && t + sh - wlo < wn) {
+            v = A.canon(A.half(x[min(max(k + DK, 0), NE - 1)]));
+            if (flags & R2_NEG) v = v == 0 ? 0u : J.f.p - v;
+            if (flags & R2_SUBFROM) {
+              const uint32_t e = t < J.epi_avail ? ev[k] : 0u;
+              v = e >= v ? e - v : e - v + J.f.p;
+            }
+            if (mul) v = mulmod_full(v, ev[k], J.f.p, J.f.ninv);
+            if (pre) pdst[t] = v;
+          } else if (pre && t < J.epi_avail) {
+            v = ev[k];
+          }
+          y[k] = v;
+        }
+      };
+      if (sh == 0) take(std::integral_constant<int, 0>());
+      else if (sh == NN / 2) take(std::integral_constant<int, HALF>());
+      else if (sh == NN / 4) take(std::integral_constant<int, NE / 4>());
+      else take(std::integral_constant<int, -(NE / 4)>());
 #pragma unroll
       for (int k = 0; k < NE; ++k) x[k] = y[k];
       __syncthreads();  // the inverse's last exchange has been read by every thread

@@ -36,7 +36,8 @@ enum {
   EW_SUB_FROM = 6,  // dst[i] = (i < avail ? src[i] : 0) - src2[i]  (square_root.hpp:93-94)
   EW_REVERSE = 7,   // dst[i] = src[n - 1 - i]                      (division.hpp:53-58)
   EW_MULADD = 8,    // dst[i] = (c ? dst[i] : 0) + src[i] * src2[i] (spectra of split products)
-  EW_RECIP = 9      // dst[i] = 1 / (c + i), 0 for c + i == 0       (exponential.hpp:194-195 table)
+  EW_RECIP = 9,     // dst[i] = 1 / (c + i), 0 for c + i == 0       (exponential.hpp:194-195 table)
+  EW_THETA = 10     // dst[i] = src[i] * i: x a'(x), so dst + 1 is a' (calculus.hpp:17-39)
 };
 struct EwRec {
   int op;

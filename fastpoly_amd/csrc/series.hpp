@@ -337,6 +337,138 @@ static ScanRec scan_rec_of(const SeriesReq& q) { return {q.a, (uint32_t)q.la, 0}
 
 // Emits the schedule of one series call given its scan; throws the reference's
 // domain errors in the reference's order.
+// Whether exp's or sqrt's doubling from m to 2m coefficients (c known to
+// m/2) runs as exp_fused_doubling / sqrt_fused_doubling: a native lazy field,
+// 2m points with a column pass the chained form serves, a step too large for
+// one launch.
+static bool exp_fused_ok(Context& c, const HostField& T, const Engine& E, size_t m, size_t hlen, size_t csize) {
+  return E.native && T.lazy && hlen == m && 2 * csize == m && ((size_t)1 << E.L) == 2 * m &&
+         !one_launch_step(c, T, E, m) && c.conv_plan(E.L).S >= 5;
+}
+
+// One doubling of exp's Newton iteration (exponential.hpp:158-204, with the
+// inverse's extension inverse.hpp:18-65) at N = 2m points, b known to m and
+// c = 1/b to m/2 coefficients, in nine launches that pass spectra from one
+// product to the next through the chained column pass (F: forward transform):
+//   B = F(b), C0 = F(c[0, m/2)), H = F(th)                     one launch
+//   E = B C0, R = H B                                          one launch
+//   F(e), e = E[m/2, m) moved down a quarter;
+//   F(r), r = R[m, 2m) moved down a half                       one launch
+//   G = F(e) C0
+//   F(c), c[m/2, m) = -G[0, m/2) moved up a quarter (stored to cc) under
+//   c[0, m/2) (read from cc)
+//   T = F(r) F(c)
+//   F(t), t = T[0, m) / (m + i)
+//   U = F(t) B
+//   b[m, 2m) = U[0, m)
+// Every window is free of wraparound at N points: b c0 has fewer than 3m/2
+// terms, e c0 and r c fewer than 2m, and (th b)'s wrapped terms land below m.
+static void exp_fused_doubling(Context& c, const HostField& T, int L, const uint32_t* th, size_t cth, uint32_t* b,
+                               size_t m, uint32_t* cc, const uint32_t* recip) {
+  const PassPlan P = c.conv_plan(L);
+  const uint32_t N = 1u << L, inv_n = inv_pow2(T, L), h = (uint32_t)(m / 2), mm = (uint32_t)m;
+  DevBuf Bs = c.buf(N), C0 = c.buf(N), Hs = c.buf(N), X = c.buf(N), Y = c.buf(N), Z = c.buf(N);
+  ColJob2 fj[3];
+  for (ColJob2& j : fj) {
+    j = ColJob2{};
+    j.f = T.dev;
+  }
+  fj[0].src = b; fj[0].dst = Bs.get(); fj[0].in_count = mm;
+  fj[1].src = cc; fj[1].dst = C0.get(); fj[1].in_count = h;
+  fj[2].src = th; fj[2].dst = Hs.get(); fj[2].in_count = (uint32_t)cth;
+  launch_cols_n(c, T.lazy, kColFwd, fj, P, 3, nullptr);
+  RowJob2 rj[2];
+  for (RowJob2& j : rj) {
+    j = RowJob2{};
+    j.f = T.dev; j.nrows = 1u << P.S; j.in_a = N; j.in_b = N; j.out_lo = 0; j.out_hi = N;
+  }
+  rj[0].src_a = Bs.get(); rj[0].src_b = C0.get(); rj[0].dst = X.get();
+  rj[1].src_a = Hs.get(); rj[1].src_b = Bs.get(); rj[1].dst = Hs.get();
+  launch_rows_group(c, T.lazy, R2M_CONV, rj, 2, P.b, 1u << P.S);
+  auto chain = [&](const uint32_t* src, uint32_t* dst, uint32_t lo, uint32_t hi, uint32_t shift) {
+    ColJob2 j{};
+    j.f = T.dev; j.inv_n = inv_n;
+    j.src = src; j.dst = dst; j.out_lo = lo; j.out_hi = hi; j.out_shift = shift;
+    return j;
+  };
+  const ColJob2 ch[2] = {chain(X.get(), Y.get(), h, mm, h), chain(Hs.get(), Z.get(), mm, N, mm)};
+  launch_cols_n(c, T.lazy, kColChain, ch, P, 2, nullptr);
+  auto rows = [&](uint32_t* a, const uint32_t* bb) {
+    RowJob2 j = rj[0];
+    j.src_a = a; j.src_b = bb; j.dst = a;
+    launch_rows(c, T.lazy, R2M_CONV, j, P.b, 1u << P.S);
+  };
+  rows(Y.get(), C0.get());
+  ColJob2 cj = chain(Y.get(), X.get(), 0, h, 0u - h);
+  cj.flags = R2_NEG | R2_PREFIX; cj.epi = cc; cj.epi_avail = h;
+  launch_cols(c, T.lazy, kColChain, cj, P);
+  rows(Z.get(), X.get());
+  ColJob2 tj = chain(Z.get(), Y.get(), 0, mm, 0);
+  tj.flags = R2_MULTAB; tj.epi = recip;
+  launch_cols(c, T.lazy, kColChain, tj, P);
+  rows(Y.get(), Bs.get());
+  ColJob2 ij = chain(Y.get(), b + m, 0, mm, 0);
+  launch_cols(c, T.lazy, kColInv, ij, P);
+}
+
+// sqrt's doubling from m to 2m coefficients (square_root.hpp:46-100, with the
+// inverse's extension inverse.hpp:18-65) at N = 2m points, c = 1/b known to
+// m/2, in seven launches (as exp_fused_doubling):
+//   B = F(b), C0 = F(c[0, m/2))                                one launch
+//   E = B C0, Q = B B                                          one launch
+//   F(e), e = E[m/2, m) moved down a quarter;
+//   F(res), res = a[v + m + i] - Q[m + i] moved down a half    one launch
+//   G = F(e) C0
+//   F(c), c[m/2, m) = -G[0, m/2) moved up a quarter (stored to cc) under
+//   c[0, m/2) (read from cc)
+//   T = F(res) F(c)
+//   b[m, 2m) = T[0, m) / 2
+// amin = a + v + m with avail words; b b has fewer than 2m terms.
+static void sqrt_fused_doubling(Context& c, const HostField& T, int L, uint32_t* b, size_t m, uint32_t* cc,
+                                const uint32_t* amin, uint32_t avail, uint32_t half_inv) {
+  const PassPlan P = c.conv_plan(L);
+  const uint32_t N = 1u << L, inv_n = inv_pow2(T, L), h = (uint32_t)(m / 2), mm = (uint32_t)m;
+  DevBuf Bs = c.buf(N), C0 = c.buf(N), X = c.buf(N), Y = c.buf(N), Z = c.buf(N);
+  ColJob2 fj[2];
+  for (ColJob2& j : fj) {
+    j = ColJob2{};
+    j.f = T.dev;
+  }
+  fj[0].src = b; fj[0].dst = Bs.get(); fj[0].in_count = mm;
+  fj[1].src = cc; fj[1].dst = C0.get(); fj[1].in_count = h;
+  launch_cols_n(c, T.lazy, kColFwd, fj, P, 2, nullptr);
+  RowJob2 rj[2];
+  for (RowJob2& j : rj) {
+    j = RowJob2{};
+    j.f = T.dev; j.nrows = 1u << P.S; j.in_a = N; j.in_b = N; j.out_lo = 0; j.out_hi = N;
+  }
+  rj[0].src_a = Bs.get(); rj[0].src_b = C0.get(); rj[0].dst = X.get();
+  rj[1].src_a = Bs.get(); rj[1].src_b = Bs.get(); rj[1].dst = Z.get();
+  launch_rows_group(c, T.lazy, R2M_CONV, rj, 2, P.b, 1u << P.S);
+  auto chain = [&](const uint32_t* src, uint32_t* dst, uint32_t lo, uint32_t hi, uint32_t shift) {
+    ColJob2 j{};
+    j.f = T.dev; j.inv_n = inv_n;
+    j.src = src; j.dst = dst; j.out_lo = lo; j.out_hi = hi; j.out_shift = shift;
+    return j;
+  };
+  ColJob2 ch[2] = {chain(X.get(), Y.get(), h, mm, h), chain(Z.get(), Z.get(), mm, N, mm)};
+  ch[1].flags = R2_SUBFROM; ch[1].epi = amin; ch[1].epi_avail = avail;
+  launch_cols_n(c, T.lazy, kColChain, ch, P, 2, nullptr);
+  auto rows = [&](uint32_t* a, const uint32_t* bb) {
+    RowJob2 j = rj[0];
+    j.src_a = a; j.src_b = bb; j.dst = a;
+    launch_rows(c, T.lazy, R2M_CONV, j, P.b, 1u << P.S);
+  };
+  rows(Y.get(), C0.get());
+  ColJob2 cj = chain(Y.get(), X.get(), 0, h, 0u - h);
+  cj.flags = R2_NEG | R2_PREFIX; cj.epi = cc; cj.epi_avail = h;
+  launch_cols(c, T.lazy, kColChain, cj, P);
+  rows(Z.get(), X.get());
+  ColJob2 ij = chain(Z.get(), b + m, 0, mm, 0);
+  ij.inv_n = T.mul(inv_n, half_inv);
+  launch_cols(c, T.lazy, kColInv, ij, P);
+}
+
 static void series_emit(Context& c, const HostField& T, const SeriesReq& q, const ScanOut& s) {
   const uint32_t* a = q.a;
   const size_t la = q.la, n = q.n;
@@ -397,11 +529,15 @@ static void series_emit(Context& c, const HostField& T, const SeriesReq& q, cons
       }
       // (No zero fill of out: the small-series launch and the Newton steps
       // below write every coefficient of b = out[0, n).)
-      DevBuf ap = c.buf(lap);
-      emit_ew(c, T, EW_DERIV, ap.get(), a, nullptr, lap, 0, 0);
+      // th = x a'(x) (th[i] = i a[i], lap + 1 words) and a' = th + 1: the
+      // fused doublings take r = (a' b)[m - 1, 2m - 1) as (th b)[m, 2m),
+      // a window at half the transform.
+      DevBuf th = c.buf(lap + 1);
+      emit_ew(c, T, EW_THETA, th.get(), a, nullptr, lap + 1, 0, 0);
+      const uint32_t* ap = th.get() + 1;
       uint32_t* b = out;
       if (T.p == 2) {  // n <= 2 (n - 1 < p): the recurrence alone
-        recur(c, T, REC_EXP, ap.get(), lap, nullptr, 0, b, 0, n, one, 0);
+        recur(c, T, REC_EXP, ap, lap, nullptr, 0, b, 0, n, one, 0);
         return;
       }
       // 1/i for the steps' t[i] /= m + i (exponential.hpp:194-195), read by the
@@ -417,7 +553,7 @@ static void series_emit(Context& c, const HostField& T, const SeriesReq& q, cons
       DevBuf cbuf = c.buf(n);
       {
         RowJob2 rj{};
-        rj.src_a = ap.get(); rj.in_a = (uint32_t)std::min(lap, n0 - 1); rj.epi = rtab.get();
+        rj.src_a = ap; rj.in_a = (uint32_t)std::min(lap, n0 - 1); rj.epi = rtab.get();
         rj.dst = b; rj.src_c = cbuf.get();
         small_series(c, T, R2M_NEXP, rj, n0);
       }
@@ -426,10 +562,15 @@ static void series_emit(Context& c, const HostField& T, const SeriesReq& q, cons
       for (size_t m = n0; m < n; m *= 2) {
         const size_t m2 = std::min(2 * m, n), hlen = m2 - m;
         if (hlen <= 8) {
-          recur(c, T, REC_EXP, ap.get(), lap, nullptr, 0, b, m, m2, one, 0);
+          recur(c, T, REC_EXP, ap, lap, nullptr, 0, b, m, m2, one, 0);
           continue;
         }
         Engine E(c, T, ceil_log2(2 * m));
+        if (rtab.get() && exp_fused_ok(c, T, E, m, hlen, csize)) {
+          exp_fused_doubling(c, T, E.L, th.get(), std::min(lap, m2 - 1) + 1, b, m, cbuf.get(), rtab.get() + m);
+          csize = m;
+          continue;
+        }
         // The c extension's first product and r = (a' b)[m - 1, ...) depend
         // only on b and the old c: one set of launches for both.
         const bool merged = csize < hlen && E.native && rtab.get() && !one_launch_step(c, T, E, m) &&
@@ -439,7 +580,7 @@ static void series_emit(Context& c, const HostField& T, const SeriesReq& q, cons
           r = c.buf(hlen);
           DevBuf e = c.buf(hlen - csize);
           const WinProd w[2] = {{b, std::min(m, hlen), cbuf.get(), csize, csize, hlen, e.get(), true, Epi()},
-                                {ap.get(), std::min(lap, m2 - 1), b, m, m - 1, m - 1 + hlen, r.get(), false, Epi()}};
+                                {ap, std::min(lap, m2 - 1), b, m, m - 1, m - 1 + hlen, r.get(), false, Epi()}};
           native_conv_multi(c, T, E.L, w, 2);
           cext_finish(c, T, e.get(), cbuf.get(), csize, hlen);
           csize = hlen;
@@ -451,7 +592,7 @@ static void series_emit(Context& c, const HostField& T, const SeriesReq& q, cons
           // The step's three products in one workgroup (R2M_EXPSTEP).
           RowJob2 rj{};
           rj.f = T.dev;
-          rj.src_a = ap.get(); rj.in_a = (uint32_t)std::min(lap, m2 - 1);
+          rj.src_a = ap; rj.in_a = (uint32_t)std::min(lap, m2 - 1);
           rj.src_b = b; rj.in_b = (uint32_t)m;
           rj.src_c = cbuf.get(); rj.aux = (uint32_t)(m - 1); rj.epi = rtab.get() + m;
           rj.dst = b + m; rj.nrows = 1; rj.out_lo = 0; rj.out_hi = (uint32_t)hlen; rj.out_shift = 0;
@@ -462,7 +603,7 @@ static void series_emit(Context& c, const HostField& T, const SeriesReq& q, cons
         }
         if (!merged) r = c.buf(hlen);
         if (E.native && rtab.get() && !merged)
-          native_conv(c, T, E.L, ap.get(), std::min(lap, m2 - 1), b, m, m - 1, m - 1 + hlen, r.get());
+          native_conv(c, T, E.L, ap, std::min(lap, m2 - 1), b, m, m - 1, m - 1 + hlen, r.get());
         if (E.native && rtab.get() && T.lazy && ((size_t)1 << E.L) == 2 * m && c.conv_plan(E.L).S >= 5) {
           // t = (r c)[0, h) / (m + i) hands itself to out = (t b)[0, h)
           // through the chained pass; b's column spectrum rides in the first
@@ -505,7 +646,7 @@ static void series_emit(Context& c, const HostField& T, const SeriesReq& q, cons
           native_conv(c, T, E.L, t.get(), hlen, b, m, 0, hlen, b + m);
           continue;
         }
-        crt_conv(c, T, E.L, ap.get(), std::min(lap, m2 - 1), b, m, m - 1, m - 1 + hlen, r.get());
+        crt_conv(c, T, E.L, ap, std::min(lap, m2 - 1), b, m, m - 1, m - 1 + hlen, r.get());
         crt_conv(c, T, E.L, r.get(), hlen, cbuf.get(), hlen, 0, hlen, t.get());
         emit_ew(c, T, EW_MUL_RECIP, t.get(), t.get(), nullptr, hlen, 0, (uint32_t)m);
         crt_conv(c, T, E.L, t.get(), hlen, b, m, 0, hlen, b + m);
@@ -558,6 +699,12 @@ static void series_emit(Context& c, const HostField& T, const SeriesReq& q, cons
           continue;
         }
         Engine E(c, T, ceil_log2(2 * m));
+        if (exp_fused_ok(c, T, E, m, hlen, csize)) {
+          sqrt_fused_doubling(c, T, E.L, b, m, cbuf.get(), av + std::min(m, au), (uint32_t)(au > m ? au - m : 0),
+                              half_inv);
+          csize = m;
+          continue;
+        }
         // res = a[v + m + i] - (b * b)[m + i]  (square_root.hpp:93-94)
         Epi sub;
         sub.flags = R2_SUBFROM;
