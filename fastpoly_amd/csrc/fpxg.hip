@@ -123,7 +123,9 @@ static void init_field(HostField& F, uint32_t p, uint32_t root) {
   const int plane_log = std::min(logmax, kPlaneLog);
   const size_t lo_n = (size_t)1 << split, hi_n = (size_t)1 << (logmax - split);
   const size_t tw_words = (2 * lo_n + 2 * hi_n + 3) & ~(size_t)3;  // keep the planes 16-byte aligned
-  std::vector<uint32_t> h(tw_words + 2 * kPlaneWords, 0);
+  // After the planes: powers of the coset generator (the primitive root) and
+  // of its inverse for exponents below 2^logmax, split like the root tables.
+  std::vector<uint32_t> h(tw_words + 2 * kPlaneWords + 2 * lo_n + 2 * hi_n, 0);
   uint32_t* lo_f = h.data();
   uint32_t* lo_i = lo_f + lo_n;
   uint32_t* hi_f = lo_i + lo_n;
@@ -142,6 +144,21 @@ static void init_field(HostField& F, uint32_t p, uint32_t root) {
     cf = F.one; ci = F.one;
     for (size_t k = 0; k < hi_n; ++k) {
       hi_f[k] = cf; hi_i[k] = ci;
+      cf = F.mul(cf, sf); ci = F.mul(ci, si);
+    }
+  }
+  {
+    uint32_t* co = h.data() + tw_words + 2 * kPlaneWords;
+    const uint32_t gi = F.inv(g);
+    uint32_t cf = F.one, ci = F.one;
+    for (size_t k = 0; k < lo_n; ++k) {
+      co[k] = cf; co[lo_n + k] = ci;
+      cf = F.mul(cf, g); ci = F.mul(ci, gi);
+    }
+    const uint32_t sf = F.pow(g, lo_n), si = F.pow(gi, lo_n);
+    cf = F.one; ci = F.one;
+    for (size_t k = 0; k < hi_n; ++k) {
+      co[2 * lo_n + k] = cf; co[2 * lo_n + hi_n + k] = ci;
       cf = F.mul(cf, sf); ci = F.mul(ci, si);
     }
   }
@@ -633,13 +650,14 @@ void ntt_forward(Context& c, const HostField& F, uint32_t* dst, const uint32_t* 
 // split product's spectra, a few launches that each fill the GPU several
 // times over instead of one under-filled launch per pass and transform.
 static void ntt_forward_multi(Context& c, const HostField& F, uint32_t* const* dst, const uint32_t* const* src,
-                              const uint32_t* in_count, int k, int L) {
+                              const uint32_t* in_count, int k, int L, const uint32_t* col_flags = nullptr) {
   // Transforms of 2^22 points and more: rows of 2^14 words, so that the row
   // passes of all k transforms run as one persistent prefetching pass.
   const PassPlan P = (L >= 22 && L - kDmaRowLog <= 12 && c.row_log_max >= kDmaRowLog)
                          ? PassPlan{L, kDmaRowLog, L - kDmaRowLog, tile_log_cols(L - kDmaRowLog, L)}
                          : c.plan(L);
   if (P.S == 0 || c.rec) {
+    if (col_flags) throw Error(E_DEVICE, "twisted transforms need a column pass and an immediate launch");
     for (int i = 0; i < k; ++i) ntt_forward(c, F, dst[i], src[i], in_count[i], L, false, true);
     return;
   }
@@ -650,6 +668,7 @@ static void ntt_forward_multi(Context& c, const HostField& F, uint32_t* const* d
     for (int i = 0; i < n; ++i) {
       cj[i] = ColJob2{};
       cj[i].src = src[i0 + i]; cj[i].dst = dst[i0 + i]; cj[i].in_count = in_count[i0 + i]; cj[i].f = F.dev;
+      cj[i].flags = col_flags ? col_flags[i0 + i] : 0u;
     }
     launch_cols_n(c, F.lazy, false, cj, P, (uint32_t)n, nullptr);
   }
@@ -952,6 +971,61 @@ static void split_conv(Context& c, const HostField& T, const uint32_t* a, size_t
   }
 }
 
+// Products of two operands of at most N = 2^M coefficients (M = min(v2(p -
+// 1), 26)) whose full length exceeds N, over a lazy NTT prime with N < p - 1:
+// c = c_lo + x^N c_hi from two N-point cyclic products, one at the roots of
+// unity w^k and one on the coset g w^k (g a primitive root, G = g^N != 1):
+// the first gives c_lo + c_hi, the second, untwisted by g^-t, c_lo + G c_hi,
+// so c_hi = (d - c1) / (G - 1) and c_lo = c1 - c_hi, in the second inverse
+// column pass's epilogue.  Four forward transforms (a and b, plain and
+// twisted by g^t in the column pass's loads) and two inverse ones, against
+// the split product's four and three; the product is unique, so the limbs
+// are the same.
+static bool coset_ok(const Context& c, const HostField& T, size_t la, size_t lb) {
+  if (T.p == 2 || T.maxlog < 16 || !T.lazy || c.rec) return false;
+  const int M = std::min(T.maxlog, 26);
+  const size_t N = (size_t)1 << M;
+  return la <= N && lb <= N && la + lb - 1 > N && (uint64_t)T.p - 1 > N && c.plan(M).S > 0;
+}
+
+static void coset_conv(Context& c, const HostField& T, const uint32_t* a, size_t la, const uint32_t* b, size_t lb,
+                       uint32_t* out, size_t outc) {
+  const int M = std::min(T.maxlog, 26);
+  const size_t N = (size_t)1 << M;
+  const bool same = a == b && la == lb;
+  DevBuf SA = c.buf(N), SA2 = c.buf(N), SB, SB2;
+  uint32_t* fd[4] = {SA.get(), SA2.get(), nullptr, nullptr};
+  const uint32_t* fs[4] = {a, a, b, b};
+  uint32_t fc[4] = {(uint32_t)la, (uint32_t)la, (uint32_t)lb, (uint32_t)lb};
+  const uint32_t ff[4] = {0u, R2_TWIST, 0u, R2_TWIST};
+  if (!same) {
+    SB = c.buf(N);
+    SB2 = c.buf(N);
+    fd[2] = SB.get();
+    fd[3] = SB2.get();
+  }
+  ntt_forward_multi(c, T, fd, fs, fc, same ? 2 : 4, M, ff);
+  const uint32_t* B1 = same ? SA.get() : SB.get();
+  const uint32_t* B2 = same ? SA2.get() : SB2.get();
+  const PassPlan P = c.plan(M);
+  DevBuf W = c.buf(N), C1 = c.buf(N);
+  RowJob2 rj{};
+  rj.f = T.dev; rj.dst = W.get(); rj.nrows = 1u << P.S; rj.in_a = (uint32_t)N; rj.out_lo = 0; rj.out_hi = (uint32_t)N;
+  rj.src_a = SA.get(); rj.src_b = B1;
+  launch_rows(c, T.lazy, R2M_SPEC_ONLY, rj, P.b, rj.nrows);
+  ColJob2 cj{};
+  cj.src = W.get(); cj.dst = C1.get(); cj.out_lo = 0; cj.out_hi = (uint32_t)N; cj.out_shift = 0;
+  cj.inv_n = inv_pow2(T, M); cj.f = T.dev;
+  launch_cols(c, T.lazy, kColInv, cj, P);
+  rj.src_a = SA2.get(); rj.src_b = B2;
+  launch_rows(c, T.lazy, R2M_SPEC_ONLY, rj, P.b, rj.nrows);
+  const uint32_t G = T.pow(T.from_int(T.root), (uint64_t)N);
+  cj.dst = out; cj.out_hi = (uint32_t)std::min(outc, N);
+  cj.flags = R2_COSET; cj.epi = C1.get(); cj.epi_avail = (uint32_t)(outc > N ? outc - N : 0);
+  cj.acc_end = T.inv(T.sub(G, T.one));
+  launch_cols(c, T.lazy, kColInv, cj, P);
+}
+
 // The path conv_dev takes for a product of la x lb coefficients (both already
 // clamped to outc), decided from the sizes and the field alone.
 enum ConvRoute { CR_DIRECT, CR_TAIL, CR_NATIVE, CR_SPLIT, CR_CRT, CR_BLOCKS };
@@ -1045,7 +1119,8 @@ void conv_dev(Context& c, const HostField& T, const uint32_t* a, size_t la, cons
       conv_core(c, T, a, la, b, lb, out, outc, false);
       return;
     case CR_SPLIT:
-      split_conv(c, T, a, la, b, lb, out, outc);
+      if (coset_ok(c, T, la, lb)) coset_conv(c, T, a, la, b, lb, out, outc);
+      else split_conv(c, T, a, la, b, lb, out, outc);
       return;
     case CR_CRT:
       crt_window(c, T, a, la, b, lb, out, outc, false);

@@ -497,7 +497,14 @@ struct RowJob2 {
 // By-value jobs per row launch: one for each of the three auxiliary primes.
 constexpr int kRowPack = 3;
 enum : uint32_t { R2_LIFT = 1, R2_CANON = 2, R2_SCALE = 4, R2_NEG = 8, R2_ACC = 16, R2_SUBFROM = 32, R2_MULTAB = 64,
-                  R2_PREFIX = 128, R2_ACCLO = 256 };
+                  R2_PREFIX = 128, R2_ACCLO = 256, R2_TWIST = 512, R2_COSET = 1024 };
+// R2_TWIST (forward column pass only): input coefficient t times g^t, g the
+// field's coset generator (so the spectrum is that of the coset g w^k).
+// R2_COSET (inverse column pass only): the output v at position t < 2^L is
+// the untwisted coset product's coefficient d = v g^-t; with c1 = epi[t]
+// (the cyclic product's coefficient) and acc_end = 1/(g^N - 1), the high
+// coefficient h = (d - c1) / (g^N - 1) goes to dst[N + t] (t < epi_avail)
+// and the low one c1 - h to dst[t].
 // R2_PREFIX (chained column pass only): the operand's positions below
 // epi_avail come from epi, and its window is also stored to epi.
 // R2_ACCLO (column inverse only): like R2_ACC for output positions below
@@ -1252,6 +1259,18 @@ struct ColJob2 {
 // the three auxiliary primes.
 constexpr int kColPack = 6;
 
+// g^e (inverse: g^-e) for e < 2^logmax, g the field's coset generator: the
+// coset tables follow the inverse twiddle planes in the field's table block,
+// split like the root tables (lo[e & mask], hi[e >> split]).
+template <class Ar>
+__device__ __forceinline__ uint32_t coset_pow(const FieldDev& f, const Ar& A, uint32_t e, bool inv) {
+  typedef __attribute__((address_space(1))) const uint32_t g32;
+  const uint32_t lo_n = 1u << f.split, hi_n = 1u << (f.logmax - f.split);
+  const uint32_t* ct = f.planes_inv + kPlaneWords + (inv ? lo_n : 0u);
+  const uint32_t* ch = f.planes_inv + kPlaneWords + 2 * lo_n + (inv ? hi_n : 0u);
+  return A.mul(*(g32*)(ch + (e >> f.split)), *(g32*)(ct + (e & (lo_n - 1))));
+}
+
 template <int S, int LOGC, int E = 4>
 struct ColShape {
   static constexpr int GT = Sched<S, E>::GT;
@@ -1378,6 +1397,20 @@ k_cols2(JobPack<ColJob2, kColPack> jp, const ColJob2* __restrict__ jobs, int L) 
 #pragma unroll
         for (int i = 0; i < NE0; ++i) x[i] = A.mul(x[i], J.f.r2);
       }
+      if (flags & R2_TWIST) {
+        // Element (u, k) is coefficient t = gid_u * M + col + k * (M << lo0):
+        // g^t from the coset tables for k = 0, then a running product.
+        const uint32_t Dg = coset_pow(J.f, A, M << lo0, false);
+#pragma unroll
+        for (int u = 0; u < PhaseGroups<S, 0, LOGC, E>::NG; ++u) {
+          uint32_t q = coset_pow(J.f, A, PhaseGroups<S, 0, LOGC, E>::gid(g, u) * M + col, false);
+#pragma unroll
+          for (int k = 0; k < (1 << r0); ++k) {
+            x[u * (1 << r0) + k] = A.mul(x[u * (1 << r0) + k], q);
+            if (k + 1 < (1 << r0)) q = A.mul(q, Dg);
+          }
+        }
+      }
     }
     const ArithS Sg(J.f);
     if constexpr (!ColTune<S>::PF) tw_load<S, 0, LOGC, E>(T0, J.f.planes_fwd, g);  // after the data: registers
@@ -1417,7 +1450,7 @@ k_cols2(JobPack<ColJob2, kColPack> jp, const ColJob2* __restrict__ jobs, int L) 
     // The epilogue operand of every output (epi for R2_SUBFROM / R2_MULTAB,
     // else the destination itself for R2_ACC / R2_ACCLO), issued with the
     // data at clamped positions; unused values are never read.
-    const bool epi_op = (flags & (R2_SUBFROM | R2_MULTAB)) != 0;
+    const bool epi_op = (flags & (R2_SUBFROM | R2_MULTAB | R2_COSET)) != 0;
     const bool acc_op = (flags & (R2_ACC | R2_ACCLO)) != 0;
     const uint32_t win = J.out_hi > J.out_shift ? J.out_hi - J.out_shift : 0u;  // output words past out_shift
     const uint32_t elen = epi_op ? ((flags & R2_SUBFROM) ? J.epi_avail : win) : (acc_op ? win : 0u);
@@ -1517,13 +1550,20 @@ k_cols2(JobPack<ColJob2, kColPack> jp, const ColJob2* __restrict__ jobs, int L) 
       return;
     }
     const uint32_t p = J.f.p;
+    // R2_COSET: g^-gi by the forward twist's running product (position gi =
+    // gid_u * M + col + k * (M << (S - r0))).
+    const bool coset = (flags & R2_COSET) != 0;
+    const uint32_t Dc = coset ? coset_pow(J.f, A, M << (S - r0), true) : 0u;
 #pragma unroll
-    for (int u = 0; u < PhaseGroups<S, 0, LOGC, E>::NG; ++u)
+    for (int u = 0; u < PhaseGroups<S, 0, LOGC, E>::NG; ++u) {
+      uint32_t cq = coset ? coset_pow(J.f, A, PhaseGroups<S, 0, LOGC, E>::gid(g, u) * M + col, true) : 0u;
 #pragma unroll
       for (int k = 0; k < (1 << r0); ++k) {
         const uint32_t e = elem<S, 0, E>(PhaseGroups<S, 0, LOGC, E>::gid(g, u), k);
         const uint32_t gi = e * M + col;
         const uint32_t v = A.canon(A.half(x[u * (1 << r0) + k]));
+        const uint32_t gq = A.canon(A.half(cq));
+        if (coset && k + 1 < (1 << r0)) cq = A.mul(cq, Dc);
         if (gi >= J.out_lo && gi < J.out_hi) {
           const uint32_t t = gi - J.out_shift;
           uint32_t w = neg ? (v == 0 ? 0u : p - v) : v;
@@ -1533,6 +1573,13 @@ k_cols2(JobPack<ColJob2, kColPack> jp, const ColJob2* __restrict__ jobs, int L) 
           }
           if (flags & R2_MULTAB)
             w = mulmod_full(w, kEvPF ? ev[u * (1 << r0) + k] : gload(J.epi + t), p, J.f.ninv);
+          if (coset) {
+            const uint32_t c1 = kEvPF ? ev[u * (1 << r0) + k] : gload(J.epi + t);
+            const uint32_t d = mulmod_full(w, gq, p, J.f.ninv);
+            const uint32_t h = mulmod_full(d >= c1 ? d - c1 : d - c1 + p, J.acc_end, p, J.f.ninv);
+            if (gi < J.epi_avail) gstore(J.dst + (M << S) + gi, h);
+            w = c1 >= h ? c1 - h : c1 - h + p;
+          }
           if ((flags & R2_ACC) || ((flags & R2_ACCLO) && t < J.acc_end)) {
             // the destination's old value: prefetched unless an epilogue operand took the registers
             const uint32_t old = (epi_op || !kEvPF) ? gload(J.dst + t) : ev[u * (1 << r0) + k];
@@ -1542,6 +1589,7 @@ k_cols2(JobPack<ColJob2, kColPack> jp, const ColJob2* __restrict__ jobs, int L) 
           gstore(J.dst + t, w);
         }
       }
+    }
   }
 }
 
