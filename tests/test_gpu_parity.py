@@ -270,23 +270,30 @@ def test_conv_tail_repair(fa, oracle, p):
     # 65537 = 2^16 + 1: largest transform 2^16, pieces of 2^15
     (65537, [(70000, 50000), (3 * 32768 + 7, 5), (65537, 65537), (40000, 130000), (1 << 16, 1 << 16),
              (1 << 19, 1 << 19)]),  # the last: 16 x 16 pieces, past the split limit (three-prime path)
-    # 786433 = 3 * 2^18 + 1: largest transform 2^18, pieces of 2^17
-    (786433, [(1 << 18, 1 << 18), (300000, 200001), (5, 262145), (131073, 393215)]),
+    # 786433 = 3 * 2^18 + 1: largest transform 2^18, pieces of 2^17; operands
+    # of at most 2^18 coefficients take the coset product (cyclic and
+    # g-twisted spectra, 786432 > 2^18)
+    (786433, [(1 << 18, 1 << 18), (300000, 200001), (5, 262145), (131073, 393215), (200000, 150000),
+              (1 << 18, 1000), (140000, 1 << 18)]),
 ])
 def test_conv_split_beyond_max_transform(fa, oracle, p, shapes):
-    """Products longer than the prime's largest transform: one spectrum per
-    half-length piece, block sums inverted and accumulated at their offsets;
-    uneven piece counts, truncated outputs and squaring included."""
+    """Products longer than the prime's largest transform N: operands of at
+    most N coefficients as the coset product when N < p - 1 (65537 has no
+    coset: p - 1 = N), else one spectrum per half-length piece, block sums
+    inverted and accumulated at their offsets; uneven piece counts, truncated
+    outputs (inside the low half) and squaring included."""
     rng = random.Random(p + 1)
     for la, lb in shapes:
         a = _rnd(oracle, p, la, rng.randrange(1 << 30))
         b = _rnd(oracle, p, lb, rng.randrange(1 << 30))
         want = oracle.conv(p, a, b)
         assert np.array_equal(fa.conv(p, a, b), want), (p, la, lb)
-        n = (la + lb) // 3 + 1
-        assert np.array_equal(fa.conv(p, a, b, n), want[:n]), (p, la, lb, n)
-    x = _rnd(oracle, p, 100000, 7)
-    assert np.array_equal(fa.conv(p, x, x), oracle.conv(p, x, x))
+        big = (p - 1) & -(p - 1)  # the largest transform
+        for n in ((la + lb) // 3 + 1, min(la + lb - 1, big + 5)):
+            assert np.array_equal(fa.conv(p, a, b, n), want[:n]), (p, la, lb, n)
+    for lx in (100000, 200000):
+        x = _rnd(oracle, p, lx, 7)
+        assert np.array_equal(fa.conv(p, x, x), oracle.conv(p, x, x)), (p, lx)
 
 
 @pytest.mark.parametrize("p,op,n,want", [
