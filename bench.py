@@ -194,7 +194,7 @@ def main():
         "value": value, "unit": "coeffs/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": ms_per_step, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "u32", "data": "synthetic (uniform [0,p) Montgomery limbs, numpy PCG64 seed 20261203)",
-        "config": {"workload": "single large transforms mod 998244353: conv 2^23 x 2^23 (split into 2^23-point native NTTs) "
+        "config": {"workload": "single large transforms mod 998244353: conv 2^23 x 2^23 (coset product of 2^23-point native NTTs) "
                                "+ inv/log/exp/sqrt at n = 2^20",
                    "modulus": P, "conv_len": N_CONV, "series_n": N_SERIES},
         "step_ms_by_op": by_op,
