@@ -193,7 +193,9 @@ __device__ __forceinline__ uint32_t ew_value(const EwRec& R, uint32_t i) {
     case EW_MUL_RECIP: return fs_mul(f, R.src[i], fs_inv(f, fs_from_int(f, R.c + i)));
     case EW_DERIV: return fs_mul(f, R.src[i + 1], fs_from_int(f, i + 1));
     case EW_THETA: return fs_mul(f, R.src[i], fs_from_int(f, i));
-    case EW_INTEGRAL: return (i >= 1 && i - 1 < R.avail) ? fs_mul(f, R.src[i - 1], fs_inv(f, fs_from_int(f, i))) : 0u;
+    case EW_INTEGRAL:  // src2: a table of 1/i, if given
+      return (i >= 1 && i - 1 < R.avail) ? fs_mul(f, R.src[i - 1], R.src2 ? R.src2[i] : fs_inv(f, fs_from_int(f, i)))
+                                         : 0u;
     case EW_REVERSE: return R.src[R.n - 1 - i];
     case EW_RECIP: return R.c + i == 0 ? 0u : fs_inv(f, fs_from_int(f, R.c + i));
     case EW_MULADD: {
@@ -248,7 +250,7 @@ __global__ void __launch_bounds__(256) k_ew(const EwRec* __restrict__ recs, cons
   make_uniform(R);
   start = __builtin_amdgcn_readfirstlane(start);
   const uint32_t end = min(R.n, start + kChunk);
-  if (R.op == EW_RECIP || R.op == EW_MUL_RECIP || R.op == EW_INTEGRAL) {  // uniform branch
+  if (R.op == EW_RECIP || R.op == EW_MUL_RECIP || (R.op == EW_INTEGRAL && !R.src2)) {  // uniform branch
     for (uint32_t i0 = start + 4 * threadIdx.x; i0 < end; i0 += 4 * blockDim.x) ew_recip4(R, i0, end);
     return;
   }
@@ -538,6 +540,26 @@ static void emit_ew(Context& c, const HostField& T, int op, uint32_t* dst, const
   const unsigned blocks = (unsigned)((n + kChunk - 1) / kChunk);
   hipLaunchKernelGGL(k_ew, dim3(blocks), dim3(256), 0, c.stream, (const EwRec*)nullptr, (const uint2*)nullptr, r);
   FPXG_HIP(hipGetLastError());
+}
+
+// A device table of 1/i for i < n (entry 0 = 0) over T's field, kept for the
+// process: computed once per modulus and grown (never shrunk) by immediate
+// calls, read by exp's Newton steps and log's integral.  Returns null where
+// none of at least n entries exists and none may be made here (recording:
+// the table would be computed before the launches recorded ahead of it run)
+// or n is above 2^24 (64 MiB).
+static const uint32_t* recip_table(Context& c, const HostField& T, size_t n) {
+  auto it = c.recip_tabs.find(T.p);
+  if (it != c.recip_tabs.end() && it->second.second >= n) return it->second.first;
+  if (c.rec || n > ((size_t)1 << 24)) return nullptr;
+  size_t len = (size_t)1 << 16;
+  while (len < n) len <<= 1;
+  uint32_t* t = nullptr;
+  FPXG_HIP(hipMalloc((void**)&t, len * 4));
+  if (it != c.recip_tabs.end()) c.recip_old.push_back(it->second.first);
+  c.recip_tabs[T.p] = {t, len};
+  emit_ew(c, T, EW_RECIP, t, nullptr, nullptr, len, 0, 0);
+  return t;
 }
 
 // Seeds (dot products of at most ~128 terms) run on one wave; longer tails on

@@ -1291,6 +1291,8 @@ void fpxg_shutdown(void) {
   }
   for (auto& kv : g_ctx->tickets) (void)hipEventDestroy(kv.second);
   if (g_ctx->scan_host) (void)hipHostFree(g_ctx->scan_host);
+  for (auto& kv : g_ctx->recip_tabs) (void)hipFree(kv.second.first);
+  for (uint32_t* t : g_ctx->recip_old) (void)hipFree(t);
   (void)hipStreamDestroy(g_ctx->stream);
   g_ctx.reset();
 }

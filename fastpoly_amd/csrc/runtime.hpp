@@ -259,6 +259,12 @@ struct Context {
   CrtFields crt_f{};  // the auxiliary primes' tables, k_cols2_crt's kernel argument
   bool crt_f_ok = false;
   std::unordered_map<uint32_t, bool> prime_cache;          // modulus -> accepted
+  // Reciprocal tables 1/i (entry 0 = 0) by modulus: the largest one and its
+  // length.  A table is only ever replaced by a longer one, and replaced
+  // tables stay allocated until shutdown (cached graphs and recorded
+  // schedules may still read them).
+  std::unordered_map<uint32_t, std::pair<uint32_t*, size_t>> recip_tabs;
+  std::vector<uint32_t*> recip_old;
   // Outstanding batch tickets (completion events).
   std::map<uint64_t, hipEvent_t> tickets;
   uint64_t next_ticket = 1;

@@ -511,7 +511,7 @@ static void series_emit(Context& c, const HostField& T, const SeriesReq& q, cons
         lp = std::min(ld + (n - 1) - 1, n - 1);
         conv_dev(c, T, d.get(), ld, ia.get(), n - 1, pr.get(), lp);
       }
-      emit_ew(c, T, EW_INTEGRAL, out, pr.get(), nullptr, n, (uint32_t)lp, 0);
+      emit_ew(c, T, EW_INTEGRAL, out, pr.get(), recip_table(c, T, n), n, (uint32_t)lp, 0);
       return;
     }
     case FPXG_EXP: {  // poly::exp (exponential.hpp:107-207)
@@ -544,16 +544,21 @@ static void series_emit(Context& c, const HostField& T, const SeriesReq& q, cons
       // small-series kernel, the direct steps (up to 2^9 points: indices
       // below 512) and, on native fields, the product epilogues.  (Other
       // fields read it only below 512.)
-      DevBuf rtab;
+      // (The process-wide table where there is one, else a table of this call.)
       const size_t nr = series_native(T, 6) ? n : std::min<size_t>(n, std::max<size_t>(kSmallSeries, 512));
-      rtab = c.buf(nr);
-      emit_ew(c, T, EW_RECIP, rtab.get(), nullptr, nullptr, nr, 0, 0);
+      DevBuf rbuf;
+      const uint32_t* rtab = recip_table(c, T, nr);
+      if (!rtab) {
+        rbuf = c.buf(nr);
+        emit_ew(c, T, EW_RECIP, rbuf.get(), nullptr, nullptr, nr, 0, 0);
+        rtab = rbuf.get();
+      }
       // The first min(n, 512) coefficients of b, and c = 1/b to half of that.
       const size_t n0 = std::min<size_t>(n, kSmallSeries);
       DevBuf cbuf = c.buf(n);
       {
         RowJob2 rj{};
-        rj.src_a = ap; rj.in_a = (uint32_t)std::min(lap, n0 - 1); rj.epi = rtab.get();
+        rj.src_a = ap; rj.in_a = (uint32_t)std::min(lap, n0 - 1); rj.epi = rtab;
         rj.dst = b; rj.src_c = cbuf.get();
         small_series(c, T, R2M_NEXP, rj, n0);
       }
@@ -566,14 +571,14 @@ static void series_emit(Context& c, const HostField& T, const SeriesReq& q, cons
           continue;
         }
         Engine E(c, T, ceil_log2(2 * m));
-        if (rtab.get() && exp_fused_ok(c, T, E, m, hlen, csize)) {
-          exp_fused_doubling(c, T, E.L, th.get(), std::min(lap, m2 - 1) + 1, b, m, cbuf.get(), rtab.get() + m);
+        if (rtab && exp_fused_ok(c, T, E, m, hlen, csize)) {
+          exp_fused_doubling(c, T, E.L, th.get(), std::min(lap, m2 - 1) + 1, b, m, cbuf.get(), rtab + m);
           csize = m;
           continue;
         }
         // The c extension's first product and r = (a' b)[m - 1, ...) depend
         // only on b and the old c: one set of launches for both.
-        const bool merged = csize < hlen && E.native && rtab.get() && !one_launch_step(c, T, E, m) &&
+        const bool merged = csize < hlen && E.native && rtab && !one_launch_step(c, T, E, m) &&
                             cext_mergeable(c, T, csize, hlen, E.L);
         DevBuf r;
         if (merged) {
@@ -588,13 +593,13 @@ static void series_emit(Context& c, const HostField& T, const SeriesReq& q, cons
           extend_inverse(c, T, b, m, cbuf.get(), csize, hlen);
           csize = hlen;
         }
-        if (rtab.get() && one_launch_step(c, T, E, m)) {
+        if (rtab && one_launch_step(c, T, E, m)) {
           // The step's three products in one workgroup (R2M_EXPSTEP).
           RowJob2 rj{};
           rj.f = T.dev;
           rj.src_a = ap; rj.in_a = (uint32_t)std::min(lap, m2 - 1);
           rj.src_b = b; rj.in_b = (uint32_t)m;
-          rj.src_c = cbuf.get(); rj.aux = (uint32_t)(m - 1); rj.epi = rtab.get() + m;
+          rj.src_c = cbuf.get(); rj.aux = (uint32_t)(m - 1); rj.epi = rtab + m;
           rj.dst = b + m; rj.nrows = 1; rj.out_lo = 0; rj.out_hi = (uint32_t)hlen; rj.out_shift = 0;
           rj.inv_n = inv_pow2(T, E.L);
           rj.flags = R2_SCALE | R2_CANON;
@@ -602,9 +607,9 @@ static void series_emit(Context& c, const HostField& T, const SeriesReq& q, cons
           continue;
         }
         if (!merged) r = c.buf(hlen);
-        if (E.native && rtab.get() && !merged)
+        if (E.native && rtab && !merged)
           native_conv(c, T, E.L, ap, std::min(lap, m2 - 1), b, m, m - 1, m - 1 + hlen, r.get());
-        if (E.native && rtab.get() && T.lazy && ((size_t)1 << E.L) == 2 * m && c.conv_plan(E.L).S >= 5) {
+        if (E.native && rtab && T.lazy && ((size_t)1 << E.L) == 2 * m && c.conv_plan(E.L).S >= 5) {
           // t = (r c)[0, h) / (m + i) hands itself to out = (t b)[0, h)
           // through the chained pass; b's column spectrum rides in the first
           // product's column launch: five launches instead of six.
@@ -627,7 +632,7 @@ static void series_emit(Context& c, const HostField& T, const SeriesReq& q, cons
           launch_rows(c, T.lazy, R2M_CONV, rj, P.b, rj.nrows);
           ColJob2 ch{};
           ch.src = R.get(); ch.dst = Tz.get(); ch.out_lo = 0; ch.out_hi = (uint32_t)hlen; ch.out_shift = 0;
-          ch.inv_n = inv_n; ch.flags = R2_MULTAB; ch.epi = rtab.get() + m; ch.f = T.dev;
+          ch.inv_n = inv_n; ch.flags = R2_MULTAB; ch.epi = rtab + m; ch.f = T.dev;
           launch_cols(c, T.lazy, kColChain, ch, P);
           rj.src_a = Tz.get(); rj.src_b = Bs.get(); rj.dst = Tz.get();
           launch_rows(c, T.lazy, R2M_CONV, rj, P.b, rj.nrows);
@@ -638,10 +643,10 @@ static void series_emit(Context& c, const HostField& T, const SeriesReq& q, cons
           continue;
         }
         DevBuf t = c.buf(hlen);
-        if (E.native && rtab.get()) {
+        if (E.native && rtab) {
           Epi ep;
           ep.flags = R2_MULTAB;
-          ep.src = rtab.get() + m;
+          ep.src = rtab + m;
           native_conv(c, T, E.L, r.get(), hlen, cbuf.get(), hlen, 0, hlen, t.get(), false, ep);
           native_conv(c, T, E.L, t.get(), hlen, b, m, 0, hlen, b + m);
           continue;
