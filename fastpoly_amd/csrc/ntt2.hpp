@@ -1132,9 +1132,9 @@ __global__ void __launch_bounds__(kMidTPO << LM) k_step_direct(RowJob2 j0, const
 // are the same.  Job fields (out_hi = N):
 //   R2M_NINV : c = 1/a mod x^N; src_a = a (in_a words, a[0] != 0), dst = c.
 //   R2M_NEXP : b = exp(a) mod x^N; src_a = a' (in_a words), epi = 1/i table,
-//              dst = b; src_c (an output here) <- c = 1/b mod x^(N/2).
+//              dst = b; src_c (an output here) <- c = 1/b mod x^N.
 //   R2M_NSQRT: b = sqrt(a) mod x^N with b[0] = aux; src_a = a (in_a words),
-//              inv_n = 1/aux, dst = b; src_c (an output) <- c = 1/b mod x^(N/2).
+//              inv_n = 1/aux, dst = b; src_c (an output) <- c = 1/b mod x^N.
 // Each doubling keeps c = 1/b to the precision of the next step's half
 // (extend_c, the inverse step with b in place of a).  kSmallThreads =
 // 1024 threads, four an output of each product (each product has at most
@@ -1227,6 +1227,14 @@ __global__ void __launch_bounds__(kSmallThreads) k_newton_small(RowJob2 j0, cons
         if (par == 0 && o < h) B[m + o] = mulmod_full(v, half, p, ninv);
       }
       __syncthreads();
+    }
+  }
+  if constexpr (MODE != R2M_NINV) {
+    // c to all N coefficients: the first Newton step after this launch needs
+    // 1/b to its full length, and b[0, N) is final here.
+    if (cp < N) {
+      extend_c(B, N, cp, N);
+      cp = N;
     }
   }
   if constexpr (MODE == R2M_NINV) {

@@ -553,7 +553,7 @@ static void series_emit(Context& c, const HostField& T, const SeriesReq& q, cons
         emit_ew(c, T, EW_RECIP, rbuf.get(), nullptr, nullptr, nr, 0, 0);
         rtab = rbuf.get();
       }
-      // The first min(n, 512) coefficients of b, and c = 1/b to half of that.
+      // The first min(n, 512) coefficients of b, and c = 1/b to as many.
       const size_t n0 = std::min<size_t>(n, kSmallSeries);
       DevBuf cbuf = c.buf(n);
       {
@@ -563,7 +563,7 @@ static void series_emit(Context& c, const HostField& T, const SeriesReq& q, cons
         small_series(c, T, R2M_NEXP, rj, n0);
       }
       if (n == n0) return;
-      size_t csize = n0 / 2;
+      size_t csize = n0;  // the small-series launch leaves c = 1/b mod x^n0
       for (size_t m = n0; m < n; m *= 2) {
         const size_t m2 = std::min(2 * m, n), hlen = m2 - m;
         if (hlen <= 8) {
@@ -686,7 +686,7 @@ static void series_emit(Context& c, const HostField& T, const SeriesReq& q, cons
       const uint32_t half_inv = T.from_int(((uint64_t)T.p + 1) / 2);
       const uint32_t tri = T.mul(T.inv(root0), half_inv);
       const uint32_t* av = a + v;
-      // The first min(inner, 512) coefficients of b, and c = 1/b to half of that.
+      // The first min(inner, 512) coefficients of b, and c = 1/b to as many.
       const size_t n0 = std::min<size_t>(inner, kSmallSeries);
       DevBuf cbuf = c.buf(inner);
       {
@@ -696,7 +696,7 @@ static void series_emit(Context& c, const HostField& T, const SeriesReq& q, cons
         small_series(c, T, R2M_NSQRT, rj, n0);
       }
       if (inner == n0) return;
-      size_t csize = n0 / 2;
+      size_t csize = n0;  // the small-series launch leaves c = 1/b mod x^n0
       for (size_t m = n0; m < inner; m *= 2) {
         const size_t m2 = std::min(2 * m, inner), hlen = m2 - m;
         if (hlen <= 8) {
