@@ -1018,7 +1018,9 @@ constexpr uint32_t kMidTPO = 4;  // threads an output of a direct middle product
 
 // sum_j X[j] Y[o + off - j] over 0 <= j < nx, 0 <= o + off - j < ny, for an
 // output o < h: the terms with j = par mod kMidTPO, added to those of the
-// output's other kMidTPO - 1 lanes (adjacent lanes of one wave).
+// output's other kMidTPO - 1 lanes (adjacent lanes of one wave).  Canonical
+// limbs; products are summed in 64 bits before each Montgomery reduction,
+// four at a time for p < 2^30 (4 p^2 < p 2^32), else two (2 p^2 < p 2^32).
 __device__ __forceinline__ uint32_t mid_product(const uint32_t* X, uint32_t nx, const uint32_t* Y, uint32_t ny,
                                                 uint32_t off, uint32_t o, uint32_t par, uint32_t h, uint32_t p,
                                                 uint32_t ninv) {
@@ -1029,6 +1031,14 @@ __device__ __forceinline__ uint32_t mid_product(const uint32_t* X, uint32_t nx, 
     const uint32_t jlo = s >= ny ? s - (ny - 1) : 0u;
     const uint32_t jhi = min(nx - 1, s);  // inclusive
     uint32_t j = jlo + ((par - jlo) & (D - 1));  // the first j >= jlo with j = par mod D
+    if (p < (1u << 30)) {  // uniform: four products below p 2^32 (4 p < 2^32), one reduction
+      for (; j + 3 * D <= jhi; j += 4 * D) {
+        const uint64_t t = (uint64_t)X[j] * Y[s - j] + (uint64_t)X[j + D] * Y[s - j - D] +
+                           (uint64_t)X[j + 2 * D] * Y[s - j - 2 * D] + (uint64_t)X[j + 3 * D] * Y[s - j - 3 * D];
+        const uint32_t z = acc + redc_full(t, p, ninv);
+        acc = z >= p ? z - p : z;
+      }
+    }
     for (; j + D <= jhi; j += 2 * D) {
       const uint32_t r = redc_full((uint64_t)X[j] * Y[s - j] + (uint64_t)X[j + D] * Y[s - j - D], p, ninv);
       const uint32_t z = acc + r;
