@@ -265,8 +265,8 @@ __global__ void __launch_bounds__(256) k_ew(const EwRec* __restrict__ recs, cons
 // tile) and the short operand are staged in LDS (zeros outside the inputs),
 // and each thread slides a four-word window down the long operand (one LDS
 // read per term for four outputs).  Two products are summed before each
-// Montgomery reduction (2 p^2 < p 2^32), so any odd prime p < 2^31 works:
-// no transform and no auxiliary primes.
+// Montgomery reduction (2 p^2 < p 2^32), four for p < 2^30, so any odd prime
+// p < 2^31 works: no transform and no auxiliary primes.
 // ---------------------------------------------------------------------------
 constexpr uint32_t kDirectShort = 256;
 constexpr uint32_t kDirectTile = 1024;
@@ -304,8 +304,11 @@ __device__ __forceinline__ void conv_direct_tile(const uint32_t* __restrict__ sh
   };
   // Two terms an iteration, their LDS reads issued together: term j uses the
   // window w, term j + 1 the window shifted by one (n1 = W[top - j - 1]).
-  uint32_t j = 0;
-  for (; j + 1 < ns; j += 2) {
+  // The 64-bit sums are reduced every iteration, or every second one for
+  // p < 2^30 (four products below p 2^32).
+  const bool pairs = p < (1u << 30);  // uniform
+  uint32_t j = 0, it = 0;
+  for (; j + 1 < ns; j += 2, ++it) {
     const uint64_t s0 = S[j], s1 = S[j + 1];
     const uint32_t n1 = W[top - j - 1];
     const uint32_t n2 = j + 2 < ns ? W[top - j - 2] : 0u;
@@ -314,7 +317,7 @@ __device__ __forceinline__ void conv_direct_tile(const uint32_t* __restrict__ sh
     a64[0] += s1 * n1;
 #pragma unroll
     for (int q = 1; q < 4; ++q) a64[q] += s1 * w[q - 1];
-    fold();
+    if (!pairs || (it & 1)) fold();
     w[3] = w[1];
     w[2] = w[0];
     w[1] = n1;
