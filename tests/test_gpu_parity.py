@@ -296,6 +296,38 @@ def test_conv_split_beyond_max_transform(fa, oracle, p, shapes):
         assert np.array_equal(fa.conv(p, x, x), oracle.conv(p, x, x)), (p, lx)
 
 
+def _eval_limbs(p, limbs, x0):
+    """Sum of v_i x0^i mod p over the values v_i of Montgomery limbs (numpy,
+    exact: every product below 2^60, every partial sum below 2^64)."""
+    n = len(limbs)
+    rinv = pow(1 << 32, p - 2, p)
+    v = limbs.astype(np.uint64) * np.uint64(rinv) % np.uint64(p)
+    blk = 4096
+    lo = np.array([pow(x0, i, p) for i in range(blk)], dtype=np.uint64)
+    hi = np.array([pow(x0, blk * j, p) for j in range((n + blk - 1) // blk)], dtype=np.uint64)
+    pw = (hi[:, None] * lo[None, :] % np.uint64(p)).reshape(-1)[:n]
+    return int((v * pw % np.uint64(p)).sum() % np.uint64(p))
+
+
+@pytest.mark.gpu
+def test_coset_product_full_size(fa):
+    """The coset product at the headline prime's largest transform, operands
+    of 2^23 and 2^23 - 12345 coefficients: the full product agrees with
+    a(x0) b(x0) at two points (a wrong low or high half fails both), and the
+    truncations inside and just past the low half are its prefixes.  (The
+    symmetric 2^23 x 2^23 product is pinned by the reference's checksum.)"""
+    rng = np.random.default_rng(23)
+    la, lb = 1 << 23, (1 << 23) - 12345
+    a = rng.integers(0, P, la, dtype=np.uint32)
+    b = rng.integers(0, P, lb, dtype=np.uint32)
+    full = fa.conv(P, a, b)
+    assert full.size == la + lb - 1
+    for x0 in (3, 123456789):
+        assert _eval_limbs(P, full, x0) == _eval_limbs(P, a, x0) * _eval_limbs(P, b, x0) % P, x0
+    for n in ((1 << 23) - 1000, (1 << 23) + 777):
+        assert np.array_equal(fa.conv(P, a, b, n), full[:n]), n
+
+
 @pytest.mark.parametrize("p,op,n,want", [
     (int(m), op, int(s), ck)
     for m, t in gi.reference_checksums().items() for op, sizes in t.items()
